@@ -1,0 +1,765 @@
+/*
+ * ak_oracle.c -- FP64 CPU oracle for the D3Q27 cumulant lattice-Boltzmann step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only the tests, the
+ * smoke check and the CPU-baseline leg of the benchmark load it.  The product
+ * path (aerokinetic_amd) never links, imports or calls it.
+ *
+ * It restates the reference algorithm with the reference's own data flow:
+ * two population arrays with one ghost layer on every side, pull streaming,
+ * boundary objects that write into ghost / solid slots between steps, and the
+ * reference's step order (src/aerokinetic/solver.py:252-279):
+ *
+ *     periodic z ghost copy -> boundary objects (farfield first, walls last)
+ *     -> momentum-exchange loads (sample steps) -> fused pull-stream + collide
+ *     -> swap.
+ *
+ * Pieces and the reference lines they follow:
+ *   - storage: zero-centred g = f - w, rho = 1 + sum g (solver.py:75-77,
+ *     convergence.py:89, docs/METHOD_AEROKINETIC_RU.txt:143-148);
+ *   - collision: compressible cumulant, shear rate omega_s on the five
+ *     deviatoric second-order cumulants, bulk rate omega_b on the trace, rate 1
+ *     for every higher cumulant group (solver.py:75-77, interface.py:127-129,
+ *     docs/METHOD_AEROKINETIC_RU.txt:222-226).  Written in f-space: central
+ *     moments, relaxation, Gaussian (zero higher cumulant) closure, chimera
+ *     back-transform;
+ *   - solid mask and wall links: even-odd ray test and first polygon crossing
+ *     (geometry.py:58-85), q = -1 fallback when q < 1/2 and the second fluid
+ *     node is solid (solver.py:100-122);
+ *   - Bouzidi interpolated bounce-back (lbmpy NoSlipLinearBouzidi as used at
+ *     solver.py:141, walls.py:21);
+ *   - characteristic farfield with NEEP reconstruction (farfield.py:34-224);
+ *   - legacy farfield: velocity bounce-back on x-, y-, y+ and the
+ *     history-carrying extrapolation outflow on x+ (solver.py:130-135);
+ *   - loads: F = sum c_d (f_out(x,d) + f_in(x+c_d, dbar)), z-moment about the
+ *     moment reference (solver.py:204-250) and the coefficient normalisation
+ *     (solver.py:27-30);
+ *   - seeded initial field and product-form equilibrium (solver.py:157-202).
+ *
+ * Direction index: i = 9 (cx+1) + 3 (cy+1) + (cz+1); opposite(i) = 26 - i.
+ * Array layout: cell (x,y,z) in ghosted coordinates, populations innermost.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NQ 27
+static int CX[NQ], CY[NQ], CZ[NQ];
+static double WT[NQ];
+static int tables_ready = 0;
+
+static void tables(void) {
+    if (tables_ready) return;
+    for (int i = 0; i < NQ; ++i) {
+        CX[i] = i / 9 - 1;
+        CY[i] = (i / 3) % 3 - 1;
+        CZ[i] = i % 3 - 1;
+        int n = abs(CX[i]) + abs(CY[i]) + abs(CZ[i]);
+        WT[i] = n == 0 ? 8.0 / 27.0 : n == 1 ? 2.0 / 27.0 : n == 2 ? 1.0 / 54.0 : 1.0 / 216.0;
+    }
+    tables_ready = 1;
+}
+
+typedef struct {
+    int x, y, z, d, elem;
+    double q;
+} wall_link;
+
+typedef struct {
+    int x, y, z, d, obj;
+    int edge[3], wrap[3];
+    double incoming, derivative;
+    long clock;
+} ff_link;
+
+typedef struct {
+    int n[3];
+    double nh[3], un_inf, ut_inf[3], P[3][3];
+    int evolve, glancing, use_axis[3];
+    double *initial; /* glancing faces: trace of the initial outgoing invariant, nx*ny*nz */
+} ff_object;
+
+typedef struct {
+    int x, y, z, d;
+    double pdf, pdf_nd;
+} out_link;
+
+typedef struct {
+    int nx, ny, nz, X, Y, Z;
+    long ncell;
+    double *src, *dst;
+    uint8_t *solid; /* ghosted xy mask, X*Y */
+    double omega_s, omega_b;
+    double U, rho0;
+    /* walls */
+    int nelem;
+    int nlinks;
+    wall_link *links;
+    long fallback_links;
+    double rx_off, ry_off, chord_cells;
+    /* farfield: 0 none (periodic x/y), 1 legacy, 2 characteristic */
+    int ff_kind;
+    int nff;
+    ff_link *ff;
+    ff_object obj[8];
+    int nubb, nout;
+    wall_link *ubb; /* q unused */
+    out_link *out;
+    long steps;
+} ako;
+
+static inline long cid(const ako *s, int x, int y, int z) {
+    return ((long)(x + 1) * s->Y + (y + 1)) * s->Z + (z + 1);
+}
+
+/* ---------------------------------------------------------------- collision */
+
+/* Compressible D3Q27 cumulant collision of one cell, f-space formulation.
+ * Rates: omega_s (deviatoric second order), omega_b (trace), 1 (all else). */
+static void collide(const double *g, double *out, double ws, double wb) {
+    double f[NQ];
+    double drho = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+    for (int i = 0; i < NQ; ++i) {
+        drho += g[i];
+        jx += CX[i] * g[i];
+        jy += CY[i] * g[i];
+        jz += CZ[i] * g[i];
+        f[i] = g[i] + WT[i];
+    }
+    const double rho = 1.0 + drho;
+    const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+    double kxx = 0, kyy = 0, kzz = 0, kxy = 0, kxz = 0, kyz = 0;
+    for (int i = 0; i < NQ; ++i) {
+        const double vx = CX[i] - ux, vy = CY[i] - uy, vz = CZ[i] - uz;
+        kxx += f[i] * vx * vx;
+        kyy += f[i] * vy * vy;
+        kzz += f[i] * vz * vz;
+        kxy += f[i] * vx * vy;
+        kxz += f[i] * vx * vz;
+        kyz += f[i] * vy * vz;
+    }
+    /* relax: trace toward rho (3 cs^2 rho), deviators and off-diagonals toward 0 */
+    const double tr = kxx + kyy + kzz;
+    const double trp = tr + wb * (rho - tr);
+    const double dxy = (1.0 - ws) * (kxx - kyy);
+    const double dxz = (1.0 - ws) * (kxx - kzz);
+    const double Kxx = (trp + dxy + dxz) / 3.0;
+    const double Kyy = (trp - 2.0 * dxy + dxz) / 3.0;
+    const double Kzz = (trp + dxy - 2.0 * dxz) / 3.0;
+    const double Kxy = (1.0 - ws) * kxy, Kxz = (1.0 - ws) * kxz, Kyz = (1.0 - ws) * kyz;
+
+    /* post-collision central moments K[a][b][c]: cumulants of order >= 3 vanish */
+    double K[3][3][3];
+    memset(K, 0, sizeof K);
+    K[0][0][0] = rho;
+    K[2][0][0] = Kxx; K[0][2][0] = Kyy; K[0][0][2] = Kzz;
+    K[1][1][0] = Kxy; K[1][0][1] = Kxz; K[0][1][1] = Kyz;
+    K[2][2][0] = (Kxx * Kyy + 2.0 * Kxy * Kxy) / rho;
+    K[2][0][2] = (Kxx * Kzz + 2.0 * Kxz * Kxz) / rho;
+    K[0][2][2] = (Kyy * Kzz + 2.0 * Kyz * Kyz) / rho;
+    K[2][1][1] = (Kxx * Kyz + 2.0 * Kxy * Kxz) / rho;
+    K[1][2][1] = (Kyy * Kxz + 2.0 * Kxy * Kyz) / rho;
+    K[1][1][2] = (Kzz * Kxy + 2.0 * Kxz * Kyz) / rho;
+    K[2][2][2] = (Kxx * Kyy * Kzz + 2.0 * (Kxx * Kyz * Kyz + Kyy * Kxz * Kxz + Kzz * Kxy * Kxy)
+                  + 8.0 * Kxy * Kxz * Kyz) / (rho * rho);
+
+    /* chimera back-transform, one axis at a time: (k0,k1,k2) about u -> populations */
+    double A[3][3][3], B[3][3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            const double k0 = K[a][b][0], k1 = K[a][b][1], k2 = K[a][b][2];
+            const double m1 = k1 + uz * k0, m2 = k2 + 2.0 * uz * k1 + uz * uz * k0;
+            A[a][b][1] = k0 - m2;
+            A[a][b][2] = 0.5 * (m2 + m1);
+            A[a][b][0] = 0.5 * (m2 - m1);
+        }
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) {
+            const double k0 = A[a][0][c], k1 = A[a][1][c], k2 = A[a][2][c];
+            const double m1 = k1 + uy * k0, m2 = k2 + 2.0 * uy * k1 + uy * uy * k0;
+            B[a][1][c] = k0 - m2;
+            B[a][2][c] = 0.5 * (m2 + m1);
+            B[a][0][c] = 0.5 * (m2 - m1);
+        }
+    for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+            const double k0 = B[0][b][c], k1 = B[1][b][c], k2 = B[2][b][c];
+            const double m1 = k1 + ux * k0, m2 = k2 + 2.0 * ux * k1 + ux * ux * k0;
+            A[1][b][c] = k0 - m2;
+            A[2][b][c] = 0.5 * (m2 + m1);
+            A[0][b][c] = 0.5 * (m2 - m1);
+        }
+    for (int i = 0; i < NQ; ++i) out[i] = A[CX[i] + 1][CY[i] + 1][CZ[i] + 1] - WT[i];
+}
+
+/* product-form equilibrium population of direction i, absolute (not shifted) */
+static double phi(int c, double q) {
+    return 2.0 / 3.0 - q * q + (1.5 * q * q - 0.5) * c * c + c * q / 2.0;
+}
+static double equilibrium(int i, double r, const double *u) {
+    return r * phi(CX[i], u[0]) * phi(CY[i], u[1]) * phi(CZ[i], u[2]);
+}
+
+/* ------------------------------------------------------------ construction */
+
+void *ako_create(int nx, int ny, int nz, double omega_s, double omega_b) {
+    tables();
+    if (nx < 4 || ny < 4 || nz < 1) return NULL;
+    ako *s = (ako *)calloc(1, sizeof(ako));
+    s->nx = nx; s->ny = ny; s->nz = nz;
+    s->X = nx + 2; s->Y = ny + 2; s->Z = nz + 2;
+    s->ncell = (long)s->X * s->Y * s->Z;
+    s->src = (double *)calloc((size_t)s->ncell * NQ, sizeof(double));
+    s->dst = (double *)calloc((size_t)s->ncell * NQ, sizeof(double));
+    s->solid = (uint8_t *)calloc((size_t)s->X * s->Y, 1);
+    s->omega_s = omega_s;
+    s->omega_b = omega_b;
+    s->rho0 = 1.0;
+    return s;
+}
+
+void ako_destroy(void *h) {
+    ako *s = (ako *)h;
+    if (!s) return;
+    free(s->src); free(s->dst); free(s->solid); free(s->links); free(s->ff);
+    free(s->ubb); free(s->out);
+    for (int k = 0; k < 8; ++k) free(s->obj[k].initial);
+    free(s);
+}
+
+/* even-odd crossing test of one point against one polygon (geometry.py:58-68) */
+static int contains(int np, const double *xy, double x, double y) {
+    int inside = 0;
+    for (int k = 0; k < np; ++k) {
+        const double ax = xy[2 * k], ay = xy[2 * k + 1];
+        const int kn = (k + 1) % np;
+        const double bx = xy[2 * kn], by = xy[2 * kn + 1];
+        if (by == ay) continue;
+        const int crosses = (ay > y) != (by > y);
+        const double xs = (bx - ax) * (y - ay) / (by - ay) + ax;
+        if (crosses && x < xs) inside ^= 1;
+    }
+    return inside;
+}
+
+/* first crossing of p + t e with the polygon, clipped to [1e-10, 1] (geometry.py:70-85) */
+static double link_fraction(int np, const double *xy, double px, double py, double ex, double ey) {
+    double best = INFINITY;
+    for (int k = 0; k < np; ++k) {
+        const double ax = xy[2 * k], ay = xy[2 * k + 1];
+        const int kn = (k + 1) % np;
+        const double edx = xy[2 * kn] - ax, edy = xy[2 * kn + 1] - ay;
+        const double den = ex * edy - ey * edx;
+        if (!(fabs(den) > 1e-14)) continue;
+        const double rx = ax - px, ry = ay - py;
+        const double t = (rx * edy - ry * edx) / den;
+        const double v = (rx * ey - ry * ex) / den;
+        if (t >= -1e-10 && t <= 1 + 1e-10 && v >= -1e-10 && v <= 1 + 1e-10 && t < best) best = t;
+    }
+    if (!isfinite(best)) return NAN;
+    return best < 1e-10 ? 1e-10 : (best > 1.0 ? 1.0 : best);
+}
+
+/* Solid mask of every ghosted xy cell and the fluid->solid link list of each
+ * element.  Coordinates: cell (i,j) centre = ((i+0.5) dx + ox, (j+0.5) dx + oy).
+ * Returns the link count, or -1 if a link has no intersection. */
+int ako_set_geometry(void *h, int nelem, const int *npts, const double *xy,
+                     double ox, double oy, double dx) {
+    ako *s = (ako *)h;
+    const int X = s->X, Y = s->Y;
+    uint8_t *elem_of = (uint8_t *)calloc((size_t)X * Y, 1); /* 0 = fluid, else element+1 */
+    const double **poly = (const double **)malloc(sizeof(double *) * nelem);
+    long off = 0;
+    for (int e = 0; e < nelem; ++e) { poly[e] = xy + 2 * off; off += npts[e]; }
+    for (int gx = 0; gx < X; ++gx)
+        for (int gy = 0; gy < Y; ++gy) {
+            const double x = ((double)(gx - 1) + 0.5) * dx + ox;
+            const double y = ((double)(gy - 1) + 0.5) * dx + oy;
+            for (int e = 0; e < nelem; ++e)
+                if (contains(npts[e], poly[e], x, y)) elem_of[gx * Y + gy] = (uint8_t)(e + 1);
+        }
+    for (long k = 0; k < (long)X * Y; ++k) s->solid[k] = elem_of[k] != 0;
+    /* links: fluid interior cell, neighbour solid (any z, ghosts included) */
+    int cap = 1024, n = 0, bad = 0;
+    wall_link *L = (wall_link *)malloc(sizeof(wall_link) * cap);
+    s->fallback_links = 0;
+    for (int e = 0; e < nelem; ++e)
+        for (int x = 0; x < s->nx; ++x)
+            for (int y = 0; y < s->ny; ++y) {
+                if (elem_of[(x + 1) * Y + (y + 1)]) continue;
+                for (int d = 0; d < NQ; ++d) {
+                    if (CX[d] == 0 && CY[d] == 0) continue;
+                    const int gx = x + CX[d], gy = y + CY[d];
+                    if (elem_of[(gx + 1) * Y + (gy + 1)] != e + 1) continue;
+                    const double px = ((double)x + 0.5) * dx + ox, py = ((double)y + 0.5) * dx + oy;
+                    const double ex = CX[d] * dx, ey = CY[d] * dx;
+                    double q = link_fraction(npts[e], poly[e], px, py, ex, ey);
+                    if (isnan(q)) { bad = 1; continue; }
+                    if (q < 0.5) {
+                        const double sx = px - ex, sy = py - ey;
+                        int blocked = 0;
+                        for (int f = 0; f < nelem; ++f) blocked |= contains(npts[f], poly[f], sx, sy);
+                        if (blocked) q = -1.0;
+                    }
+                    for (int z = 0; z < s->nz; ++z) {
+                        if (n == cap) { cap *= 2; L = (wall_link *)realloc(L, sizeof(wall_link) * cap); }
+                        L[n].x = x; L[n].y = y; L[n].z = z; L[n].d = d; L[n].q = q; L[n].elem = e;
+                        if (q == -1.0) s->fallback_links++;
+                        ++n;
+                    }
+                }
+            }
+    free(elem_of);
+    free(poly);
+    free(s->links);
+    s->links = L;
+    s->nlinks = n;
+    s->nelem = nelem;
+    return bad ? -1 : n;
+}
+
+int ako_link_count(void *h) { return ((ako *)h)->nlinks; }
+long ako_fallback_links(void *h) { return ((ako *)h)->fallback_links; }
+
+void ako_get_links(void *h, int *x, int *y, int *z, int *d, double *q, int *elem) {
+    ako *s = (ako *)h;
+    for (int k = 0; k < s->nlinks; ++k) {
+        x[k] = s->links[k].x; y[k] = s->links[k].y; z[k] = s->links[k].z;
+        d[k] = s->links[k].d; q[k] = s->links[k].q; elem[k] = s->links[k].elem;
+    }
+}
+
+void ako_get_solid(void *h, uint8_t *mask) { /* interior nx*ny, x-major */
+    ako *s = (ako *)h;
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y) mask[(long)x * s->ny + y] = s->solid[(x + 1) * s->Y + (y + 1)];
+}
+
+/* lever-arm offsets (cell-centre position = index + 0.5 + off, lattice units) and the
+ * coefficient normalisation (chord in cells) */
+void ako_set_load_frame(void *h, double rx_off, double ry_off, double chord_cells, double U) {
+    ako *s = (ako *)h;
+    s->rx_off = rx_off; s->ry_off = ry_off; s->chord_cells = chord_cells; s->U = U;
+}
+
+/* ---------------------------------------------------------------- init */
+
+/* Seeded initial field of the reference (solver.py:157-202), evaluated on every
+ * interior cell; ghosts stay zero; solid cells get u = 0.  Product-form
+ * equilibrium with rho = 1, stored shifted by the weights. */
+void ako_init_seeded(void *h, double U, double amp, double phy, double phz, double ky, double kz,
+                     double lex, double ley, double ox, double oy, double dx) {
+    ako *s = (ako *)h;
+    memset(s->src, 0, sizeof(double) * s->ncell * NQ);
+    memset(s->dst, 0, sizeof(double) * s->ncell * NQ);
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y)
+            for (int z = 0; z < s->nz; ++z) {
+                const double xx = ((double)x + 0.5) * dx + ox, yy = ((double)y + 0.5) * dx + oy;
+                const double zz = ((double)z + 0.5) * dx;
+                const double rx = xx - lex, ry = yy - ley;
+                const double env = exp(-((rx - 0.5) / 0.75) * ((rx - 0.5) / 0.75));
+                double u[3];
+                u[0] = U;
+                u[1] = amp * sin(ky * ry + phy) * cos(kz * zz + phz) * env;
+                u[2] = -amp * ky / kz * cos(ky * ry + phy) * sin(kz * zz + phz) * env;
+                if (s->solid[(x + 1) * s->Y + (y + 1)]) u[0] = u[1] = u[2] = 0.0;
+                double *g = s->src + cid(s, x, y, z) * NQ;
+                for (int i = 0; i < NQ; ++i) g[i] = equilibrium(i, 1.0, u) - WT[i];
+            }
+    s->steps = 0;
+}
+
+/* populations in the (nx, ny, nz, 27) interior layout */
+void ako_set_pdf(void *h, const double *g) {
+    ako *s = (ako *)h;
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y)
+            for (int z = 0; z < s->nz; ++z)
+                memcpy(s->src + cid(s, x, y, z) * NQ, g + (((long)x * s->ny + y) * s->nz + z) * NQ,
+                       sizeof(double) * NQ);
+}
+
+void ako_get_pdf(void *h, double *g) {
+    ako *s = (ako *)h;
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y)
+            for (int z = 0; z < s->nz; ++z)
+                memcpy(g + (((long)x * s->ny + y) * s->nz + z) * NQ, s->src + cid(s, x, y, z) * NQ,
+                       sizeof(double) * NQ);
+}
+
+/* -------------------------------------------------------------- farfield */
+
+static void macro_at(const ako *s, int x, int y, int z, double *rho, double *u) {
+    const double *g = s->src + cid(s, x, y, z) * NQ;
+    double r = 0, jx = 0, jy = 0, jz = 0;
+    for (int i = 0; i < NQ; ++i) { r += g[i]; jx += CX[i] * g[i]; jy += CY[i] * g[i]; jz += CZ[i] * g[i]; }
+    r += 1.0;
+    *rho = r; u[0] = jx / r; u[1] = jy / r; u[2] = jz / r;
+}
+
+static int wrapz(const ako *s, int z) { return ((z % s->nz) + s->nz) % s->nz; }
+
+/* transverse-transport derivative F of one evolving link (farfield.py:162-181, 98-109) */
+static double ff_derivative(const ako *s, const ff_object *o, const int foot[3], const int edge[3],
+                            double rho, const double *u, double rm) {
+    const double cs = sqrt(1.0 / 3.0);
+    double F = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        if (!o->use_axis[a]) continue;
+        const int e = edge[a];
+        const int off1 = e == -1 ? 1 : -1;
+        const int off2 = e == 0 ? 1 : -2 * e;
+        const double c0 = 1.5 * e;
+        const double c1 = e == 0 ? -0.5 : -2.0 * e;
+        const double c2 = e == 0 ? 0.5 : 0.5 * e;
+        int p1[3] = {foot[0], foot[1], foot[2]}, p2[3] = {foot[0], foot[1], foot[2]};
+        p1[a] += off1; p2[a] += off2;
+        p1[2] = wrapz(s, p1[2]); p2[2] = wrapz(s, p2[2]);
+        double r1, r2, u1[3], u2[3];
+        macro_at(s, p1[0], p1[1], p1[2], &r1, u1);
+        macro_at(s, p2[0], p2[1], p2[2], &r2, u2);
+        double pdu = 0.0;
+        for (int b = 0; b < 3; ++b) pdu += o->P[a][b] * (c0 * u[b] + c1 * u1[b] + c2 * u2[b]);
+        const double rm1 = o->nh[0] * u1[0] + o->nh[1] * u1[1] + o->nh[2] * u1[2] - cs * log(r1 / s->rho0);
+        const double rm2 = o->nh[0] * u2[0] + o->nh[1] * u2[1] + o->nh[2] * u2[2] - cs * log(r2 / s->rho0);
+        F += o->un_inf * pdu - o->ut_inf[a] * (c0 * rm + c1 * rm1 + c2 * rm2);
+        (void)rho;
+    }
+    return F;
+}
+
+static void build_characteristic(ako *s) {
+    const double U = s->U, cs = sqrt(1.0 / 3.0);
+    int k = 0;
+    for (int sx = -1; sx <= 1; ++sx)
+        for (int sy = -1; sy <= 1; ++sy) {
+            if (!sx && !sy) continue;
+            ff_object *o = &s->obj[k++];
+            o->n[0] = sx; o->n[1] = sy; o->n[2] = 0;
+            const double len = sqrt((double)(sx * sx + sy * sy));
+            o->nh[0] = sx / len; o->nh[1] = sy / len; o->nh[2] = 0.0;
+            const double vel[3] = {U, 0.0, 0.0};
+            const double ndotv = sx * U; /* unnormalised, decides the regime */
+            o->evolve = ndotv >= 0 && U != 0.0;
+            o->glancing = o->evolve && ndotv == 0.0;
+            o->un_inf = o->nh[0] * vel[0] + o->nh[1] * vel[1] + o->nh[2] * vel[2];
+            for (int a = 0; a < 3; ++a) {
+                o->ut_inf[a] = vel[a] - o->un_inf * o->nh[a];
+                int any = 0;
+                for (int b = 0; b < 3; ++b) {
+                    o->P[a][b] = (a == b ? 1.0 : 0.0) - o->nh[a] * o->nh[b];
+                    if (o->P[a][b] != 0.0) any = 1;
+                }
+                o->use_axis[a] = !(o->ut_inf[a] == 0.0 && !any);
+            }
+            free(o->initial);
+            o->initial = NULL;
+            if (o->glancing) {
+                o->initial = (double *)malloc(sizeof(double) * s->nx * s->ny * s->nz);
+                for (int x = 0; x < s->nx; ++x)
+                    for (int y = 0; y < s->ny; ++y)
+                        for (int z = 0; z < s->nz; ++z) {
+                            double r, u[3];
+                            macro_at(s, x, y, z, &r, u);
+                            o->initial[((long)x * s->ny + y) * s->nz + z] =
+                                (u[0] * o->nh[0] + u[1] * o->nh[1] + u[2] * o->nh[2]) - cs * log(r / s->rho0) - o->un_inf;
+                        }
+            }
+        }
+    /* links, objects in the reference's installation order */
+    int cap = 4096, n = 0;
+    ff_link *L = (ff_link *)malloc(sizeof(ff_link) * cap);
+    for (int ob = 0; ob < 8; ++ob) {
+        const ff_object *o = &s->obj[ob];
+        for (int x = 0; x < s->nx; ++x)
+            for (int y = 0; y < s->ny; ++y) {
+                if (x > 0 && x < s->nx - 1 && y > 0 && y < s->ny - 1) continue;
+                for (int z = 0; z < s->nz; ++z)
+                    for (int d = 0; d < NQ; ++d) {
+                        const int gx = x + CX[d], gy = y + CY[d];
+                        const int tx = gx < 0 ? -1 : gx >= s->nx ? 1 : 0;
+                        const int ty = gy < 0 ? -1 : gy >= s->ny ? 1 : 0;
+                        if ((!tx && !ty) || tx != o->n[0] || ty != o->n[1]) continue;
+                        if (n == cap) { cap *= 2; L = (ff_link *)realloc(L, sizeof(ff_link) * cap); }
+                        ff_link *l = &L[n++];
+                        memset(l, 0, sizeof *l);
+                        l->x = x; l->y = y; l->z = z; l->d = d; l->obj = ob;
+                        int foot[3] = {x + CX[d] - o->n[0], y + CY[d] - o->n[1], z + CZ[d]};
+                        l->wrap[2] = foot[2] < 0 ? s->nz : foot[2] >= s->nz ? -s->nz : 0;
+                        foot[2] += l->wrap[2];
+                        const int shape[2] = {s->nx, s->ny};
+                        for (int a = 0; a < 2; ++a)
+                            l->edge[a] = foot[a] == 0 ? -1 : foot[a] == shape[a] - 1 ? 1 : 0;
+                        if (o->glancing) { l->clock = 0; continue; }
+                        if (!o->evolve) { l->incoming = o->un_inf; l->derivative = 0.0; continue; }
+                        double r, u[3];
+                        macro_at(s, foot[0], foot[1], foot[2], &r, u);
+                        const double rm = o->nh[0] * u[0] + o->nh[1] * u[1] + o->nh[2] * u[2] - cs * log(r / s->rho0);
+                        const double F = ff_derivative(s, o, foot, l->edge, r, u, rm);
+                        l->incoming = rm - F;
+                        l->derivative = F;
+                    }
+            }
+    }
+    free(s->ff);
+    s->ff = L;
+    s->nff = n;
+}
+
+static void build_legacy(ako *s) {
+    int cu = 4096, co = 1024, nu = 0, no = 0;
+    wall_link *Lu = (wall_link *)malloc(sizeof(wall_link) * cu);
+    out_link *Lo = (out_link *)malloc(sizeof(out_link) * co);
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y) {
+            if (x > 0 && x < s->nx - 1 && y > 0 && y < s->ny - 1) continue;
+            for (int z = 0; z < s->nz; ++z)
+                for (int d = 0; d < NQ; ++d) {
+                    const int gx = x + CX[d], gy = y + CY[d];
+                    if (gx >= s->nx) {
+                        if (no == co) { co *= 2; Lo = (out_link *)realloc(Lo, sizeof(out_link) * co); }
+                        out_link *l = &Lo[no++];
+                        l->x = x; l->y = y; l->z = z; l->d = d;
+                        /* both history values start at the zero-centred free-stream
+                         * equilibrium of the incoming direction (initial_density=1,
+                         * initial_velocity=(U,0,0), solver.py:133-134); pinned by the
+                         * recorded legacy load history to 1e-13 over 900 steps */
+                        const int id = NQ - 1 - d;
+                        const double uinf[3] = {s->U, 0.0, 0.0};
+                        l->pdf_nd = equilibrium(id, 1.0, uinf) - WT[id];
+                        l->pdf = l->pdf_nd;
+                    } else if (gx < 0 || gy < 0 || gy >= s->ny) {
+                        if (nu == cu) { cu *= 2; Lu = (wall_link *)realloc(Lu, sizeof(wall_link) * cu); }
+                        wall_link *l = &Lu[nu++];
+                        l->x = x; l->y = y; l->z = z; l->d = d; l->q = 0; l->elem = 0;
+                    }
+                }
+        }
+    free(s->ubb); free(s->out);
+    s->ubb = Lu; s->nubb = nu;
+    s->out = Lo; s->nout = no;
+}
+
+/* kind: 0 = none (periodic x/y), 1 = legacy, 2 = characteristic.  Reads the
+ * current populations, so call after the initial state is set. */
+int ako_set_farfield(void *h, int kind) {
+    ako *s = (ako *)h;
+    s->ff_kind = kind;
+    if (kind == 2) build_characteristic(s);
+    else if (kind == 1) build_legacy(s);
+    return kind == 2 ? s->nff : kind == 1 ? s->nubb + s->nout : 0;
+}
+
+/* ---------------------------------------------------------------- step */
+
+static void sync_periodic(ako *s) {
+    const int Z = s->Z;
+    for (int gx = 0; gx < s->X; ++gx)
+        for (int gy = 0; gy < s->Y; ++gy) {
+            double *col = s->src + ((long)gx * s->Y + gy) * Z * NQ;
+            memcpy(col, col + (long)(Z - 2) * NQ, sizeof(double) * NQ);
+            memcpy(col + (long)(Z - 1) * NQ, col + NQ, sizeof(double) * NQ);
+        }
+    if (s->ff_kind == 0) { /* fully periodic box: x and y ghosts as well */
+        const int X = s->X, Y = s->Y;
+        for (int gy = 0; gy < Y; ++gy)
+            for (int gz = 0; gz < Z; ++gz) {
+                memcpy(s->src + (((long)0 * Y + gy) * Z + gz) * NQ, s->src + (((long)(X - 2) * Y + gy) * Z + gz) * NQ, sizeof(double) * NQ);
+                memcpy(s->src + (((long)(X - 1) * Y + gy) * Z + gz) * NQ, s->src + (((long)1 * Y + gy) * Z + gz) * NQ, sizeof(double) * NQ);
+            }
+        for (int gx = 0; gx < X; ++gx)
+            for (int gz = 0; gz < Z; ++gz) {
+                memcpy(s->src + (((long)gx * Y + 0) * Z + gz) * NQ, s->src + (((long)gx * Y + (Y - 2)) * Z + gz) * NQ, sizeof(double) * NQ);
+                memcpy(s->src + (((long)gx * Y + (Y - 1)) * Z + gz) * NQ, s->src + (((long)gx * Y + 1) * Z + gz) * NQ, sizeof(double) * NQ);
+            }
+    }
+}
+
+static void apply_characteristic(ako *s) {
+    const double cs = sqrt(1.0 / 3.0);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->nff; ++k) {
+        ff_link *l = &s->ff[k];
+        const ff_object *o = &s->obj[l->obj];
+        const int d = l->d, id = NQ - 1 - d;
+        const int foot[3] = {l->x + CX[d] - o->n[0], l->y + CY[d] - o->n[1], l->z + CZ[d] + l->wrap[2]};
+        double rho, u[3];
+        macro_at(s, foot[0], foot[1], foot[2], &rho, u);
+        const double un = o->nh[0] * u[0] + o->nh[1] * u[1] + o->nh[2] * u[2];
+        const double lr = log(rho / s->rho0);
+        const double rm = un - cs * lr, rp = un + cs * lr;
+        double incoming, F = 0.0;
+        if (o->evolve && !o->glancing) {
+            F = ff_derivative(s, o, foot, l->edge, rho, u, rm);
+            incoming = l->incoming + (F + l->derivative) / 2.0;
+        } else if (o->glancing) {
+            /* trace of the stored initial invariant along x at the total shift U*clock */
+            const double target = (double)foot[0] - s->U * (double)l->clock;
+            const int lower = (int)floor(target);
+            const double frac = target - lower;
+            double sum = 0.0;
+            for (int bit = 0; bit < 2; ++bit) {
+                const int kx = lower + bit;
+                const double wgt = bit ? frac : 1.0 - frac;
+                double v = 0.0;
+                if (kx >= 0 && kx < s->nx) v = o->initial[((long)kx * s->ny + foot[1]) * s->nz + foot[2]];
+                sum += wgt * v;
+            }
+            incoming = o->un_inf + sum;
+        } else {
+            incoming = o->un_inf;
+        }
+        const double unb = (rp + incoming) / 2.0;
+        const double rhob = s->rho0 * exp((rp - incoming) / (2.0 * cs));
+        const double vinf[3] = {s->U, 0.0, 0.0};
+        double ub[3];
+        for (int a = 0; a < 3; ++a)
+            ub[a] = o->nh[a] * unb + (unb >= 0 ? u[a] - o->nh[a] * un : vinf[a] - o->nh[a] * o->un_inf);
+        const double fout = s->src[cid(s, foot[0], foot[1], foot[2]) * NQ + id];
+        const double val = fout + equilibrium(id, rhob, ub) - equilibrium(id, rho, u);
+        s->src[cid(s, l->x + CX[d], l->y + CY[d], l->z + CZ[d]) * NQ + id] = val;
+        if (o->evolve && !o->glancing) { l->incoming = incoming; l->derivative = F; }
+        if (o->glancing) l->clock += 1;
+    }
+}
+
+static void apply_legacy(ako *s) {
+    /* velocity bounce-back; rho from the fluid cell's own populations */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->nubb; ++k) {
+        const wall_link *l = &s->ubb[k];
+        const double *g = s->src + cid(s, l->x, l->y, l->z) * NQ;
+        double r = 0.0;
+        for (int i = 0; i < NQ; ++i) r += g[i];
+        r += 1.0;
+        const int d = l->d;
+        const double cu = CX[d] * s->U;
+        s->src[cid(s, l->x + CX[d], l->y + CY[d], l->z + CZ[d]) * NQ + (NQ - 1 - d)] = g[d] - 6.0 * WT[d] * cu * r;
+    }
+    /* extrapolation outflow, after the velocity bounce-back (it may read its output) */
+    const double c = sqrt(1.0 / 3.0);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->nout; ++k) {
+        out_link *l = &s->out[k];
+        const int d = l->d, id = NQ - 1 - d;
+        const double val = l->pdf * c + (1.0 - c) * l->pdf_nd;
+        s->src[cid(s, l->x + CX[d], l->y + CY[d], l->z + CZ[d]) * NQ + id] = val;
+        l->pdf = s->src[cid(s, l->x, l->y + CY[d], l->z + CZ[d]) * NQ + id];
+        l->pdf_nd = val;
+    }
+}
+
+static void apply_walls(ako *s) {
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->nlinks; ++k) {
+        const wall_link *l = &s->links[k];
+        const int d = l->d, id = NQ - 1 - d;
+        const double *gx = s->src + cid(s, l->x, l->y, l->z) * NQ;
+        const double fxf = gx[d], fxf_inv = gx[id];
+        const double q = l->q;
+        double val;
+        if (q >= 0.5) {
+            val = 0.5 * (fxf + fxf_inv * (2.0 * q - 1.0)) / q;
+        } else if (q > 0.0) {
+            const double d2 = s->src[cid(s, l->x - CX[d], l->y - CY[d], l->z - CZ[d]) * NQ + d];
+            val = 2.0 * q * fxf + (1.0 - 2.0 * q) * d2;
+        } else {
+            val = fxf;
+        }
+        s->src[cid(s, l->x + CX[d], l->y + CY[d], l->z + CZ[d]) * NQ + id] = val;
+    }
+}
+
+static void measure_loads(const ako *s, double *per_elem) {
+    for (int e = 0; e < s->nelem; ++e) per_elem[4 * e] = per_elem[4 * e + 1] = per_elem[4 * e + 2] = per_elem[4 * e + 3] = 0.0;
+    for (int k = 0; k < s->nlinks; ++k) {
+        const wall_link *l = &s->links[k];
+        const int d = l->d, id = NQ - 1 - d;
+        const double out = s->src[cid(s, l->x, l->y, l->z) * NQ + d];
+        const double in = s->src[cid(s, l->x + CX[d], l->y + CY[d], l->z + CZ[d]) * NQ + id];
+        const double v = out + in;
+        const double fx = CX[d] * v, fy = CY[d] * v, fz = CZ[d] * v;
+        const double rx = l->x + 0.5 + s->rx_off, ry = l->y + 0.5 + s->ry_off;
+        double *pe = per_elem + 4 * l->elem;
+        pe[0] += fx; pe[1] += fy; pe[2] += fz; pe[3] += rx * fy - ry * fx;
+    }
+}
+
+static void stream_collide(ako *s) {
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < nx; ++x)
+        for (int y = 0; y < ny; ++y) {
+            if (s->solid[(x + 1) * s->Y + (y + 1)]) continue;
+            for (int z = 0; z < nz; ++z) {
+                double g[NQ];
+                for (int i = 0; i < NQ; ++i) g[i] = s->src[cid(s, x - CX[i], y - CY[i], z - CZ[i]) * NQ + i];
+                collide(g, s->dst + cid(s, x, y, z) * NQ, s->omega_s, s->omega_b);
+            }
+        }
+    double *t = s->src; s->src = s->dst; s->dst = t;
+}
+
+/* One reference time step.  If loads != NULL the per-element loads
+ * [Fx, Fy, Fz, Mz] (lattice units) are written after the boundary objects,
+ * and coef (if not NULL) receives cl, cd, cm. */
+void ako_step(void *h, double *loads, double *coef) {
+    ako *s = (ako *)h;
+    sync_periodic(s);
+    if (s->ff_kind == 2) apply_characteristic(s);
+    else if (s->ff_kind == 1) apply_legacy(s);
+    apply_walls(s);
+    if (loads) {
+        measure_loads(s, loads);
+        if (coef) {
+            double tot[4] = {0, 0, 0, 0};
+            for (int e = 0; e < s->nelem; ++e)
+                for (int c = 0; c < 4; ++c) tot[c] += loads[4 * e + c];
+            const double qa = 0.5 * s->U * s->U * s->chord_cells * s->nz;
+            coef[0] = tot[1] / qa;
+            coef[1] = tot[0] / qa;
+            coef[2] = -tot[3] / (qa * s->chord_cells);
+        }
+    }
+    stream_collide(s);
+    s->steps++;
+}
+
+/* rho and u of interior cells from the current (post-collision) state; returns
+ * local Mach max and |rho-1| max over fluid cells in out[0], out[1], min rho in out[2] */
+void ako_diagnostics(void *h, double *out) {
+    ako *s = (ako *)h;
+    double umax2 = 0.0, derr = 0.0, rmin = INFINITY;
+    int finite = 1;
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y) {
+            if (s->solid[(x + 1) * s->Y + (y + 1)]) continue;
+            for (int z = 0; z < s->nz; ++z) {
+                double r, u[3];
+                macro_at(s, x, y, z, &r, u);
+                if (!isfinite(r) || !isfinite(u[0]) || !isfinite(u[1]) || !isfinite(u[2])) finite = 0;
+                const double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                if (u2 > umax2) umax2 = u2;
+                if (fabs(r - 1.0) > derr) derr = fabs(r - 1.0);
+                if (r < rmin) rmin = r;
+            }
+        }
+    out[0] = sqrt(3.0) * sqrt(umax2);
+    out[1] = derr;
+    out[2] = rmin;
+    out[3] = finite;
+}
+
+int ako_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,45 @@
+"""Extract golden vectors (data files only) from the reference's recorded runs.
+
+Run in a container that has the reference checkout; the outputs are committed
+so the tests never need it.  Sources (reference paths):
+  docs/verification_stage1_20261106/corrected32_t12/{project.json,history.csv}
+  docs/verification_farfield_20261107/airfoil/characteristic_down3/{project.json,history.csv}
+  docs/verification_stage1_20261106/profile_threads1.json (last_sample after 272 steps)
+  tests/fixtures/legacy051_project.json (fingerprint fixture)
+"""
+import csv
+import json
+import shutil
+import sys
+from pathlib import Path
+
+REF = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+OUT = Path(__file__).resolve().parent
+
+CASES = {
+    "n32_legacy_down3": ("docs/verification_stage1_20261106/corrected32_t12", 240),
+    "n32_characteristic_down3": ("docs/verification_farfield_20261107/airfoil/characteristic_down3", 400),
+}
+
+for name, (rel, last_step) in CASES.items():
+    dst = OUT / name
+    dst.mkdir(exist_ok=True)
+    shutil.copyfile(REF / rel / "project.json", dst / "project.json")
+    with open(REF / rel / "history.csv", newline="") as src, open(dst / "history_head.csv", "w", newline="") as out:
+        reader = csv.DictReader(src)
+        writer = csv.DictWriter(out, reader.fieldnames)
+        writer.writeheader()
+        for row in reader:
+            if int(row["step"]) > last_step:
+                break
+            writer.writerow(row)
+
+profile = json.loads((REF / "docs/verification_stage1_20261106/profile_threads1.json").read_text())
+(OUT / "profile_n32_naca0012_81.json").write_text(json.dumps({
+    "case": {"naca0012_n_side": 81, "chord": 1.0, "moment_reference": [0.25, 0.0],
+             "flow": {"reynolds": 30000, "alpha_deg": 4, "physical_chord_m": 0.15},
+             "grid": {"cells_per_chord": 32, "upstream_chords": 1.5, "downstream_chords": 3,
+                      "transverse_chords": 1.5, "span_chords": 0.5, "memory_limit_gib": 2},
+             "farfield": "legacy", "steps": 272, "sample_every": 8},
+    "shape": profile["shape"], "last_sample": profile["last_sample"]}, indent=2) + "\n")
+shutil.copyfile(REF / "tests/fixtures/legacy051_project.json", OUT / "legacy051_project.json")
