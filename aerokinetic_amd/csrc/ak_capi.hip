@@ -1,0 +1,750 @@
+// ak_capi.hip -- C ABI of the gfx950 D3Q27 cumulant step (declared in include/aerokinetic.h).
+//
+// Host-side orchestration only: buffer sizing, boundary item tables, launch order.
+// Per step: boundary kernels (farfield, walls) write the incoming populations of
+// boundary links where the next AA sweep reads them; on sampled steps the wall
+// kernel also emits load partials, a one-block kernel folds them into a history
+// row, and the sweep folds the diagnostics (max |u|, max |rho-1|, min rho,
+// non-finite flag) into the same row.  No host synchronisation in the step path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aerokinetic.h"
+#include "ak_kernels.hpp"
+
+using namespace ak;
+
+namespace {
+
+std::string g_error;
+
+struct Lattice {
+  int dev = 0, nx = 0, ny = 0, nz = 0, storage = 0;
+  double ws = 1.0, wb = 1.0;
+  float scale = 1.0f, inv_scale = 1.0f;
+  Desc L{};
+  size_t pop_elems = 0, pop_bytes = 0, elem_size = 8;
+  void *pop = nullptr;
+  hipStream_t stream = nullptr;
+  uint8_t *d_solid = nullptr;
+  // walls
+  int nlinks = 0, nelem = 1;
+  int32_t *d_wxy = nullptr;
+  int8_t *d_wdir = nullptr, *d_welem = nullptr;
+  double *d_wq = nullptr;
+  double rx_off = 0, ry_off = 0, chord = 1, U = 0;
+  double *d_partials = nullptr;
+  int wall_blocks = 0;
+  // farfield
+  int ff_kind = AK_FARFIELD_PERIODIC;
+  FFParams ffp{};
+  int nff = 0;
+  int32_t *d_ffg = nullptr, *d_ffz = nullptr, *d_ffmask = nullptr;
+  int8_t *d_ffobj = nullptr;
+  double *d_inc = nullptr, *d_der = nullptr, *d_rows = nullptr;
+  long long *d_clock = nullptr;
+  int nleg = 0;
+  int32_t *d_legx = nullptr, *d_legz = nullptr;
+  int8_t *d_legdir = nullptr, *d_legkind = nullptr;
+  double *d_pdf = nullptr, *d_pdfnd = nullptr;
+  // diagnostics / history
+  DiagAcc *d_diag = nullptr;
+  double *d_hist = nullptr;
+  int hist_cap = 0, hist_rows = 0, hist_width = 0;
+  std::vector<double> host_hist;
+  long long sweeps = 0;  // sweeps since the initial state: parity of the AA form
+  SweepGeom geom{};
+  std::string err;
+  std::vector<void *> owned;
+};
+
+int fail(Lattice *s, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (s) s->err = buf;
+  g_error = buf;
+  return -1;
+}
+
+#define HIPCHK(s, call)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) return fail((s), "%s: %s", #call, hipGetErrorString(e_));        \
+  } while (0)
+
+template <typename T>
+int dalloc(Lattice *s, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(s, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  s->owned.push_back(*p);
+  return 0;
+}
+
+void dfree(Lattice *s, void *p) {
+  if (!p) return;
+  auto it = std::find(s->owned.begin(), s->owned.end(), p);
+  if (it != s->owned.end()) s->owned.erase(it);
+  (void)hipFree(p);
+}
+
+template <typename T>
+int upload(Lattice *s, T **dst, const std::vector<T> &v) {
+  if (*dst) { dfree(s, *dst); *dst = nullptr; }
+  if (dalloc(s, dst, v.size())) return -1;
+  if (!v.empty()) HIPCHK(s, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+void set_geom(Lattice *s, int bx, int by) {
+  s->geom.bx = bx;
+  s->geom.by = by;
+  s->geom.gx = (s->nx + bx - 1) / bx;
+  s->geom.gy = (s->ny + by - 1) / by;
+  s->geom.gz = s->nz;
+  s->geom.nblocks = s->geom.gx * s->geom.gy * s->geom.gz;
+}
+
+// ---------------------------------------------------------------- launches
+
+template <int S>
+void launch_sweep(Lattice *s, bool odd, bool diag) {
+  using T = typename St<S>::T;
+  using C = typename St<S>::C;
+  T *A = (T *)s->pop;
+  const dim3 grid(s->geom.nblocks), block(s->geom.bx * s->geom.by);
+  const bool pxy = s->ff_kind == AK_FARFIELD_PERIODIC;
+  const C ws = (C)s->ws, wb = (C)s->wb;
+#define AK_SWEEP(O, P, D) \
+  hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, A, s->d_solid, s->L, s->geom, ws, wb, s->scale, s->inv_scale, s->d_diag)
+  if (odd) {
+    if (pxy) { if (diag) AK_SWEEP(true, true, true); else AK_SWEEP(true, true, false); }
+    else { if (diag) AK_SWEEP(true, false, true); else AK_SWEEP(true, false, false); }
+  } else {
+    if (pxy) { if (diag) AK_SWEEP(false, true, true); else AK_SWEEP(false, true, false); }
+    else { if (diag) AK_SWEEP(false, false, true); else AK_SWEEP(false, false, false); }
+  }
+#undef AK_SWEEP
+}
+
+template <int S, bool PE>
+void launch_boundary(Lattice *s, bool loads) {
+  using T = typename St<S>::T;
+  T *A = (T *)s->pop;
+  if (s->ff_kind == AK_FARFIELD_CHARACTERISTIC && s->nff) {
+    FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff};
+    hipLaunchKernelGGL((k_farfield<S, PE>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, A, s->L, I,
+                       s->ffp, s->scale, s->inv_scale);
+  } else if (s->ff_kind == AK_FARFIELD_LEGACY && s->nleg) {
+    LegacyLinks K{s->d_legx, s->d_legz, s->d_legdir, s->d_legkind, s->d_pdf, s->d_pdfnd, s->nleg};
+    hipLaunchKernelGGL((k_legacy<S, PE>), dim3(blocks_for(s->nleg, 128)), dim3(128), 0, s->stream, A, s->L, K,
+                       s->U, s->scale, s->inv_scale);
+  }
+  if (s->nlinks) {
+    WallLinks W{s->d_wxy, s->d_wdir, s->d_wq, s->d_welem, s->nlinks, s->nelem, s->rx_off, s->ry_off};
+    if (loads)
+      hipLaunchKernelGGL((k_walls<S, PE, true>), dim3(s->wall_blocks), dim3(256), 0, s->stream, A, s->L, W, s->scale,
+                         s->inv_scale, s->d_partials);
+    else
+      hipLaunchKernelGGL((k_walls<S, PE, false>), dim3(s->wall_blocks), dim3(256), 0, s->stream, A, s->L, W, s->scale,
+                         s->inv_scale, s->d_partials);
+  }
+}
+
+template <int S>
+void one_step(Lattice *s, bool sample, double step_number) {
+  const bool pe = (s->sweeps % 2) == 0;
+  if (pe) launch_boundary<S, true>(s, sample);
+  else launch_boundary<S, false>(s, sample);
+  double *row = s->d_hist + (size_t)s->hist_rows * s->hist_width;
+  if (sample) {
+    const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
+    hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, s->stream, s->d_partials,
+                       s->nlinks ? s->wall_blocks : 0, s->nelem, qa, s->chord, step_number, row, s->d_diag);
+  }
+  launch_sweep<S>(s, pe, sample);
+  if (sample) {
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, row);
+    s->hist_rows++;
+  }
+  s->sweeps++;
+}
+
+int flush_history(Lattice *s) {
+  if (!s->hist_rows) return 0;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const size_t n = (size_t)s->hist_rows * s->hist_width;
+  const size_t old = s->host_hist.size();
+  s->host_hist.resize(old + n);
+  HIPCHK(s, hipMemcpy(s->host_hist.data() + old, s->d_hist, n * sizeof(double), hipMemcpyDeviceToHost));
+  s->hist_rows = 0;
+  return 0;
+}
+
+template <int S, bool PE>
+void launch_export(Lattice *s, double *d_out) {
+  using T = typename St<S>::T;
+  const long long n = (long long)s->nx * s->ny * s->nz;
+  if (s->ff_kind == AK_FARFIELD_PERIODIC)
+    hipLaunchKernelGGL((k_export<S, PE, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, d_out, s->inv_scale);
+  else
+    hipLaunchKernelGGL((k_export<S, PE, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, d_out, s->inv_scale);
+}
+
+template <int S, bool PE>
+void launch_import(Lattice *s, const double *d_in) {
+  using T = typename St<S>::T;
+  const long long n = (long long)s->nx * s->ny * s->nz;
+  if (s->ff_kind == AK_FARFIELD_PERIODIC)
+    hipLaunchKernelGGL((k_import<S, PE, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (T *)s->pop, s->L, d_in, s->scale);
+  else
+    hipLaunchKernelGGL((k_import<S, PE, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (T *)s->pop, s->L, d_in, s->scale);
+}
+
+#define AK_DISPATCH(s, FN, ...)            \
+  do {                                     \
+    switch ((s)->storage) {                \
+      case 0: FN<0>(__VA_ARGS__); break;   \
+      case 1: FN<1>(__VA_ARGS__); break;   \
+      default: FN<2>(__VA_ARGS__); break;  \
+    }                                      \
+  } while (0)
+
+template <int S>
+void export_any(Lattice *s, double *d) {
+  if (s->sweeps % 2 == 0) launch_export<S, true>(s, d);
+  else launch_export<S, false>(s, d);
+}
+template <int S>
+void import_any(Lattice *s, const double *d) {
+  if (s->sweeps % 2 == 0) launch_import<S, true>(s, d);
+  else launch_import<S, false>(s, d);
+}
+
+Lattice *H(void *h) { return (Lattice *)h; }
+
+int check_ready(Lattice *s) {
+  if (!s) return fail(nullptr, "null lattice handle");
+  if (!s->pop) return fail(s, "population buffer not bound (ak_bind_populations)");
+  HIPCHK(s, hipSetDevice(s->dev));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ak_version(void) { return "aerokinetic-mi355x 0.1 (gfx950 HIP, AA-pattern D3Q27 cumulant)"; }
+
+int ak_last_error(void *handle, char *buf, size_t len) {
+  const std::string &e = handle ? H(handle)->err : g_error;
+  if (buf && len) {
+    std::snprintf(buf, len, "%s", e.c_str());
+  }
+  return (int)e.size();
+}
+
+int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int storage, double omega_shear,
+                      double omega_bulk, double fp16_scale, int history_capacity, void *stream) {
+  if (!handle) return fail(nullptr, "null handle pointer");
+  *handle = nullptr;
+  if (nx < 4 || ny < 4 || nz < 1 || nx > 65000 || ny > 65000)
+    return fail(nullptr, "lattice %dx%dx%d outside 4..65000 x 4..65000 x >=1", nx, ny, nz);
+  if (storage < 0 || storage > 2) return fail(nullptr, "unknown storage precision %d", storage);
+  if (!(omega_shear > 0 && omega_shear < 2) || !(omega_bulk > 0 && omega_bulk < 2))
+    return fail(nullptr, "relaxation rates must lie in (0, 2)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+    return fail(nullptr, "HIP device %d not available (%d devices)", device, ndev);
+  if (hipSetDevice(device) != hipSuccess) return fail(nullptr, "hipSetDevice(%d) failed", device);
+  Lattice *s = new Lattice();
+  s->dev = device; s->nx = nx; s->ny = ny; s->nz = nz; s->storage = storage;
+  s->ws = omega_shear; s->wb = omega_bulk;
+  s->stream = (hipStream_t)stream;
+  s->elem_size = storage == 0 ? 8 : storage == 1 ? 4 : 2;
+  if (storage == 2) {
+    const double sc = fp16_scale > 0 ? fp16_scale : 65536.0;
+    const double lg = std::log2(sc);
+    if (lg != std::floor(lg) || sc < 1.0 || sc > 1048576.0) {
+      delete s;
+      return fail(nullptr, "fp16 scale must be a power of two in [1, 2^20]");
+    }
+    s->scale = (float)sc;
+    s->inv_scale = (float)(1.0 / sc);
+  }
+  // 64-byte aligned interior rows; pitch >= nx + 2 holds both x ghosts
+  const long long align = 64 / (long long)s->elem_size;
+  long long P = ((long long)nx + 2 + align - 1) / align * align;
+  s->L.nx = nx; s->L.ny = ny; s->L.nz = nz;
+  s->L.P = P;
+  s->L.PS = P * (ny + 2);
+  s->L.QS = s->L.PS * nz;
+  s->L.base = align;
+  s->pop_elems = (size_t)(s->L.base + 27 * s->L.QS + align);
+  s->pop_bytes = s->pop_elems * s->elem_size;
+  set_geom(s, 64, 4);
+  s->hist_cap = history_capacity > 0 ? history_capacity : 4096;
+  s->hist_width = AK_HIST_FIXED + 4 * MAX_ELEM;
+  if (dalloc(s, &s->d_solid, (size_t)(nx + 2) * (ny + 2)) || dalloc(s, &s->d_diag, 1) ||
+      dalloc(s, &s->d_hist, (size_t)s->hist_cap * s->hist_width)) {
+    for (void *p : s->owned) (void)hipFree(p);
+    delete s;
+    return -1;
+  }
+  (void)hipMemset(s->d_solid, 0, (size_t)(nx + 2) * (ny + 2));
+  *handle = s;
+  return 0;
+}
+
+int ak_lattice_destroy(void *handle) {
+  Lattice *s = H(handle);
+  if (!s) return 0;
+  (void)hipSetDevice(s->dev);
+  if (s->stream) (void)hipStreamSynchronize(s->stream); else (void)hipDeviceSynchronize();
+  for (void *p : s->owned) (void)hipFree(p);
+  delete s;
+  return 0;
+}
+
+size_t ak_population_bytes(void *handle) { return handle ? H(handle)->pop_bytes : 0; }
+
+int ak_bind_populations(void *handle, void *device_ptr, size_t bytes) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  if (!device_ptr) return fail(s, "null population buffer");
+  if (bytes < s->pop_bytes) return fail(s, "population buffer of %zu bytes < required %zu", bytes, s->pop_bytes);
+  if (((uintptr_t)device_ptr) % 64) return fail(s, "population buffer must be 64-byte aligned");
+  s->pop = device_ptr;
+  HIPCHK(s, hipSetDevice(s->dev));
+  // ghosts and padding start at zero
+  hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s->stream, (uint8_t *)s->pop, (long long)s->pop_bytes, (uint8_t)0);
+  HIPCHK(s, hipGetLastError());
+  return 0;
+}
+
+int ak_set_solid(void *handle, const uint8_t *mask_xy) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  std::vector<uint8_t> m((size_t)(s->nx + 2) * (s->ny + 2), 0);
+  for (int x = 0; x < s->nx; ++x)
+    for (int y = 0; y < s->ny; ++y) m[(size_t)(y + 1) * (s->nx + 2) + (x + 1)] = mask_xy[(size_t)x * s->ny + y] ? 1 : 0;
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipMemcpy(s->d_solid, m.data(), m.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int ak_set_walls(void *handle, int n, const int32_t *x, const int32_t *y, const int32_t *d, const double *q,
+                 const int32_t *element, int nelem, double rx_off, double ry_off, double chord_cells,
+                 double lattice_speed) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  if (nelem < 1 || nelem > MAX_ELEM) return fail(s, "element count %d outside 1..%d", nelem, MAX_ELEM);
+  std::vector<int32_t> xy(n);
+  std::vector<int8_t> dir(n), el(n);
+  std::vector<double> qq(n);
+  for (int k = 0; k < n; ++k) {
+    if (x[k] < 0 || x[k] >= s->nx || y[k] < 0 || y[k] >= s->ny) return fail(s, "wall link %d outside the lattice", k);
+    if (d[k] < 0 || d[k] > 26 || d[k] == 13) return fail(s, "wall link %d has direction %d", k, d[k]);
+    if (element[k] < 0 || element[k] >= nelem) return fail(s, "wall link %d has element %d", k, element[k]);
+    const int gx = x[k] + cxq(d[k]), gy = y[k] + cyq(d[k]);
+    if (gx < -1 || gx > s->nx || gy < -1 || gy > s->ny) return fail(s, "wall link %d leaves the ghost layer", k);
+    if (!(q[k] == -1.0 || (q[k] > 0.0 && q[k] <= 1.0))) return fail(s, "wall link %d has q=%g", k, q[k]);
+    xy[k] = x[k] | (y[k] << 16);
+    dir[k] = (int8_t)d[k];
+    el[k] = (int8_t)element[k];
+    qq[k] = q[k];
+  }
+  HIPCHK(s, hipSetDevice(s->dev));
+  if (upload(s, &s->d_wxy, xy) || upload(s, &s->d_wdir, dir) || upload(s, &s->d_wq, qq) || upload(s, &s->d_welem, el))
+    return -1;
+  s->nlinks = n;
+  s->nelem = nelem;
+  s->rx_off = rx_off; s->ry_off = ry_off; s->chord = chord_cells; s->U = lattice_speed;
+  s->wall_blocks = (int)blocks_for((long long)n * s->nz, 256);
+  if (s->d_partials) { dfree(s, s->d_partials); s->d_partials = nullptr; }
+  if (dalloc(s, &s->d_partials, (size_t)std::max(s->wall_blocks, 1) * nelem * 5)) return -1;
+  return 0;
+}
+
+int ak_init_seeded(void *handle, const double *p) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  SeedParams sp{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10]};
+  const long long n = (long long)s->nx * s->ny * s->nz;
+  hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s->stream, (uint8_t *)s->pop, (long long)s->pop_bytes, (uint8_t)0);
+  switch (s->storage) {
+    case 0: hipLaunchKernelGGL((k_init_seeded<0>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (double *)s->pop, s->d_solid, s->L, sp, s->scale); break;
+    case 1: hipLaunchKernelGGL((k_init_seeded<1>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (float *)s->pop, s->d_solid, s->L, sp, s->scale); break;
+    default: hipLaunchKernelGGL((k_init_seeded<2>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (_Float16 *)s->pop, s->d_solid, s->L, sp, s->scale); break;
+  }
+  HIPCHK(s, hipGetLastError());
+  s->sweeps = 0;
+  s->hist_rows = 0;
+  s->host_hist.clear();
+  s->U = p[0];
+  return 0;
+}
+
+int ak_set_farfield(void *handle, int kind, double U) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  if (kind < 0 || kind > 2) return fail(s, "unknown farfield kind %d", kind);
+  if (s->sweeps % 2 != 0) return fail(s, "set the farfield at an even step (initial state)");
+  s->ff_kind = kind;
+  s->U = U;
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  if (kind == AK_FARFIELD_CHARACTERISTIC) {
+    // objects in the order normals (-1,-1),(-1,0),(-1,1),(0,-1),(0,1),(1,-1),(1,0),(1,1)
+    int k = 0;
+    for (int sx = -1; sx <= 1; ++sx)
+      for (int sy = -1; sy <= 1; ++sy) {
+        if (!sx && !sy) continue;
+        FFObj &o = s->ffp.o[k++];
+        o.n[0] = sx; o.n[1] = sy; o.n[2] = 0;
+        const double len = std::sqrt((double)(sx * sx + sy * sy));
+        o.nh[0] = sx / len; o.nh[1] = sy / len; o.nh[2] = 0.0;
+        const double vel[3] = {U, 0.0, 0.0};
+        const double ndv = sx * U;
+        o.evolve = ndv >= 0 && U != 0.0;
+        o.glancing = o.evolve && ndv == 0.0;
+        o.un_inf = o.nh[0] * vel[0] + o.nh[1] * vel[1] + o.nh[2] * vel[2];
+        for (int a = 0; a < 3; ++a) {
+          o.ut_inf[a] = vel[a] - o.un_inf * o.nh[a];
+          bool any = false;
+          for (int b = 0; b < 3; ++b) {
+            o.P[a][b] = (a == b ? 1.0 : 0.0) - o.nh[a] * o.nh[b];
+            if (o.P[a][b] != 0.0) any = true;
+          }
+          o.use_axis[a] = !(o.ut_inf[a] == 0.0 && !any);
+        }
+        o.side = sy < 0 ? 0 : 1;
+      }
+    s->ffp.U = U;
+    s->ffp.rho0 = 1.0;
+    // items: every exterior ghost cell of each object, with the links entering the fluid
+    std::vector<int32_t> g, gz, mask;
+    std::vector<int8_t> obj;
+    for (int ob = 0; ob < 8; ++ob) {
+      const FFObj &o = s->ffp.o[ob];
+      for (int gx = -1; gx <= nx; ++gx)
+        for (int gy = -1; gy <= ny; ++gy) {
+          const int tx = gx < 0 ? -1 : gx >= nx ? 1 : 0, ty = gy < 0 ? -1 : gy >= ny ? 1 : 0;
+          if ((!tx && !ty) || tx != o.n[0] || ty != o.n[1]) continue;
+          int m = 0;
+          for (int d = 0; d < 27; ++d) {
+            const int fx = gx - cxq(d), fy = gy - cyq(d);
+            if (fx >= 0 && fx < nx && fy >= 0 && fy < ny) m |= 1 << d;
+          }
+          if (!m) continue;
+          for (int z = 0; z < nz; ++z) {
+            g.push_back((gx + 1) | ((gy + 1) << 16));
+            gz.push_back(z);
+            mask.push_back(m);
+            obj.push_back((int8_t)ob);
+          }
+        }
+    }
+    s->nff = (int)g.size();
+    if (upload(s, &s->d_ffg, g) || upload(s, &s->d_ffz, gz) || upload(s, &s->d_ffmask, mask) || upload(s, &s->d_ffobj, obj))
+      return -1;
+    std::vector<double> zeros(s->nff, 0.0);
+    std::vector<long long> zc(s->nff, 0);
+    if (upload(s, &s->d_inc, zeros) || upload(s, &s->d_der, zeros) || upload(s, &s->d_clock, zc)) return -1;
+    std::vector<double> rows((size_t)2 * nz * nx, 0.0);
+    if (upload(s, &s->d_rows, rows)) return -1;
+    FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff};
+    const long long nr = 2LL * nz * nx;
+    switch (s->storage) {
+      case 0:
+        hipLaunchKernelGGL((k_glancing_rows<0>), dim3(blocks_for(nr, 256)), dim3(256), 0, s->stream, (const double *)s->pop, s->L, s->d_rows, 1.0, s->inv_scale);
+        hipLaunchKernelGGL((k_farfield_init<0>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, (const double *)s->pop, s->L, I, s->ffp, s->inv_scale);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_glancing_rows<1>), dim3(blocks_for(nr, 256)), dim3(256), 0, s->stream, (const float *)s->pop, s->L, s->d_rows, 1.0, s->inv_scale);
+        hipLaunchKernelGGL((k_farfield_init<1>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, (const float *)s->pop, s->L, I, s->ffp, s->inv_scale);
+        break;
+      default:
+        hipLaunchKernelGGL((k_glancing_rows<2>), dim3(blocks_for(nr, 256)), dim3(256), 0, s->stream, (const _Float16 *)s->pop, s->L, s->d_rows, 1.0, s->inv_scale);
+        hipLaunchKernelGGL((k_farfield_init<2>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, (const _Float16 *)s->pop, s->L, I, s->ffp, s->inv_scale);
+        break;
+    }
+    HIPCHK(s, hipGetLastError());
+  } else if (kind == AK_FARFIELD_LEGACY) {
+    // velocity bounce-back on x-, y-, y+ ghosts; outflow on x+ ghosts (corners included)
+    std::vector<int32_t> xs, zs;
+    std::vector<int8_t> dir, knd;
+    std::vector<double> pdf, pdfnd;
+    for (int x = 0; x < nx; ++x)
+      for (int y = 0; y < ny; ++y) {
+        if (x > 0 && x < nx - 1 && y > 0 && y < ny - 1) continue;
+        for (int z = 0; z < nz; ++z)
+          for (int d = 0; d < 27; ++d) {
+            const int gx = x + cxq(d), gy = y + cyq(d);
+            int kind_ = -1;
+            if (gx >= nx) kind_ = 1;
+            else if (gx < 0 || gy < 0 || gy >= ny) kind_ = 0;
+            if (kind_ < 0) continue;
+            xs.push_back((x + 1) | ((y + 1) << 16));
+            zs.push_back(z);
+            dir.push_back((int8_t)d);
+            knd.push_back((int8_t)kind_);
+            // outflow history starts at the shifted free-stream equilibrium of the incoming direction
+            const int id = 26 - d;
+            auto ph = [&](int c, double u) { return c == 0 ? 2.0 / 3.0 - u * u : (1.0 / 3.0 + u * u + c * u) * 0.5; };
+            const double e = ph(cxq(id), U) * ph(cyq(id), 0.0) * ph(czq(id), 0.0) - wq(id);
+            pdf.push_back(kind_ == 1 ? e : 0.0);
+            pdfnd.push_back(kind_ == 1 ? e : 0.0);
+          }
+      }
+    // velocity bounce-back links first: the outflow of a corner reads their values
+    std::vector<size_t> order(xs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return knd[a] < knd[b]; });
+    auto perm = [&](auto &v) { auto c = v; for (size_t i = 0; i < order.size(); ++i) v[i] = c[order[i]]; };
+    perm(xs); perm(zs); perm(dir); perm(knd); perm(pdf); perm(pdfnd);
+    s->nleg = (int)xs.size();
+    if (upload(s, &s->d_legx, xs) || upload(s, &s->d_legz, zs) || upload(s, &s->d_legdir, dir) ||
+        upload(s, &s->d_legkind, knd) || upload(s, &s->d_pdf, pdf) || upload(s, &s->d_pdfnd, pdfnd))
+      return -1;
+  }
+  return 0;
+}
+
+int ak_set_sweep_tile(void *handle, int bx, int by) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  if (bx < 64 || bx % 64 || by < 1 || bx * by > 256) return fail(s, "tile %dx%d: bx multiple of 64, bx*by <= 256", bx, by);
+  set_geom(s, bx, by);
+  return 0;
+}
+
+int ak_run(void *handle, int nsteps, long long first_step, int sample_every) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  for (int k = 0; k < nsteps; ++k) {
+    const long long number = first_step + k + 1;
+    const bool sample = sample_every > 0 && number % sample_every == 0;
+    if (sample && s->hist_rows >= s->hist_cap && flush_history(s)) return -1;
+    switch (s->storage) {
+      case 0: one_step<0>(s, sample, (double)number); break;
+      case 1: one_step<1>(s, sample, (double)number); break;
+      default: one_step<2>(s, sample, (double)number); break;
+    }
+  }
+  HIPCHK(s, hipGetLastError());
+  return 0;
+}
+
+// Same launches as ak_run, with HIP events around each sweep and each boundary phase
+// on the lattice's stream.  out = {sweep ms total, boundary ms total, sweeps}.
+int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_every, double *out) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  std::vector<hipEvent_t> ev(4 * (size_t)nsteps);
+  for (auto &e : ev) HIPCHK(s, hipEventCreate(&e));
+  for (int k = 0; k < nsteps; ++k) {
+    const long long number = first_step + k + 1;
+    const bool sample = sample_every > 0 && number % sample_every == 0;
+    if (sample && s->hist_rows >= s->hist_cap && flush_history(s)) return -1;
+    const bool pe = (s->sweeps % 2) == 0;
+    HIPCHK(s, hipEventRecord(ev[4 * k], s->stream));
+    switch (s->storage) {
+      case 0: if (pe) launch_boundary<0, true>(s, sample); else launch_boundary<0, false>(s, sample); break;
+      case 1: if (pe) launch_boundary<1, true>(s, sample); else launch_boundary<1, false>(s, sample); break;
+      default: if (pe) launch_boundary<2, true>(s, sample); else launch_boundary<2, false>(s, sample); break;
+    }
+    double *row = s->d_hist + (size_t)s->hist_rows * s->hist_width;
+    if (sample) {
+      const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
+      hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, s->stream, s->d_partials,
+                         s->nlinks ? s->wall_blocks : 0, s->nelem, qa, s->chord, (double)number, row, s->d_diag);
+    }
+    HIPCHK(s, hipEventRecord(ev[4 * k + 1], s->stream));
+    HIPCHK(s, hipEventRecord(ev[4 * k + 2], s->stream));
+    switch (s->storage) {
+      case 0: launch_sweep<0>(s, pe, sample); break;
+      case 1: launch_sweep<1>(s, pe, sample); break;
+      default: launch_sweep<2>(s, pe, sample); break;
+    }
+    HIPCHK(s, hipEventRecord(ev[4 * k + 3], s->stream));
+    if (sample) {
+      hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, row);
+      s->hist_rows++;
+    }
+    s->sweeps++;
+  }
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  double sweep = 0.0, bnd = 0.0;
+  for (int k = 0; k < nsteps; ++k) {
+    float a = 0.f, b = 0.f;
+    HIPCHK(s, hipEventElapsedTime(&a, ev[4 * k], ev[4 * k + 1]));
+    HIPCHK(s, hipEventElapsedTime(&b, ev[4 * k + 2], ev[4 * k + 3]));
+    bnd += a;
+    sweep += b;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  out[0] = sweep;
+  out[1] = bnd;
+  out[2] = nsteps;
+  return 0;
+}
+
+int ak_history_width(void *handle) { return handle ? H(handle)->hist_width : 0; }
+
+int ak_read_history(void *handle, double *out, int max_rows) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  if (flush_history(s)) return -1;
+  const int rows = (int)(s->host_hist.size() / s->hist_width);
+  const int n = std::min(rows, max_rows);
+  if (out && n) std::memcpy(out, s->host_hist.data(), (size_t)n * s->hist_width * sizeof(double));
+  s->host_hist.erase(s->host_hist.begin(), s->host_hist.begin() + (size_t)n * s->hist_width);
+  return n;
+}
+
+int ak_get_populations(void *handle, double *host_out) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  const size_t n = (size_t)s->nx * s->ny * s->nz * 27;
+  double *d = nullptr;
+  HIPCHK(s, hipMalloc((void **)&d, n * sizeof(double)));
+  AK_DISPATCH(s, export_any, s, d);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpy(host_out, d, n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(s, "population export: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int ak_set_populations(void *handle, const double *host_in) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  const size_t n = (size_t)s->nx * s->ny * s->nz * 27;
+  double *d = nullptr;
+  HIPCHK(s, hipMalloc((void **)&d, n * sizeof(double)));
+  hipError_t e = hipMemcpy(d, host_in, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    AK_DISPATCH(s, import_any, s, d);
+    e = hipStreamSynchronize(s->stream);
+  }
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(s, "population import: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int ak_get_macroscopic(void *handle, double *rho, double *u) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  const size_t ncell = (size_t)s->nx * s->ny * s->nz;
+  std::vector<double> g(ncell * 27);
+  if (ak_get_populations(handle, g.data())) return -1;
+  for (size_t c = 0; c < ncell; ++c) {
+    double r = 0, jx = 0, jy = 0, jz = 0;
+    for (int j = 0; j < 27; ++j) {
+      const double v = g[c * 27 + j];
+      r += v; jx += cxq(j) * v; jy += cyq(j) * v; jz += czq(j) * v;
+    }
+    r += 1.0;
+    rho[c] = r;
+    u[3 * c] = jx / r; u[3 * c + 1] = jy / r; u[3 * c + 2] = jz / r;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+template <int S, bool PE>
+void launch_diag_state(Lattice *s, double *part, unsigned nb) {
+  using T = typename St<S>::T;
+  if (s->ff_kind == AK_FARFIELD_PERIODIC)
+    hipLaunchKernelGGL((k_diag_state<S, PE, true>), dim3(nb), dim3(256), 0, s->stream, (const T *)s->pop, s->d_solid, s->L, s->inv_scale, part);
+  else
+    hipLaunchKernelGGL((k_diag_state<S, PE, false>), dim3(nb), dim3(256), 0, s->stream, (const T *)s->pop, s->d_solid, s->L, s->inv_scale, part);
+}
+template <int S>
+void diag_any(Lattice *s, double *part, unsigned nb) {
+  if (s->sweeps % 2 == 0) launch_diag_state<S, true>(s, part, nb);
+  else launch_diag_state<S, false>(s, part, nb);
+}
+template <int S, bool PE>
+void launch_plane(Lattice *s, int z, double *r, double *u) {
+  using T = typename St<S>::T;
+  const long long n = (long long)s->nx * s->ny;
+  if (s->ff_kind == AK_FARFIELD_PERIODIC)
+    hipLaunchKernelGGL((k_plane<S, PE, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, z, s->inv_scale, r, u);
+  else
+    hipLaunchKernelGGL((k_plane<S, PE, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, z, s->inv_scale, r, u);
+}
+template <int S>
+void plane_any(Lattice *s, int z, double *r, double *u) {
+  if (s->sweeps % 2 == 0) launch_plane<S, true>(s, z, r, u);
+  else launch_plane<S, false>(s, z, r, u);
+}
+}  // namespace
+
+extern "C" {
+
+int ak_diagnostics(void *handle, double *out) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  const unsigned nb = blocks_for((long long)s->nx * s->ny * s->nz, 256);
+  double *d = nullptr;
+  HIPCHK(s, hipMalloc((void **)&d, (size_t)nb * 5 * sizeof(double)));
+  AK_DISPATCH(s, diag_any, s, d, nb);
+  std::vector<double> h((size_t)nb * 5);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(s, "diagnostics: %s", hipGetErrorString(e));
+  double sum = 0.0, u2 = 0.0, dr = 0.0, rmin = 1e300, bad = 0.0;
+  for (unsigned b = 0; b < nb; ++b) {
+    sum += h[5 * b]; u2 = std::max(u2, h[5 * b + 1]); dr = std::max(dr, h[5 * b + 2]);
+    rmin = std::min(rmin, h[5 * b + 3]); bad = std::max(bad, h[5 * b + 4]);
+  }
+  out[0] = std::sqrt(3.0) * std::sqrt(u2);
+  out[1] = dr;
+  out[2] = rmin;
+  out[3] = bad;
+  out[4] = sum;
+  return 0;
+}
+
+int ak_get_plane(void *handle, int z, double *rho, double *u) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  if (z < 0 || z >= s->nz) return fail(s, "plane z=%d outside 0..%d", z, s->nz - 1);
+  const size_t n = (size_t)s->nx * s->ny;
+  double *d = nullptr;
+  HIPCHK(s, hipMalloc((void **)&d, n * 4 * sizeof(double)));
+  AK_DISPATCH(s, plane_any, s, z, d, d + n);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpy(rho, d, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(u, d + n, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(s, "plane export: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int ak_synchronize(void *handle) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,630 @@
+// ak_kernels.hpp -- gfx950 kernels of one lattice step (sweep, walls, farfield, reductions).
+#pragma once
+#include "ak_device.hpp"
+
+namespace ak {
+
+// ------------------------------------------------------------------ sweep
+// One thread per interior cell; blocks tile (x, y) of one z plane.  The 1-D grid is
+// remapped so that each XCD (blocks are dealt round-robin over 8 XCDs) owns a
+// contiguous slab of tiles and the neighbour rows it gathers stay in its own L2.
+struct SweepGeom {
+  int bx, by;      // block tile (threads) in x, y
+  int gx, gy, gz;  // tiles per axis
+  int nblocks;
+};
+
+struct DiagAcc {  // order-independent maxima/minima as bit patterns of non-negative doubles
+  unsigned long long u2max, drmax, rhomin, nonfinite;
+};
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, xcd = b % 8, k = b / 8;
+  return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+template <typename C>
+__device__ __forceinline__ C wave_max(C v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) { C t = __shfl_xor(v, o, 64); v = t > v ? t : v; }
+  return v;
+}
+template <typename C>
+__device__ __forceinline__ C wave_min(C v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) { C t = __shfl_xor(v, o, 64); v = t < v ? t : v; }
+  return v;
+}
+
+template <int S, bool ODD, bool PXY, bool DIAG>
+__global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A,
+                                               const uint8_t *__restrict__ solid, Desc L, SweepGeom G,
+                                               typename St<S>::C ws, typename St<S>::C wb, float scale,
+                                               float inv_scale, DiagAcc *__restrict__ diag) {
+  using C = typename St<S>::C;
+  const int lb = xcd_remap(blockIdx.x, G.nblocks);
+  const int tx = lb % G.gx, ty = (lb / G.gx) % G.gy, tz = lb / (G.gx * G.gy);
+  const int x = tx * G.bx + (int)(threadIdx.x % G.bx);
+  const int y = ty * G.by + (int)(threadIdx.x / G.bx);
+  const int z = tz;
+  bool active = x < L.nx && y < L.ny;
+  if (active) active = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)] == 0;
+  C drho = 0, u2 = 0;
+  if (active) {
+    const long long cell = L.base + cell_of(L, x, y, z);
+    C g[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      long long src;
+      if constexpr (ODD) src = cell + (long long)opp(q) * L.QS + nb_off<PXY>(L, x, y, z, -cxq(q), -cyq(q), -czq(q));
+      else src = cell + (long long)q * L.QS;
+      g[q] = ld<S>(A + src, inv_scale);
+    }
+    collide<C>(g, ws, wb, drho, u2);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      long long dst;
+      if constexpr (ODD) dst = cell + (long long)q * L.QS + nb_off<PXY>(L, x, y, z, cxq(q), cyq(q), czq(q));
+      else dst = cell + (long long)opp(q) * L.QS;
+      stv<S>(A + dst, g[q], scale);
+    }
+  }
+  if constexpr (DIAG) {
+    // rho and u are conserved by the collision: the pre-collision moments are the
+    // post-collision state's.  Non-finite values raise a flag.
+    double vu = active ? (double)u2 : 0.0;
+    double vd = active ? fabs((double)drho) : 0.0;
+    double vr = active ? 1.0 + (double)drho : 1e300;
+    int bad = active && !(isfinite(vu) && isfinite(vd)) ? 1 : 0;
+    if (!isfinite(vu)) vu = 0.0;
+    if (!isfinite(vd)) vd = 0.0;
+    if (!isfinite(vr)) vr = 1e300;
+    vu = wave_max(vu); vd = wave_max(vd); vr = wave_min(vr);
+    bad = __any(bad) ? 1 : 0;
+    __shared__ double su[4], sd[4], sr[4];
+    __shared__ int sb[4];
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) { su[w] = vu; sd[w] = vd; sr[w] = vr; sb[w] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = (blockDim.x + 63) / 64;
+      for (int k = 1; k < nw; ++k) {
+        vu = su[k] > vu ? su[k] : vu; vd = sd[k] > vd ? sd[k] : vd;
+        vr = sr[k] < vr ? sr[k] : vr; bad |= sb[k];
+      }
+      atomicMax(&diag->u2max, (unsigned long long)__double_as_longlong(vu));
+      atomicMax(&diag->drmax, (unsigned long long)__double_as_longlong(vd));
+      if (vr > 0.0) atomicMin(&diag->rhomin, (unsigned long long)__double_as_longlong(vr));
+      else atomicMax(&diag->nonfinite, 2ull);  // non-positive density
+      if (bad) atomicMax(&diag->nonfinite, 1ull);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ walls
+// Bouzidi interpolated bounce-back over the compacted fluid->solid link list, fused
+// with the momentum-exchange loads.  One thread per (link, z).  Per-block sums of
+// [Fx, Fy, Fz, Mz, mass] for every element are written in a fixed order; the final
+// reduction sums the blocks in a fixed order, so loads are deterministic.
+struct WallLinks {
+  const int32_t *xy;     // x | (y << 16)
+  const int8_t *dir;
+  const double *q;
+  const int8_t *elem;
+  int n, nelem;
+  double rx_off, ry_off;
+};
+
+constexpr int MAX_ELEM = 8;
+
+template <int S, bool PE, bool LOADS>
+__global__ __launch_bounds__(256) void k_walls(typename St<S>::T *__restrict__ A, Desc L, WallLinks W,
+                                               float scale, float inv_scale, double *__restrict__ partials) {
+  using C = typename St<S>::C;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)W.n * L.nz;
+  double f[5] = {0, 0, 0, 0, 0};
+  int el = -1;
+  if (t < total) {
+    const int k = (int)(t / L.nz), z = (int)(t % L.nz);
+    const int xy = W.xy[k];
+    const int x = xy & 0xffff, y = xy >> 16;
+    const int d = W.dir[k], id = opp(d);
+    const C q = (C)W.q[k];
+    const C fxf = ld<S>(A + loc_out<PE>(L, x, y, z, d), inv_scale);
+    const C fxi = ld<S>(A + loc_out<PE>(L, x, y, z, id), inv_scale);
+    C val;
+    if (q >= C(0.5)) {
+      val = C(0.5) * (fxf + fxi * (C(2) * q - C(1))) / q;
+    } else if (q > C(0)) {
+      // f_out(x - c_d, d): the second fluid node along the link
+      const int zz = (int)(((z - czq(d)) % L.nz + L.nz) % L.nz);
+      const C d2 = ld<S>(A + loc_out<PE>(L, x - cxq(d), y - cyq(d), zz, d), inv_scale);
+      val = C(2) * q * fxf + (C(1) - C(2) * q) * d2;
+    } else {
+      val = fxf;
+    }
+    stv<S>(A + loc_in<PE>(L, x, y, z, id), val, scale);
+    if constexpr (LOADS) {
+      // force on the solid: c_d (outgoing + reflected); lever arm at the fluid cell centre
+      const double v = (double)fxf + (double)val;
+      f[0] = cxq(d) * v; f[1] = cyq(d) * v; f[2] = czq(d) * v;
+      const double rx = x + 0.5 + W.rx_off, ry = y + 0.5 + W.ry_off;
+      f[3] = rx * f[1] - ry * f[0];
+      f[4] = (double)val - (double)fxf;
+      el = W.elem[k];
+    }
+  }
+  if constexpr (LOADS) {
+    __shared__ double red[4][5];
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    for (int e = 0; e < W.nelem; ++e) {
+      double v[5];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        double a = el == e ? f[c] : 0.0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
+        v[c] = a;
+      }
+      if (lane == 0)
+        for (int c = 0; c < 5; ++c) red[w][c] = v[c];
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const int nw = blockDim.x / 64;
+        for (int c = 0; c < 5; ++c) {
+          double s = red[0][c];
+          for (int k = 1; k < nw; ++k) s += red[k][c];
+          partials[((long long)blockIdx.x * W.nelem + e) * 5 + c] = s;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Sum the per-block wall partials (fixed order), convert to coefficients and write
+// the loads part of a history row; clear the diagnostics accumulator.
+__global__ void k_loads_final(const double *__restrict__ partials, int nblocks, int nelem, double qa,
+                              double chord, double step, double *__restrict__ row, DiagAcc *diag) {
+  __shared__ double acc[MAX_ELEM * 5][64];
+  const int lane = threadIdx.x;  // 64 threads
+  for (int c = 0; c < nelem * 5; ++c) {
+    double s = 0.0;
+    for (int b = lane; b < nblocks; b += 64) s += partials[(long long)b * nelem * 5 + c];
+    acc[c][lane] = s;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double tot[5] = {0, 0, 0, 0, 0};
+    for (int e = 0; e < nelem; ++e) {
+      double v[5];
+      for (int c = 0; c < 5; ++c) {
+        double s = 0.0;
+        for (int k = 0; k < 64; ++k) s += acc[e * 5 + c][k];
+        v[c] = s;
+        tot[c] += s;
+      }
+      for (int c = 0; c < 4; ++c) row[9 + e * 4 + c] = v[c];
+    }
+    row[0] = step;
+    row[1] = tot[1] / qa;
+    row[2] = tot[0] / qa;
+    row[3] = -tot[3] / (qa * chord);
+    row[8] = tot[4];
+    diag->u2max = 0ull;
+    diag->drmax = 0ull;
+    diag->rhomin = (unsigned long long)__double_as_longlong(1e300);
+    diag->nonfinite = 0ull;
+  }
+}
+
+__global__ void k_diag_final(const DiagAcc *diag, double *__restrict__ row) {
+  if (threadIdx.x == 0) {
+    row[4] = sqrt(3.0) * sqrt(__longlong_as_double((long long)diag->u2max));
+    row[5] = __longlong_as_double((long long)diag->drmax);
+    row[6] = __longlong_as_double((long long)diag->rhomin);
+    row[7] = (double)diag->nonfinite;
+  }
+}
+
+// ------------------------------------------------------------------ farfield
+// Characteristic farfield (isothermal Riemann invariants, NEEP reconstruction, linear
+// transverse transport).  Work item = one exterior ghost cell g of one face/edge
+// object; every link entering the fluid from g shares the foot g - n, hence the
+// same boundary state, which is computed once per item.
+struct FFItems {
+  const int32_t *g;        // ghost cell: (x+1) | (y+1) << 16
+  const int32_t *gz;       // its z in [0, nz)
+  const int8_t *obj;       // object 0..7
+  const int32_t *dmask;    // bit d set: link from g - c_d (direction d) exists
+  double *incoming, *derivative;
+  long long *clock;
+  const double *initial;   // glancing trace rows: [side][z][x], side 0 = y low, 1 = y high
+  int n;
+};
+
+struct FFObj {
+  int n[3];
+  double nh[3], un_inf, ut_inf[3], P[3][3];
+  int evolve, glancing, use_axis[3], side;
+};
+
+struct FFParams {
+  FFObj o[8];
+  double U, rho0;
+};
+
+template <int S, bool PE>
+__device__ __forceinline__ void macro_cell(const typename St<S>::T *A, const Desc &L, int x, int y, int z,
+                                           float inv_scale, typename St<S>::C &drho, typename St<S>::C (&u)[3]) {
+  using C = typename St<S>::C;
+  C r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+    const C v = ld<S>(A + loc_out<PE>(L, x, y, z, j), inv_scale);
+    r += v;
+    if (cxq(j)) jx += cxq(j) * v;
+    if (cyq(j)) jy += cyq(j) * v;
+    if (czq(j)) jz += czq(j) * v;
+  }
+  drho = r;
+  const C rho = C(1) + r;
+  u[0] = jx / rho; u[1] = jy / rho; u[2] = jz / rho;
+}
+
+template <typename C>
+__device__ __forceinline__ C log_rho(C drho, C rho0) {
+  // ln(rho / rho0) with rho = 1 + drho
+  if constexpr (sizeof(C) == 8) return log((C(1) + drho) / rho0);
+  else return log1pf(drho) - logf(rho0);
+}
+
+template <int S, bool PE>
+__device__ double ff_transport(const typename St<S>::T *A, const Desc &L, const FFObj &o, const int foot[3],
+                               const int edge[2], typename St<S>::C rm, const typename St<S>::C (&u)[3],
+                               float inv_scale, double rho0) {
+  using C = typename St<S>::C;
+  const C cs = (C)0.57735026918962576451;
+  C F = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (!o.use_axis[a]) continue;
+    const int e = a < 2 ? edge[a] : 0;
+    const int off1 = e == -1 ? 1 : -1;
+    const int off2 = e == 0 ? 1 : -2 * e;
+    const C c0 = C(1.5) * e;
+    const C c1 = e == 0 ? C(-0.5) : C(-2) * e;
+    const C c2 = e == 0 ? C(0.5) : C(0.5) * e;
+    int p1[3] = {foot[0], foot[1], foot[2]}, p2[3] = {foot[0], foot[1], foot[2]};
+    p1[a] += off1; p2[a] += off2;
+    p1[2] = ((p1[2] % L.nz) + L.nz) % L.nz;
+    p2[2] = ((p2[2] % L.nz) + L.nz) % L.nz;
+    C d1, d2, u1[3], u2[3];
+    macro_cell<S, PE>(A, L, p1[0], p1[1], p1[2], inv_scale, d1, u1);
+    macro_cell<S, PE>(A, L, p2[0], p2[1], p2[2], inv_scale, d2, u2);
+    C pdu = 0;
+    for (int b = 0; b < 3; ++b) pdu += (C)o.P[a][b] * (c0 * u[b] + c1 * u1[b] + c2 * u2[b]);
+    const C rm1 = (C)o.nh[0] * u1[0] + (C)o.nh[1] * u1[1] + (C)o.nh[2] * u1[2] - cs * log_rho<C>(d1, (C)rho0);
+    const C rm2 = (C)o.nh[0] * u2[0] + (C)o.nh[1] * u2[1] + (C)o.nh[2] * u2[2] - cs * log_rho<C>(d2, (C)rho0);
+    F += (C)o.un_inf * pdu - (C)o.ut_inf[a] * (c0 * rm + c1 * rm1 + c2 * rm2);
+  }
+  return (double)F;
+}
+
+__device__ __forceinline__ void ff_unpack(int32_t p, int &gx, int &gy) {
+  gx = (p & 0xffff) - 1;
+  gy = ((p >> 16) & 0xffff) - 1;
+}
+
+template <int S, bool PE>
+__global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict__ A, Desc L, FFItems I,
+                                                  FFParams Pm, float scale, float inv_scale) {
+  using C = typename St<S>::C;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= I.n) return;
+  int gx, gy;
+  ff_unpack(I.g[k], gx, gy);
+  const int gz = I.gz[k];
+  const FFObj &o = Pm.o[I.obj[k]];
+  int foot[3] = {gx - o.n[0], gy - o.n[1], gz};
+  const int edge[2] = {foot[0] == 0 ? -1 : (foot[0] == L.nx - 1 ? 1 : 0),
+                       foot[1] == 0 ? -1 : (foot[1] == L.ny - 1 ? 1 : 0)};
+  C drho, u[3];
+  macro_cell<S, PE>(A, L, foot[0], foot[1], foot[2], inv_scale, drho, u);
+  const C cs = (C)0.57735026918962576451;
+  const C un = (C)o.nh[0] * u[0] + (C)o.nh[1] * u[1] + (C)o.nh[2] * u[2];
+  const C lr = log_rho<C>(drho, (C)Pm.rho0);
+  const C rm = un - cs * lr, rp = un + cs * lr;
+  C incoming;
+  if (o.evolve && !o.glancing) {
+    const double F = ff_transport<S, PE>(A, L, o, foot, edge, rm, u, inv_scale, Pm.rho0);
+    const double inc = I.incoming[k] + (F + I.derivative[k]) / 2.0;
+    I.incoming[k] = inc;
+    I.derivative[k] = F;
+    incoming = (C)inc;
+  } else if (o.glancing) {
+    // one interpolation of the stored initial invariant at the total traced shift
+    const long long clk = I.clock[k];
+    const double target = (double)foot[0] - Pm.U * (double)clk;
+    const int lower = (int)floor(target);
+    const double frac = target - lower;
+    const double *row = I.initial + ((long long)o.side * L.nz + foot[2]) * L.nx;
+    double s = 0.0;
+    for (int bit = 0; bit < 2; ++bit) {
+      const int kx = lower + bit;
+      const double wgt = bit ? frac : 1.0 - frac;
+      const double v = (kx >= 0 && kx < L.nx) ? row[kx] : 0.0;
+      s += wgt * v;
+    }
+    incoming = (C)(o.un_inf + s);
+    I.clock[k] = clk + 1;
+  } else {
+    incoming = (C)o.un_inf;
+  }
+  const C unb = (rp + incoming) * C(0.5);
+  const C rho = C(1) + drho;
+  const C rhob = (C)Pm.rho0 * exp((rp - incoming) / (C(2) * cs));
+  C ub[3];
+  const C vinf[3] = {(C)Pm.U, C(0), C(0)};
+  for (int a = 0; a < 3; ++a)
+    ub[a] = (C)o.nh[a] * unb + (unb >= C(0) ? u[a] - (C)o.nh[a] * un : vinf[a] - (C)o.nh[a] * (C)o.un_inf);
+  const int mask = I.dmask[k];
+#pragma unroll
+  for (int d = 0; d < NQ; ++d) {
+    if (!((mask >> d) & 1)) continue;
+    const int id = opp(d);
+    const C fo = ld<S>(A + loc_out<PE>(L, foot[0], foot[1], foot[2], id), inv_scale);
+    const C val = fo + (equilibrium<C>(id, rhob, ub[0], ub[1], ub[2]) - equilibrium<C>(id, rho, u[0], u[1], u[2]));
+    // fluid cell x = g - c_d reads direction id
+    const int xz = ((gz - czq(d)) % L.nz + L.nz) % L.nz;
+    stv<S>(A + loc_in<PE>(L, gx - cxq(d), gy - cyq(d), xz, id), val, scale);
+  }
+}
+
+// Initial farfield state from the initial populations (post-even form): incoming =
+// R- - F and derivative = F on evolving objects (a backward Taylor seed), and the
+// trace rows of glancing objects.
+template <int S>
+__global__ void k_farfield_init(const typename St<S>::T *__restrict__ A, Desc L, FFItems I, FFParams Pm,
+                                float inv_scale) {
+  using C = typename St<S>::C;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= I.n) return;
+  int gx, gy;
+  ff_unpack(I.g[k], gx, gy);
+  const int gz = I.gz[k];
+  const FFObj &o = Pm.o[I.obj[k]];
+  int foot[3] = {gx - o.n[0], gy - o.n[1], gz};
+  const int edge[2] = {foot[0] == 0 ? -1 : (foot[0] == L.nx - 1 ? 1 : 0),
+                       foot[1] == 0 ? -1 : (foot[1] == L.ny - 1 ? 1 : 0)};
+  if (o.glancing) { I.clock[k] = 0; return; }
+  if (!o.evolve) { I.incoming[k] = o.un_inf; I.derivative[k] = 0.0; return; }
+  C drho, u[3];
+  macro_cell<S, true>(A, L, foot[0], foot[1], foot[2], inv_scale, drho, u);
+  const C cs = (C)0.57735026918962576451;
+  const C rm = (C)o.nh[0] * u[0] + (C)o.nh[1] * u[1] + (C)o.nh[2] * u[2] - cs * log_rho<C>(drho, (C)Pm.rho0);
+  const double F = ff_transport<S, true>(A, L, o, foot, edge, rm, u, inv_scale, Pm.rho0);
+  I.incoming[k] = (double)rm - F;
+  I.derivative[k] = F;
+}
+
+// Initial outgoing invariant of the two y boundary rows (glancing objects):
+// rows[side][z][x] = u.n - cs ln(rho/rho0) - un_inf with n = (0, -1, 0) / (0, 1, 0).
+template <int S>
+__global__ void k_glancing_rows(const typename St<S>::T *__restrict__ A, Desc L, double *rows, double rho0,
+                                float inv_scale) {
+  using C = typename St<S>::C;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2LL * L.nz * L.nx) return;
+  const int x = (int)(t % L.nx), z = (int)((t / L.nx) % L.nz), side = (int)(t / ((long long)L.nx * L.nz));
+  const int y = side == 0 ? 0 : L.ny - 1;
+  C drho, u[3];
+  macro_cell<S, true>(A, L, x, y, z, inv_scale, drho, u);
+  const double ny_ = side == 0 ? -1.0 : 1.0;
+  rows[t] = ny_ * (double)u[1] - 0.57735026918962576451 * (double)log_rho<C>(drho, (C)rho0);
+}
+
+// ------------------------------------------------------------------ legacy farfield
+// Velocity bounce-back on x-, y-, y+ (density of the fluid cell) and the extrapolation
+// outflow on x+ (c-weighted blend of the last outgoing population of the tangential
+// neighbour and the previous boundary value).  Work item = one link.
+struct LegacyLinks {
+  const int32_t *x;   // fluid cell packed like FFItems::g: (x+1) | (y+1) << 16
+  const int32_t *z;
+  const int8_t *dir;
+  const int8_t *kind; // 0 velocity bounce-back, 1 outflow
+  double *pdf, *pdf_nd;
+  int n;
+};
+
+template <int S, bool PE>
+__device__ __forceinline__ typename St<S>::C ubb_value(const typename St<S>::T *A, const Desc &L, int x, int y,
+                                                      int z, int d, double U, float inv_scale) {
+  using C = typename St<S>::C;
+  C r = 0;
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) r += ld<S>(A + loc_out<PE>(L, x, y, z, j), inv_scale);
+  const C rho = C(1) + r;
+  const C fo = ld<S>(A + loc_out<PE>(L, x, y, z, d), inv_scale);
+  return fo - C(6) * (C)wq(d) * (C)(cxq(d) * U) * rho;
+}
+
+template <int S, bool PE>
+__global__ __launch_bounds__(128) void k_legacy(typename St<S>::T *__restrict__ A, Desc L, LegacyLinks K,
+                                                double U, float scale, float inv_scale) {
+  using C = typename St<S>::C;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K.n) return;
+  int x, y;
+  ff_unpack(K.x[k], x, y);
+  const int z = K.z[k];
+  const int d = K.dir[k], id = opp(d);
+  C val;
+  if (K.kind[k] == 0) {
+    val = ubb_value<S, PE>(A, L, x, y, z, d, U, inv_scale);
+  } else {
+    const double c = 0.57735026918962576451;
+    const double v = K.pdf[k] * c + (1.0 - c) * K.pdf_nd[k];
+    val = (C)v;
+    // new history: f_out(t, id) of the tangential neighbour t = x + c_d - n
+    const int ty = y + cyq(d), tz = ((z + czq(d)) % L.nz + L.nz) % L.nz;
+    double nv;
+    if (ty < 0 || ty >= L.ny) {
+      // t is a velocity-bounce-back ghost: its slot holds the bounce-back value of
+      // link (t - c_d, d), written in this same boundary pass
+      nv = (double)ubb_value<S, PE>(A, L, x - 1, ty - cyq(d), z, d, U, inv_scale);
+    } else {
+      nv = (double)ld<S>(A + loc_out<PE>(L, x, ty, tz, id), inv_scale);
+    }
+    K.pdf[k] = nv;
+    K.pdf_nd[k] = v;
+  }
+  stv<S>(A + loc_in<PE>(L, x, y, z, id), val, scale);
+}
+
+// ------------------------------------------------------------------ init / exchange
+struct SeedParams {
+  double U, amp, phy, phz, ky, kz, lex, ley, ox, oy, dx;
+};
+
+// Seeded initial field, product-form equilibrium with rho = 1, written in post-even
+// form.  Evaluated in FP64 for every storage precision.
+template <int S>
+__global__ void k_init_seeded(typename St<S>::T *__restrict__ A, const uint8_t *__restrict__ solid, Desc L,
+                              SeedParams p, float scale) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L.nx * L.ny * L.nz) return;
+  const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
+  const double xx = ((double)x + 0.5) * p.dx + p.ox, yy = ((double)y + 0.5) * p.dx + p.oy;
+  const double zz = ((double)z + 0.5) * p.dx;
+  const double rx = xx - p.lex, ry = yy - p.ley;
+  const double e = (rx - 0.5) / 0.75;
+  const double env = exp(-e * e);
+  double ux = p.U;
+  double uy = p.amp * sin(p.ky * ry + p.phy) * cos(p.kz * zz + p.phz) * env;
+  double uz = -p.amp * p.ky / p.kz * cos(p.ky * ry + p.phy) * sin(p.kz * zz + p.phz) * env;
+  if (solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]) ux = uy = uz = 0.0;
+  const long long c = L.base + cell_of(L, x, y, z);
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    const double g = equilibrium<double>(i, 1.0, ux, uy, uz) - wq(i);
+    stv<S>(A + c + (long long)opp(i) * L.QS, (typename St<S>::C)g, scale);
+  }
+}
+
+template <int S, bool PE, bool PXY>
+__global__ void k_export(const typename St<S>::T *__restrict__ A, Desc L, double *__restrict__ out,
+                         float inv_scale) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L.nx * L.ny * L.nz) return;
+  const int z = (int)(t % L.nz), y = (int)((t / L.nz) % L.ny), x = (int)(t / ((long long)L.nz * L.ny));
+  const long long c = L.base + cell_of(L, x, y, z);
+  for (int j = 0; j < NQ; ++j) {
+    long long src;
+    if constexpr (PE) src = c + (long long)opp(j) * L.QS;
+    else src = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
+    out[t * NQ + j] = (double)ld<S>(A + src, inv_scale);
+  }
+}
+
+template <int S, bool PE, bool PXY>
+__global__ void k_import(typename St<S>::T *__restrict__ A, Desc L, const double *__restrict__ in, float scale) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L.nx * L.ny * L.nz) return;
+  const int z = (int)(t % L.nz), y = (int)((t / L.nz) % L.ny), x = (int)(t / ((long long)L.nz * L.ny));
+  const long long c = L.base + cell_of(L, x, y, z);
+  for (int j = 0; j < NQ; ++j) {
+    long long dst;
+    if constexpr (PE) dst = c + (long long)opp(j) * L.QS;
+    else dst = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
+    stv<S>(A + dst, (typename St<S>::C)in[t * NQ + j], scale);
+  }
+}
+
+// Read-only state diagnostics: per-block [sum rho, max |u|^2, max |rho-1|, min rho, bad]
+// over fluid cells; the host folds the blocks in a fixed order.
+template <int S, bool PE, bool PXY>
+__global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__restrict__ A,
+                                                    const uint8_t *__restrict__ solid, Desc L,
+                                                    float inv_scale, double *__restrict__ part) {
+  using C = typename St<S>::C;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)L.nx * L.ny * L.nz;
+  double sr = 0.0, vu = 0.0, vd = 0.0, vr = 1e300;
+  int bad = 0;
+  if (t < n) {
+    const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
+    if (!solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]) {
+      const long long c = L.base + cell_of(L, x, y, z);
+      C r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        long long src;
+        if constexpr (PE) src = c + (long long)opp(j) * L.QS;
+        else src = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
+        const C v = ld<S>(A + src, inv_scale);
+        r += v;
+        if (cxq(j)) jx += cxq(j) * v;
+        if (cyq(j)) jy += cyq(j) * v;
+        if (czq(j)) jz += czq(j) * v;
+      }
+      const double rho = 1.0 + (double)r;
+      const double ux = (double)jx / rho, uy = (double)jy / rho, uz = (double)jz / rho;
+      const double u2 = ux * ux + uy * uy + uz * uz;
+      if (!(isfinite(rho) && isfinite(u2))) bad = 1;
+      else { sr = rho; vu = u2; vd = fabs((double)r); vr = rho; }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    sr += __shfl_xor(sr, o, 64);
+    double a = __shfl_xor(vu, o, 64); vu = a > vu ? a : vu;
+    a = __shfl_xor(vd, o, 64); vd = a > vd ? a : vd;
+    a = __shfl_xor(vr, o, 64); vr = a < vr ? a : vr;
+  }
+  bad = __any(bad) ? 1 : 0;
+  __shared__ double sh[4][5];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) { sh[w][0] = sr; sh[w][1] = vu; sh[w][2] = vd; sh[w][3] = vr; sh[w][4] = bad; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double o0 = sh[0][0], o1 = sh[0][1], o2 = sh[0][2], o3 = sh[0][3], o4 = sh[0][4];
+    for (int k = 1; k < 4; ++k) {
+      o0 += sh[k][0]; o1 = sh[k][1] > o1 ? sh[k][1] : o1; o2 = sh[k][2] > o2 ? sh[k][2] : o2;
+      o3 = sh[k][3] < o3 ? sh[k][3] : o3; o4 = sh[k][4] > o4 ? sh[k][4] : o4;
+    }
+    double *p = part + (long long)blockIdx.x * 5;
+    p[0] = o0; p[1] = o1; p[2] = o2; p[3] = o3; p[4] = o4;
+  }
+}
+
+// rho and u of one z plane (interior x, y), x-major output: rho[x*ny + y], u[(x*ny + y)*3 + a]
+template <int S, bool PE, bool PXY>
+__global__ void k_plane(const typename St<S>::T *__restrict__ A, Desc L, int z, float inv_scale,
+                        double *__restrict__ rho_out, double *__restrict__ u_out) {
+  using C = typename St<S>::C;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L.nx * L.ny) return;
+  const int y = (int)(t % L.ny), x = (int)(t / L.ny);
+  const long long c = L.base + cell_of(L, x, y, z);
+  C r = 0, jx = 0, jy = 0, jz = 0;
+  for (int j = 0; j < NQ; ++j) {
+    long long src;
+    if constexpr (PE) src = c + (long long)opp(j) * L.QS;
+    else src = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
+    const C v = ld<S>(A + src, inv_scale);
+    r += v; jx += cxq(j) * v; jy += cyq(j) * v; jz += czq(j) * v;
+  }
+  const double rho = 1.0 + (double)r;
+  rho_out[t] = rho;
+  u_out[3 * t] = (double)jx / rho; u_out[3 * t + 1] = (double)jy / rho; u_out[3 * t + 2] = (double)jz / rho;
+}
+
+template <typename T>
+__global__ void k_fill(T *p, long long n, T v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = t; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace ak

@@ -1,0 +1,280 @@
+"""Uniform-grid solver: the drop-in `Solver` of the reference on one MI355X.
+
+Public surface of the reference (`src/aerokinetic/solver.py:27-537`):
+
+* `coefficients(force, moment_z, speed, chord_cells, span_cells)` (solver.py:27-30);
+* `Solver(section, flow, grid, progress=None)` with `step(sample)`,
+  `diagnostics()`, `run(...)`, `save_frame(...)` and the `NumericalFailure`
+  error (solver.py:23-24, 252-537);
+* outputs `history.csv` (step,t_star,time_s,cl,cd,cm,local_mach_max,
+  density_error_max), `metadata.json`, `summary.json`, `project.json` and
+  mid-span `frames/*.npz`.
+
+All numerics run in the native HIP library: one AA-pattern lattice in HBM,
+boundary kernels before each sweep, loads and low-Mach guards reduced on the
+device into a history buffer read back in batches.  Extra keyword-only
+arguments select the storage precision ("fp64", "fp32", "fp16"), the device and
+an explicit plan (fixed lattice shapes).
+"""
+from __future__ import annotations
+
+from dataclasses import asdict
+import csv
+import json
+import math
+import time
+from pathlib import Path
+
+import numpy as np
+
+from .config import EXTERIOR_FARFIELD, Flow, Grid, Plan
+from .geometry import Section
+from ._io import atomic_binary, write_json
+from .lattice import Lattice
+from .project import Project, RunSettings
+from .walls import labels, wall_links
+from . import _lib
+
+
+class NumericalFailure(RuntimeError):
+    pass
+
+
+def coefficients(force, moment_z, speed, chord_cells, span_cells):
+    """Force on the solid: +Cl up, +Cd downstream, +Cm nose up."""
+    q_area = 0.5 * speed**2 * chord_cells * span_cells
+    return float(force[1] / q_area), float(force[0] / q_area), float(-moment_z / (q_area * chord_cells))
+
+
+def seed_parameters(section, flow, plan):
+    """Scalars of the seeded initial field (solver.py:167-197): phases from the flow seed,
+    leading edge in the normalized frame, wave numbers of the transverse/spanwise modes."""
+    rng = np.random.default_rng(flow.seed)
+    phase_y, phase_z = rng.uniform(0, 2 * np.pi, 2)
+    le = section.leading_edge_normalized(flow.alpha_deg)
+    return {"U": flow.lattice_speed, "amp": flow.perturbation * flow.lattice_speed,
+            "phase_y": float(phase_y), "phase_z": float(phase_z),
+            "ky": 2 * np.pi / (plan.shape[1] * plan.dx), "kz": 2 * np.pi / plan.span,
+            "lex": float(le[0]), "ley": float(le[1])}
+
+
+class Solver:
+    HISTORY_COLUMNS = ["step", "t_star", "time_s", "cl", "cd", "cm", "local_mach_max", "density_error_max"]
+
+    def __init__(self, section: Section, flow: Flow = Flow(), grid: Grid = Grid(), progress=None, *,
+                 storage="fp64", device=0, plan: Plan | None = None, fp16_scale=0.0, tile=None):
+        self.section, self.flow, self.grid = section, flow, grid
+        if grid.farfield == EXTERIOR_FARFIELD:
+            raise ValueError("exterior_regularized_fd_v1 is not provided by this package; "
+                             "use the characteristic or legacy farfield")
+        if storage not in _lib.STORE:
+            raise ValueError("storage must be fp64, fp32 or fp16")
+        self.plan = plan or Plan.create(section, flow, grid)
+        self.storage = storage
+        self.contours = section.normalized(flow.alpha_deg)
+        if self.plan.estimated_memory_gib > grid.memory_limit_gib:
+            raise MemoryError(f"Estimated {self.plan.estimated_memory_gib:.2f} GiB exceeds the explicit "
+                              f"limit {grid.memory_limit_gib:.2f} GiB. Inspect Plan before choosing a larger limit.")
+        self._progress = progress or (lambda message: None)
+        nx, ny, nz = self.plan.shape
+        lab = labels(self.contours, (nx, ny), self.plan.origin, self.plan.dx)
+        self._solid_xy = lab[1:-1, 1:-1] > 0
+        for k, e in enumerate(self.contours):
+            if not (lab == k + 1).any():
+                raise ValueError(f"Element {e.name} has no solid cells; increase resolution")
+        self._fluid = np.broadcast_to(~self._solid_xy[:, :, None], self.plan.shape)
+        self._progress("Building the D3Q27 cumulant lattice on the GPU")
+        self.lattice = Lattice(nx, ny, nz, storage=storage, omega_shear=self.plan.relaxation_rate,
+                               omega_bulk=1.0, fp16_scale=fp16_scale, device=device)
+        if tile:
+            self.lattice.set_tile(*tile)
+        self.lattice.set_solid(self._solid_xy)
+        self.links, fallback = wall_links(self.contours, (nx, ny), self.plan.origin, self.plan.dx)
+        self.wall_fallback_links = fallback * nz
+        self.lattice.set_walls(self.links, len(self.contours), self.plan.origin[0] / self.plan.dx,
+                               self.plan.origin[1] / self.plan.dx, grid.cells_per_chord, flow.lattice_speed)
+        self._initialize()
+        self.last_loads = None
+
+    # ------------------------------------------------------------------ state
+    def _initialize(self):
+        self.lattice.init_seeded(seed_parameters(self.section, self.flow, self.plan), self.plan.origin, self.plan.dx)
+        self.lattice.set_farfield(self.grid.farfield, self.flow.lattice_speed)
+        self.steps = 0
+
+    def _record(self, row):
+        nel = len(self.contours)
+        rec = {"cl": float(row[1]), "cd": float(row[2]), "cm": float(row[3]),
+               "element_lattice_loads": np.array(row[_lib.HIST_FIXED:_lib.HIST_FIXED + 4 * nel]).reshape(nel, 4),
+               "step": int(row[0]), "t_star": (int(row[0]) - 0.5) * self.plan.dt_convective,
+               "local_mach_max": float(row[4]), "density_error_max": float(row[5]),
+               "density_min": float(row[6]), "nonfinite": float(row[7])}
+        if self.plan.dt_seconds is not None:
+            rec["time_s"] = (int(row[0]) - 0.5) * self.plan.dt_seconds
+        return rec
+
+    def advance(self, nsteps, sample_every=0):
+        """Advance nsteps; returns the records of the sampled steps (device-batched)."""
+        if nsteps <= 0:
+            return []
+        self.lattice.run(nsteps, self.steps, sample_every)
+        self.steps += nsteps
+        if sample_every <= 0:
+            return []
+        return [self._record(r) for r in self.lattice.read_history()]
+
+    def step(self, sample=False):
+        """One time step; with sample=True returns the loads of this step (solver.py:252-279)."""
+        recs = self.advance(1, 1 if sample else 0)
+        if not sample:
+            return None
+        rec = recs[-1]
+        loads = {k: rec[k] for k in ("cl", "cd", "cm", "element_lattice_loads", "step", "t_star")}
+        if "time_s" in rec:
+            loads["time_s"] = rec["time_s"]
+        self.last_loads = loads
+        self._last_diag = rec
+        return loads
+
+    def diagnostics(self):
+        """Low-Mach validity guards of the current state (solver.py:281-297)."""
+        mach, derr, rmin, bad, rsum = self.lattice.diagnostics()
+        if bad or not (math.isfinite(mach) and math.isfinite(derr)):
+            raise NumericalFailure("Non-finite fluid state; the run is invalid")
+        if mach > 0.3 or derr > 0.1 or rmin <= 0:
+            raise NumericalFailure(f"Low-Mach validity check failed: local Ma={mach:.3g}, |rho-1|max={derr:.3g}")
+        nfluid = int(self._fluid.sum())
+        return {"local_mach_max": float(mach), "density_mean": float(rsum / nfluid),
+                "density_error_max": float(derr), "wall_fallback_links": self.wall_fallback_links}
+
+    def populations(self):
+        """Post-collision shifted populations g[x, y, z, 27] (FP64 copy)."""
+        return self.lattice.populations()
+
+    def set_populations(self, g):
+        self.lattice.set_populations(g)
+
+    # ------------------------------------------------------------------ outputs
+    def save_frame(self, path, max_pixels=800):
+        """Mid-span speed, vorticity and Cp planes (solver.py:312-334)."""
+        nx, ny, nz = self.plan.shape
+        stride = max(1, math.ceil(max(nx, ny) / max_pixels))
+        rho, vel = self.lattice.plane(nz // 2)
+        rho, vel = rho[::stride, ::stride], vel[::stride, ::stride]
+        U = self.flow.lattice_speed
+        speed = np.linalg.norm(vel, axis=-1) / U
+        h = stride * self.plan.dx
+        omega_z = (np.gradient(vel[..., 1], h, axis=0) - np.gradient(vel[..., 0], h, axis=1)) / U
+        cp = (rho - 1) / (1.5 * U**2)
+        mask = self._solid_xy[::stride, ::stride]
+        for a in (speed, omega_z, cp):
+            a[mask] = np.nan
+        with atomic_binary(path) as handle:
+            np.savez_compressed(handle, speed=speed, omega_z=omega_z, cp=cp, mask=mask,
+                                origin=self.plan.origin, dx=self.plan.dx * stride,
+                                t_star=self.steps * self.plan.dt_convective)
+
+    def run(self, duration_convective, output, sample_every=20, frame_every=200, average_after=0.0,
+            check_every=100, callback=None, stop_event=None, checkpoint_every=1000, resume=False,
+            termination=None, residual_callback=None):
+        """Run a further t* = t U / c interval into a new/empty directory (solver.py:353-537).
+
+        callback(solver, record) may return False to stop.  Mean/RMS coefficients
+        are plain sample statistics over t_star >= average_after.
+        """
+        from .convergence import StopCriteria
+        from .timing import duration_steps
+        from . import __version__
+        if resume:
+            raise NotImplementedError("resume requires device checkpoints, which this package does not write yet")
+        termination = termination or StopCriteria()
+        settings = RunSettings(duration_convective, sample_every, frame_every, check_every, average_after,
+                               checkpoint_every, termination)
+        project = Project(self.section, self.flow, self.grid, settings, "+".join(e.name for e in self.section.elements))
+        output = Path(output)
+        if output.exists() and any(output.iterdir()):
+            raise FileExistsError("Use a new/empty output directory to avoid mixing histories")
+        output.mkdir(parents=True, exist_ok=True)
+        (output / "frames").mkdir(exist_ok=True)
+        end_step = self.steps + duration_steps(duration_convective, self.plan.dt_convective)
+        if termination.max_steps:
+            end_step = min(end_step, self.steps + termination.max_steps)
+        metadata = {"version": __version__, "flow": asdict(self.flow), "grid": asdict(self.grid),
+                    "plan": self.plan.as_dict(), "reference_chord_input_units": self.section.chord,
+                    "moment_reference_input_units": self.section.moment_reference,
+                    "contours": [{"name": e.name, "xy": e.xy.tolist()} for e in self.section.elements],
+                    "coefficient_convention": "Cl up; Cd downstream; Cm nose-up about moment_reference; area=c*periodic_span",
+                    "force_time": "midpoint of boundary momentum-transfer interval",
+                    "status": "running", "experimental_validation": "not_performed",
+                    "case_fingerprint": project.case_fingerprint, "run": asdict(settings),
+                    "start_step": self.steps, "start_t_star": self.steps * self.plan.dt_convective,
+                    "resume_supported": False, "end_step": end_step,
+                    "device": {"backend": "hip-gfx950", "storage": self.storage,
+                               "library": _lib.load().ak_version().decode()}}
+        project.save(output / "project.json")
+        write_json(output / "metadata.json", metadata)
+        rows, state, reason = [], "completed", "duration_reached"
+        start = time.monotonic()
+        history = output / "history.csv"
+        try:
+            self.diagnostics()
+            self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
+            with history.open("w", newline="", encoding="utf-8") as handle:
+                writer = csv.DictWriter(handle, self.HISTORY_COLUMNS)
+                writer.writeheader()
+                stop = False
+                while self.steps < end_step and not stop:
+                    if stop_event is not None and stop_event.is_set():
+                        state, reason = "stopped", "user_stopped"
+                        break
+                    # next event: a sample (when a callback must see it), a check, a frame, the end
+                    nxt = end_step
+                    for every in (check_every, frame_every) + ((sample_every,) if callback else ()):
+                        if every:
+                            nxt = min(nxt, (self.steps // every + 1) * every)
+                    if not callback:
+                        nxt = min(nxt, self.steps + 256 * max(sample_every, 1))
+                    recs = self.advance(nxt - self.steps, sample_every)
+                    for rec in recs:
+                        if rec["nonfinite"] or not all(math.isfinite(rec[k]) for k in ("cl", "cd", "cm")):
+                            raise NumericalFailure("Non-finite loads; the run is invalid")
+                        if rec["local_mach_max"] > 0.3 or rec["density_error_max"] > 0.1 or rec["density_min"] <= 0:
+                            raise NumericalFailure(f"Low-Mach validity check failed: local Ma={rec['local_mach_max']:.3g}, "
+                                                   f"|rho-1|max={rec['density_error_max']:.3g}")
+                        row = {k: rec.get(k, "") for k in self.HISTORY_COLUMNS}
+                        writer.writerow(row)
+                        rows.append(row)
+                        self.last_loads = rec
+                        if callback is not None and callback(self, rec) is False:
+                            state, reason, stop = "stopped", "user_stopped", True
+                            break
+                    handle.flush()
+                    if check_every and self.steps % check_every == 0:
+                        self.diagnostics()
+                    if (frame_every and self.steps % frame_every == 0) or self.steps == end_step:
+                        self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
+                    if termination.wall_time_limit_s and time.monotonic() - start > termination.wall_time_limit_s:
+                        state, reason = "stopped", "wall_time_limit"
+                        break
+                self.diagnostics()
+                self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
+        except BaseException as error:
+            state, reason = "failed", "failed"
+            metadata["failure"] = f"{type(error).__name__}: {error}"
+            raise
+        finally:
+            sel = np.array([[r["cl"], r["cd"], r["cm"]] for r in rows if r["t_star"] >= average_after])
+            ok = len(sel) and state != "failed"
+            summary = {"status": state, "steps": self.steps, "t_star": self.steps * self.plan.dt_convective,
+                       "wall_clock_seconds": time.monotonic() - start, "samples": len(rows),
+                       "averaging_samples": len(sel), "average_after_t_star": average_after,
+                       "mean": dict(zip(("cl", "cd", "cm"), map(float, sel.mean(axis=0)))) if ok else None,
+                       "rms_fluctuation": dict(zip(("cl", "cd", "cm"), map(float, sel.std(axis=0)))) if ok else None,
+                       "wall_fallback_links": self.wall_fallback_links,
+                       "experimental_validation": "not_performed", "termination_reason": reason,
+                       "termination": asdict(termination)}
+            metadata["termination_reason"] = reason
+            metadata["status"] = state
+            write_json(output / "metadata.json", metadata)
+            write_json(output / "summary.json", summary)
+        return summary

@@ -1,0 +1,132 @@
+/*
+ * aerokinetic.h -- C ABI of the MI355X D3Q27 cumulant lattice-Boltzmann step.
+ *
+ * One opaque lattice = one uniform D3Q27 level of one airfoil case on one GPU.
+ * All entry points return 0 on success and a negative code on failure; the
+ * message is available from ak_last_error().  No entry point throws, and none
+ * allocates device memory in the step path (ak_run / ak_step_*): buffers are
+ * created by ak_lattice_create / ak_set_* or bound by the caller.
+ *
+ * Each entry point replaces a piece of the reference's lbmpy/pystencils kernel
+ * plug-in boundary (paths inside the reference repository):
+ *
+ *   ak_lattice_create        LatticeBoltzmannStep(...) kernel build
+ *                            (src/aerokinetic/solver.py:58-91, interface.py:116-130)
+ *   ak_set_solid/ak_set_walls  solid masks + NoSlipLinearBouzidi index lists
+ *                            (solver.py:93-122,139-143, walls.py:21-49)
+ *   ak_set_farfield          install_characteristic_farfield / UBB +
+ *                            ExtrapolationOutflow (farfield.py:209-224, solver.py:130-138)
+ *   ak_init_seeded           _initialize + set_pdf_fields_from_macroscopic_values
+ *                            (solver.py:157-202)
+ *   ak_run                   the per-step body of Solver.step: _sync_src,
+ *                            boundary_handling, _loads, run_kernel(stream_collide),
+ *                            swap (solver.py:252-279)
+ *   ak_read_history          loads + diagnostics of sampled steps
+ *                            (solver.py:204-250, 281-297)
+ *   ak_diagnostics           post_run + the low-Mach guards of Solver.diagnostics
+ *                            (solver.py:281-297)
+ *   ak_get_plane             the mid-span gather of save_frame (solver.py:312-334)
+ *   ak_get/set_populations   gather_array / cpu_arrays of the pdf field (checkpoint.py:32-68)
+ *
+ * Population exchange layout (host, FP64): g[x][y][z][27], zero-centred
+ * g = f - w, direction i = 9 (cx+1) + 3 (cy+1) + (cz+1).
+ */
+#ifndef AEROKINETIC_H
+#define AEROKINETIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* storage precision of the populations */
+enum { AK_STORE_F64 = 0, AK_STORE_F32 = 1, AK_STORE_F16 = 2 };
+/* outer boundary */
+enum { AK_FARFIELD_PERIODIC = 0, AK_FARFIELD_LEGACY = 1, AK_FARFIELD_CHARACTERISTIC = 2 };
+/* one history row per sampled step (doubles) */
+enum {
+  AK_HIST_STEP = 0, AK_HIST_CL = 1, AK_HIST_CD = 2, AK_HIST_CM = 3,
+  AK_HIST_MACH = 4, AK_HIST_DENSITY_ERR = 5, AK_HIST_RHO_MIN = 6, AK_HIST_NONFINITE = 7,
+  AK_HIST_MASS_EXCHANGE = 8, AK_HIST_FIXED = 9 /* then nelem*4 per-element loads Fx,Fy,Fz,Mz */
+};
+
+/* Create a lattice of nx*ny*nz fluid cells (z periodic) on `device`.
+ * omega_shear / omega_bulk: relaxation rates; fp16_scale: power-of-two scale of
+ * FP16 shifted storage (ignored otherwise; 0 selects the default);
+ * history_capacity: sampled rows kept on the device between reads;
+ * stream: hipStream_t for every launch (NULL = default stream). */
+int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int storage,
+                      double omega_shear, double omega_bulk, double fp16_scale,
+                      int history_capacity, void *stream);
+int ak_lattice_destroy(void *handle);
+
+/* Bytes the caller must provide for the population buffer, and binding of a
+ * caller-owned device buffer (e.g. a torch tensor's data_ptr()). */
+size_t ak_population_bytes(void *handle);
+int ak_bind_populations(void *handle, void *device_ptr, size_t bytes);
+
+/* Solid mask of the interior xy cells (host, nx*ny bytes, x-major: m[x*ny+y]). */
+int ak_set_solid(void *handle, const uint8_t *mask_xy);
+
+/* Fluid-to-solid links of the xy plane, extruded over z (host arrays of n):
+ * fluid cell (x, y), direction d (solid at x + c_d), Bouzidi q (-1 = halfway
+ * fallback), element index.  Lever arm = (index + 0.5 + r_off) in lattice units;
+ * chord_cells, lattice_speed normalise the coefficients. */
+int ak_set_walls(void *handle, int n, const int32_t *x, const int32_t *y, const int32_t *d,
+                 const double *q, const int32_t *element, int nelem, double rx_off, double ry_off,
+                 double chord_cells, double lattice_speed);
+
+/* Outer boundary; reads the current populations, so call after the initial state. */
+int ak_set_farfield(void *handle, int kind, double lattice_speed);
+
+/* Seeded initial field (u_x = U plus the spanwise-periodic Gaussian-envelope
+ * perturbation; u = 0 in solid cells; rho = 1) as product-form equilibrium.
+ * p = {U, amp, phase_y, phase_z, ky, kz, lex, ley, ox, oy, dx}. */
+int ak_init_seeded(void *handle, const double *p);
+
+/* Thread tile of the sweep (bx a multiple of 64, bx*by <= 256); set by the tile autotuner. */
+int ak_set_sweep_tile(void *handle, int bx, int by);
+
+/* Advance `nsteps` reference steps.  Steps with (first_step + k + 1) % sample_every == 0
+ * (k = 0..nsteps-1; sample_every <= 0 disables) record a history row. */
+int ak_run(void *handle, int nsteps, long long first_step, int sample_every);
+
+/* ak_run with HIP events on the lattice's stream around every sweep and every
+ * boundary phase (measurement only; synchronises at the end).
+ * out = {total sweep milliseconds, total boundary milliseconds, steps}. */
+int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_every, double *out);
+
+/* Copy up to max_rows recorded rows to host (row length ak_history_width) and
+ * clear the device history; returns the row count (or < 0 on error). */
+int ak_history_width(void *handle);
+int ak_read_history(void *handle, double *out, int max_rows);
+
+/* Populations of the interior, FP64, in the exchange layout (synchronises). */
+int ak_get_populations(void *handle, double *host_out);
+int ak_set_populations(void *handle, const double *host_in);
+
+/* Macroscopic fields of the interior: rho[x][y][z], u[x][y][z][3] (host, FP64). */
+int ak_get_macroscopic(void *handle, double *rho, double *u);
+
+/* Diagnostics of the current state over fluid cells (synchronises):
+ * out = {local Mach max, max |rho-1|, min rho, non-finite flag, sum of rho}. */
+int ak_diagnostics(void *handle, double *out);
+
+/* rho[x][y] and u[x][y][3] of one z plane (host, FP64; synchronises). */
+int ak_get_plane(void *handle, int z, double *rho, double *u);
+
+/* Wait for all work of this lattice. */
+int ak_synchronize(void *handle);
+
+/* Last error message of this handle (or of the library when handle is NULL). */
+int ak_last_error(void *handle, char *buf, size_t len);
+
+/* Library build identifier. */
+const char *ak_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

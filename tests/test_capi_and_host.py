@@ -1,0 +1,106 @@
+"""CPU-side checks: the C ABI library loads and exports every declared symbol; host
+logic (case model, wall links) against the reference's rules and the oracle."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from aerokinetic_amd import _lib
+from aerokinetic_amd.config import Flow, Grid, Plan
+from aerokinetic_amd.geometry import Contour, Section, naca0012
+from aerokinetic_amd.project import Project
+from aerokinetic_amd.timing import duration_steps
+from aerokinetic_amd.walls import extrude, labels, wall_links
+from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "aerokinetic.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char \*)\s*(ak_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_lib.exported_symbols())
+    assert b"gfx950" in lib.ak_version()
+
+
+def test_product_links_equal_independent_oracle_links():
+    project = Project.load(ROOT / "tests" / "golden" / "n32_legacy_down3" / "project.json")
+    plan = project.plan()
+    contours = project.section.normalized(project.flow.alpha_deg)
+    links, fallback = wall_links(contours, plan.shape[:2], plan.origin, plan.dx)
+    o = O.Oracle(*plan.shape, plan.relaxation_rate)
+    o.set_geometry(contours, plan.origin, plan.dx)
+    ol = o.links()
+    ext = extrude(links, plan.shape[2])
+    a = np.lexsort((ext["z"], ext["d"], ext["y"], ext["x"], ext["element"]))
+    b = np.lexsort((ol["z"], ol["d"], ol["y"], ol["x"], ol["element"]))
+    for k in ("x", "y", "z", "d", "element"):
+        np.testing.assert_array_equal(ext[k][a], ol[k][b])
+    np.testing.assert_array_equal(ext["q"][a], ol["q"][b])
+    assert fallback * plan.shape[2] == o.fallback_links()
+    np.testing.assert_array_equal(labels(contours, plan.shape[:2], plan.origin, plan.dx)[1:-1, 1:-1] > 0, o.solid())
+
+
+def test_link_fraction_against_exact_plane():
+    # tests/test_geometry_and_units.py:6-10 of the reference
+    box = Contour([[0.25, -1], [2, -1], [2, 1], [0.25, 1]])
+    p = np.array([[0, 0], [0, 0.5], [0, -0.25]])
+    offsets = np.array([[1, 0], [1, 1], [1, -1]])
+    np.testing.assert_allclose(box.link_fraction(p, offsets), 0.25)
+
+
+def test_links_land_on_the_polygon():
+    # tests/test_numerics.py:40-56 of the reference: q puts the wall point on the contour
+    section = Section((naca0012(31),))
+    flow, grid = Flow(perturbation=0), Grid(cells_per_chord=16, upstream_chords=1, downstream_chords=2,
+                                            transverse_chords=1, memory_limit_gib=1)
+    plan = Plan.create(section, flow, grid)
+    contours = section.normalized(flow.alpha_deg)
+    links, _ = wall_links(contours, plan.shape[:2], plan.origin, plan.dx)
+    from aerokinetic_amd.d3q27 import CX, CY
+    d = links["d"]
+    pts = np.column_stack(((links["x"] + 0.5) * plan.dx + plan.origin[0], (links["y"] + 0.5) * plan.dx + plan.origin[1]))
+    q = links["q"]
+    assert (q > 0).all()
+    dirs = np.column_stack((CX[d], CY[d])) * plan.dx
+    wall = pts + q[:, None] * dirs
+    eps = 1e-6 * dirs
+    assert not contours[0].contains(*(wall - eps).T).any()
+    assert contours[0].contains(*(wall + eps).T).all()
+
+
+def test_units_and_duration_rounding():
+    a = Contour([[0, 0], [1, 0], [1, 0.1], [0, 0.1]])
+    b = Contour([[1.2, 0], [2, -0.1], [2, -0.2], [1.2, -0.1]])
+    section = Section((a, b))
+    assert section.chord == pytest.approx(np.hypot(2, 0.15))
+    flow = Flow(reynolds=30000, physical_chord_m=0.15)
+    grid = Grid(cells_per_chord=64)
+    plan = Plan.create(section, flow, grid)
+    assert flow.lattice_speed * grid.cells_per_chord / plan.viscosity == pytest.approx(30000)
+    assert plan.dt_seconds * flow.physical_speed / flow.physical_chord_m == pytest.approx(plan.dt_convective)
+    assert duration_steps(0.0175, 0.04 / 16) == 7
+    assert duration_steps(1.0, 0.1) == 10
+    with pytest.raises(ValueError):
+        Flow(reynolds=5000)
+    with pytest.raises(ValueError, match="farfield"):
+        Grid(farfield="misspelled")
+
+
+def test_explicit_plan_puts_leading_edge_on_requested_cell():
+    from aerokinetic_amd import cases
+    section, flow, grid, plan = cases.NACA0012_C32_384x256x8.build()
+    assert plan.shape == (384, 256, 8)
+    le = section.leading_edge_normalized(flow.alpha_deg)
+    np.testing.assert_allclose((le[0] - plan.origin[0]) / plan.dx, 128)
+    np.testing.assert_allclose((le[1] - plan.origin[1]) / plan.dx, 128)
+    assert flow.lattice_speed == pytest.approx(0.1 / np.sqrt(3))
