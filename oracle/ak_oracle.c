@@ -706,6 +706,69 @@ static void stream_collide(ako *s) {
     double *t = s->src; s->src = s->dst; s->dst = t;
 }
 
+/* ---- collide_stream building blocks (refined levels, interface.py:274-283) ---- */
+
+/* collision in place on the interior (solid cells skipped) */
+void ako_collide(void *h) {
+    ako *s = (ako *)h;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y) {
+            if (s->solid[(x + 1) * s->Y + (y + 1)]) continue;
+            for (int z = 0; z < s->nz; ++z) {
+                double *g = s->src + cid(s, x, y, z) * NQ;
+                double o[NQ];
+                collide(g, o, s->omega_s, s->omega_b);
+                memcpy(g, o, sizeof o);
+            }
+        }
+}
+
+/* periodic ghost copy and boundary objects (farfield, walls) on the current state */
+void ako_boundaries(void *h) {
+    ako *s = (ako *)h;
+    sync_periodic(s);
+    if (s->ff_kind == 2) apply_characteristic(s);
+    else if (s->ff_kind == 1) apply_legacy(s);
+    apply_walls(s);
+}
+
+/* pull streaming of every interior cell, then swap */
+void ako_stream(void *h) {
+    ako *s = (ako *)h;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y)
+            for (int z = 0; z < s->nz; ++z) {
+                double *d = s->dst + cid(s, x, y, z) * NQ;
+                for (int i = 0; i < NQ; ++i) d[i] = s->src[cid(s, x - CX[i], y - CY[i], z - CZ[i]) * NQ + i];
+            }
+    double *t = s->src; s->src = s->dst; s->dst = t;
+}
+
+/* Wall momentum exchange after the boundary objects (walls.py:68-95): links whose
+ * fluid cell lies in [lo, hi); lever arm ((index + 0.5) dx + origin) in common units.
+ * out = {Fx, Fy, Fz, Mz, mass exchange, links} */
+void ako_measure(void *h, const int *lo, const int *hi, double dx, double ox, double oy, double *out) {
+    ako *s = (ako *)h;
+    double f[5] = {0, 0, 0, 0, 0};
+    long n = 0;
+    for (int k = 0; k < s->nlinks; ++k) {
+        const wall_link *l = &s->links[k];
+        if (l->x < lo[0] || l->x >= hi[0] || l->y < lo[1] || l->y >= hi[1] || l->z < lo[2] || l->z >= hi[2]) continue;
+        const int d = l->d, id = NQ - 1 - d;
+        const double o = s->src[cid(s, l->x, l->y, l->z) * NQ + d];
+        const double in = s->src[cid(s, l->x + CX[d], l->y + CY[d], l->z + CZ[d]) * NQ + id];
+        const double v = o + in;
+        const double fx = CX[d] * v, fy = CY[d] * v;
+        const double rx = (l->x + 0.5) * dx + ox, ry = (l->y + 0.5) * dx + oy;
+        f[0] += fx; f[1] += fy; f[2] += CZ[d] * v; f[3] += rx * fy - ry * fx; f[4] += in - o;
+        ++n;
+    }
+    for (int c = 0; c < 5; ++c) out[c] = f[c];
+    out[5] = (double)n;
+}
+
 /* One reference time step.  If loads != NULL the per-element loads
  * [Fx, Fy, Fz, Mz] (lattice units) are written after the boundary objects,
  * and coef (if not NULL) receives cl, cd, cm. */

@@ -57,6 +57,10 @@ def lib():
         L.ako_step.argtypes = [ctypes.c_void_p, _c_double_p, _c_double_p]
         L.ako_diagnostics.argtypes = [ctypes.c_void_p, _c_double_p]
         L.ako_threads.restype = ctypes.c_int
+        for name in ("ako_collide", "ako_boundaries", "ako_stream"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+        L.ako_measure.argtypes = [ctypes.c_void_p, _c_int_p, _c_int_p, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, _c_double_p]
         _lib = L
     return _lib
 

@@ -6,6 +6,8 @@ so the tests never need it.  Sources (reference paths):
   docs/verification_farfield_20261107/airfoil/characteristic_down3/{project.json,history.csv}
   docs/verification_stage1_20261106/profile_threads1.json (last_sample after 272 steps)
   tests/fixtures/legacy051_project.json (fingerprint fixture)
+  docs/verification_040/{startup_project.json,startup_history.csv} (two-level startup, every 4 coarse steps)
+  docs/verification_multielement_050/fowler/{project.json,history.csv} (two elements, every coarse step)
 """
 import csv
 import json
@@ -26,6 +28,23 @@ for name, (rel, last_step) in CASES.items():
     dst.mkdir(exist_ok=True)
     shutil.copyfile(REF / rel / "project.json", dst / "project.json")
     with open(REF / rel / "history.csv", newline="") as src, open(dst / "history_head.csv", "w", newline="") as out:
+        reader = csv.DictReader(src)
+        writer = csv.DictWriter(out, reader.fieldnames)
+        writer.writeheader()
+        for row in reader:
+            if int(row["step"]) > last_step:
+                break
+            writer.writerow(row)
+
+REFINED = {
+    "refined_startup_n32": ("docs/verification_040", "startup_project.json", "startup_history.csv", 40),
+    "refined_fowler_n32": ("docs/verification_multielement_050/fowler", "project.json", "history.csv", 32),
+}
+for name, (rel, proj, hist, last_step) in REFINED.items():
+    dst = OUT / name
+    dst.mkdir(exist_ok=True)
+    shutil.copyfile(REF / rel / proj, dst / "project.json")
+    with open(REF / rel / hist, newline="") as src, open(dst / "history_head.csv", "w", newline="") as out:
         reader = csv.DictReader(src)
         writer = csv.DictWriter(out, reader.fieldnames)
         writer.writeheader()

@@ -78,3 +78,23 @@ def test_fingerprint_fixture():
     p = Project.load(GOLDEN / "legacy051_project.json")
     assert p.case_fingerprint == "9ae371cecf492f67494d9f6ac6eb70f2a33cc4e6ca680a7088b6ce146206f48f"
     assert "farfield" not in p.as_dict()["grid"]
+
+
+def _refined_history(name):
+    with open(GOLDEN / name / "history_head.csv", newline="") as fh:
+        return {int(r["step"]): {k: float(r[k]) for k in ("cl", "cd", "cm")} for r in csv.DictReader(fh)}
+
+
+@pytest.mark.parametrize("name,steps", [("refined_startup_n32", 12), ("refined_fowler_n32", 6)])
+def test_refined_oracle_pins_transfer_interface_and_fine_walls(name, steps):
+    """Two-level 2:1 step: prolongation in time, reflux, restriction, fine Bouzidi walls, loads."""
+    from aerokinetic_amd.refined_plan import RefinedPlan, load_refined_project
+    from oracle.refined_oracle import RefinedOracle
+    project, settings = load_refined_project(GOLDEN / name / "project.json")
+    ref = _refined_history(name)
+    ro = RefinedOracle(project, settings, RefinedPlan.create(project, settings))
+    for _ in range(steps):
+        rec = ro.step()
+        if rec["step"] in ref:
+            for k in ("cl", "cd", "cm"):
+                assert rec[k] == pytest.approx(ref[rec["step"]][k], abs=1e-12)
