@@ -66,3 +66,31 @@ NACA0012_C128_1536x1024x64 = UniformCase("naca0012 c128 1536x1024x64 a8 Re1e5 Ma
 def sampled_naca_section(camber, thickness, position=0.4, n_side=161, closed_te=True):
     """NACA 4-digit section with real-valued camber / thickness fractions."""
     return Section((naca4_profile(camber, position, thickness, n_side, closed_te, "NACA"),), 1.0, (0.25, 0.0))
+
+
+@dataclass(frozen=True)
+class RefinedCase:
+    """Two-level workload: coarse lattice, explicit fine box, legacy coarse farfield."""
+    name: str
+    coarse: UniformCase
+    box_lower: tuple
+    box_upper: tuple
+    coarse_bulk_tau: float = 1.0
+
+    def build(self):
+        """(project, settings, plan) of this workload."""
+        from .project import Project, RunSettings
+        from .refined_plan import RefinedPlan, RefinementSettings
+        section, flow, grid, base = self.coarse.build()
+        project = Project(section, flow, grid, RunSettings(), self.name)
+        settings = RefinementSettings(coarse_bulk_relaxation_time=self.coarse_bulk_tau)
+        plan = RefinedPlan.explicit(project, settings, base, self.box_lower, self.box_upper)
+        return project, settings, plan
+
+
+# Two-level case: coarse 256x192x8 (chord 32 coarse = 64 fine cells) with a fine patch of
+# 160x96x16 fine cells (80x48x8 coarse) around a NACA0012 at 6 deg, Re 3e4, Ma 0.1.
+NACA0012_REFINED_256x192x8 = RefinedCase(
+    "naca0012 two-level coarse 256x192x8 fine 160x96x16 c64f a6 Re3e4 Ma0.1",
+    UniformCase("coarse", (256, 192, 8), 32, 6.0, 3e4, 0.1, (80, 96), farfield="legacy"),
+    (62, 72), (142, 120))

@@ -126,7 +126,7 @@ void launch_sweep(Lattice *s, bool odd, bool diag) {
   using C = typename St<S>::C;
   T *A = (T *)s->pop;
   const dim3 grid(s->geom.nblocks), block(s->geom.bx * s->geom.by);
-  const bool pxy = s->ff_kind == AK_FARFIELD_PERIODIC;
+  const bool pxy = s->ff_kind == AK_FARFIELD_PERIODIC;  // AK_FARFIELD_NONE keeps ghost indexing
   const C ws = (C)s->ws, wb = (C)s->wb;
 #define AK_SWEEP(O, P, D) \
   hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, A, s->d_solid, s->L, s->geom, ws, wb, s->scale, s->inv_scale, s->d_diag)
@@ -154,7 +154,7 @@ void launch_boundary(Lattice *s, bool loads) {
                        s->U, s->scale, s->inv_scale);
   }
   if (s->nlinks) {
-    WallLinks W{s->d_wxy, s->d_wdir, s->d_wq, s->d_welem, s->nlinks, s->nelem, s->rx_off, s->ry_off};
+    WallLinks W{s->d_wxy, s->d_wdir, s->d_wq, s->d_welem, s->nlinks, s->nelem, s->rx_off, s->ry_off, 1.0};
     if (loads)
       hipLaunchKernelGGL((k_walls<S, PE, true>), dim3(s->wall_blocks), dim3(256), 0, s->stream, A, s->L, W, s->scale,
                          s->inv_scale, s->d_partials);
@@ -173,7 +173,8 @@ void one_step(Lattice *s, bool sample, double step_number) {
   if (sample) {
     const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
     hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, s->stream, s->d_partials,
-                       s->nlinks ? s->wall_blocks : 0, s->nelem, qa, s->chord, step_number, row, s->d_diag);
+                       s->nlinks ? s->wall_blocks : 0, s->nelem, 1.0 / qa, 1.0 / (qa * s->chord), 1.0,
+                       step_number, row, s->d_diag);
   }
   launch_sweep<S>(s, pe, sample);
   if (sample) {
@@ -340,7 +341,7 @@ int ak_set_solid(void *handle, const uint8_t *mask_xy) {
   if (!s) return fail(nullptr, "null lattice handle");
   std::vector<uint8_t> m((size_t)(s->nx + 2) * (s->ny + 2), 0);
   for (int x = 0; x < s->nx; ++x)
-    for (int y = 0; y < s->ny; ++y) m[(size_t)(y + 1) * (s->nx + 2) + (x + 1)] = mask_xy[(size_t)x * s->ny + y] ? 1 : 0;
+    for (int y = 0; y < s->ny; ++y) m[(size_t)(y + 1) * (s->nx + 2) + (x + 1)] = mask_xy[(size_t)x * s->ny + y] > 2 ? 1 : mask_xy[(size_t)x * s->ny + y];
   HIPCHK(s, hipSetDevice(s->dev));
   HIPCHK(s, hipMemcpy(s->d_solid, m.data(), m.size(), hipMemcpyHostToDevice));
   return 0;
@@ -401,8 +402,9 @@ int ak_init_seeded(void *handle, const double *p) {
 int ak_set_farfield(void *handle, int kind, double U) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
-  if (kind < 0 || kind > 2) return fail(s, "unknown farfield kind %d", kind);
-  if (s->sweeps % 2 != 0) return fail(s, "set the farfield at an even step (initial state)");
+  if (kind < 0 || kind > 3) return fail(s, "unknown farfield kind %d", kind);
+  if (kind == AK_FARFIELD_CHARACTERISTIC && s->sweeps % 2 != 0)
+    return fail(s, "set the characteristic farfield on the initial state");
   s->ff_kind = kind;
   s->U = U;
   const int nx = s->nx, ny = s->ny, nz = s->nz;
@@ -570,7 +572,8 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
     if (sample) {
       const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
       hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, s->stream, s->d_partials,
-                         s->nlinks ? s->wall_blocks : 0, s->nelem, qa, s->chord, (double)number, row, s->d_diag);
+                         s->nlinks ? s->wall_blocks : 0, s->nelem, 1.0 / qa, 1.0 / (qa * s->chord), 1.0,
+                         (double)number, row, s->d_diag);
     }
     HIPCHK(s, hipEventRecord(ev[4 * k + 1], s->stream));
     HIPCHK(s, hipEventRecord(ev[4 * k + 2], s->stream));
@@ -748,3 +751,5 @@ int ak_synchronize(void *handle) {
 }
 
 }  // extern "C"
+
+#include "ak_refined_capi.hpp"

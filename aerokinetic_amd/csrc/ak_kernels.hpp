@@ -48,7 +48,8 @@ __global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A
   const int y = ty * G.by + (int)(threadIdx.x / G.bx);
   const int z = tz;
   bool active = x < L.nx && y < L.ny;
-  if (active) active = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)] == 0;
+  uint8_t mk = 1;
+  if (active) { mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]; active = mk != 1; }
   C drho = 0, u2 = 0;
   if (active) {
     const long long cell = L.base + cell_of(L, x, y, z);
@@ -72,10 +73,11 @@ __global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A
   if constexpr (DIAG) {
     // rho and u are conserved by the collision: the pre-collision moments are the
     // post-collision state's.  Non-finite values raise a flag.
-    double vu = active ? (double)u2 : 0.0;
-    double vd = active ? fabs((double)drho) : 0.0;
-    double vr = active ? 1.0 + (double)drho : 1e300;
-    int bad = active && !(isfinite(vu) && isfinite(vd)) ? 1 : 0;
+    const bool counted = active && mk == 0;
+    double vu = counted ? (double)u2 : 0.0;
+    double vd = counted ? fabs((double)drho) : 0.0;
+    double vr = counted ? 1.0 + (double)drho : 1e300;
+    int bad = counted && !(isfinite(vu) && isfinite(vd)) ? 1 : 0;
     if (!isfinite(vu)) vu = 0.0;
     if (!isfinite(vd)) vd = 0.0;
     if (!isfinite(vr)) vr = 1e300;
@@ -112,7 +114,7 @@ struct WallLinks {
   const double *q;
   const int8_t *elem;
   int n, nelem;
-  double rx_off, ry_off;
+  double rx_off, ry_off, lever;  // lever arm = (index + 0.5 + off) * lever
 };
 
 constexpr int MAX_ELEM = 8;
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(256) void k_walls(typename St<S>::T *__restrict__ A
       // force on the solid: c_d (outgoing + reflected); lever arm at the fluid cell centre
       const double v = (double)fxf + (double)val;
       f[0] = cxq(d) * v; f[1] = cyq(d) * v; f[2] = czq(d) * v;
-      const double rx = x + 0.5 + W.rx_off, ry = y + 0.5 + W.ry_off;
+      const double rx = (x + 0.5 + W.rx_off) * W.lever, ry = (y + 0.5 + W.ry_off) * W.lever;
       f[3] = rx * f[1] - ry * f[0];
       f[4] = (double)val - (double)fxf;
       el = W.elem[k];
@@ -185,8 +187,9 @@ __global__ __launch_bounds__(256) void k_walls(typename St<S>::T *__restrict__ A
 
 // Sum the per-block wall partials (fixed order), convert to coefficients and write
 // the loads part of a history row; clear the diagnostics accumulator.
-__global__ void k_loads_final(const double *__restrict__ partials, int nblocks, int nelem, double qa,
-                              double chord, double step, double *__restrict__ row, DiagAcc *diag) {
+// cl = Fy fscale, cd = Fx fscale, cm = -Mz mscale, mass exchange = mass * vscale.
+__global__ void k_loads_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
+                              double mscale, double vscale, double step, double *__restrict__ row, DiagAcc *diag) {
   __shared__ double acc[MAX_ELEM * 5][64];
   const int lane = threadIdx.x;  // 64 threads
   for (int c = 0; c < nelem * 5; ++c) {
@@ -208,10 +211,10 @@ __global__ void k_loads_final(const double *__restrict__ partials, int nblocks, 
       for (int c = 0; c < 4; ++c) row[9 + e * 4 + c] = v[c];
     }
     row[0] = step;
-    row[1] = tot[1] / qa;
-    row[2] = tot[0] / qa;
-    row[3] = -tot[3] / (qa * chord);
-    row[8] = tot[4];
+    row[1] = tot[1] * fscale;
+    row[2] = tot[0] * fscale;
+    row[3] = -tot[3] * mscale;
+    row[8] = tot[4] * vscale;
     diag->u2max = 0ull;
     diag->drmax = 0ull;
     diag->rhomin = (unsigned long long)__double_as_longlong(1e300);
@@ -527,6 +530,16 @@ __global__ void k_export(const typename St<S>::T *__restrict__ A, Desc L, double
   }
 }
 
+// Incoming (pre-collision) populations f_in(x, i) as the next sweep would gather them.
+template <int S, bool PE>
+__global__ void k_export_in(const typename St<S>::T *__restrict__ A, Desc L, double *__restrict__ out,
+                            float inv_scale) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L.nx * L.ny * L.nz) return;
+  const int z = (int)(t % L.nz), y = (int)((t / L.nz) % L.ny), x = (int)(t / ((long long)L.nz * L.ny));
+  for (int j = 0; j < NQ; ++j) out[t * NQ + j] = (double)ld<S>(A + loc_in<PE>(L, x, y, z, j), inv_scale);
+}
+
 template <int S, bool PE, bool PXY>
 __global__ void k_import(typename St<S>::T *__restrict__ A, Desc L, const double *__restrict__ in, float scale) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -554,7 +567,7 @@ __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__r
   int bad = 0;
   if (t < n) {
     const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
-    if (!solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]) {
+    if (solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)] == 0) {
       const long long c = L.base + cell_of(L, x, y, z);
       C r = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll

@@ -1,0 +1,336 @@
+// ak_refined.hpp -- kernels of the two-level 2:1 coupling (coarse lattice + padded fine box).
+//
+// Both levels are AA lattices whose state between sweeps is post-collision; the
+// pre-collision value of a cell is what the next sweep gathers (loc_in).  Every
+// interface operation therefore reads or writes loc_in slots between sweeps:
+//   donor moments   -- gather pre-collision coarse values of the donor block (box +- 3)
+//   prolong/inject  -- time-interpolated, 5-point prolonged, stress-rescaled and
+//                      regularised populations written into the fine donor cells' slots
+//   face flux       -- exact moment flux of pull streaming through the box faces,
+//                      from post-collision values (coarse once, fine per substep)
+//   reflux          -- equilibrium correction of the coarse cells outside the faces
+//   restrict        -- 2x2x2 mean of fine moments, rescaled and regularised, written
+//                      into the covered coarse cells' slots
+// Transfer arithmetic is FP64 for every storage precision.
+#pragma once
+#include "ak_kernels.hpp"
+
+namespace ak {
+
+struct Levels {
+  // stress scales coarse->fine and fine->coarse: tau*dt ratios of shear and bulk
+  double shear_cf, bulk_cf, shear_fc, bulk_fc;
+};
+
+struct BoxGeom {
+  int lo[3], hi[3];  // coarse cell bounds (z spans the whole periodic span)
+  int nb[3];         // box shape in coarse cells
+  int pad;           // fine donor layers (2)
+};
+
+enum { TE_DENSITY = 1, TE_NEGATIVE = 2, TE_MACH = 4, TE_NONFINITE = 8 };
+
+__device__ __forceinline__ void moments_of(const double (&g)[NQ], double (&m)[10], int &err) {
+  double d = 0, jx = 0, jy = 0, jz = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0, mn = 1e300;
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    const double v = g[i];
+    d += v;
+    jx += cxq(i) * v; jy += cyq(i) * v; jz += czq(i) * v;
+    sxx += cxq(i) * cxq(i) * v; syy += cyq(i) * cyq(i) * v; szz += czq(i) * czq(i) * v;
+    sxy += cxq(i) * cyq(i) * v; sxz += cxq(i) * czq(i) * v; syz += cyq(i) * czq(i) * v;
+    const double f = v + wq(i);
+    mn = f < mn ? f : mn;
+  }
+  const double rho = d + 1.0;
+  if (!(rho > 0)) err |= TE_DENSITY;
+  if (mn < 0) err |= TE_NEGATIVE;
+  if (!isfinite(rho) || !isfinite(jx + jy + jz)) err |= TE_NONFINITE;
+  if (sqrt(jx * jx + jy * jy + jz * jz) / rho / 0.57735026918962576451 > 0.3) err |= TE_MACH;
+  m[0] = rho; m[1] = jx; m[2] = jy; m[3] = jz;
+  m[4] = sxx - d / 3.0 - jx * jx / rho;
+  m[5] = syy - d / 3.0 - jy * jy / rho;
+  m[6] = szz - d / 3.0 - jz * jz / rho;
+  m[7] = sxy - jx * jy / rho;
+  m[8] = sxz - jx * jz / rho;
+  m[9] = syz - jy * jz / rho;
+}
+
+// tensor-product equilibrium + second-order Hermite stress, shifted by the weights
+__device__ __forceinline__ void populations_of(const double (&m)[10], double (&g)[NQ], int &err) {
+  const double rho = m[0];
+  if (!(rho > 0)) err |= TE_DENSITY;
+  const double u[3] = {m[1] / rho, m[2] / rho, m[3] / rho};
+  if (sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / 0.57735026918962576451 > 0.3) err |= TE_MACH;
+  double ph[3][3];
+  for (int a = 0; a < 3; ++a) {
+    ph[a][1] = 2.0 / 3.0 - u[a] * u[a];
+    ph[a][0] = (1.0 / 3.0 + u[a] * u[a] - u[a]) * 0.5;
+    ph[a][2] = (1.0 / 3.0 + u[a] * u[a] + u[a]) * 0.5;
+  }
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    const int cx = cxq(i), cy = cyq(i), cz = czq(i);
+    const double eq = rho * ph[0][cx + 1] * ph[1][cy + 1] * ph[2][cz + 1];
+    const double h = (cx * cx - 1.0 / 3.0) * m[4] + (cy * cy - 1.0 / 3.0) * m[5] + (cz * cz - 1.0 / 3.0) * m[6]
+                   + 2.0 * (cx * cy * m[7] + cx * cz * m[8] + cy * cz * m[9]);
+    const double p = eq + wq(i) * 4.5 * h;
+    if (!(p >= 0)) err |= (isfinite(p) ? TE_NEGATIVE : TE_NONFINITE);
+    g[i] = p - wq(i);
+  }
+}
+
+__device__ __forceinline__ void rescale(double (&m)[10], double shear, double bulk) {
+  const double iso = (m[4] + m[5] + m[6]) / 3.0;
+  for (int k = 4; k < 10; ++k) m[k] *= shear;
+  for (int k = 4; k < 7; ++k) m[k] += (bulk - shear) * iso;
+}
+
+template <int S, bool PE>
+__device__ __forceinline__ void gather_in(const typename St<S>::T *A, const Desc &L, int x, int y, int z, float inv,
+                                          double (&g)[NQ]) {
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) g[i] = (double)ld<S>(A + loc_in<PE>(L, x, y, z, i), inv);
+}
+template <int S, bool PE>
+__device__ __forceinline__ void scatter_in(typename St<S>::T *A, const Desc &L, int x, int y, int z, float scale,
+                                           const double (&g)[NQ]) {
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) stv<S>(A + loc_in<PE>(L, x, y, z, i), (typename St<S>::C)g[i], scale);
+}
+
+__device__ __forceinline__ bool in_core(const BoxGeom &B, int bx, int by) {
+  // covered cell (box coordinates) more than three rows inside the box in x and y
+  return bx >= 3 && bx < B.nb[0] - 3 && by >= 3 && by < B.nb[1] - 3;
+}
+
+struct FreeState {
+  double g[NQ];  // free-stream shifted populations
+  double m[10];  // their moments
+};
+
+// Pre-collision moments of the coarse donor block (box +- 3 cells; z wraps).
+template <int S, bool PE>
+__global__ void k_donor_moments(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F,
+                                float inv_scale, double *__restrict__ M, int *__restrict__ err) {
+  const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)dx * dy * dz) return;
+  const int i = (int)(t % dx), j = (int)((t / dx) % dy), k = (int)(t / ((long long)dx * dy));
+  const int x = B.lo[0] - 3 + i, y = B.lo[1] - 3 + j;
+  const int z = ((B.lo[2] - 3 + k) % L.nz + L.nz) % L.nz;
+  double m[10];
+  int e = 0;
+  const int bx = x - B.lo[0], by = y - B.lo[1];
+  if (bx >= 0 && bx < B.nb[0] && by >= 0 && by < B.nb[1] && in_core(B, bx, by)) {
+    for (int c = 0; c < 10; ++c) m[c] = F.m[c];
+  } else {
+    double g[NQ];
+    gather_in<S, PE>(A, L, x, y, z, inv_scale, g);
+    moments_of(g, m, e);
+  }
+  for (int c = 0; c < 10; ++c) M[t * 10 + c] = m[c];
+  if (e) atomicOr(err, e);
+}
+
+// Fine donor cells (the two pad layers in x and y) from the time-interpolated,
+// prolonged, rescaled coarse moments.
+template <int S, bool PE>
+__global__ void k_prolong_inject(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V,
+                                 const double *__restrict__ M0, const double *__restrict__ M1, double frac,
+                                 float scale, int *__restrict__ err) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L.nx * L.ny * L.nz) return;
+  const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
+  const int p = B.pad;
+  if (x >= p && x < L.nx - p && y >= p && y < L.ny - p) return;  // owned fine cell
+  const int dx = B.nb[0] + 6, dy = B.nb[1] + 6;
+  // parent (in the donor block, halo 2 removed) and child side on each axis
+  const int pz = z + p;  // padded z coordinate of the periodic fine span
+  const int P[3] = {x / 2 + 2, y / 2 + 2, pz / 2 + 2};
+  const int sd[3] = {x & 1, y & 1, pz & 1};
+  const double wlo[5] = {-3.0 / 128.0, 11.0 / 64.0, 1.0, -11.0 / 64.0, 3.0 / 128.0};
+  const double whi[5] = {3.0 / 128.0, -11.0 / 64.0, 1.0, 11.0 / 64.0, -3.0 / 128.0};
+  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c = -2; c <= 2; ++c) {
+    const double wc = sd[2] ? whi[c + 2] : wlo[c + 2];
+    for (int b = -2; b <= 2; ++b) {
+      const double wb = wc * (sd[1] ? whi[b + 2] : wlo[b + 2]);
+      for (int a = -2; a <= 2; ++a) {
+        const double w = wb * (sd[0] ? whi[a + 2] : wlo[a + 2]);
+        const long long idx = (((long long)(P[2] + c) * dy + (P[1] + b)) * dx + (P[0] + a)) * 10;
+        for (int k = 0; k < 10; ++k) m[k] += w * ((1.0 - frac) * M0[idx + k] + frac * M1[idx + k]);
+      }
+    }
+  }
+  rescale(m, V.shear_cf, V.bulk_cf);
+  double g[NQ];
+  int e = 0;
+  populations_of(m, g, e);
+  scatter_in<S, PE>(A, L, x, y, z, scale, g);
+  if (e) atomicOr(err, e);
+}
+
+// Face flux registers.  Face f = 2*axis + (side > 0), axis in {0 (x faces), 1 (y faces)}.
+// Register layout reg[f][t][z][4], t along the in-plane tangential axis (box range).
+struct FluxTerm {
+  int8_t d, s0, s1;  // direction, tangential shifts (in-plane axis, z)
+  double coef[4];    // weight * normal * [1, c]
+};
+struct FluxTable {  // host-built, copied once into a device buffer
+  FluxTerm term[2][52];
+  int n[2];
+};
+
+template <int S, bool PE>
+__device__ __forceinline__ void flux_at(const typename St<S>::T *A, const Desc &L, const FluxTable *__restrict__ T, int axis,
+                                        int face_cell, int tpos, int zpos, float inv, double (&out)[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  for (int k = 0; k < T->n[axis]; ++k) {
+    const FluxTerm &tm = T->term[axis][k];
+    const int nrm = axis == 0 ? cxq(tm.d) : cyq(tm.d);
+    const int a = face_cell + (nrm < 0 ? 1 : 0);
+    const int tt = tpos - tm.s0;
+    const int zz = ((zpos - tm.s1) % L.nz + L.nz) % L.nz;
+    const int x = axis == 0 ? a : tt, y = axis == 0 ? tt : a;
+    const double v = (double)ld<S>(A + loc_out<PE>(L, x, y, zz, tm.d), inv);
+    for (int c = 0; c < 4; ++c) out[c] += v * tm.coef[c];
+  }
+}
+
+template <int S, bool PE>
+__global__ void k_flux_coarse(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T,
+                              double vol, float inv, double *__restrict__ reg) {
+  const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4LL * nt * L.nz) return;
+  const int zpos = (int)(t % L.nz), ti = (int)((t / L.nz) % nt), f = (int)(t / ((long long)L.nz * nt));
+  const int axis = f / 2, side = f % 2;
+  const int other = 1 - axis;
+  if (ti >= B.nb[other]) return;
+  const int face_cell = side ? B.hi[axis] - 1 : B.lo[axis] - 1;
+  double o[4];
+  flux_at<S, PE>(A, L, T, axis, face_cell, B.lo[other] + ti, zpos, inv, o);
+  double *r = reg + (((long long)f * nt + ti) * L.nz + zpos) * 4;
+  for (int c = 0; c < 4; ++c) r[c] = o[c] * vol;
+}
+
+// Fine flux summed over the 2x2 children of each coarse face element, accumulated over substeps.
+template <int S, bool PE>
+__global__ void k_flux_fine(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T,
+                            double vol, float inv, int nz_coarse, int accumulate, double *__restrict__ reg) {
+  const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4LL * nt * nz_coarse) return;
+  const int zc = (int)(t % nz_coarse), ti = (int)((t / nz_coarse) % nt), f = (int)(t / ((long long)nz_coarse * nt));
+  const int axis = f / 2, side = f % 2;
+  const int other = 1 - axis;
+  if (ti >= B.nb[other]) return;
+  const int n_axis = 2 * B.nb[axis];
+  const int face_cell = side ? B.pad + n_axis - 1 : B.pad - 1;
+  double s[4] = {0, 0, 0, 0};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      double o[4];
+      flux_at<S, PE>(A, L, T, axis, face_cell, B.pad + 2 * ti + a, 2 * zc + b, inv, o);
+      for (int c = 0; c < 4; ++c) s[c] += o[c];
+    }
+  double *r = reg + (((long long)f * nt + ti) * nz_coarse + zc) * 4;
+  for (int c = 0; c < 4; ++c) r[c] = (accumulate ? r[c] : 0.0) + s[c] * vol;
+}
+
+// Equilibrium reflux on the coarse cells just outside each face.
+template <int S, bool PE>
+__global__ void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const double *__restrict__ rc,
+                         const double *__restrict__ rf, double inv_vol, float scale, float inv,
+                         int *__restrict__ err) {
+  const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4LL * nt * L.nz) return;
+  const int zpos = (int)(t % L.nz), ti = (int)((t / L.nz) % nt), f = (int)(t / ((long long)L.nz * nt));
+  const int axis = f / 2, side = f % 2;
+  const int other = 1 - axis;
+  if (ti >= B.nb[other]) return;
+  const int a = side ? B.hi[axis] : B.lo[axis] - 1;
+  const int x = axis == 0 ? a : B.lo[other] + ti, y = axis == 0 ? B.lo[other] + ti : a;
+  const long long ri = (((long long)f * nt + ti) * L.nz + zpos) * 4;
+  const double sgn = side ? 1.0 : -1.0;
+  double g[NQ], m[10], mo[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, mn[10], po[NQ], pn[NQ];
+  int e = 0;
+  gather_in<S, PE>(A, L, x, y, zpos, inv, g);
+  moments_of(g, m, e);
+  for (int c = 0; c < 4; ++c) mo[c] = m[c];
+  for (int c = 0; c < 10; ++c) mn[c] = mo[c];
+  for (int c = 0; c < 4; ++c) mn[c] += sgn * (rf[ri + c] - rc[ri + c]) * inv_vol;
+  populations_of(mn, pn, e);
+  populations_of(mo, po, e);
+  for (int i = 0; i < NQ; ++i) g[i] += pn[i] - po[i];
+  moments_of(g, m, e);
+  scatter_in<S, PE>(A, L, x, y, zpos, scale, g);
+  if (e) atomicOr(err, e);
+}
+
+// Covered coarse cells from the fine state: free stream in the virtual core, otherwise
+// the 2x2x2 mean of the children's moments, rescaled fine->coarse and regularised.
+template <int S, bool PEC, bool PEF>
+__global__ void k_restrict(typename St<S>::T *__restrict__ Ac, Desc Lc, const typename St<S>::T *__restrict__ Af,
+                           Desc Lf, BoxGeom B, Levels V, FreeState F, float scale, float inv,
+                           int *__restrict__ err) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)B.nb[0] * B.nb[1] * B.nb[2]) return;
+  const int bx = (int)(t % B.nb[0]), by = (int)((t / B.nb[0]) % B.nb[1]), bz = (int)(t / ((long long)B.nb[0] * B.nb[1]));
+  const int x = B.lo[0] + bx, y = B.lo[1] + by, z = B.lo[2] + bz;
+  double g[NQ];
+  int e = 0;
+  if (in_core(B, bx, by)) {
+    for (int i = 0; i < NQ; ++i) g[i] = F.g[i];
+  } else {
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 2; ++c)
+      for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 2; ++a) {
+          double gf[NQ], m[10];
+          gather_in<S, PEF>(Af, Lf, B.pad + 2 * bx + a, B.pad + 2 * by + b, 2 * bz + c, inv, gf);
+          moments_of(gf, m, e);
+          for (int k = 0; k < 10; ++k) acc[k] += m[k];
+        }
+    for (int k = 0; k < 10; ++k) acc[k] *= 0.125;
+    rescale(acc, V.shear_fc, V.bulk_fc);
+    populations_of(acc, g, e);
+  }
+  scatter_in<S, PEC>(Ac, Lc, x, y, z, scale, g);
+  if (e) atomicOr(err, e);
+}
+
+// Initial pre-collision state of one level: rho = 1, seeded momentum, regularised
+// product-form populations (no stress), written into the gather slots of a
+// lattice whose next sweep is even (streamed form).
+struct LevelSeed {
+  double U, amp, phy, phz, ky, kz, lex, ley;  // seeded field (common frame)
+  double ox, oy, oz, dx;                      // cell (i,j,k) centre = ((i+0.5) dx + o)
+  double px, py;                              // plan origin added to x, y
+};
+
+template <int S>
+__global__ void k_init_level(typename St<S>::T *__restrict__ A, const uint8_t *__restrict__ mask, Desc L, LevelSeed p,
+                             float scale, int *__restrict__ err) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)L.nx * L.ny * L.nz) return;
+  const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
+  const double xx = ((double)x + 0.5) * p.dx + p.ox + p.px, yy = ((double)y + 0.5) * p.dx + p.oy + p.py;
+  const double zz = ((double)z + 0.5) * p.dx + p.oz;
+  const double rx = xx - p.lex, ry = yy - p.ley;
+  const double e = (rx - 0.5) / 0.75;
+  const double env = exp(-e * e);
+  double m[10] = {1.0, p.U, 0, 0, 0, 0, 0, 0, 0, 0};
+  m[2] = p.amp * sin(p.ky * ry + p.phy) * cos(p.kz * zz + p.phz) * env;
+  m[3] = -p.amp * p.ky / p.kz * cos(p.ky * ry + p.phy) * sin(p.kz * zz + p.phz) * env;
+  if (mask[(long long)(y + 1) * (L.nx + 2) + (x + 1)] == 1) m[1] = m[2] = m[3] = 0.0;
+  double g[NQ];
+  int er = 0;
+  populations_of(m, g, er);
+  scatter_in<S, false>(A, L, x, y, z, scale, g);
+  if (er) atomicOr(err, er);
+}
+
+}  // namespace ak

@@ -1,0 +1,402 @@
+// ak_refined_capi.hpp -- C ABI of the two-level 2:1 refined step (included by ak_capi.hip).
+//
+// One refined case = a coarse lattice (legacy farfield, z periodic, covered cells kept
+// as donors) + a fine lattice over the box plus two donor layers in x and y (z
+// periodic, Bouzidi walls).  One coarse step:
+//   donor moments (before) -> coarse sweep -> coarse face flux -> coarse boundary
+//   -> donor moments (after) -> 2 x [prolong+inject, fine sweep, fine face flux,
+//   fine walls + loads] -> reflux -> restriction.
+#include "ak_refined.hpp"
+
+namespace {
+
+struct Refined {
+  Lattice *c = nullptr, *f = nullptr;
+  BoxGeom B{};
+  Levels V{};
+  FreeState F{};
+  double dxc = 0, dtc = 0, dxf = 0, dtf = 0, span = 0, U = 0;
+  double *d_before = nullptr, *d_after = nullptr, *d_regc = nullptr, *d_regf = nullptr, *d_part = nullptr;
+  int *d_err = nullptr;
+  FluxTable T{};
+  FluxTable *d_T = nullptr;
+  int nt = 0, wall_blocks = 0;
+  long long steps = 0;
+};
+
+Refined *R(void *h) { return (Refined *)h; }
+
+// host copies of the transfer algebra (same formulas as the device)
+void host_populations(const double m[10], double g[27]) {
+  const double rho = m[0];
+  const double u[3] = {m[1] / rho, m[2] / rho, m[3] / rho};
+  double ph[3][3];
+  for (int a = 0; a < 3; ++a) {
+    ph[a][1] = 2.0 / 3.0 - u[a] * u[a];
+    ph[a][0] = (1.0 / 3.0 + u[a] * u[a] - u[a]) * 0.5;
+    ph[a][2] = (1.0 / 3.0 + u[a] * u[a] + u[a]) * 0.5;
+  }
+  for (int i = 0; i < 27; ++i) {
+    const int cx = cxq(i), cy = cyq(i), cz = czq(i);
+    const double eq = rho * ph[0][cx + 1] * ph[1][cy + 1] * ph[2][cz + 1];
+    const double h = (cx * cx - 1.0 / 3.0) * m[4] + (cy * cy - 1.0 / 3.0) * m[5] + (cz * cz - 1.0 / 3.0) * m[6]
+                   + 2.0 * (cx * cy * m[7] + cx * cz * m[8] + cy * cz * m[9]);
+    g[i] = eq + wq(i) * 4.5 * h - wq(i);
+  }
+}
+void host_moments(const double g[27], double m[10]) {
+  double d = 0, j[3] = {0, 0, 0}, s[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 27; ++i) {
+    const int c[3] = {cxq(i), cyq(i), czq(i)};
+    d += g[i];
+    for (int a = 0; a < 3; ++a) j[a] += c[a] * g[i];
+    s[0] += c[0] * c[0] * g[i]; s[1] += c[1] * c[1] * g[i]; s[2] += c[2] * c[2] * g[i];
+    s[3] += c[0] * c[1] * g[i]; s[4] += c[0] * c[2] * g[i]; s[5] += c[1] * c[2] * g[i];
+  }
+  const double rho = d + 1.0;
+  m[0] = rho; m[1] = j[0]; m[2] = j[1]; m[3] = j[2];
+  m[4] = s[0] - d / 3.0 - j[0] * j[0] / rho;
+  m[5] = s[1] - d / 3.0 - j[1] * j[1] / rho;
+  m[6] = s[2] - d / 3.0 - j[2] * j[2] / rho;
+  m[7] = s[3] - j[0] * j[1] / rho;
+  m[8] = s[4] - j[0] * j[2] / rho;
+  m[9] = s[5] - j[1] * j[2] / rho;
+}
+
+void build_flux_table(FluxTable &T) {
+  for (int axis = 0; axis < 2; ++axis) {
+    const int other = 1 - axis;
+    int n = 0;
+    for (int d = 0; d < 27; ++d) {
+      const int c[3] = {cxq(d), cyq(d), czq(d)};
+      const int nrm = c[axis];
+      if (!nrm) continue;
+      // tangential axes in ascending order: the in-plane one and z
+      int moving[2], m = 0;
+      if (c[other]) moving[m++] = other;
+      if (c[2]) moving[m++] = 2;
+      auto fact = [](int k) { int r = 1; for (int i = 2; i <= k; ++i) r *= i; return r; };
+      for (int mask = 0; mask < (1 << m); ++mask) {
+        const int cnt = __builtin_popcount(mask);
+        const double w = (double)fact(cnt) * fact(m - cnt) / fact(m + 1);
+        FluxTerm &tm = T.term[axis][n++];
+        tm.d = (int8_t)d;
+        tm.s0 = 0; tm.s1 = 0;
+        for (int k = 0; k < m; ++k)
+          if (mask & (1 << k)) {
+            if (moving[k] == other) tm.s0 = (int8_t)c[other];
+            else tm.s1 = (int8_t)c[2];
+          }
+        tm.coef[0] = w * nrm;
+        for (int a = 0; a < 3; ++a) tm.coef[1 + a] = w * nrm * c[a];
+      }
+    }
+    T.n[axis] = n;
+  }
+}
+
+// ev (optional, 8 events): step start/end, coarse sweep start/end, fine sweep 0 and 1 start/end
+template <int S>
+void refined_coarse_step(Refined *r, bool sample, double number, double *row, hipEvent_t *ev = nullptr) {
+  Lattice *c = r->c, *f = r->f;
+  if (ev) (void)hipEventRecord(ev[0], c->stream);
+  using T_ = typename St<S>::T;
+  const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
+  const unsigned flux_blocks = blocks_for(4LL * r->nt * c->nz, 128);
+  // donor moments at time t (pre-collision coarse state)
+  if (c->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_before, r->d_err);
+  else
+    hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_before, r->d_err);
+  // coarse collide (AA sweep gathers the pre-collision state)
+  if (ev) (void)hipEventRecord(ev[2], c->stream);
+  launch_sweep<S>(c, c->sweeps % 2 == 0, false);
+  if (ev) (void)hipEventRecord(ev[3], c->stream);
+  c->sweeps++;
+  const bool cpe = c->sweeps % 2 == 0;
+  const double volc = r->dxc * r->dxc * r->dxc, volf = r->dxf * r->dxf * r->dxf;
+  if (cpe) {
+    hipLaunchKernelGGL((k_flux_coarse<S, true>), dim3(flux_blocks), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
+    launch_boundary<S, true>(c, false);
+    hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
+  } else {
+    hipLaunchKernelGGL((k_flux_coarse<S, false>), dim3(flux_blocks), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
+    launch_boundary<S, false>(c, false);
+    hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
+  }
+  const long long nf = (long long)f->nx * f->ny * f->nz;
+  const unsigned fflux_blocks = blocks_for(4LL * r->nt * c->nz, 128);
+  for (int sub = 0; sub < 2; ++sub) {
+    const double frac = sub * 0.5;
+    if (f->sweeps % 2 == 0)
+      hipLaunchKernelGGL((k_prolong_inject<S, true>), dim3(blocks_for(nf, 128)), dim3(128), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_before, r->d_after, frac, f->scale, r->d_err);
+    else
+      hipLaunchKernelGGL((k_prolong_inject<S, false>), dim3(blocks_for(nf, 128)), dim3(128), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_before, r->d_after, frac, f->scale, r->d_err);
+    if (ev) (void)hipEventRecord(ev[4 + 2 * sub], f->stream);
+    launch_sweep<S>(f, f->sweeps % 2 == 0, false);
+    if (ev) (void)hipEventRecord(ev[5 + 2 * sub], f->stream);
+    f->sweeps++;
+    WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
+    double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;
+    if (f->sweeps % 2 == 0) {
+      hipLaunchKernelGGL((k_flux_fine<S, true>), dim3(fflux_blocks), dim3(128), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, c->nz, sub, r->d_regf);
+      if (f->nlinks) hipLaunchKernelGGL((k_walls<S, true, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
+    } else {
+      hipLaunchKernelGGL((k_flux_fine<S, false>), dim3(fflux_blocks), dim3(128), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, c->nz, sub, r->d_regf);
+      if (f->nlinks) hipLaunchKernelGGL((k_walls<S, false, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
+    }
+  }
+  const double inv_volc = 1.0 / volc;
+  if (c->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_reflux<S, true>), dim3(flux_blocks), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
+  else
+    hipLaunchKernelGGL((k_reflux<S, false>), dim3(flux_blocks), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
+  const long long nbox = (long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2];
+  const bool pc = c->sweeps % 2 == 0, pf = f->sweeps % 2 == 0;
+#define AK_RESTRICT(A_, B_) \
+  hipLaunchKernelGGL((k_restrict<S, A_, B_>), dim3(blocks_for(nbox, 128)), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, (const T_ *)f->pop, f->L, r->B, r->V, r->F, c->scale, f->inv_scale, r->d_err)
+  if (pc) { if (pf) AK_RESTRICT(true, true); else AK_RESTRICT(true, false); }
+  else { if (pf) AK_RESTRICT(false, true); else AK_RESTRICT(false, false); }
+#undef AK_RESTRICT
+  if (sample) {
+    // impulse over the two substeps: F dx_f^3 (dx_f/dt_f) / dt_c; q_area = span / 2
+    const double fscale = volf * (r->dxf / r->dtf) / r->dtc / (0.5 * r->span);
+    hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, c->stream, r->d_part, f->nlinks ? 2 * r->wall_blocks : 0,
+                       f->nelem, fscale, fscale, volf, number, row, c->d_diag);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, c->stream, c->d_diag, row);
+  }
+  if (ev) (void)hipEventRecord(ev[1], c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const int32_t *box_lo,
+                      const int32_t *box_hi, int storage, const double *level, double lattice_speed,
+                      double fp16_scale, int history_capacity, void *stream) {
+  if (!handle) return fail(nullptr, "null handle pointer");
+  *handle = nullptr;
+  // level = {dx_c, dt_c, tau_shear_c, tau_bulk_c}
+  const double dxc = level[0], dtc = level[1], tsc = level[2], tbc = level[3];
+  if (!(dxc > 0 && dtc > 0 && tsc > 0.5 && tbc > 0.5)) return fail(nullptr, "invalid coarse level");
+  for (int a = 0; a < 2; ++a)
+    if (box_lo[a] < 3 || box_hi[a] > (a ? ny : nx) - 3 || box_hi[a] - box_lo[a] < 8)
+      return fail(nullptr, "fine box must keep three donor cells inside the domain and span >= 8 cells");
+  if (box_lo[2] != 0 || box_hi[2] != nz) return fail(nullptr, "fine box must span the periodic z axis");
+  Refined *r = new Refined();
+  const double tsf = 0.5 + 2 * (tsc - 0.5), tbf = 0.5 + 2 * (tbc - 0.5);
+  r->dxc = dxc; r->dtc = dtc; r->dxf = dxc / 2; r->dtf = dtc / 2; r->U = lattice_speed;
+  r->span = nz * dxc;
+  r->V.shear_cf = tsf * r->dtf / (tsc * dtc);
+  r->V.bulk_cf = tbf * r->dtf / (tbc * dtc);
+  r->V.shear_fc = tsc * dtc / (tsf * r->dtf);
+  r->V.bulk_fc = tbc * dtc / (tbf * r->dtf);
+  for (int a = 0; a < 3; ++a) { r->B.lo[a] = box_lo[a]; r->B.hi[a] = box_hi[a]; r->B.nb[a] = box_hi[a] - box_lo[a]; }
+  r->B.pad = 2;
+  double ff[10] = {1.0, lattice_speed, 0, 0, 0, 0, 0, 0, 0, 0};
+  host_populations(ff, r->F.g);
+  host_moments(r->F.g, r->F.m);
+  build_flux_table(r->T);
+  r->nt = std::max(r->B.nb[0], r->B.nb[1]);
+  void *hc = nullptr, *hf = nullptr;
+  if (ak_lattice_create(&hc, device, nx, ny, nz, storage, 1.0 / tsc, 1.0 / tbc, fp16_scale, history_capacity, stream)) {
+    delete r;
+    return -1;
+  }
+  if (ak_lattice_create(&hf, device, 2 * r->B.nb[0] + 4, 2 * r->B.nb[1] + 4, 2 * nz, storage, 1.0 / tsf, 1.0 / tbf,
+                        fp16_scale, 16, stream)) {
+    ak_lattice_destroy(hc);
+    delete r;
+    return -1;
+  }
+  r->c = (Lattice *)hc;
+  r->f = (Lattice *)hf;
+  r->f->ff_kind = AK_FARFIELD_NONE;
+  const size_t ndon = (size_t)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6) * 10;
+  const size_t nreg = (size_t)4 * r->nt * nz * 4;
+  if (dalloc(r->c, &r->d_before, ndon) || dalloc(r->c, &r->d_after, ndon) || dalloc(r->c, &r->d_regc, nreg) ||
+      dalloc(r->c, &r->d_regf, nreg) || dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
+    ak_lattice_destroy(hf); ak_lattice_destroy(hc);
+    delete r;
+    return -1;
+  }
+  (void)hipMemset(r->d_err, 0, sizeof(int));
+  if (hipMemcpy(r->d_T, &r->T, sizeof(FluxTable), hipMemcpyHostToDevice) != hipSuccess) {
+    ak_lattice_destroy(hf); ak_lattice_destroy(hc);
+    delete r;
+    return fail(nullptr, "flux table upload failed");
+  }
+  *handle = r;
+  return 0;
+}
+
+int ak_refined_destroy(void *handle) {
+  Refined *r = R(handle);
+  if (!r) return 0;
+  ak_lattice_destroy(r->f);  // partials are owned by the fine lattice
+  ak_lattice_destroy(r->c);  // transfer buffers are owned by the coarse lattice
+  delete r;
+  return 0;
+}
+
+void *ak_refined_level(void *handle, int level) {
+  Refined *r = R(handle);
+  return r ? (level ? (void *)r->f : (void *)r->c) : nullptr;
+}
+
+// Fine-level walls (padded fine coordinates); lever arm in common units about
+// the common origin: ((index + 0.5) dx_f + origin_f).
+int ak_refined_set_walls(void *handle, int n, const int32_t *x, const int32_t *y, const int32_t *d, const double *q,
+                         const int32_t *element, int nelem, double origin_fx, double origin_fy) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  if (ak_set_walls(r->f, n, x, y, d, q, element, nelem, origin_fx / r->dxf, origin_fy / r->dxf, 1.0, r->U)) return -1;
+  r->wall_blocks = r->f->wall_blocks;
+  if (r->d_part) { dfree(r->f, r->d_part); r->d_part = nullptr; }
+  if (dalloc(r->f, &r->d_part, (size_t)std::max(r->wall_blocks, 1) * nelem * 5 * 2)) return -1;
+  return 0;
+}
+
+// Initial state: seeded field on both levels (p = {U, amp, phy, phz, ky, kz, lex, ley,
+// plan_ox, plan_oy}), restriction of the fine state into the covered coarse cells,
+// legacy farfield on the coarse level.
+int ak_refined_init_seeded(void *handle, const double *p) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  Lattice *c = r->c, *f = r->f;
+  if (check_ready(c) || check_ready(f)) return -1;
+  for (Lattice *s : {c, f}) {
+    hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s->stream, (uint8_t *)s->pop, (long long)s->pop_bytes, (uint8_t)0);
+    s->sweeps = 1;  // pre-collision state stored in streamed form: the next sweep is even
+    s->hist_rows = 0;
+    s->host_hist.clear();
+  }
+  LevelSeed sc{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], 0.0, 0.0, 0.0, r->dxc, p[8], p[9]};
+  LevelSeed sf = sc;
+  sf.dx = r->dxf;
+  sf.ox = r->B.lo[0] * r->dxc - r->B.pad * r->dxf;
+  sf.oy = r->B.lo[1] * r->dxc - r->B.pad * r->dxf;
+  (void)hipMemset(r->d_err, 0, sizeof(int));
+  const long long ncv = (long long)c->nx * c->ny * c->nz, nfv = (long long)f->nx * f->ny * f->nz;
+  switch (c->storage) {
+    case 0:
+      hipLaunchKernelGGL((k_init_level<0>), dim3(blocks_for(ncv, 256)), dim3(256), 0, c->stream, (double *)c->pop, c->d_solid, c->L, sc, c->scale, r->d_err);
+      hipLaunchKernelGGL((k_init_level<0>), dim3(blocks_for(nfv, 256)), dim3(256), 0, f->stream, (double *)f->pop, f->d_solid, f->L, sf, f->scale, r->d_err);
+      hipLaunchKernelGGL((k_restrict<0, false, false>), dim3(blocks_for((long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2], 128)), dim3(128), 0, c->stream, (double *)c->pop, c->L, (const double *)f->pop, f->L, r->B, r->V, r->F, c->scale, f->inv_scale, r->d_err);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_init_level<1>), dim3(blocks_for(ncv, 256)), dim3(256), 0, c->stream, (float *)c->pop, c->d_solid, c->L, sc, c->scale, r->d_err);
+      hipLaunchKernelGGL((k_init_level<1>), dim3(blocks_for(nfv, 256)), dim3(256), 0, f->stream, (float *)f->pop, f->d_solid, f->L, sf, f->scale, r->d_err);
+      hipLaunchKernelGGL((k_restrict<1, false, false>), dim3(blocks_for((long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2], 128)), dim3(128), 0, c->stream, (float *)c->pop, c->L, (const float *)f->pop, f->L, r->B, r->V, r->F, c->scale, f->inv_scale, r->d_err);
+      break;
+    default:
+      hipLaunchKernelGGL((k_init_level<2>), dim3(blocks_for(ncv, 256)), dim3(256), 0, c->stream, (_Float16 *)c->pop, c->d_solid, c->L, sc, c->scale, r->d_err);
+      hipLaunchKernelGGL((k_init_level<2>), dim3(blocks_for(nfv, 256)), dim3(256), 0, f->stream, (_Float16 *)f->pop, f->d_solid, f->L, sf, f->scale, r->d_err);
+      hipLaunchKernelGGL((k_restrict<2, false, false>), dim3(blocks_for((long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2], 128)), dim3(128), 0, c->stream, (_Float16 *)c->pop, c->L, (const _Float16 *)f->pop, f->L, r->B, r->V, r->F, c->scale, f->inv_scale, r->d_err);
+      break;
+  }
+  HIPCHK(c, hipGetLastError());
+  if (ak_set_farfield(c, AK_FARFIELD_LEGACY, r->U)) return -1;
+  r->steps = 0;
+  return 0;
+}
+
+int ak_refined_run(void *handle, int nsteps, long long first_step, int sample_every) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  Lattice *c = r->c;
+  if (check_ready(c) || check_ready(r->f)) return -1;
+  for (int k = 0; k < nsteps; ++k) {
+    const long long number = first_step + k + 1;
+    const bool sample = sample_every > 0 && number % sample_every == 0;
+    if (sample && c->hist_rows >= c->hist_cap && flush_history(c)) return -1;
+    double *row = c->d_hist + (size_t)c->hist_rows * c->hist_width;
+    switch (c->storage) {
+      case 0: refined_coarse_step<0>(r, sample, (double)number, row); break;
+      case 1: refined_coarse_step<1>(r, sample, (double)number, row); break;
+      default: refined_coarse_step<2>(r, sample, (double)number, row); break;
+    }
+    if (sample) c->hist_rows++;
+    r->steps++;
+  }
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// ak_refined_run with HIP events (measurement): out = {total ms, coarse sweep ms, fine sweep ms,
+// coarse steps}; synchronises at the end.
+int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int sample_every, double *out) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  Lattice *c = r->c;
+  if (check_ready(c) || check_ready(r->f)) return -1;
+  std::vector<hipEvent_t> ev(8 * (size_t)nsteps);
+  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  for (int k = 0; k < nsteps; ++k) {
+    const long long number = first_step + k + 1;
+    const bool sample = sample_every > 0 && number % sample_every == 0;
+    if (sample && c->hist_rows >= c->hist_cap && flush_history(c)) return -1;
+    double *row = c->d_hist + (size_t)c->hist_rows * c->hist_width;
+    switch (c->storage) {
+      case 0: refined_coarse_step<0>(r, sample, (double)number, row, &ev[8 * k]); break;
+      case 1: refined_coarse_step<1>(r, sample, (double)number, row, &ev[8 * k]); break;
+      default: refined_coarse_step<2>(r, sample, (double)number, row, &ev[8 * k]); break;
+    }
+    if (sample) c->hist_rows++;
+    r->steps++;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double tot = 0, cs = 0, fs = 0;
+  for (int k = 0; k < nsteps; ++k) {
+    float a = 0, b = 0, d0 = 0, d1 = 0;
+    hipEvent_t *e = &ev[8 * k];
+    HIPCHK(c, hipEventElapsedTime(&a, e[0], e[1]));
+    HIPCHK(c, hipEventElapsedTime(&b, e[2], e[3]));
+    HIPCHK(c, hipEventElapsedTime(&d0, e[4], e[5]));
+    HIPCHK(c, hipEventElapsedTime(&d1, e[6], e[7]));
+    tot += a; cs += b; fs += d0 + d1;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  out[0] = tot; out[1] = cs; out[2] = fs; out[3] = nsteps;
+  return 0;
+}
+
+// Transfer error flags raised so far (bit 1 density, 2 negative population, 4 Mach, 8 non-finite); clears them.
+int ak_refined_transfer_errors(void *handle) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  int e = 0;
+  HIPCHK(r->c, hipStreamSynchronize(r->c->stream));
+  HIPCHK(r->c, hipMemcpy(&e, r->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  (void)hipMemset(r->d_err, 0, sizeof(int));
+  return e;
+}
+
+// Pre-collision populations of one level (0 coarse, 1 fine padded), exchange layout.
+int ak_refined_get_populations(void *handle, int level, double *host_out) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  Lattice *s = level ? r->f : r->c;
+  if (check_ready(s)) return -1;
+  const size_t n = (size_t)s->nx * s->ny * s->nz;
+  double *d = nullptr;
+  HIPCHK(s, hipMalloc((void **)&d, n * 27 * sizeof(double)));
+  const unsigned nb = blocks_for((long long)n, 256);
+  const bool pe = s->sweeps % 2 == 0;
+  switch (s->storage) {
+    case 0: if (pe) hipLaunchKernelGGL((k_export_in<0, true>), dim3(nb), dim3(256), 0, s->stream, (const double *)s->pop, s->L, d, s->inv_scale);
+            else hipLaunchKernelGGL((k_export_in<0, false>), dim3(nb), dim3(256), 0, s->stream, (const double *)s->pop, s->L, d, s->inv_scale); break;
+    case 1: if (pe) hipLaunchKernelGGL((k_export_in<1, true>), dim3(nb), dim3(256), 0, s->stream, (const float *)s->pop, s->L, d, s->inv_scale);
+            else hipLaunchKernelGGL((k_export_in<1, false>), dim3(nb), dim3(256), 0, s->stream, (const float *)s->pop, s->L, d, s->inv_scale); break;
+    default: if (pe) hipLaunchKernelGGL((k_export_in<2, true>), dim3(nb), dim3(256), 0, s->stream, (const _Float16 *)s->pop, s->L, d, s->inv_scale);
+             else hipLaunchKernelGGL((k_export_in<2, false>), dim3(nb), dim3(256), 0, s->stream, (const _Float16 *)s->pop, s->L, d, s->inv_scale); break;
+  }
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpy(host_out, d, n * 27 * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(s, "refined export: %s", hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"

@@ -142,6 +142,24 @@ class RefinedPlan:
                    ("Static two-level grid; fine cells/chord = 2 * coarse cells/chord.",
                     "All walls on the fine level; legacy farfield on the coarse level."))
 
+    @classmethod
+    def explicit(cls, project: Project, refinement: RefinementSettings, base, lower, upper):
+        """Plan with an explicit fine box (half-open coarse bounds in x, y; full span in z)."""
+        box = FineBox((int(lower[0]), int(lower[1]), 0), (int(upper[0]), int(upper[1]), base.shape[2]))
+        pts = np.concatenate([e.xy for e in project.section.normalized(project.flow.alpha_deg)])
+        lo_pt, hi_pt = pts.min(axis=0), pts.max(axis=0)
+        org = np.asarray(base.origin)
+        dist = np.r_[(lo_pt - (org + np.array(box.lower[:2]) * base.dx)) / base.dx,
+                     ((org + np.array(box.upper[:2]) * base.dx) - hi_pt) / base.dx]
+        if dist.min() < refinement.guard_coarse_cells - 1e-10:
+            raise ValueError("Wall/interface guard too small; enlarge the refinement box")
+        coarse = AcousticLevel(base.dx, base.dt_convective, 1 / base.relaxation_rate, refinement.coarse_bulk_relaxation_time)
+        padded = tuple(2 * n + 4 for n in box.shape)
+        cells = math.prod(n + 2 for n in base.shape) + math.prod(n + 2 for n in padded)
+        memory = (cells * (8 * 27 * 8 + 4 * 10 * 8 + 4 * 8 + 128) + 256 * 2**20) / 2**30
+        return cls(base.shape, base.origin, box, coarse, coarse.refined(), base.span, base.dt_seconds, memory,
+                   float(dist.min()), ("Static two-level grid with an explicit fine box.",))
+
     @property
     def fine_shape(self):
         return tuple(2 * n for n in self.box.shape)

@@ -1,20 +1,29 @@
-"""Throughput of the D3Q27 cumulant step on one MI355X, against the HBM roofline.
+"""Throughput of the D3Q27 cumulant lattice-Boltzmann step on one MI355X.
 
-Workload (default): NACA0012, chord 128 cells at 8 deg, Re 1e5, Ma 0.1, uniform
-lattice 1536x1024x64 (100.7M cells), z periodic, characteristic farfield on x/y,
-Bouzidi walls, Cl/Cd/Cm and the low-Mach guards recorded on every step.  One
-step = one full lattice-Boltzmann time step of that case (boundary kernels +
-fused AA stream-collide sweep + load/guard reductions).  Populations are
-resident in HBM before timing starts.  The headline value is the FP16
-shifted-storage run (FP32 collision); the FP64 and FP32 storage runs of the same
-case are reported beside it, with the loads of each compared to FP64.
+Headline workload (the two-level configuration): NACA0012, chord 64 fine cells
+(32 coarse) at 6 deg, Re 3e4, Ma 0.1; coarse lattice 256x192x8 (legacy farfield,
+z periodic) with a fine patch of 160x96x16 fine cells (80x48x8 coarse) sub-cycled
+twice per coarse step; Bouzidi walls on the fine level; Cl/Cd/Cm recorded every
+coarse step.  One step = one coarse step: donor moments, coarse sweep, coarse face
+flux and farfield, two fine substeps (prolongation in time, fine sweep, fine face
+flux, walls + loads), reflux and restriction.  Populations are resident in HBM
+before timing.  MLUPS counts coarse cells + 2 x fine cells per coarse step (the
+reference's own two-level MLUPS convention).  `value` is the FP64-storage run; the
+FP32-storage run of the same case is reported beside it, with its loads compared
+to FP64.
 
-roofline: algorithmic bytes of one sweep launch = fluid cells x 2 x 27 x bytes per
-population (each population read once and written once, AA pattern), divided
-by the sweep kernel's mean duration from HIP events on the lattice's stream.
+roofline: the dominant kernel (largest total time inside the step) is the fine
+sweep; its algorithmic bytes per launch = updated fine cells x 2 x 27 x bytes per
+population (one read + one write per population, AA pattern), divided by its mean
+duration from HIP events recorded on the stream it is launched on.
 
-cpu_baseline: the FP64 C/OpenMP oracle (same algorithm, reference data flow) on
-the host, on the same workload cut to a 1536x1024x8 span slice.
+uniform_sweep: the same step on a uniform 1536x1024x64 lattice (100.7M cells,
+NACA0012 chord 128 at 8 deg, Re 1e5, characteristic farfield) in FP64, FP32 and
+FP16 shifted storage -- the HBM-roofline view of the fused AA sweep at full size.
+
+cpu_baseline: the FP64 oracle (C/OpenMP lattices + NumPy transfer, the
+reference's data flow) on the host, same two-level case, a bounded number of
+coarse steps.
 """
 from __future__ import annotations
 
@@ -33,14 +42,58 @@ BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
 METRIC = "MLUPS & %HBM roofline, FP64 vs FP16-store; cases/hour + p50/p99 latency"
 
 
-def gpu_run(case, storage, steps, warmup, profile_steps):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def roofline(bytes_per_launch, seconds_per_launch):
+    achieved = bytes_per_launch / seconds_per_launch / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": None, "algorithmic_bytes_per_launch": bytes_per_launch}
+
+
+def refined_gpu(case, storage, steps, warmup, profile_steps):
+    import numpy as np
+    import torch
+    from aerokinetic_amd.refined import RefinedSolver
+    project, settings, plan = case.build()
+    s = RefinedSolver(project, settings, plan=plan, storage=storage)
+    fine_owned = 8 * plan.box.shape[0] * plan.box.shape[1] * plan.box.shape[2]
+    coarse = int(np.prod(plan.shape))
+    updates = coarse + 2 * fine_owned
+    s.advance(warmup, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = s.advance(steps, 1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tot, cs, fs = s.run_profiled(profile_steps)
+    fnx, fny = s.fine_shape[0] + 4, s.fine_shape[1] + 4
+    fine_cells = (fnx * fny - int(s._fine_solid_xy.sum())) * s.fine_shape[2]
+    b = BYTES[storage]
+    fine_sweep = fs / (2 * profile_steps) / 1e3
+    coarse_sweep = cs / profile_steps / 1e3
+    out = {"storage": storage, "mlups": updates * steps / dt / 1e6, "ms_per_step": dt / steps * 1e3,
+           "updates_per_step": updates, "coarse_cells": coarse, "fine_owned_cells": fine_owned,
+           "event_ms_per_step": tot / profile_steps, "coarse_sweep_ms": coarse_sweep * 1e3,
+           "fine_sweep_ms": fine_sweep * 1e3,
+           "roofline": roofline(fine_cells * 2 * 27 * b, fine_sweep),
+           "coarse_sweep_roofline": roofline(coarse * 2 * 27 * b, coarse_sweep),
+           "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
+           "diag": s.diagnostics()}
+    s.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def uniform_gpu(case, storage, steps, warmup, profile_steps):
     import numpy as np
     import torch
     from aerokinetic_amd.solver import Solver
     section, flow, grid, plan = case.build()
     s = Solver(section, flow, grid, storage=storage, plan=plan)
     cells = int(np.prod(plan.shape))
-    fluid_cells = int(s._fluid.sum())
+    fluid = int(s._fluid.sum())
     s.advance(warmup, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -50,90 +103,97 @@ def gpu_run(case, storage, steps, warmup, profile_steps):
     sweep_ms, bnd_ms = s.lattice.run_profiled(profile_steps, s.steps, 1)
     s.steps += profile_steps
     s.lattice.read_history()
-    sweep_avg = sweep_ms / profile_steps / 1e3
-    traffic = fluid_cells * 2 * 27 * BYTES[storage]
-    achieved = traffic / sweep_avg / 1e9
+    sweep = sweep_ms / profile_steps / 1e3
     out = {"storage": storage, "mlups": cells * steps / dt / 1e6, "ms_per_step": dt / steps * 1e3,
-           "sweep_ms": sweep_avg * 1e3, "boundary_ms": bnd_ms / profile_steps,
-           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                        "algorithmic_bytes_per_launch": traffic},
-           "cells": cells, "fluid_cells": fluid_cells,
-           "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
-           "diag": s.diagnostics()}
+           "sweep_ms": sweep * 1e3, "boundary_ms": bnd_ms / profile_steps,
+           "roofline": roofline(fluid * 2 * 27 * BYTES[storage], sweep), "cells": cells,
+           "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs])}
     s.lattice.close()
     del s
     torch.cuda.empty_cache()
     return out
 
 
-def cpu_baseline(case, seconds):
-    """FP64 oracle on the host: same case, span cut to 8 cells, timed for ~`seconds`."""
+def compare_loads(results, ref="fp64"):
+    import numpy as np
+    out = {}
+    for st, r in results.items():
+        if st == ref or ref not in results:
+            continue
+        a, b = r["loads"], results[ref]["loads"]
+        out[st] = {"max_abs_dcoef_vs_fp64": dict(zip(("cl", "cd", "cm"), map(float, np.abs(a - b).max(axis=0)))),
+                   "mean_coef_vs_fp64": {k: [float(a[:, i].mean()), float(b[:, i].mean())]
+                                         for i, k in enumerate(("cl", "cd", "cm"))}}
+    return out
+
+
+def cpu_baseline_refined(case, seconds):
     from oracle import oracle as O
-    sub = case.with_shape((case.shape[0], case.shape[1], 8))
-    section, flow, grid, plan = sub.build()
-    o = O.from_case(section, flow, grid, plan)
-    cells = plan.shape[0] * plan.shape[1] * plan.shape[2]
-    for _ in range(2):
-        o.step(sample=True)
+    from oracle.refined_oracle import RefinedOracle
+    project, settings, plan = case.build()
+    ro = RefinedOracle(project, settings, plan)
+    updates = plan.shape[0] * plan.shape[1] * plan.shape[2] + 2 * 8 * plan.box.shape[0] * plan.box.shape[1] * plan.box.shape[2]
+    ro.step()
     n, t0 = 0, time.perf_counter()
     while True:
-        o.step(sample=True)
+        ro.step()
         n += 1
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": cells * n / dt / 1e6, "unit": "MLUPS", "cores": O.Oracle.threads(), "kind": "port",
-            "sample": f"FP64 C/OpenMP oracle, {plan.shape[0]}x{plan.shape[1]}x{plan.shape[2]} span slice of the "
-                      f"same case, 2 warm-up + {n} timed steps with loads every step ({dt:.1f} s)"}
+    return {"value": updates * n / dt / 1e6, "unit": "MLUPS", "cores": O.Oracle.threads(), "kind": "port",
+            "sample": f"FP64 oracle (C/OpenMP lattices, NumPy transfer) on the same two-level case, "
+                      f"1 warm-up + {n} timed coarse steps ({dt:.1f} s)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=("c128", "c32"), default="c128")
-    ap.add_argument("--storages", default="fp16,fp64,fp32")
-    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--storages", default="fp64,fp32")
+    ap.add_argument("--profile-steps", type=int, default=40)
+    ap.add_argument("--uniform-storages", default="fp16,fp64,fp32", help="empty to skip the 100M-cell sweep")
+    ap.add_argument("--uniform-steps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build runs a case on a single MI355X; "
+        print("bench.py measures one GPU: a case runs on a single MI355X here; "
               "multi-GPU runs are not supported (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     from aerokinetic_amd import cases
-    case = cases.NACA0012_C128_1536x1024x64 if args.workload == "c128" else cases.NACA0012_C32_384x256x8
-    storages = [s.strip() for s in args.storages.split(",") if s.strip()]
+    case = cases.NACA0012_REFINED_256x192x8
+    storages = [s for s in args.storages.split(",") if s]
     results = {}
     for st in storages:
-        print(f"[bench] {case.name} storage={st}", file=sys.stderr, flush=True)
-        results[st] = gpu_run(case, st, args.steps, args.warmup, args.profile_steps)
+        log(f"two-level {st}")
+        results[st] = refined_gpu(case, st, args.steps, args.warmup, args.profile_steps)
         r = results[st]
-        print(f"[bench]   {r['mlups']:.0f} MLUPS, sweep {r['sweep_ms']:.3f} ms, "
-              f"{r['roofline']['achieved']:.0f} GB/s ({100 * r['roofline']['frac']:.1f}% of HBM peak)",
-              file=sys.stderr, flush=True)
+        log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, fine sweep {r['fine_sweep_ms']:.4f} ms "
+            f"({100 * r['roofline']['frac']:.1f}% HBM), coarse sweep {r['coarse_sweep_ms']:.4f} ms")
+    uni = {}
+    ucase = cases.NACA0012_C128_1536x1024x64
+    for st in [s for s in args.uniform_storages.split(",") if s]:
+        log(f"uniform 100M {st}")
+        uni[st] = uniform_gpu(ucase, st, args.uniform_steps, 10, 20)
+        r = uni[st]
+        log(f"  {r['mlups']:.0f} MLUPS, sweep {r['sweep_ms']:.3f} ms ({100 * r['roofline']['frac']:.1f}% HBM)")
+    cpu = cpu_baseline_refined(case, args.cpu_seconds) if args.cpu_seconds > 0 else None
     head = results[storages[0]]
-    import numpy as np
-    extra = {}
-    for st, r in results.items():
-        e = {k: r[k] for k in ("mlups", "ms_per_step", "sweep_ms", "boundary_ms", "roofline")}
-        if "fp64" in results and st != "fp64":
-            a, b = r["loads"], results["fp64"]["loads"]
-            e["max_abs_dcoef_vs_fp64"] = dict(zip(("cl", "cd", "cm"), map(float, np.abs(a - b).max(axis=0))))
-            e["mean_coef_rel_err_vs_fp64"] = dict(zip(("cl", "cd", "cm"), map(
-                float, np.abs(a.mean(axis=0) - b.mean(axis=0)) / np.abs(b.mean(axis=0)))))
-        e["local_mach_max"] = r["diag"]["local_mach_max"]
-        extra[st] = e
-    cpu = cpu_baseline(case, args.cpu_seconds) if args.cpu_seconds > 0 else None
+    strip = lambda r: {k: v for k, v in r.items() if k not in ("loads", "diag")}
     line = {"metric": METRIC, "value": head["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None,
             "dtype": "f64" if storages[0] == "fp64" else "f32", "data": "synthetic",
-            "config": {"workload": case.name + ", characteristic farfield, Bouzidi walls, loads+guards every step",
-                       "storage": storages[0], "cells": head["cells"], "fluid_cells": head["fluid_cells"],
+            "config": {"workload": case.name + ", legacy coarse farfield, fine Bouzidi walls, loads every coarse step",
+                       "storage": storages[0], "updates_per_step": head["updates_per_step"],
                        "parallelism": "single GPU"},
-            "roofline": head["roofline"], "cpu_baseline": cpu, "by_storage": extra}
+            "roofline": head["roofline"], "cpu_baseline": cpu,
+            "by_storage": {st: strip(r) for st, r in results.items()},
+            "loads_vs_fp64": compare_loads(results),
+            "uniform_sweep": {"workload": ucase.name + ", characteristic farfield, loads every step",
+                              "by_storage": {st: strip(r) for st, r in uni.items()},
+                              "loads_vs_fp64": compare_loads(uni)}}
     print(json.dumps(line))
 
 

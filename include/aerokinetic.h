@@ -44,7 +44,8 @@ extern "C" {
 /* storage precision of the populations */
 enum { AK_STORE_F64 = 0, AK_STORE_F32 = 1, AK_STORE_F16 = 2 };
 /* outer boundary */
-enum { AK_FARFIELD_PERIODIC = 0, AK_FARFIELD_LEGACY = 1, AK_FARFIELD_CHARACTERISTIC = 2 };
+enum { AK_FARFIELD_PERIODIC = 0, AK_FARFIELD_LEGACY = 1, AK_FARFIELD_CHARACTERISTIC = 2,
+       AK_FARFIELD_NONE = 3 /* x/y ghosts left to the caller (refined fine level) */ };
 /* one history row per sampled step (doubles) */
 enum {
   AK_HIST_STEP = 0, AK_HIST_CL = 1, AK_HIST_CD = 2, AK_HIST_CM = 3,
@@ -67,7 +68,8 @@ int ak_lattice_destroy(void *handle);
 size_t ak_population_bytes(void *handle);
 int ak_bind_populations(void *handle, void *device_ptr, size_t bytes);
 
-/* Solid mask of the interior xy cells (host, nx*ny bytes, x-major: m[x*ny+y]). */
+/* Cell mask of the interior xy cells (host, nx*ny bytes, x-major: m[x*ny+y]):
+ * 0 fluid, 1 solid (not updated), 2 fluid excluded from the diagnostics. */
 int ak_set_solid(void *handle, const uint8_t *mask_xy);
 
 /* Fluid-to-solid links of the xy plane, extruded over z (host arrays of n):
