@@ -14,8 +14,15 @@ struct SweepGeom {
   int nblocks;
 };
 
-struct DiagAcc {  // order-independent maxima/minima as bit patterns of non-negative doubles
+// Order-independent maxima/minima as bit patterns of non-negative doubles.  Blocks
+// update one of DIAG_SLOTS slots (block index modulo the slot count) so that atomics
+// from hundreds of thousands of blocks never pile onto one address.
+constexpr int DIAG_SLOTS = 1024;
+struct DiagSlot {
   unsigned long long u2max, drmax, rhomin, nonfinite;
+};
+struct DiagAcc {
+  DiagSlot s[DIAG_SLOTS];
 };
 
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -94,11 +101,12 @@ __global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A
         vu = su[k] > vu ? su[k] : vu; vd = sd[k] > vd ? sd[k] : vd;
         vr = sr[k] < vr ? sr[k] : vr; bad |= sb[k];
       }
-      atomicMax(&diag->u2max, (unsigned long long)__double_as_longlong(vu));
-      atomicMax(&diag->drmax, (unsigned long long)__double_as_longlong(vd));
-      if (vr > 0.0) atomicMin(&diag->rhomin, (unsigned long long)__double_as_longlong(vr));
-      else atomicMax(&diag->nonfinite, 2ull);  // non-positive density
-      if (bad) atomicMax(&diag->nonfinite, 1ull);
+      DiagSlot *sl = &diag->s[blockIdx.x % DIAG_SLOTS];
+      atomicMax(&sl->u2max, (unsigned long long)__double_as_longlong(vu));
+      atomicMax(&sl->drmax, (unsigned long long)__double_as_longlong(vd));
+      if (vr > 0.0) atomicMin(&sl->rhomin, (unsigned long long)__double_as_longlong(vr));
+      else atomicMax(&sl->nonfinite, 2ull);  // non-positive density
+      if (bad) atomicMax(&sl->nonfinite, 1ull);
     }
   }
 }
@@ -215,19 +223,35 @@ __global__ void k_loads_final(const double *__restrict__ partials, int nblocks, 
     row[2] = tot[0] * fscale;
     row[3] = -tot[3] * mscale;
     row[8] = tot[4] * vscale;
-    diag->u2max = 0ull;
-    diag->drmax = 0ull;
-    diag->rhomin = (unsigned long long)__double_as_longlong(1e300);
-    diag->nonfinite = 0ull;
+  }
+  for (int k = lane; k < DIAG_SLOTS; k += 64) {
+    diag->s[k].u2max = 0ull;
+    diag->s[k].drmax = 0ull;
+    diag->s[k].rhomin = (unsigned long long)__double_as_longlong(1e300);
+    diag->s[k].nonfinite = 0ull;
   }
 }
 
 __global__ void k_diag_final(const DiagAcc *diag, double *__restrict__ row) {
+  // 64 threads fold the slots; then a fixed-order wave reduction
+  unsigned long long u = 0ull, d = 0ull, r = (unsigned long long)__double_as_longlong(1e300), b = 0ull;
+  for (int k = threadIdx.x; k < DIAG_SLOTS; k += 64) {
+    const DiagSlot &sl = diag->s[k];
+    u = sl.u2max > u ? sl.u2max : u;
+    d = sl.drmax > d ? sl.drmax : d;
+    r = sl.rhomin < r ? sl.rhomin : r;
+    b = sl.nonfinite > b ? sl.nonfinite : b;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned long long u2 = __shfl_xor(u, o, 64), d2 = __shfl_xor(d, o, 64);
+    const unsigned long long r2 = __shfl_xor(r, o, 64), b2 = __shfl_xor(b, o, 64);
+    u = u2 > u ? u2 : u; d = d2 > d ? d2 : d; r = r2 < r ? r2 : r; b = b2 > b ? b2 : b;
+  }
   if (threadIdx.x == 0) {
-    row[4] = sqrt(3.0) * sqrt(__longlong_as_double((long long)diag->u2max));
-    row[5] = __longlong_as_double((long long)diag->drmax);
-    row[6] = __longlong_as_double((long long)diag->rhomin);
-    row[7] = (double)diag->nonfinite;
+    row[4] = sqrt(3.0) * sqrt(__longlong_as_double((long long)u));
+    row[5] = __longlong_as_double((long long)d);
+    row[6] = __longlong_as_double((long long)r);
+    row[7] = (double)b;
   }
 }
 

@@ -134,34 +134,70 @@ __global__ void k_donor_moments(const typename St<S>::T *__restrict__ A, Desc L,
 }
 
 // Fine donor cells (the two pad layers in x and y) from the time-interpolated,
-// prolonged, rescaled coarse moments.
+// prolonged, rescaled coarse moments.  The 5-point conservative prolongation is
+// separable, so it runs as three one-axis passes over the donor block:
+//   x pass (with the time interpolation): (nbx+6, nby+6, nz+6) parents -> (2nbx+4, nby+6, nz+6)
+//   y pass: -> (2nbx+4, 2nby+4, nz+6), only columns that contain donor cells
+//   z pass + stress rescale + regularised rebuild, only donor cells, written into
+//   the gather slots of the fine lattice.
+// Child 2P of parent P takes a0 - off, child 2P+1 takes a0 + off, with
+// off = 11/64 (a1 - a_-1) - 3/128 (a2 - a_-2).
+__device__ __forceinline__ double prolong_weight(int side, int k) {
+  // k = -2..2
+  const double lo[5] = {-3.0 / 128.0, 11.0 / 64.0, 1.0, -11.0 / 64.0, 3.0 / 128.0};
+  return side ? -lo[k + 2] + (k == 0 ? 2.0 : 0.0) : lo[k + 2];
+}
+
+__global__ void k_prolong_x(const double *__restrict__ M0, const double *__restrict__ M1, double frac, BoxGeom B,
+                            double *__restrict__ X) {
+  const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)cxn * dy * dz) return;
+  const int cx = (int)(t % cxn), qy = (int)((t / cxn) % dy), qz = (int)(t / ((long long)cxn * dy));
+  const int P = cx / 2 + 2, sd = cx & 1;
+  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = -2; a <= 2; ++a) {
+    const double w = prolong_weight(sd, a);
+    const long long idx = (((long long)qz * dy + qy) * dx + (P + a)) * 10;
+    for (int k = 0; k < 10; ++k) m[k] += w * ((1.0 - frac) * M0[idx + k] + frac * M1[idx + k]);
+  }
+  double *o = X + t * 10;
+  for (int k = 0; k < 10; ++k) o[k] = m[k];
+}
+
+__global__ void k_prolong_y(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y) {
+  const int dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4, cyn = 2 * B.nb[1] + 4;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)cxn * cyn * dz) return;
+  const int cx = (int)(t % cxn), cy = (int)((t / cxn) % cyn), qz = (int)(t / ((long long)cxn * cyn));
+  const int p = B.pad;
+  if (cx >= p && cx < cxn - p && cy >= p && cy < cyn - p) return;  // owned column
+  const int P = cy / 2 + 2, sd = cy & 1;
+  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = -2; b <= 2; ++b) {
+    const double w = prolong_weight(sd, b);
+    const double *q = X + (((long long)qz * dy + (P + b)) * cxn + cx) * 10;
+    for (int k = 0; k < 10; ++k) m[k] += w * q[k];
+  }
+  double *o = Y + t * 10;
+  for (int k = 0; k < 10; ++k) o[k] = m[k];
+}
+
 template <int S, bool PE>
-__global__ void k_prolong_inject(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V,
-                                 const double *__restrict__ M0, const double *__restrict__ M1, double frac,
-                                 float scale, int *__restrict__ err) {
+__global__ void k_prolong_z_inject(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V,
+                                   const double *__restrict__ Y, float scale, int *__restrict__ err) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)L.nx * L.ny * L.nz) return;
   const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
   const int p = B.pad;
   if (x >= p && x < L.nx - p && y >= p && y < L.ny - p) return;  // owned fine cell
-  const int dx = B.nb[0] + 6, dy = B.nb[1] + 6;
-  // parent (in the donor block, halo 2 removed) and child side on each axis
   const int pz = z + p;  // padded z coordinate of the periodic fine span
-  const int P[3] = {x / 2 + 2, y / 2 + 2, pz / 2 + 2};
-  const int sd[3] = {x & 1, y & 1, pz & 1};
-  const double wlo[5] = {-3.0 / 128.0, 11.0 / 64.0, 1.0, -11.0 / 64.0, 3.0 / 128.0};
-  const double whi[5] = {3.0 / 128.0, -11.0 / 64.0, 1.0, 11.0 / 64.0, -3.0 / 128.0};
+  const int P = pz / 2 + 2, sd = pz & 1;
   double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int c = -2; c <= 2; ++c) {
-    const double wc = sd[2] ? whi[c + 2] : wlo[c + 2];
-    for (int b = -2; b <= 2; ++b) {
-      const double wb = wc * (sd[1] ? whi[b + 2] : wlo[b + 2]);
-      for (int a = -2; a <= 2; ++a) {
-        const double w = wb * (sd[0] ? whi[a + 2] : wlo[a + 2]);
-        const long long idx = (((long long)(P[2] + c) * dy + (P[1] + b)) * dx + (P[0] + a)) * 10;
-        for (int k = 0; k < 10; ++k) m[k] += w * ((1.0 - frac) * M0[idx + k] + frac * M1[idx + k]);
-      }
-    }
+    const double w = prolong_weight(sd, c);
+    const double *q = Y + (((long long)(P + c) * L.ny + y) * L.nx + x) * 10;
+    for (int k = 0; k < 10; ++k) m[k] += w * q[k];
   }
   rescale(m, V.shear_cf, V.bulk_cf);
   double g[NQ];
@@ -172,7 +208,6 @@ __global__ void k_prolong_inject(typename St<S>::T *__restrict__ A, Desc L, BoxG
 }
 
 // Face flux registers.  Face f = 2*axis + (side > 0), axis in {0 (x faces), 1 (y faces)}.
-// Register layout reg[f][t][z][4], t along the in-plane tangential axis (box range).
 struct FluxTerm {
   int8_t d, s0, s1;  // direction, tangential shifts (in-plane axis, z)
   double coef[4];    // weight * normal * [1, c]
@@ -182,61 +217,72 @@ struct FluxTable {  // host-built, copied once into a device buffer
   int n[2];
 };
 
+// One wavefront per face element: lane k evaluates term k (< 64), then a fixed-order
+// butterfly sums the wave, so the register value does not depend on scheduling.
 template <int S, bool PE>
-__device__ __forceinline__ void flux_at(const typename St<S>::T *A, const Desc &L, const FluxTable *__restrict__ T, int axis,
-                                        int face_cell, int tpos, int zpos, float inv, double (&out)[4]) {
+__device__ __forceinline__ void flux_wave(const typename St<S>::T *A, const Desc &L, const FluxTable *__restrict__ T,
+                                          int axis, int face_cell, int tpos, int zpos, float inv, int lane,
+                                          double (&out)[4]) {
   out[0] = out[1] = out[2] = out[3] = 0.0;
-  for (int k = 0; k < T->n[axis]; ++k) {
-    const FluxTerm &tm = T->term[axis][k];
+  if (lane < T->n[axis]) {
+    const FluxTerm &tm = T->term[axis][lane];
     const int nrm = axis == 0 ? cxq(tm.d) : cyq(tm.d);
     const int a = face_cell + (nrm < 0 ? 1 : 0);
     const int tt = tpos - tm.s0;
     const int zz = ((zpos - tm.s1) % L.nz + L.nz) % L.nz;
     const int x = axis == 0 ? a : tt, y = axis == 0 ? tt : a;
     const double v = (double)ld<S>(A + loc_out<PE>(L, x, y, zz, tm.d), inv);
-    for (int c = 0; c < 4; ++c) out[c] += v * tm.coef[c];
+    for (int c = 0; c < 4; ++c) out[c] = v * tm.coef[c];
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    for (int c = 0; c < 4; ++c) out[c] += __shfl_xor(out[c], o, 64);
+}
+
+// Coarse face registers reg[f][t][z][4] (integrated over one coarse step: x dx_c^3).
+template <int S, bool PE>
+__global__ __launch_bounds__(256) void k_flux_coarse(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
+                                                     const FluxTable *__restrict__ T, double vol, float inv,
+                                                     double *__restrict__ reg) {
+  const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
+  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (w >= 4LL * nt * L.nz) return;
+  const int zpos = (int)(w % L.nz), ti = (int)((w / L.nz) % nt), f = (int)(w / ((long long)L.nz * nt));
+  const int axis = f / 2, side = f % 2;
+  const int other = 1 - axis;
+  if (ti >= B.nb[other]) return;  // whole wave leaves together
+  const int face_cell = side ? B.hi[axis] - 1 : B.lo[axis] - 1;
+  double o[4];
+  flux_wave<S, PE>(A, L, T, axis, face_cell, B.lo[other] + ti, zpos, inv, lane, o);
+  if (lane == 0) {
+    double *r = reg + (((long long)f * nt + ti) * L.nz + zpos) * 4;
+    for (int c = 0; c < 4; ++c) r[c] = o[c] * vol;
   }
 }
 
+// Fine face registers per fine face element regf[f][t_f][z_f][4] (x dx_f^3), summed over
+// the two substeps (accumulate = 1 on the second); the reflux folds the 2x2 children.
 template <int S, bool PE>
-__global__ void k_flux_coarse(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T,
-                              double vol, float inv, double *__restrict__ reg) {
-  const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 4LL * nt * L.nz) return;
-  const int zpos = (int)(t % L.nz), ti = (int)((t / L.nz) % nt), f = (int)(t / ((long long)L.nz * nt));
+__global__ __launch_bounds__(256) void k_flux_fine(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
+                                                   const FluxTable *__restrict__ T, double vol, float inv,
+                                                   int accumulate, double *__restrict__ regf) {
+  const int nt = 2 * (B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1]);
+  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (w >= 4LL * nt * L.nz) return;
+  const int zf = (int)(w % L.nz), tf = (int)((w / L.nz) % nt), f = (int)(w / ((long long)L.nz * nt));
   const int axis = f / 2, side = f % 2;
   const int other = 1 - axis;
-  if (ti >= B.nb[other]) return;
-  const int face_cell = side ? B.hi[axis] - 1 : B.lo[axis] - 1;
-  double o[4];
-  flux_at<S, PE>(A, L, T, axis, face_cell, B.lo[other] + ti, zpos, inv, o);
-  double *r = reg + (((long long)f * nt + ti) * L.nz + zpos) * 4;
-  for (int c = 0; c < 4; ++c) r[c] = o[c] * vol;
-}
-
-// Fine flux summed over the 2x2 children of each coarse face element, accumulated over substeps.
-template <int S, bool PE>
-__global__ void k_flux_fine(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T,
-                            double vol, float inv, int nz_coarse, int accumulate, double *__restrict__ reg) {
-  const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 4LL * nt * nz_coarse) return;
-  const int zc = (int)(t % nz_coarse), ti = (int)((t / nz_coarse) % nt), f = (int)(t / ((long long)nz_coarse * nt));
-  const int axis = f / 2, side = f % 2;
-  const int other = 1 - axis;
-  if (ti >= B.nb[other]) return;
+  if (tf >= 2 * B.nb[other]) return;
   const int n_axis = 2 * B.nb[axis];
   const int face_cell = side ? B.pad + n_axis - 1 : B.pad - 1;
-  double s[4] = {0, 0, 0, 0};
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      double o[4];
-      flux_at<S, PE>(A, L, T, axis, face_cell, B.pad + 2 * ti + a, 2 * zc + b, inv, o);
-      for (int c = 0; c < 4; ++c) s[c] += o[c];
-    }
-  double *r = reg + (((long long)f * nt + ti) * nz_coarse + zc) * 4;
-  for (int c = 0; c < 4; ++c) r[c] = (accumulate ? r[c] : 0.0) + s[c] * vol;
+  double o[4];
+  flux_wave<S, PE>(A, L, T, axis, face_cell, B.pad + tf, zf, inv, lane, o);
+  if (lane == 0) {
+    double *r = regf + (((long long)f * nt + tf) * L.nz + zf) * 4;
+    for (int c = 0; c < 4; ++c) r[c] = (accumulate ? r[c] : 0.0) + o[c] * vol;
+  }
 }
 
 // Equilibrium reflux on the coarse cells just outside each face.
@@ -244,6 +290,7 @@ template <int S, bool PE>
 __global__ void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const double *__restrict__ rc,
                          const double *__restrict__ rf, double inv_vol, float scale, float inv,
                          int *__restrict__ err) {
+  // rf holds fine face elements [f][2 nt][2 nz][4]
   const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4LL * nt * L.nz) return;
@@ -255,13 +302,19 @@ __global__ void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, c
   const int x = axis == 0 ? a : B.lo[other] + ti, y = axis == 0 ? B.lo[other] + ti : a;
   const long long ri = (((long long)f * nt + ti) * L.nz + zpos) * 4;
   const double sgn = side ? 1.0 : -1.0;
+  double ff[4] = {0, 0, 0, 0};
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      const double *q = rf + (((long long)f * 2 * nt + 2 * ti + a) * 2 * L.nz + 2 * zpos + b) * 4;
+      for (int c = 0; c < 4; ++c) ff[c] += q[c];
+    }
   double g[NQ], m[10], mo[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, mn[10], po[NQ], pn[NQ];
   int e = 0;
   gather_in<S, PE>(A, L, x, y, zpos, inv, g);
   moments_of(g, m, e);
   for (int c = 0; c < 4; ++c) mo[c] = m[c];
   for (int c = 0; c < 10; ++c) mn[c] = mo[c];
-  for (int c = 0; c < 4; ++c) mn[c] += sgn * (rf[ri + c] - rc[ri + c]) * inv_vol;
+  for (int c = 0; c < 4; ++c) mn[c] += sgn * (ff[c] - rc[ri + c]) * inv_vol;
   populations_of(mn, pn, e);
   populations_of(mo, po, e);
   for (int i = 0; i < NQ; ++i) g[i] += pn[i] - po[i];

@@ -17,6 +17,7 @@ struct Refined {
   FreeState F{};
   double dxc = 0, dtc = 0, dxf = 0, dtf = 0, span = 0, U = 0;
   double *d_before = nullptr, *d_after = nullptr, *d_regc = nullptr, *d_regf = nullptr, *d_part = nullptr;
+  double *d_px = nullptr, *d_py = nullptr;  // prolongation passes
   int *d_err = nullptr;
   FluxTable T{};
   FluxTable *d_T = nullptr;
@@ -103,6 +104,7 @@ void refined_coarse_step(Refined *r, bool sample, double number, double *row, hi
   using T_ = typename St<S>::T;
   const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
   const unsigned flux_blocks = blocks_for(4LL * r->nt * c->nz, 128);
+  const unsigned cwave_blocks = blocks_for(4LL * r->nt * c->nz * 64, 256);
   // donor moments at time t (pre-collision coarse state)
   if (c->sweeps % 2 == 0)
     hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_before, r->d_err);
@@ -116,22 +118,26 @@ void refined_coarse_step(Refined *r, bool sample, double number, double *row, hi
   const bool cpe = c->sweeps % 2 == 0;
   const double volc = r->dxc * r->dxc * r->dxc, volf = r->dxf * r->dxf * r->dxf;
   if (cpe) {
-    hipLaunchKernelGGL((k_flux_coarse<S, true>), dim3(flux_blocks), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
+    hipLaunchKernelGGL((k_flux_coarse<S, true>), dim3(cwave_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
     launch_boundary<S, true>(c, false);
     hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
   } else {
-    hipLaunchKernelGGL((k_flux_coarse<S, false>), dim3(flux_blocks), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
+    hipLaunchKernelGGL((k_flux_coarse<S, false>), dim3(cwave_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
     launch_boundary<S, false>(c, false);
     hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
   }
   const long long nf = (long long)f->nx * f->ny * f->nz;
-  const unsigned fflux_blocks = blocks_for(4LL * r->nt * c->nz, 128);
+  const unsigned fflux_blocks = blocks_for(4LL * (2 * r->nt) * f->nz * 64, 256);
   for (int sub = 0; sub < 2; ++sub) {
     const double frac = sub * 0.5;
+    const long long npx = (long long)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
+    const long long npy = (long long)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (r->B.nb[2] + 6);
+    hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(npx, 256)), dim3(256), 0, f->stream, r->d_before, r->d_after, frac, r->B, r->d_px);
+    hipLaunchKernelGGL(k_prolong_y, dim3(blocks_for(npy, 256)), dim3(256), 0, f->stream, r->d_px, r->B, r->d_py);
     if (f->sweeps % 2 == 0)
-      hipLaunchKernelGGL((k_prolong_inject<S, true>), dim3(blocks_for(nf, 128)), dim3(128), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_before, r->d_after, frac, f->scale, r->d_err);
+      hipLaunchKernelGGL((k_prolong_z_inject<S, true>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
     else
-      hipLaunchKernelGGL((k_prolong_inject<S, false>), dim3(blocks_for(nf, 128)), dim3(128), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_before, r->d_after, frac, f->scale, r->d_err);
+      hipLaunchKernelGGL((k_prolong_z_inject<S, false>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
     if (ev) (void)hipEventRecord(ev[4 + 2 * sub], f->stream);
     launch_sweep<S>(f, f->sweeps % 2 == 0, false);
     if (ev) (void)hipEventRecord(ev[5 + 2 * sub], f->stream);
@@ -139,10 +145,10 @@ void refined_coarse_step(Refined *r, bool sample, double number, double *row, hi
     WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
     double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;
     if (f->sweeps % 2 == 0) {
-      hipLaunchKernelGGL((k_flux_fine<S, true>), dim3(fflux_blocks), dim3(128), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, c->nz, sub, r->d_regf);
+      hipLaunchKernelGGL((k_flux_fine<S, true>), dim3(fflux_blocks), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, sub, r->d_regf);
       if (f->nlinks) hipLaunchKernelGGL((k_walls<S, true, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
     } else {
-      hipLaunchKernelGGL((k_flux_fine<S, false>), dim3(fflux_blocks), dim3(128), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, c->nz, sub, r->d_regf);
+      hipLaunchKernelGGL((k_flux_fine<S, false>), dim3(fflux_blocks), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, sub, r->d_regf);
       if (f->nlinks) hipLaunchKernelGGL((k_walls<S, false, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
     }
   }
@@ -216,7 +222,9 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
   const size_t ndon = (size_t)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6) * 10;
   const size_t nreg = (size_t)4 * r->nt * nz * 4;
   if (dalloc(r->c, &r->d_before, ndon) || dalloc(r->c, &r->d_after, ndon) || dalloc(r->c, &r->d_regc, nreg) ||
-      dalloc(r->c, &r->d_regf, nreg) || dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
+      dalloc(r->c, &r->d_regf, 4 * nreg) ||
+      dalloc(r->c, &r->d_px, (size_t)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (nz + 6) * 10) ||
+      dalloc(r->c, &r->d_py, (size_t)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (nz + 6) * 10) || dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
     delete r;
     return -1;
