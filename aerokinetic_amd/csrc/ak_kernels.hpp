@@ -43,6 +43,28 @@ __device__ __forceinline__ C wave_min(C v) {
   return v;
 }
 
+// Keep a wave-uniform pointer in scalar registers (the compiler otherwise may carry the
+// 27 direction bases of the in-place sweep as 64-bit vector values).
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ gchar *uniform_ptr(const void *p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (gchar *)(((unsigned long long)hi << 32) | lo);
+}
+template <int S>
+__device__ __forceinline__ typename St<S>::C ld_g(const gchar *p, float inv_scale) {
+  typedef __attribute__((address_space(1))) const typename St<S>::T gT;
+  if constexpr (S == 2) return (float)(*(const gT *)p) * inv_scale;
+  else return *(const gT *)p;
+}
+template <int S>
+__device__ __forceinline__ void st_g(gchar *p, typename St<S>::C v, float scale) {
+  typedef __attribute__((address_space(1))) typename St<S>::T gT;
+  if constexpr (S == 2) *(gT *)p = (_Float16)(v * scale);
+  else *(gT *)p = v;
+}
+
 template <int S, bool ODD, bool PXY, bool DIAG>
 __global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A,
                                                const uint8_t *__restrict__ solid, Desc L, SweepGeom G,
@@ -59,38 +81,74 @@ __global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A
   if (active) { mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]; active = mk != 1; }
   C drho = 0, u2 = 0;
   if (active) {
-    const long long cell = L.base + cell_of(L, x, y, z);
     C g[NQ];
+    if constexpr (PXY) {
+      const long long cell = L.base + cell_of(L, x, y, z);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      long long src;
-      if constexpr (ODD) src = cell + (long long)opp(q) * L.QS + nb_off<PXY>(L, x, y, z, -cxq(q), -cyq(q), -czq(q));
-      else src = cell + (long long)q * L.QS;
-      g[q] = ld<S>(A + src, inv_scale);
-    }
-    collide<C>(g, ws, wb, drho, u2);
+      for (int q = 0; q < NQ; ++q) {
+        long long src;
+        if constexpr (ODD) src = cell + (long long)opp(q) * L.QS + nb_off<PXY>(L, x, y, z, -cxq(q), -cyq(q), -czq(q));
+        else src = cell + (long long)q * L.QS;
+        g[q] = ld<S>(A + src, inv_scale);
+      }
+      collide<C>(g, ws, wb, drho, u2);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      long long dst;
-      if constexpr (ODD) dst = cell + (long long)q * L.QS + nb_off<PXY>(L, x, y, z, cxq(q), cyq(q), czq(q));
-      else dst = cell + (long long)opp(q) * L.QS;
-      stv<S>(A + dst, g[q], scale);
+      for (int q = 0; q < NQ; ++q) {
+        long long dst;
+        if constexpr (ODD) dst = cell + (long long)q * L.QS + nb_off<PXY>(L, x, y, z, cxq(q), cyq(q), czq(q));
+        else dst = cell + (long long)opp(q) * L.QS;
+        stv<S>(A + dst, g[q], scale);
+      }
+    } else {
+      // A block covers one z plane, so every direction's shift (including the periodic
+      // z wrap) is wave-uniform: a scalar base per direction plus one 32-bit offset.
+      // byte offset of the cell inside a direction plane (32 bits): global loads then use a
+      // scalar base plus this vector offset, with no 64-bit vector address arithmetic
+      const unsigned boff = (unsigned)((y + 1) * L.P + x) * (unsigned)sizeof(typename St<S>::T);
+      const long long plane = L.base + (long long)z * L.PS;
+      const long long zp = (z == L.nz - 1) ? -(long long)(L.nz - 1) * L.PS : L.PS;
+      const long long zm = (z == 0) ? (long long)(L.nz - 1) * L.PS : -L.PS;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        long long d;
+        if constexpr (ODD) {
+          const int cz = -czq(q);
+          d = (long long)opp(q) * L.QS - cxq(q) - (long long)cyq(q) * L.P + (cz == 1 ? zp : cz == -1 ? zm : 0);
+        } else {
+          d = (long long)q * L.QS;
+        }
+        const gchar *bq = uniform_ptr(A + (plane + d));
+        g[q] = ld_g<S>(bq + boff, inv_scale);
+      }
+      collide<C>(g, ws, wb, drho, u2);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        long long d;
+        if constexpr (ODD) {
+          const int cz = czq(q);
+          d = (long long)q * L.QS + cxq(q) + (long long)cyq(q) * L.P + (cz == 1 ? zp : cz == -1 ? zm : 0);
+        } else {
+          d = (long long)opp(q) * L.QS;
+        }
+        gchar *bq = uniform_ptr(A + (plane + d));
+        st_g<S>(bq + boff, g[q], scale);
+      }
     }
   }
   if constexpr (DIAG) {
     // rho and u are conserved by the collision: the pre-collision moments are the
-    // post-collision state's.  Non-finite values raise a flag.
+    // post-collision state's.  Wave reductions run in the collision's compute type.
     const bool counted = active && mk == 0;
-    double vu = counted ? (double)u2 : 0.0;
-    double vd = counted ? fabs((double)drho) : 0.0;
-    double vr = counted ? 1.0 + (double)drho : 1e300;
+    C vu = counted ? u2 : C(0);
+    C vd = counted ? (drho < C(0) ? -drho : drho) : C(0);
+    C vr = counted ? C(1) + drho : C(3.0e38);
     int bad = counted && !(isfinite(vu) && isfinite(vd)) ? 1 : 0;
-    if (!isfinite(vu)) vu = 0.0;
-    if (!isfinite(vd)) vd = 0.0;
-    if (!isfinite(vr)) vr = 1e300;
+    if (!isfinite(vu)) vu = C(0);
+    if (!isfinite(vd)) vd = C(0);
+    if (!isfinite(vr)) vr = C(3.0e38);
     vu = wave_max(vu); vd = wave_max(vd); vr = wave_min(vr);
     bad = __any(bad) ? 1 : 0;
-    __shared__ double su[4], sd[4], sr[4];
+    __shared__ C su[4], sd[4], sr[4];
     __shared__ int sb[4];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     if (lane == 0) { su[w] = vu; sd[w] = vd; sr[w] = vr; sb[w] = bad; }
@@ -102,9 +160,9 @@ __global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A
         vr = sr[k] < vr ? sr[k] : vr; bad |= sb[k];
       }
       DiagSlot *sl = &diag->s[blockIdx.x % DIAG_SLOTS];
-      atomicMax(&sl->u2max, (unsigned long long)__double_as_longlong(vu));
-      atomicMax(&sl->drmax, (unsigned long long)__double_as_longlong(vd));
-      if (vr > 0.0) atomicMin(&sl->rhomin, (unsigned long long)__double_as_longlong(vr));
+      atomicMax(&sl->u2max, (unsigned long long)__double_as_longlong((double)vu));
+      atomicMax(&sl->drmax, (unsigned long long)__double_as_longlong((double)vd));
+      if (vr > C(0)) atomicMin(&sl->rhomin, (unsigned long long)__double_as_longlong((double)vr >= 3.0e38 ? 1e300 : (double)vr));
       else atomicMax(&sl->nonfinite, 2ull);  // non-positive density
       if (bad) atomicMax(&sl->nonfinite, 1ull);
     }

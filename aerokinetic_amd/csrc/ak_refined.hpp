@@ -154,6 +154,9 @@ __global__ void k_prolong_x(const double *__restrict__ M0, const double *__restr
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)cxn * dy * dz) return;
   const int cx = (int)(t % cxn), qy = (int)((t / cxn) % dy), qz = (int)(t / ((long long)cxn * dy));
+  // the y pass only reads columns of donor cells: every row next to an x pad, and only
+  // the parent rows of the y pads (qy <= 4 or qy >= nby+1) elsewhere
+  if (cx >= B.pad && cx < cxn - B.pad && qy > 4 && qy < B.nb[1] + 1) return;
   const int P = cx / 2 + 2, sd = cx & 1;
   double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int a = -2; a <= 2; ++a) {
@@ -285,54 +288,99 @@ __global__ __launch_bounds__(256) void k_flux_fine(const typename St<S>::T *__re
   }
 }
 
-// Equilibrium reflux on the coarse cells just outside each face.
+// Equilibrium reflux on the coarse cells just outside each face: one wavefront per
+// cell, lane i (< 27) owns population i; density and momentum by fixed-order butterflies.
+__device__ __forceinline__ double eq_dir(int i, double rho, double ux, double uy, double uz) {
+  const double u[3] = {ux, uy, uz};
+  const int c[3] = {cxq(i), cyq(i), czq(i)};
+  double e = rho;
+  for (int a = 0; a < 3; ++a)
+    e *= c[a] == 0 ? 2.0 / 3.0 - u[a] * u[a] : (1.0 / 3.0 + u[a] * u[a] + c[a] * u[a]) * 0.5;
+  return e;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 template <int S, bool PE>
-__global__ void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const double *__restrict__ rc,
-                         const double *__restrict__ rf, double inv_vol, float scale, float inv,
-                         int *__restrict__ err) {
+__global__ __launch_bounds__(256) void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
+                                                const double *__restrict__ rc, const double *__restrict__ rf,
+                                                double inv_vol, float scale, float inv, int *__restrict__ err) {
   // rf holds fine face elements [f][2 nt][2 nz][4]
   const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 4LL * nt * L.nz) return;
-  const int zpos = (int)(t % L.nz), ti = (int)((t / L.nz) % nt), f = (int)(t / ((long long)L.nz * nt));
+  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x % 64;
+  if (w >= 4LL * nt * L.nz) return;
+  const int zpos = (int)(w % L.nz), ti = (int)((w / L.nz) % nt), f = (int)(w / ((long long)L.nz * nt));
   const int axis = f / 2, side = f % 2;
   const int other = 1 - axis;
-  if (ti >= B.nb[other]) return;
+  if (ti >= B.nb[other]) return;  // uniform across the wave
   const int a = side ? B.hi[axis] : B.lo[axis] - 1;
   const int x = axis == 0 ? a : B.lo[other] + ti, y = axis == 0 ? B.lo[other] + ti : a;
   const long long ri = (((long long)f * nt + ti) * L.nz + zpos) * 4;
   const double sgn = side ? 1.0 : -1.0;
-  double ff[4] = {0, 0, 0, 0};
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      const double *q = rf + (((long long)f * 2 * nt + 2 * ti + a) * 2 * L.nz + 2 * zpos + b) * 4;
-      for (int c = 0; c < 4; ++c) ff[c] += q[c];
-    }
-  double g[NQ], m[10], mo[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, mn[10], po[NQ], pn[NQ];
+  double corr[4];
+  for (int c = 0; c < 4; ++c) {
+    double ff = 0.0;
+    for (int p = 0; p < 2; ++p)
+      for (int q = 0; q < 2; ++q) ff += rf[(((long long)f * 2 * nt + 2 * ti + p) * 2 * L.nz + 2 * zpos + q) * 4 + c];
+    corr[c] = sgn * (ff - rc[ri + c]) * inv_vol;
+  }
+  const int i = lane < NQ ? lane : 0;
+  const long long slot = loc_in<PE>(L, x, y, zpos, i);
+  double g = lane < NQ ? (double)ld<S>(A + slot, inv) : 0.0;
+  const double d = wave_sum(g);
+  const double jx = wave_sum(cxq(i) * g * (lane < NQ));
+  const double jy = wave_sum(cyq(i) * g * (lane < NQ));
+  const double jz = wave_sum(czq(i) * g * (lane < NQ));
   int e = 0;
-  gather_in<S, PE>(A, L, x, y, zpos, inv, g);
+  const double rho = d + 1.0, rho2 = rho + corr[0];
+  if (!(rho > 0) || !(rho2 > 0)) e |= TE_DENSITY;
+  const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+  const double vx = (jx + corr[1]) / rho2, vy = (jy + corr[2]) / rho2, vz = (jz + corr[3]) / rho2;
+  if (sqrt(ux * ux + uy * uy + uz * uz) > 0.3 * 0.57735026918962576451 ||
+      sqrt(vx * vx + vy * vy + vz * vz) > 0.3 * 0.57735026918962576451)
+    e |= TE_MACH;
+  const double eo = eq_dir(i, rho, ux, uy, uz), en = eq_dir(i, rho2, vx, vy, vz);
+  if (lane < NQ) {
+    if (!(eo >= 0) || !(en >= 0)) e |= TE_NEGATIVE;
+    g += en - eo;
+    if (!(g + wq(i) >= 0)) e |= isfinite(g) ? TE_NEGATIVE : TE_NONFINITE;
+    stv<S>(A + slot, (typename St<S>::C)g, scale);
+  }
+  if (e) atomicOr(err, e);
+}
+
+// Moments of every owned fine cell outside the fine virtual core (first restriction stage).
+template <int S, bool PE>
+__global__ void k_fine_moments(const typename St<S>::T *__restrict__ Af, Desc Lf, BoxGeom B, float inv,
+                               double *__restrict__ Mf, int *__restrict__ err) {
+  const int fx = 2 * B.nb[0], fy = 2 * B.nb[1], fz = 2 * B.nb[2];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)fx * fy * fz) return;
+  const int x = (int)(t % fx), y = (int)((t / fx) % fy), z = (int)(t / ((long long)fx * fy));
+  if (in_core(B, x / 2, y / 2)) return;
+  double g[NQ], m[10];
+  int e = 0;
+  gather_in<S, PE>(Af, Lf, B.pad + x, B.pad + y, z, inv, g);
   moments_of(g, m, e);
-  for (int c = 0; c < 4; ++c) mo[c] = m[c];
-  for (int c = 0; c < 10; ++c) mn[c] = mo[c];
-  for (int c = 0; c < 4; ++c) mn[c] += sgn * (ff[c] - rc[ri + c]) * inv_vol;
-  populations_of(mn, pn, e);
-  populations_of(mo, po, e);
-  for (int i = 0; i < NQ; ++i) g[i] += pn[i] - po[i];
-  moments_of(g, m, e);
-  scatter_in<S, PE>(A, L, x, y, zpos, scale, g);
+  for (int k = 0; k < 10; ++k) Mf[t * 10 + k] = m[k];
   if (e) atomicOr(err, e);
 }
 
 // Covered coarse cells from the fine state: free stream in the virtual core, otherwise
-// the 2x2x2 mean of the children's moments, rescaled fine->coarse and regularised.
-template <int S, bool PEC, bool PEF>
-__global__ void k_restrict(typename St<S>::T *__restrict__ Ac, Desc Lc, const typename St<S>::T *__restrict__ Af,
-                           Desc Lf, BoxGeom B, Levels V, FreeState F, float scale, float inv,
-                           int *__restrict__ err) {
+// the 2x2x2 mean of the children's moments (from k_fine_moments), rescaled fine->coarse
+// and regularised.
+template <int S, bool PEC>
+__global__ void k_restrict(typename St<S>::T *__restrict__ Ac, Desc Lc, const double *__restrict__ Mf, BoxGeom B,
+                           Levels V, FreeState F, float scale, int *__restrict__ err) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)B.nb[0] * B.nb[1] * B.nb[2]) return;
   const int bx = (int)(t % B.nb[0]), by = (int)((t / B.nb[0]) % B.nb[1]), bz = (int)(t / ((long long)B.nb[0] * B.nb[1]));
   const int x = B.lo[0] + bx, y = B.lo[1] + by, z = B.lo[2] + bz;
+  const int fx = 2 * B.nb[0], fy = 2 * B.nb[1];
   double g[NQ];
   int e = 0;
   if (in_core(B, bx, by)) {
@@ -342,9 +390,7 @@ __global__ void k_restrict(typename St<S>::T *__restrict__ Ac, Desc Lc, const ty
     for (int c = 0; c < 2; ++c)
       for (int b = 0; b < 2; ++b)
         for (int a = 0; a < 2; ++a) {
-          double gf[NQ], m[10];
-          gather_in<S, PEF>(Af, Lf, B.pad + 2 * bx + a, B.pad + 2 * by + b, 2 * bz + c, inv, gf);
-          moments_of(gf, m, e);
+          const double *m = Mf + (((long long)(2 * bz + c) * fy + (2 * by + b)) * fx + (2 * bx + a)) * 10;
           for (int k = 0; k < 10; ++k) acc[k] += m[k];
         }
     for (int k = 0; k < 10; ++k) acc[k] *= 0.125;

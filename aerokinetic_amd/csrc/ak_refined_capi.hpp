@@ -18,6 +18,7 @@ struct Refined {
   double dxc = 0, dtc = 0, dxf = 0, dtf = 0, span = 0, U = 0;
   double *d_before = nullptr, *d_after = nullptr, *d_regc = nullptr, *d_regf = nullptr, *d_part = nullptr;
   double *d_px = nullptr, *d_py = nullptr;  // prolongation passes
+  double *d_mf = nullptr;                   // fine moments for the restriction
   int *d_err = nullptr;
   FluxTable T{};
   FluxTable *d_T = nullptr;
@@ -96,6 +97,22 @@ void build_flux_table(FluxTable &T) {
   }
 }
 
+template <int S>
+void launch_restrict(Refined *r) {
+  Lattice *c = r->c, *f = r->f;
+  using T_ = typename St<S>::T;
+  const long long nfo = 8LL * r->B.nb[0] * r->B.nb[1] * r->B.nb[2];
+  const long long nbox = (long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2];
+  if (f->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_fine_moments<S, true>), dim3(blocks_for(nfo, 256)), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, f->inv_scale, r->d_mf, r->d_err);
+  else
+    hipLaunchKernelGGL((k_fine_moments<S, false>), dim3(blocks_for(nfo, 256)), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, f->inv_scale, r->d_mf, r->d_err);
+  if (c->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_restrict<S, true>), dim3(blocks_for(nbox, 128)), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, (const double *)r->d_mf, r->B, r->V, r->F, c->scale, r->d_err);
+  else
+    hipLaunchKernelGGL((k_restrict<S, false>), dim3(blocks_for(nbox, 128)), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, (const double *)r->d_mf, r->B, r->V, r->F, c->scale, r->d_err);
+}
+
 // ev (optional, 8 events): step start/end, coarse sweep start/end, fine sweep 0 and 1 start/end
 template <int S>
 void refined_coarse_step(Refined *r, bool sample, double number, double *row, hipEvent_t *ev = nullptr) {
@@ -103,7 +120,6 @@ void refined_coarse_step(Refined *r, bool sample, double number, double *row, hi
   if (ev) (void)hipEventRecord(ev[0], c->stream);
   using T_ = typename St<S>::T;
   const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
-  const unsigned flux_blocks = blocks_for(4LL * r->nt * c->nz, 128);
   const unsigned cwave_blocks = blocks_for(4LL * r->nt * c->nz * 64, 256);
   // donor moments at time t (pre-collision coarse state)
   if (c->sweeps % 2 == 0)
@@ -154,16 +170,10 @@ void refined_coarse_step(Refined *r, bool sample, double number, double *row, hi
   }
   const double inv_volc = 1.0 / volc;
   if (c->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_reflux<S, true>), dim3(flux_blocks), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
+    hipLaunchKernelGGL((k_reflux<S, true>), dim3(cwave_blocks), dim3(256), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
   else
-    hipLaunchKernelGGL((k_reflux<S, false>), dim3(flux_blocks), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
-  const long long nbox = (long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2];
-  const bool pc = c->sweeps % 2 == 0, pf = f->sweeps % 2 == 0;
-#define AK_RESTRICT(A_, B_) \
-  hipLaunchKernelGGL((k_restrict<S, A_, B_>), dim3(blocks_for(nbox, 128)), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, (const T_ *)f->pop, f->L, r->B, r->V, r->F, c->scale, f->inv_scale, r->d_err)
-  if (pc) { if (pf) AK_RESTRICT(true, true); else AK_RESTRICT(true, false); }
-  else { if (pf) AK_RESTRICT(false, true); else AK_RESTRICT(false, false); }
-#undef AK_RESTRICT
+    hipLaunchKernelGGL((k_reflux<S, false>), dim3(cwave_blocks), dim3(256), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
+  launch_restrict<S>(r);
   if (sample) {
     // impulse over the two substeps: F dx_f^3 (dx_f/dt_f) / dt_c; q_area = span / 2
     const double fscale = volf * (r->dxf / r->dtf) / r->dtc / (0.5 * r->span);
@@ -224,7 +234,8 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
   if (dalloc(r->c, &r->d_before, ndon) || dalloc(r->c, &r->d_after, ndon) || dalloc(r->c, &r->d_regc, nreg) ||
       dalloc(r->c, &r->d_regf, 4 * nreg) ||
       dalloc(r->c, &r->d_px, (size_t)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (nz + 6) * 10) ||
-      dalloc(r->c, &r->d_py, (size_t)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (nz + 6) * 10) || dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
+      dalloc(r->c, &r->d_py, (size_t)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (nz + 6) * 10) ||
+      dalloc(r->c, &r->d_mf, (size_t)8 * r->B.nb[0] * r->B.nb[1] * nz * 10) || dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
     delete r;
     return -1;
@@ -291,17 +302,17 @@ int ak_refined_init_seeded(void *handle, const double *p) {
     case 0:
       hipLaunchKernelGGL((k_init_level<0>), dim3(blocks_for(ncv, 256)), dim3(256), 0, c->stream, (double *)c->pop, c->d_solid, c->L, sc, c->scale, r->d_err);
       hipLaunchKernelGGL((k_init_level<0>), dim3(blocks_for(nfv, 256)), dim3(256), 0, f->stream, (double *)f->pop, f->d_solid, f->L, sf, f->scale, r->d_err);
-      hipLaunchKernelGGL((k_restrict<0, false, false>), dim3(blocks_for((long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2], 128)), dim3(128), 0, c->stream, (double *)c->pop, c->L, (const double *)f->pop, f->L, r->B, r->V, r->F, c->scale, f->inv_scale, r->d_err);
+      launch_restrict<0>(r);
       break;
     case 1:
       hipLaunchKernelGGL((k_init_level<1>), dim3(blocks_for(ncv, 256)), dim3(256), 0, c->stream, (float *)c->pop, c->d_solid, c->L, sc, c->scale, r->d_err);
       hipLaunchKernelGGL((k_init_level<1>), dim3(blocks_for(nfv, 256)), dim3(256), 0, f->stream, (float *)f->pop, f->d_solid, f->L, sf, f->scale, r->d_err);
-      hipLaunchKernelGGL((k_restrict<1, false, false>), dim3(blocks_for((long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2], 128)), dim3(128), 0, c->stream, (float *)c->pop, c->L, (const float *)f->pop, f->L, r->B, r->V, r->F, c->scale, f->inv_scale, r->d_err);
+      launch_restrict<1>(r);
       break;
     default:
       hipLaunchKernelGGL((k_init_level<2>), dim3(blocks_for(ncv, 256)), dim3(256), 0, c->stream, (_Float16 *)c->pop, c->d_solid, c->L, sc, c->scale, r->d_err);
       hipLaunchKernelGGL((k_init_level<2>), dim3(blocks_for(nfv, 256)), dim3(256), 0, f->stream, (_Float16 *)f->pop, f->d_solid, f->L, sf, f->scale, r->d_err);
-      hipLaunchKernelGGL((k_restrict<2, false, false>), dim3(blocks_for((long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2], 128)), dim3(128), 0, c->stream, (_Float16 *)c->pop, c->L, (const _Float16 *)f->pop, f->L, r->B, r->V, r->F, c->scale, f->inv_scale, r->d_err);
+      launch_restrict<2>(r);
       break;
   }
   HIPCHK(c, hipGetLastError());
