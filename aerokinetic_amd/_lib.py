@@ -49,6 +49,7 @@ _SIGS = {
     "ak_set_sweep_tile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ak_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int]),
     "ak_run_profiled": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _dp]),
+    "ak_set_graph_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ak_history_width": (ctypes.c_int, [_vp]),
     "ak_read_history": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "ak_get_populations": (ctypes.c_int, [_vp, _dp]),
@@ -59,6 +60,19 @@ _SIGS = {
     "ak_synchronize": (ctypes.c_int, [_vp]),
     "ak_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
     "ak_version": (ctypes.c_char_p, []),
+    "ak_refined_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         _ip, _ip, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         _vp]),
+    "ak_refined_destroy": (ctypes.c_int, [_vp]),
+    "ak_refined_level": (_vp, [_vp, ctypes.c_int]),
+    "ak_refined_set_walls": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _dp, _ip, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double]),
+    "ak_refined_init_seeded": (ctypes.c_int, [_vp, _dp]),
+    "ak_refined_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int]),
+    "ak_refined_run_profiled": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _dp]),
+    "ak_refined_transfer_errors": (ctypes.c_int, [_vp]),
+    "ak_refined_set_graph_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ak_refined_get_populations": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
 }
 
 _lib = None

@@ -25,6 +25,18 @@ namespace {
 
 std::string g_error;
 
+// Instantiated HIP graphs of one step, one slot per launch variant (AA parity,
+// sampled or not, ...).  A slot is valid while `key` matches the generation of
+// the state it captured (kernel arguments: buffers, tables, tile shape).
+struct GraphCache {
+  hipGraphExec_t exec[8] = {};
+  unsigned long long key[8] = {};
+  void clear() {
+    for (auto &e : exec)
+      if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+  }
+};
+
 struct Lattice {
   int dev = 0, nx = 0, ny = 0, nz = 0, storage = 0;
   double ws = 1.0, wb = 1.0;
@@ -33,6 +45,7 @@ struct Lattice {
   size_t pop_elems = 0, pop_bytes = 0, elem_size = 8;
   void *pop = nullptr;
   hipStream_t stream = nullptr;
+  bool own_stream = false;
   uint8_t *d_solid = nullptr;
   // walls
   int nlinks = 0, nelem = 1;
@@ -60,7 +73,12 @@ struct Lattice {
   int hist_cap = 0, hist_rows = 0, hist_width = 0;
   std::vector<double> host_hist;
   long long sweeps = 0;  // sweeps since the initial state: parity of the AA form
-  SweepGeom geom{};
+  StepCtl *d_ctl = nullptr;
+  // step graphs: replayed by ak_run while no setter has changed a kernel argument
+  bool use_graphs = true;
+  unsigned long long gen = 1;
+  GraphCache graphs;
+  SweepGeom geom{}, geom_small{};
   std::string err;
   std::vector<void *> owned;
 };
@@ -125,11 +143,13 @@ void launch_sweep(Lattice *s, bool odd, bool diag) {
   using T = typename St<S>::T;
   using C = typename St<S>::C;
   T *A = (T *)s->pop;
-  const dim3 grid(s->geom.nblocks), block(s->geom.bx * s->geom.by);
   const bool pxy = s->ff_kind == AK_FARFIELD_PERIODIC;  // AK_FARFIELD_NONE keeps ghost indexing
+  // the periodic form is bounded to 256-thread blocks; larger tiles fall back to 64x4 there
+  const SweepGeom &G = pxy && s->geom.bx * s->geom.by > 256 ? s->geom_small : s->geom;
+  const dim3 grid(G.nblocks), block(G.bx * G.by);
   const C ws = (C)s->ws, wb = (C)s->wb;
 #define AK_SWEEP(O, P, D) \
-  hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, A, s->d_solid, s->L, s->geom, ws, wb, s->scale, s->inv_scale, s->d_diag)
+  hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, A, s->d_solid, s->L, G, ws, wb, s->scale, s->inv_scale, s->d_diag)
   if (odd) {
     if (pxy) { if (diag) AK_SWEEP(true, true, true); else AK_SWEEP(true, true, false); }
     else { if (diag) AK_SWEEP(true, false, true); else AK_SWEEP(true, false, false); }
@@ -164,24 +184,57 @@ void launch_boundary(Lattice *s, bool loads) {
   }
 }
 
+void launch_loads_final(Lattice *s) {
+  const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
+  hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, s->stream, s->d_partials, s->nlinks ? s->wall_blocks : 0,
+                     s->nelem, 1.0 / qa, 1.0 / (qa * s->chord), 1.0, s->d_ctl, s->d_hist, s->d_diag);
+}
+
+// One uniform step; the step number and history row come from s->d_ctl.
 template <int S>
-void one_step(Lattice *s, bool sample, double step_number) {
+void one_step(Lattice *s, bool sample) {
   const bool pe = (s->sweeps % 2) == 0;
   if (pe) launch_boundary<S, true>(s, sample);
   else launch_boundary<S, false>(s, sample);
-  double *row = s->d_hist + (size_t)s->hist_rows * s->hist_width;
-  if (sample) {
-    const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
-    hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, s->stream, s->d_partials,
-                       s->nlinks ? s->wall_blocks : 0, s->nelem, 1.0 / qa, 1.0 / (qa * s->chord), 1.0,
-                       step_number, row, s->d_diag);
-  }
+  if (sample) launch_loads_final(s);
   launch_sweep<S>(s, pe, sample);
-  if (sample) {
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, row);
-    s->hist_rows++;
-  }
+  if (sample) hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, s->d_ctl, s->d_hist);
+  else hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s->stream, s->d_ctl);
   s->sweeps++;
+}
+
+void set_ctl(Lattice *s, long long step, int row) {
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, s->stream, s->d_ctl, step, row, s->hist_width);
+}
+
+// Enqueue one step on `st` through the graph in slot `slot` of `gc`: captured from
+// `launch` (which also advances the host-side counters) on first use or after a
+// generation change, replayed otherwise (then `advance` updates the counters).
+template <class Launch, class Advance>
+int graph_step(Lattice *s, hipStream_t st, GraphCache &gc, int slot, unsigned long long key, Launch &&launch,
+               Advance &&advance) {
+  if (gc.exec[slot] && gc.key[slot] != key) {
+    (void)hipGraphExecDestroy(gc.exec[slot]);
+    gc.exec[slot] = nullptr;
+  }
+  if (!gc.exec[slot]) {
+    hipGraph_t g = nullptr;
+    HIPCHK(s, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    launch();
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess) return fail(s, "step graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiateWithFlags(&gc.exec[slot], g, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      gc.exec[slot] = nullptr;
+      return fail(s, "step graph instantiation: %s", hipGetErrorString(e));
+    }
+    gc.key[slot] = key;
+  } else {
+    advance();
+  }
+  HIPCHK(s, hipGraphLaunch(gc.exec[slot], st));
+  return 0;
 }
 
 int flush_history(Lattice *s) {
@@ -275,19 +328,31 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
   s->dev = device; s->nx = nx; s->ny = ny; s->nz = nz; s->storage = storage;
   s->ws = omega_shear; s->wb = omega_bulk;
   s->stream = (hipStream_t)stream;
+  if (!s->stream) {
+    // step graphs are captured on the lattice's stream, which cannot be the null stream;
+    // a blocking stream keeps the ordering against work on the null stream
+    if (hipStreamCreate(&s->stream) != hipSuccess) {
+      delete s;
+      return fail(nullptr, "hipStreamCreate failed");
+    }
+    s->own_stream = true;
+  }
   s->elem_size = storage == 0 ? 8 : storage == 1 ? 4 : 2;
   if (storage == 2) {
     const double sc = fp16_scale > 0 ? fp16_scale : 65536.0;
     const double lg = std::log2(sc);
     if (lg != std::floor(lg) || sc < 1.0 || sc > 1048576.0) {
+      if (s->own_stream) (void)hipStreamDestroy(s->stream);
       delete s;
       return fail(nullptr, "fp16 scale must be a power of two in [1, 2^20]");
     }
     s->scale = (float)sc;
     s->inv_scale = (float)(1.0 / sc);
   }
-  // 64-byte aligned interior rows; pitch >= nx + 2 holds both x ghosts
-  const long long align = 64 / (long long)s->elem_size;
+  // interior rows start on a 128-byte cache line (x = 0 first; the x = -1 ghost is the
+  // last element of the previous row's pad), so a wave's 64 consecutive cells of one
+  // direction plane cover whole lines; pitch >= nx + 2 holds both x ghosts
+  const long long align = 128 / (long long)s->elem_size;
   long long P = ((long long)nx + 2 + align - 1) / align * align;
   s->L.nx = nx; s->L.ny = ny; s->L.nz = nz;
   s->L.P = P;
@@ -297,15 +362,18 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
   s->pop_elems = (size_t)(s->L.base + 27 * s->L.QS + align);
   s->pop_bytes = s->pop_elems * s->elem_size;
   set_geom(s, 64, 4);
+  s->geom_small = s->geom;
   s->hist_cap = history_capacity > 0 ? history_capacity : 4096;
   s->hist_width = AK_HIST_FIXED + 4 * MAX_ELEM;
   if (dalloc(s, &s->d_solid, (size_t)(nx + 2) * (ny + 2)) || dalloc(s, &s->d_diag, 1) ||
-      dalloc(s, &s->d_hist, (size_t)s->hist_cap * s->hist_width)) {
+      dalloc(s, &s->d_hist, (size_t)s->hist_cap * s->hist_width) || dalloc(s, &s->d_ctl, 1)) {
     for (void *p : s->owned) (void)hipFree(p);
+    if (s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return -1;
   }
   (void)hipMemset(s->d_solid, 0, (size_t)(nx + 2) * (ny + 2));
+  (void)hipMemset(s->d_hist, 0, (size_t)s->hist_cap * s->hist_width * sizeof(double));
   *handle = s;
   return 0;
 }
@@ -314,8 +382,10 @@ int ak_lattice_destroy(void *handle) {
   Lattice *s = H(handle);
   if (!s) return 0;
   (void)hipSetDevice(s->dev);
-  if (s->stream) (void)hipStreamSynchronize(s->stream); else (void)hipDeviceSynchronize();
+  (void)hipStreamSynchronize(s->stream);
+  s->graphs.clear();
   for (void *p : s->owned) (void)hipFree(p);
+  if (s->own_stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return 0;
 }
@@ -325,9 +395,10 @@ size_t ak_population_bytes(void *handle) { return handle ? H(handle)->pop_bytes 
 int ak_bind_populations(void *handle, void *device_ptr, size_t bytes) {
   Lattice *s = H(handle);
   if (!s) return fail(nullptr, "null lattice handle");
+  s->gen++;  // invalidates captured step graphs
   if (!device_ptr) return fail(s, "null population buffer");
   if (bytes < s->pop_bytes) return fail(s, "population buffer of %zu bytes < required %zu", bytes, s->pop_bytes);
-  if (((uintptr_t)device_ptr) % 64) return fail(s, "population buffer must be 64-byte aligned");
+  if (((uintptr_t)device_ptr) % 128) return fail(s, "population buffer must be 128-byte aligned");
   s->pop = device_ptr;
   HIPCHK(s, hipSetDevice(s->dev));
   // ghosts and padding start at zero
@@ -339,6 +410,7 @@ int ak_bind_populations(void *handle, void *device_ptr, size_t bytes) {
 int ak_set_solid(void *handle, const uint8_t *mask_xy) {
   Lattice *s = H(handle);
   if (!s) return fail(nullptr, "null lattice handle");
+  s->gen++;  // invalidates captured step graphs
   std::vector<uint8_t> m((size_t)(s->nx + 2) * (s->ny + 2), 0);
   for (int x = 0; x < s->nx; ++x)
     for (int y = 0; y < s->ny; ++y) m[(size_t)(y + 1) * (s->nx + 2) + (x + 1)] = mask_xy[(size_t)x * s->ny + y] > 2 ? 1 : mask_xy[(size_t)x * s->ny + y];
@@ -352,6 +424,7 @@ int ak_set_walls(void *handle, int n, const int32_t *x, const int32_t *y, const 
                  double lattice_speed) {
   Lattice *s = H(handle);
   if (!s) return fail(nullptr, "null lattice handle");
+  s->gen++;  // invalidates captured step graphs
   if (nelem < 1 || nelem > MAX_ELEM) return fail(s, "element count %d outside 1..%d", nelem, MAX_ELEM);
   std::vector<int32_t> xy(n);
   std::vector<int8_t> dir(n), el(n);
@@ -383,6 +456,7 @@ int ak_set_walls(void *handle, int n, const int32_t *x, const int32_t *y, const 
 int ak_init_seeded(void *handle, const double *p) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
+  s->gen++;  // invalidates captured step graphs
   SeedParams sp{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10]};
   const long long n = (long long)s->nx * s->ny * s->nz;
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s->stream, (uint8_t *)s->pop, (long long)s->pop_bytes, (uint8_t)0);
@@ -402,6 +476,7 @@ int ak_init_seeded(void *handle, const double *p) {
 int ak_set_farfield(void *handle, int kind, double U) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
+  s->gen++;  // invalidates captured step graphs
   if (kind < 0 || kind > 3) return fail(s, "unknown farfield kind %d", kind);
   if (kind == AK_FARFIELD_CHARACTERISTIC && s->sweeps % 2 != 0)
     return fail(s, "set the characteristic farfield on the initial state");
@@ -528,7 +603,9 @@ int ak_set_farfield(void *handle, int kind, double U) {
 int ak_set_sweep_tile(void *handle, int bx, int by) {
   Lattice *s = H(handle);
   if (!s) return fail(nullptr, "null lattice handle");
-  if (bx < 64 || bx % 64 || by < 1 || bx * by > 256) return fail(s, "tile %dx%d: bx multiple of 64, bx*by <= 256", bx, by);
+  s->gen++;  // invalidates captured step graphs
+  if (bx < 64 || bx % 64 || by < 1 || bx * by > 1024)
+    return fail(s, "tile %dx%d: bx multiple of 64, bx*by <= 1024", bx, by);
   set_geom(s, bx, by);
   return 0;
 }
@@ -536,17 +613,37 @@ int ak_set_sweep_tile(void *handle, int bx, int by) {
 int ak_run(void *handle, int nsteps, long long first_step, int sample_every) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
+  if (nsteps > 0) set_ctl(s, first_step + 1, s->hist_rows);
   for (int k = 0; k < nsteps; ++k) {
     const long long number = first_step + k + 1;
     const bool sample = sample_every > 0 && number % sample_every == 0;
-    if (sample && s->hist_rows >= s->hist_cap && flush_history(s)) return -1;
-    switch (s->storage) {
-      case 0: one_step<0>(s, sample, (double)number); break;
-      case 1: one_step<1>(s, sample, (double)number); break;
-      default: one_step<2>(s, sample, (double)number); break;
+    if (sample && s->hist_rows >= s->hist_cap) {
+      if (flush_history(s)) return -1;
+      set_ctl(s, number, 0);
     }
+    auto launch = [&] {
+      switch (s->storage) {
+        case 0: one_step<0>(s, sample); break;
+        case 1: one_step<1>(s, sample); break;
+        default: one_step<2>(s, sample); break;
+      }
+    };
+    if (s->use_graphs) {
+      const int slot = (int)(s->sweeps % 2) * 2 + (sample ? 1 : 0);
+      if (graph_step(s, s->stream, s->graphs, slot, s->gen, launch, [&] { s->sweeps++; })) return -1;
+    } else {
+      launch();
+    }
+    if (sample) s->hist_rows++;
   }
   HIPCHK(s, hipGetLastError());
+  return 0;
+}
+
+int ak_set_graph_mode(void *handle, int enable) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  s->use_graphs = enable != 0;
   return 0;
 }
 
@@ -557,10 +654,14 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
   if (check_ready(s)) return -1;
   std::vector<hipEvent_t> ev(4 * (size_t)nsteps);
   for (auto &e : ev) HIPCHK(s, hipEventCreate(&e));
+  if (nsteps > 0) set_ctl(s, first_step + 1, s->hist_rows);
   for (int k = 0; k < nsteps; ++k) {
     const long long number = first_step + k + 1;
     const bool sample = sample_every > 0 && number % sample_every == 0;
-    if (sample && s->hist_rows >= s->hist_cap && flush_history(s)) return -1;
+    if (sample && s->hist_rows >= s->hist_cap) {
+      if (flush_history(s)) return -1;
+      set_ctl(s, number, 0);
+    }
     const bool pe = (s->sweeps % 2) == 0;
     HIPCHK(s, hipEventRecord(ev[4 * k], s->stream));
     switch (s->storage) {
@@ -568,13 +669,7 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
       case 1: if (pe) launch_boundary<1, true>(s, sample); else launch_boundary<1, false>(s, sample); break;
       default: if (pe) launch_boundary<2, true>(s, sample); else launch_boundary<2, false>(s, sample); break;
     }
-    double *row = s->d_hist + (size_t)s->hist_rows * s->hist_width;
-    if (sample) {
-      const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
-      hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, s->stream, s->d_partials,
-                         s->nlinks ? s->wall_blocks : 0, s->nelem, 1.0 / qa, 1.0 / (qa * s->chord), 1.0,
-                         (double)number, row, s->d_diag);
-    }
+    if (sample) launch_loads_final(s);
     HIPCHK(s, hipEventRecord(ev[4 * k + 1], s->stream));
     HIPCHK(s, hipEventRecord(ev[4 * k + 2], s->stream));
     switch (s->storage) {
@@ -584,8 +679,10 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
     }
     HIPCHK(s, hipEventRecord(ev[4 * k + 3], s->stream));
     if (sample) {
-      hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, row);
+      hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, s->d_ctl, s->d_hist);
       s->hist_rows++;
+    } else {
+      hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s->stream, s->d_ctl);
     }
     s->sweeps++;
   }
@@ -635,6 +732,7 @@ int ak_get_populations(void *handle, double *host_out) {
 int ak_set_populations(void *handle, const double *host_in) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
+  s->gen++;  // invalidates captured step graphs
   const size_t n = (size_t)s->nx * s->ny * s->nz * 27;
   double *d = nullptr;
   HIPCHK(s, hipMalloc((void **)&d, n * sizeof(double)));

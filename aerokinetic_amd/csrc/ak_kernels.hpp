@@ -66,7 +66,7 @@ __device__ __forceinline__ void st_g(gchar *p, typename St<S>::C v, float scale)
 }
 
 template <int S, bool ODD, bool PXY, bool DIAG>
-__global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A,
+__global__ __launch_bounds__(PXY ? 256 : 1024) void k_sweep(typename St<S>::T *__restrict__ A,
                                                const uint8_t *__restrict__ solid, Desc L, SweepGeom G,
                                                typename St<S>::C ws, typename St<S>::C wb, float scale,
                                                float inv_scale, DiagAcc *__restrict__ diag) {
@@ -148,8 +148,8 @@ __global__ __launch_bounds__(256) void k_sweep(typename St<S>::T *__restrict__ A
     if (!isfinite(vr)) vr = C(3.0e38);
     vu = wave_max(vu); vd = wave_max(vd); vr = wave_min(vr);
     bad = __any(bad) ? 1 : 0;
-    __shared__ C su[4], sd[4], sr[4];
-    __shared__ int sb[4];
+    __shared__ C su[16], sd[16], sr[16];  // one entry per wave, up to 1024 threads
+    __shared__ int sb[16];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     if (lane == 0) { su[w] = vu; sd[w] = vd; sr[w] = vr; sb[w] = bad; }
     __syncthreads();
@@ -254,8 +254,33 @@ __global__ __launch_bounds__(256) void k_walls(typename St<S>::T *__restrict__ A
 // Sum the per-block wall partials (fixed order), convert to coefficients and write
 // the loads part of a history row; clear the diagnostics accumulator.
 // cl = Fy fscale, cd = Fx fscale, cm = -Mz mscale, mass exchange = mass * vscale.
+// Step number and history-row cursor of a lattice, kept in device memory so that a
+// captured step (HIP graph) replays with the right row and step number.  At the
+// start of a step `step` is that step's number; the kernel that ends a step
+// advances it (and `row` after a sampled step).
+struct StepCtl {
+  long long step;
+  int row, width;
+};
+
+__global__ void k_set_ctl(StepCtl *ctl, long long step, int row, int width) {
+  if (threadIdx.x == 0) {
+    ctl->step = step;
+    ctl->row = row;
+    ctl->width = width;
+  }
+}
+
+// End of an unsampled step.
+__global__ void k_tick(StepCtl *ctl) {
+  if (threadIdx.x == 0) ctl->step += 1;
+}
+
+// Fold the wall-load partials into the current history row.  diag != nullptr:
+// also clear the diagnostic slots for the sweep that follows (k_diag_final ends
+// the step); diag == nullptr: this kernel ends the sampled step.
 __global__ void k_loads_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
-                              double mscale, double vscale, double step, double *__restrict__ row, DiagAcc *diag) {
+                              double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
   __shared__ double acc[MAX_ELEM * 5][64];
   const int lane = threadIdx.x;  // 64 threads
   for (int c = 0; c < nelem * 5; ++c) {
@@ -265,6 +290,7 @@ __global__ void k_loads_final(const double *__restrict__ partials, int nblocks, 
   }
   __syncthreads();
   if (lane == 0) {
+    double *row = hist + (long long)ctl->row * ctl->width;
     double tot[5] = {0, 0, 0, 0, 0};
     for (int e = 0; e < nelem; ++e) {
       double v[5];
@@ -276,12 +302,17 @@ __global__ void k_loads_final(const double *__restrict__ partials, int nblocks, 
       }
       for (int c = 0; c < 4; ++c) row[9 + e * 4 + c] = v[c];
     }
-    row[0] = step;
+    row[0] = (double)ctl->step;
     row[1] = tot[1] * fscale;
     row[2] = tot[0] * fscale;
     row[3] = -tot[3] * mscale;
     row[8] = tot[4] * vscale;
+    if (!diag) {
+      ctl->step += 1;
+      ctl->row += 1;
+    }
   }
+  if (!diag) return;
   for (int k = lane; k < DIAG_SLOTS; k += 64) {
     diag->s[k].u2max = 0ull;
     diag->s[k].drmax = 0ull;
@@ -290,7 +321,7 @@ __global__ void k_loads_final(const double *__restrict__ partials, int nblocks, 
   }
 }
 
-__global__ void k_diag_final(const DiagAcc *diag, double *__restrict__ row) {
+__global__ void k_diag_final(const DiagAcc *diag, StepCtl *ctl, double *__restrict__ hist) {
   // 64 threads fold the slots; then a fixed-order wave reduction
   unsigned long long u = 0ull, d = 0ull, r = (unsigned long long)__double_as_longlong(1e300), b = 0ull;
   for (int k = threadIdx.x; k < DIAG_SLOTS; k += 64) {
@@ -306,10 +337,13 @@ __global__ void k_diag_final(const DiagAcc *diag, double *__restrict__ row) {
     u = u2 > u ? u2 : u; d = d2 > d ? d2 : d; r = r2 < r ? r2 : r; b = b2 > b ? b2 : b;
   }
   if (threadIdx.x == 0) {
+    double *row = hist + (long long)ctl->row * ctl->width;
     row[4] = sqrt(3.0) * sqrt(__longlong_as_double((long long)u));
     row[5] = __longlong_as_double((long long)d);
     row[6] = __longlong_as_double((long long)r);
     row[7] = (double)b;
+    ctl->step += 1;
+    ctl->row += 1;
   }
 }
 

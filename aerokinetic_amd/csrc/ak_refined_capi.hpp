@@ -24,6 +24,9 @@ struct Refined {
   FluxTable *d_T = nullptr;
   int nt = 0, wall_blocks = 0;
   long long steps = 0;
+  hipStream_t own_stream = nullptr;  // created when the caller passes the null stream
+  bool use_graphs = true;
+  GraphCache graphs;  // slot = coarse parity * 4 + fine parity * 2 + sampled
 };
 
 Refined *R(void *h) { return (Refined *)h; }
@@ -114,8 +117,9 @@ void launch_restrict(Refined *r) {
 }
 
 // ev (optional, 8 events): step start/end, coarse sweep start/end, fine sweep 0 and 1 start/end
+// One coarse step; its number and history row come from the coarse lattice's d_ctl.
 template <int S>
-void refined_coarse_step(Refined *r, bool sample, double number, double *row, hipEvent_t *ev = nullptr) {
+void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
   Lattice *c = r->c, *f = r->f;
   if (ev) (void)hipEventRecord(ev[0], c->stream);
   using T_ = typename St<S>::T;
@@ -175,11 +179,13 @@ void refined_coarse_step(Refined *r, bool sample, double number, double *row, hi
     hipLaunchKernelGGL((k_reflux<S, false>), dim3(cwave_blocks), dim3(256), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
   launch_restrict<S>(r);
   if (sample) {
-    // impulse over the two substeps: F dx_f^3 (dx_f/dt_f) / dt_c; q_area = span / 2
+    // impulse over the two substeps: F dx_f^3 (dx_f/dt_f) / dt_c; q_area = span / 2.  The
+    // refined history carries loads only (field diagnostics are read on demand).
     const double fscale = volf * (r->dxf / r->dtf) / r->dtc / (0.5 * r->span);
     hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, c->stream, r->d_part, f->nlinks ? 2 * r->wall_blocks : 0,
-                       f->nelem, fscale, fscale, volf, number, row, c->d_diag);
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, c->stream, c->d_diag, row);
+                       f->nelem, fscale, fscale, volf, c->d_ctl, c->d_hist, (DiagAcc *)nullptr);
+  } else {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
   }
   if (ev) (void)hipEventRecord(ev[1], c->stream);
 }
@@ -201,6 +207,14 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
       return fail(nullptr, "fine box must keep three donor cells inside the domain and span >= 8 cells");
   if (box_lo[2] != 0 || box_hi[2] != nz) return fail(nullptr, "fine box must span the periodic z axis");
   Refined *r = new Refined();
+  if (!stream) {
+    // both levels share one stream (the coarse and fine kernels of a step are ordered on it)
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&r->own_stream) != hipSuccess) {
+      delete r;
+      return fail(nullptr, "hipStreamCreate failed");
+    }
+    stream = r->own_stream;
+  }
   const double tsf = 0.5 + 2 * (tsc - 0.5), tbf = 0.5 + 2 * (tbc - 0.5);
   r->dxc = dxc; r->dtc = dtc; r->dxf = dxc / 2; r->dtf = dtc / 2; r->U = lattice_speed;
   r->span = nz * dxc;
@@ -217,12 +231,14 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
   r->nt = std::max(r->B.nb[0], r->B.nb[1]);
   void *hc = nullptr, *hf = nullptr;
   if (ak_lattice_create(&hc, device, nx, ny, nz, storage, 1.0 / tsc, 1.0 / tbc, fp16_scale, history_capacity, stream)) {
+    if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
     delete r;
     return -1;
   }
   if (ak_lattice_create(&hf, device, 2 * r->B.nb[0] + 4, 2 * r->B.nb[1] + 4, 2 * nz, storage, 1.0 / tsf, 1.0 / tbf,
                         fp16_scale, 16, stream)) {
     ak_lattice_destroy(hc);
+    if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
     delete r;
     return -1;
   }
@@ -237,12 +253,14 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
       dalloc(r->c, &r->d_py, (size_t)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (nz + 6) * 10) ||
       dalloc(r->c, &r->d_mf, (size_t)8 * r->B.nb[0] * r->B.nb[1] * nz * 10) || dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
+    if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
     delete r;
     return -1;
   }
   (void)hipMemset(r->d_err, 0, sizeof(int));
   if (hipMemcpy(r->d_T, &r->T, sizeof(FluxTable), hipMemcpyHostToDevice) != hipSuccess) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
+    if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
     delete r;
     return fail(nullptr, "flux table upload failed");
   }
@@ -253,8 +271,12 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
 int ak_refined_destroy(void *handle) {
   Refined *r = R(handle);
   if (!r) return 0;
+  (void)hipSetDevice(r->c->dev);
+  (void)hipStreamSynchronize(r->c->stream);
+  r->graphs.clear();
   ak_lattice_destroy(r->f);  // partials are owned by the fine lattice
   ak_lattice_destroy(r->c);  // transfer buffers are owned by the coarse lattice
+  if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
   delete r;
   return 0;
 }
@@ -288,6 +310,7 @@ int ak_refined_init_seeded(void *handle, const double *p) {
   for (Lattice *s : {c, f}) {
     hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s->stream, (uint8_t *)s->pop, (long long)s->pop_bytes, (uint8_t)0);
     s->sweeps = 1;  // pre-collision state stored in streamed form: the next sweep is even
+    s->gen++;
     s->hist_rows = 0;
     s->host_hist.clear();
   }
@@ -324,22 +347,41 @@ int ak_refined_init_seeded(void *handle, const double *p) {
 int ak_refined_run(void *handle, int nsteps, long long first_step, int sample_every) {
   Refined *r = R(handle);
   if (!r) return fail(nullptr, "null refined handle");
-  Lattice *c = r->c;
-  if (check_ready(c) || check_ready(r->f)) return -1;
+  Lattice *c = r->c, *f = r->f;
+  if (check_ready(c) || check_ready(f)) return -1;
+  if (nsteps > 0) set_ctl(c, first_step + 1, c->hist_rows);
   for (int k = 0; k < nsteps; ++k) {
     const long long number = first_step + k + 1;
     const bool sample = sample_every > 0 && number % sample_every == 0;
-    if (sample && c->hist_rows >= c->hist_cap && flush_history(c)) return -1;
-    double *row = c->d_hist + (size_t)c->hist_rows * c->hist_width;
-    switch (c->storage) {
-      case 0: refined_coarse_step<0>(r, sample, (double)number, row); break;
-      case 1: refined_coarse_step<1>(r, sample, (double)number, row); break;
-      default: refined_coarse_step<2>(r, sample, (double)number, row); break;
+    if (sample && c->hist_rows >= c->hist_cap) {
+      if (flush_history(c)) return -1;
+      set_ctl(c, number, 0);
+    }
+    auto launch = [&] {
+      switch (c->storage) {
+        case 0: refined_coarse_step<0>(r, sample); break;
+        case 1: refined_coarse_step<1>(r, sample); break;
+        default: refined_coarse_step<2>(r, sample); break;
+      }
+    };
+    if (r->use_graphs) {
+      const int slot = (int)(c->sweeps % 2) * 4 + (int)(f->sweeps % 2) * 2 + (sample ? 1 : 0);
+      const unsigned long long key = (c->gen << 32) ^ f->gen;
+      if (graph_step(c, c->stream, r->graphs, slot, key, launch, [&] { c->sweeps += 1; f->sweeps += 2; })) return -1;
+    } else {
+      launch();
     }
     if (sample) c->hist_rows++;
     r->steps++;
   }
   HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+int ak_refined_set_graph_mode(void *handle, int enable) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  r->use_graphs = enable != 0;
   return 0;
 }
 
@@ -352,15 +394,18 @@ int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int 
   if (check_ready(c) || check_ready(r->f)) return -1;
   std::vector<hipEvent_t> ev(8 * (size_t)nsteps);
   for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  if (nsteps > 0) set_ctl(c, first_step + 1, c->hist_rows);
   for (int k = 0; k < nsteps; ++k) {
     const long long number = first_step + k + 1;
     const bool sample = sample_every > 0 && number % sample_every == 0;
-    if (sample && c->hist_rows >= c->hist_cap && flush_history(c)) return -1;
-    double *row = c->d_hist + (size_t)c->hist_rows * c->hist_width;
+    if (sample && c->hist_rows >= c->hist_cap) {
+      if (flush_history(c)) return -1;
+      set_ctl(c, number, 0);
+    }
     switch (c->storage) {
-      case 0: refined_coarse_step<0>(r, sample, (double)number, row, &ev[8 * k]); break;
-      case 1: refined_coarse_step<1>(r, sample, (double)number, row, &ev[8 * k]); break;
-      default: refined_coarse_step<2>(r, sample, (double)number, row, &ev[8 * k]); break;
+      case 0: refined_coarse_step<0>(r, sample, &ev[8 * k]); break;
+      case 1: refined_coarse_step<1>(r, sample, &ev[8 * k]); break;
+      default: refined_coarse_step<2>(r, sample, &ev[8 * k]); break;
     }
     if (sample) c->hist_rows++;
     r->steps++;

@@ -38,10 +38,10 @@ class Lattice:
         _lib.check(rc, None, "ak_lattice_create")
         self.h = h
         self.nbytes = int(lib.ak_population_bytes(h))
-        # caller-owned population buffer; 64-byte aligned view
-        self._buf = torch.empty(self.nbytes + 64, dtype=torch.uint8, device=self.torch_device)
+        # caller-owned population buffer; 128-byte aligned view
+        self._buf = torch.empty(self.nbytes + 128, dtype=torch.uint8, device=self.torch_device)
         base = self._buf.data_ptr()
-        self._offset = (-base) % 64
+        self._offset = (-base) % 128
         _lib.check(lib.ak_bind_populations(h, _lib._vp(base + self._offset), self.nbytes), h, "ak_bind_populations")
         self.hist_width = int(lib.ak_history_width(h))
 
@@ -82,6 +82,10 @@ class Lattice:
         arr = np.array([p["U"], p["amp"], p["phase_y"], p["phase_z"], p["ky"], p["kz"], p["lex"], p["ley"],
                         origin[0], origin[1], dx], dtype=np.float64)
         _lib.check(self.lib.ak_init_seeded(self.h, _lib.dptr(arr)), self.h, "ak_init_seeded")
+
+    def set_graphs(self, enable):
+        """Replay each step variant from a captured HIP graph (default) or launch directly."""
+        _lib.check(self.lib.ak_set_graph_mode(self.h, 1 if enable else 0), self.h, "ak_set_graph_mode")
 
     def set_tile(self, bx, by):
         _lib.check(self.lib.ak_set_sweep_tile(self.h, int(bx), int(by)), self.h, "ak_set_sweep_tile")

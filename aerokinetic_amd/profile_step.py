@@ -1,0 +1,104 @@
+"""Run a fixed number of steps of a named workload (a short driver for profilers).
+
+    python -m aerokinetic_amd.profile_step --workload uniform-100m --storage fp16 --steps 10
+    python -m aerokinetic_amd.profile_step --workload two-level --storage fp64 --steps 40
+    python -m aerokinetic_amd.profile_step --workload copy --copy-gib 4 --steps 10
+
+Workloads: `uniform-c32` (384x256x8), `uniform-100m` (1536x1024x64) and
+`two-level` (coarse 256x192x8 + fine 160x96x16), all NACA0012 (see
+aerokinetic_amd.cases), and `copy`, a device-to-device copy of a buffer of known
+size (the calibration kernel for memory counters).  Each workload prints one
+line per phase with HIP-event timings of the lattice's stream, so a profiler's
+per-kernel averages can be set beside them.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+
+
+def _log(msg):
+    print(f"[profile_step] {msg}", flush=True)
+
+
+def run_uniform(case, storage, steps, warmup, sample_every):
+    import numpy as np
+    import torch
+    from .solver import Solver
+    section, flow, grid, plan = case.build()
+    s = Solver(section, flow, grid, storage=storage, plan=plan)
+    fluid = int(s._fluid.sum())
+    _log(f"{case.name} storage {storage}: {int(np.prod(plan.shape))} cells, {fluid} fluid")
+    s.advance(warmup, sample_every)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.advance(steps, sample_every)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    sweep_ms, bnd_ms = s.lattice.run_profiled(steps, s.steps, sample_every)
+    s.steps += steps
+    s.lattice.read_history()
+    _log(f"steps {steps}: wall {dt / steps * 1e3:.4f} ms/step (graph replay); direct launches with events: "
+         f"sweep {sweep_ms / steps:.4f} ms, boundary {bnd_ms / steps:.4f} ms")
+    s.lattice.close()
+
+
+def run_two_level(case, storage, steps, warmup, sample_every):
+    from .refined import RefinedSolver
+    project, settings, plan = case.build()
+    s = RefinedSolver(project, settings, plan=plan, storage=storage)
+    _log(f"{case.name} storage {storage}")
+    import torch
+    s.advance(warmup, sample_every)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.advance(steps, sample_every)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tot, cs, fs = s.run_profiled(steps, sample_every)
+    _log(f"steps {steps}: wall {dt / steps * 1e3:.4f} ms/step (graph replay); direct launches with events "
+         f"{tot / steps:.4f} ms/step, coarse sweep {cs / steps:.4f} ms, fine sweep {fs / (2 * steps):.4f} ms "
+         f"per substep")
+    s.close()
+
+
+def run_copy(gib, steps):
+    import torch
+    n = int(gib * 2**30)
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    _log(f"copy {n} bytes: {ms:.4f} ms per copy, {2 * n / ms / 1e6:.1f} GB/s (read + write)")
+
+
+def main(argv=None):
+    from . import cases
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--workload", choices=("uniform-c32", "uniform-100m", "two-level", "copy"), required=True)
+    ap.add_argument("--storage", choices=("fp64", "fp32", "fp16"), default="fp64")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--sample-every", type=int, default=1)
+    ap.add_argument("--copy-gib", type=float, default=4.0)
+    a = ap.parse_args(argv)
+    if a.workload == "copy":
+        run_copy(a.copy_gib, a.steps)
+    elif a.workload == "two-level":
+        run_two_level(cases.NACA0012_REFINED_256x192x8, a.storage, a.steps, a.warmup, a.sample_every)
+    else:
+        case = cases.NACA0012_C32_384x256x8 if a.workload == "uniform-c32" else cases.NACA0012_C128_1536x1024x64
+        run_uniform(case, a.storage, a.steps, a.warmup, a.sample_every)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

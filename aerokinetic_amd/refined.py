@@ -34,25 +34,7 @@ _vp, _dp, _ip = _lib._vp, _lib._dp, _lib._ip
 
 
 def _bind():
-    L = _lib.load()
-    sig = {
-        "ak_refined_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                             _ip, _ip, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                                             _vp]),
-        "ak_refined_destroy": (ctypes.c_int, [_vp]),
-        "ak_refined_level": (_vp, [_vp, ctypes.c_int]),
-        "ak_refined_set_walls": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _dp, _ip, ctypes.c_int,
-                                                ctypes.c_double, ctypes.c_double]),
-        "ak_refined_init_seeded": (ctypes.c_int, [_vp, _dp]),
-        "ak_refined_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int]),
-        "ak_refined_transfer_errors": (ctypes.c_int, [_vp]),
-        "ak_refined_run_profiled": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _dp]),
-        "ak_refined_get_populations": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
-    }
-    for name, (res, args) in sig.items():
-        fn = getattr(L, name)
-        fn.restype, fn.argtypes = res, args
-    return L
+    return _lib.load()
 
 
 class TransferError(ValueError):
@@ -64,7 +46,7 @@ class RefinedSolver:
     pad = 2
 
     def __init__(self, project: Project, refinement=RefinementSettings(), progress=None, *, storage="fp64",
-                 device=0, base_plan=None, plan=None, fp16_scale=0.0):
+                 device=0, base_plan=None, plan=None, fp16_scale=0.0, graphs=True):
         import torch
         if not torch.cuda.is_available():
             raise _lib.NativeLibraryError("no HIP device visible to PyTorch-ROCm")
@@ -90,12 +72,13 @@ class RefinedSolver:
                                             _lib.STORE[storage], _lib.dptr(lev), self.flow.lattice_speed, fp16_scale,
                                             4096, _vp(stream.cuda_stream)), None, "ak_refined_create")
         self.h = h
+        _lib.check(self.L.ak_refined_set_graph_mode(h, 1 if graphs else 0), None, "ak_refined_set_graph_mode")
         self._levels = [self.L.ak_refined_level(h, 0), self.L.ak_refined_level(h, 1)]
         self._bufs = []
         for lvl in self._levels:
             n = int(self.L.ak_population_bytes(lvl))
-            buf = torch.empty(n + 64, dtype=torch.uint8, device=torch.device("cuda", device))
-            off = (-buf.data_ptr()) % 64
+            buf = torch.empty(n + 128, dtype=torch.uint8, device=torch.device("cuda", device))
+            off = (-buf.data_ptr()) % 128
             _lib.check(self.L.ak_bind_populations(lvl, _vp(buf.data_ptr() + off), n), lvl, "ak_bind_populations")
             self._bufs.append(buf)
         # cell masks: coarse covered cells are computed donors (2); fine solid (1), donor layers (2)

@@ -62,7 +62,7 @@ class Solver:
     HISTORY_COLUMNS = ["step", "t_star", "time_s", "cl", "cd", "cm", "local_mach_max", "density_error_max"]
 
     def __init__(self, section: Section, flow: Flow = Flow(), grid: Grid = Grid(), progress=None, *,
-                 storage="fp64", device=0, plan: Plan | None = None, fp16_scale=0.0, tile=None):
+                 storage="fp64", device=0, plan: Plan | None = None, fp16_scale=0.0, tile=None, graphs=True):
         self.section, self.flow, self.grid = section, flow, grid
         if grid.farfield == EXTERIOR_FARFIELD:
             raise ValueError("exterior_regularized_fd_v1 is not provided by this package; "
@@ -88,6 +88,7 @@ class Solver:
                                omega_bulk=1.0, fp16_scale=fp16_scale, device=device)
         if tile:
             self.lattice.set_tile(*tile)
+        self.lattice.set_graphs(graphs)
         self.lattice.set_solid(self._solid_xy)
         self.links, fallback = wall_links(self.contours, (nx, ny), self.plan.origin, self.plan.dx)
         self.wall_fallback_links = fallback * nz

@@ -27,6 +27,9 @@
  *                            (solver.py:281-297)
  *   ak_get_plane             the mid-span gather of save_frame (solver.py:312-334)
  *   ak_get/set_populations   gather_array / cpu_arrays of the pdf field (checkpoint.py:32-68)
+ *   ak_refined_*             RefinedSolver + D3Q27Transfer + StreamingFaceFlux: the two-level
+ *                            2:1 sub-cycled step (refined_solver.py:116-345,
+ *                            interface.py:116-352, refinement.py:117-254)
  *
  * Population exchange layout (host, FP64): g[x][y][z][27], zero-centred
  * g = f - w, direction i = 9 (cx+1) + 3 (cy+1) + (cz+1).
@@ -64,7 +67,7 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
 int ak_lattice_destroy(void *handle);
 
 /* Bytes the caller must provide for the population buffer, and binding of a
- * caller-owned device buffer (e.g. a torch tensor's data_ptr()). */
+ * caller-owned, 128-byte aligned device buffer (e.g. a torch tensor's data_ptr()). */
 size_t ak_population_bytes(void *handle);
 int ak_bind_populations(void *handle, void *device_ptr, size_t bytes);
 
@@ -95,6 +98,13 @@ int ak_set_sweep_tile(void *handle, int bx, int by);
  * (k = 0..nsteps-1; sample_every <= 0 disables) record a history row. */
 int ak_run(void *handle, int nsteps, long long first_step, int sample_every);
 
+/* Step graphs (default on): ak_run captures each launch variant of a step (AA parity,
+ * sampled or not) as a HIP graph on first use and replays it; any setter call
+ * (ak_set_*, ak_bind_populations, ak_init_seeded) invalidates the captured graphs.
+ * enable = 0 launches every kernel directly.  Replaces the per-step Python kernel
+ * dispatch of Solver.step (solver.py:252-279). */
+int ak_set_graph_mode(void *handle, int enable);
+
 /* ak_run with HIP events on the lattice's stream around every sweep and every
  * boundary phase (measurement only; synchronises at the end).
  * out = {total sweep milliseconds, total boundary milliseconds, steps}. */
@@ -121,6 +131,57 @@ int ak_get_plane(void *handle, int z, double *rho, double *u);
 
 /* Wait for all work of this lattice. */
 int ak_synchronize(void *handle);
+
+/* ---- Two-level 2:1 refined step -------------------------------------------------------------
+ * One refined handle owns a coarse lattice (nx*ny*nz, legacy farfield, z periodic) and a fine
+ * lattice over the half-open coarse box [box_lo, box_hi) in x/y (full span in z), padded by two
+ * fine cells on every x/y side.  Stored state on both levels is pre-collision
+ * (collide_stream order).  One coarse step = donor moments, coarse sweep, coarse face flux and
+ * farfield, two fine substeps (time-interpolated prolongation, fine sweep, fine face flux,
+ * walls + loads), reflux of the face-flux mismatch, restriction into the covered coarse cells. */
+
+/* RefinedSolver.__init__ / _build (refined_solver.py:116-215, interface.py:116-130,230-283).
+ * level = {dx_c, dt_c, tau_shear_c, tau_bulk_c} of the coarse level (the fine level is its
+ * acoustic refinement, refinement.py:58-65); lattice_speed = U (coarse lattice units). */
+int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const int32_t *box_lo,
+                      const int32_t *box_hi, int storage, const double *level, double lattice_speed,
+                      double fp16_scale, int history_capacity, void *stream);
+int ak_refined_destroy(void *handle);
+
+/* Lattice handle of one level (0 coarse, 1 fine) for the uniform entry points
+ * (populations binding, masks, diagnostics, history reads). */
+void *ak_refined_level(void *handle, int level);
+
+/* Fine-level Bouzidi links in padded fine coordinates (refined_solver.py:186-203,
+ * walls.py:21-49); lever arms about (origin_fx, origin_fy), common units. */
+int ak_refined_set_walls(void *handle, int n, const int32_t *x, const int32_t *y, const int32_t *d,
+                         const double *q, const int32_t *element, int nelem, double origin_fx,
+                         double origin_fy);
+
+/* Seeded initial field on both levels, restriction into the covered coarse cells and the
+ * legacy farfield state (refined_solver.py:216-247, solver.py:157-202).
+ * p = {U, amp, phase_y, phase_z, ky, kz, le_x, le_y, origin_x, origin_y}. */
+int ak_refined_init_seeded(void *handle, const double *p);
+
+/* nsteps coarse steps (RefinedSolver.step, refined_solver.py:311-345, interface.py:311-352);
+ * loads and diagnostics of the sampled steps go to the coarse level's history rows. */
+int ak_refined_run(void *handle, int nsteps, long long first_step, int sample_every);
+
+/* Graph replay of the whole coarse step (both substeps and all transfers), as
+ * ak_set_graph_mode; replaces the Python orchestration of _step_once
+ * (interface.py:311-352) and RefinedSolver.step (refined_solver.py:311-345). */
+int ak_refined_set_graph_mode(void *handle, int enable);
+
+/* ak_refined_run with HIP events (measurement; synchronises):
+ * out = {total ms, coarse sweep ms, fine sweep ms, steps}. */
+int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int sample_every, double *out);
+
+/* Transfer guard flags raised since the last call, then cleared (refinement.py:159-163,187-202):
+ * 1 density <= 0, 2 negative population, 4 local Mach > 0.3, 8 non-finite. */
+int ak_refined_transfer_errors(void *handle);
+
+/* Pre-collision populations of one level (0 coarse, 1 fine padded), exchange layout. */
+int ak_refined_get_populations(void *handle, int level, double *host_out);
 
 /* Last error message of this handle (or of the library when handle is NULL). */
 int ak_last_error(void *handle, char *buf, size_t len);

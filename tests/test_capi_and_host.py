@@ -1,6 +1,7 @@
 """CPU-side checks: the C ABI library loads and exports every declared symbol; host
 logic (case model, wall links) against the reference's rules and the oracle."""
 import re
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -19,7 +20,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def declared_symbols():
     text = (ROOT / "include" / "aerokinetic.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char \*)\s*(ak_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char \*|void \*)\s*(ak_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -29,6 +30,9 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) == set(_lib.exported_symbols())
+    dyn = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True, check=True)
+    exported = {ln.split()[-1] for ln in dyn.stdout.splitlines() if " T ak_" in ln}
+    assert exported == set(names)
     assert b"gfx950" in lib.ak_version()
 
 
