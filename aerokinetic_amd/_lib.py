@@ -60,6 +60,7 @@ _SIGS = {
     "ak_synchronize": (ctypes.c_int, [_vp]),
     "ak_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
     "ak_version": (ctypes.c_char_p, []),
+    "ak_device_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     "ak_refined_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _ip, _ip, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          _vp]),

@@ -7,6 +7,7 @@
 // row, and the sweep folds the diagnostics (max |u|, max |rho-1|, min rho,
 // non-finite flag) into the same row.  No host synchronisation in the step path.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -138,8 +139,10 @@ void set_geom(Lattice *s, int bx, int by) {
 
 // ---------------------------------------------------------------- launches
 
+// With e0/e1 the launch records the kernel's own start and end on those events
+// (hipExtLaunchKernel), so the measured time is the sweep alone.
 template <int S>
-void launch_sweep(Lattice *s, bool odd, bool diag) {
+void launch_sweep(Lattice *s, bool odd, bool diag, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   using T = typename St<S>::T;
   using C = typename St<S>::C;
   T *A = (T *)s->pop;
@@ -148,8 +151,15 @@ void launch_sweep(Lattice *s, bool odd, bool diag) {
   const SweepGeom &G = pxy && s->geom.bx * s->geom.by > 256 ? s->geom_small : s->geom;
   const dim3 grid(G.nblocks), block(G.bx * G.by);
   const C ws = (C)s->ws, wb = (C)s->wb;
-#define AK_SWEEP(O, P, D) \
-  hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, A, s->d_solid, s->L, G, ws, wb, s->scale, s->inv_scale, s->d_diag)
+#define AK_SWEEP(O, P, D)                                                                                           \
+  do {                                                                                                              \
+    if (e0)                                                                                                         \
+      hipExtLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, e0, e1, 0, A, s->d_solid, s->L, G, ws, \
+                            wb, s->scale, s->inv_scale, s->d_diag);                                                 \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, A, s->d_solid, s->L, G, ws, wb, s->scale, \
+                         s->inv_scale, s->d_diag);                                                                  \
+  } while (0)
   if (odd) {
     if (pxy) { if (diag) AK_SWEEP(true, true, true); else AK_SWEEP(true, true, false); }
     else { if (diag) AK_SWEEP(true, false, true); else AK_SWEEP(true, false, false); }
@@ -647,8 +657,9 @@ int ak_set_graph_mode(void *handle, int enable) {
   return 0;
 }
 
-// Same launches as ak_run, with HIP events around each sweep and each boundary phase
-// on the lattice's stream.  out = {sweep ms total, boundary ms total, sweeps}.
+// Same launches as ak_run, launched directly, with HIP events around each boundary phase
+// on the lattice's stream and on each sweep launch itself (the kernel's own start and
+// end).  out = {sweep ms total, boundary ms total, sweeps}.
 int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_every, double *out) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
@@ -671,13 +682,11 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
     }
     if (sample) launch_loads_final(s);
     HIPCHK(s, hipEventRecord(ev[4 * k + 1], s->stream));
-    HIPCHK(s, hipEventRecord(ev[4 * k + 2], s->stream));
     switch (s->storage) {
-      case 0: launch_sweep<0>(s, pe, sample); break;
-      case 1: launch_sweep<1>(s, pe, sample); break;
-      default: launch_sweep<2>(s, pe, sample); break;
+      case 0: launch_sweep<0>(s, pe, sample, ev[4 * k + 2], ev[4 * k + 3]); break;
+      case 1: launch_sweep<1>(s, pe, sample, ev[4 * k + 2], ev[4 * k + 3]); break;
+      default: launch_sweep<2>(s, pe, sample, ev[4 * k + 2], ev[4 * k + 3]); break;
     }
-    HIPCHK(s, hipEventRecord(ev[4 * k + 3], s->stream));
     if (sample) {
       hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, s->d_ctl, s->d_hist);
       s->hist_rows++;
@@ -746,25 +755,6 @@ int ak_set_populations(void *handle, const double *host_in) {
   return 0;
 }
 
-int ak_get_macroscopic(void *handle, double *rho, double *u) {
-  Lattice *s = H(handle);
-  if (check_ready(s)) return -1;
-  const size_t ncell = (size_t)s->nx * s->ny * s->nz;
-  std::vector<double> g(ncell * 27);
-  if (ak_get_populations(handle, g.data())) return -1;
-  for (size_t c = 0; c < ncell; ++c) {
-    double r = 0, jx = 0, jy = 0, jz = 0;
-    for (int j = 0; j < 27; ++j) {
-      const double v = g[c * 27 + j];
-      r += v; jx += cxq(j) * v; jy += cyq(j) * v; jz += czq(j) * v;
-    }
-    r += 1.0;
-    rho[c] = r;
-    u[3 * c] = jx / r; u[3 * c + 1] = jy / r; u[3 * c + 2] = jz / r;
-  }
-  return 0;
-}
-
 }  // extern "C"
 
 namespace {
@@ -824,6 +814,31 @@ int ak_diagnostics(void *handle, double *out) {
   return 0;
 }
 
+int ak_get_macroscopic(void *handle, double *rho, double *u) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  // one z plane at a time on the device (the plane kernel), scattered to [x][y][z] on the host
+  const size_t n = (size_t)s->nx * s->ny;
+  std::vector<double> h(4 * n);
+  double *d = nullptr;
+  HIPCHK(s, hipMalloc((void **)&d, n * 4 * sizeof(double)));
+  hipError_t e = hipSuccess;
+  for (int z = 0; z < s->nz && e == hipSuccess; ++z) {
+    AK_DISPATCH(s, plane_any, s, z, d, d + n);
+    e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d, 4 * n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) break;
+    for (size_t c = 0; c < n; ++c) {
+      const size_t o = c * s->nz + z;
+      rho[o] = h[c];
+      u[3 * o] = h[n + 3 * c]; u[3 * o + 1] = h[n + 3 * c + 1]; u[3 * o + 2] = h[n + 3 * c + 2];
+    }
+  }
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(s, "macroscopic export: %s", hipGetErrorString(e));
+  return 0;
+}
+
 int ak_get_plane(void *handle, int z, double *rho, double *u) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
@@ -845,6 +860,21 @@ int ak_synchronize(void *handle) {
   if (!s) return fail(nullptr, "null lattice handle");
   HIPCHK(s, hipSetDevice(s->dev));
   HIPCHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ak_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
+  if (!dst || !src) return fail(nullptr, "null copy buffer");
+  if (((uintptr_t)dst | (uintptr_t)src | bytes) % 8) return fail(nullptr, "copy needs 8-byte aligned buffers and size");
+  // chunks of 2^33 bytes keep the grid below 2^32 work-items
+  const size_t chunk = (size_t)1 << 33;
+  for (size_t off = 0; off < bytes; off += chunk) {
+    const long long n = (long long)(std::min(chunk, bytes - off) / 8);
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks_for(n, 512)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint64_t *)((const char *)src + off), (uint64_t *)((char *)dst + off), n);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(nullptr, "ak_device_copy: %s", hipGetErrorString(e));
   return 0;
 }
 

@@ -169,6 +169,20 @@ __global__ __launch_bounds__(PXY ? 256 : 1024) void k_sweep(typename St<S>::T *_
   }
 }
 
+// ------------------------------------------------------------------ copy
+// Streaming device copy: two 8-byte words per thread, both loads issued before the
+// stores (the attainable HBM read+write bandwidth, and the calibration kernel for the
+// memory counters).
+__global__ __launch_bounds__(256) void k_stream_copy(const uint64_t *__restrict__ a, uint64_t *__restrict__ b,
+                                                     long long n) {
+  const long long i = blockIdx.x * 512LL + threadIdx.x, j = i + 256;
+  uint64_t v0 = 0, v1 = 0;
+  if (i < n) v0 = a[i];
+  if (j < n) v1 = a[j];
+  if (i < n) b[i] = v0;
+  if (j < n) b[j] = v1;
+}
+
 // ------------------------------------------------------------------ walls
 // Bouzidi interpolated bounce-back over the compacted fluid->solid link list, fused
 // with the momentum-exchange loads.  One thread per (link, z).  Per-block sums of

@@ -117,6 +117,7 @@ void launch_restrict(Refined *r) {
 }
 
 // ev (optional, 8 events): step start/end, coarse sweep start/end, fine sweep 0 and 1 start/end
+// (the sweep events hold the kernels' own start and end times)
 // One coarse step; its number and history row come from the coarse lattice's d_ctl.
 template <int S>
 void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
@@ -131,9 +132,7 @@ void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
   else
     hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_before, r->d_err);
   // coarse collide (AA sweep gathers the pre-collision state)
-  if (ev) (void)hipEventRecord(ev[2], c->stream);
-  launch_sweep<S>(c, c->sweeps % 2 == 0, false);
-  if (ev) (void)hipEventRecord(ev[3], c->stream);
+  launch_sweep<S>(c, c->sweeps % 2 == 0, false, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
   c->sweeps++;
   const bool cpe = c->sweeps % 2 == 0;
   const double volc = r->dxc * r->dxc * r->dxc, volf = r->dxf * r->dxf * r->dxf;
@@ -158,9 +157,7 @@ void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
       hipLaunchKernelGGL((k_prolong_z_inject<S, true>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
     else
       hipLaunchKernelGGL((k_prolong_z_inject<S, false>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
-    if (ev) (void)hipEventRecord(ev[4 + 2 * sub], f->stream);
-    launch_sweep<S>(f, f->sweeps % 2 == 0, false);
-    if (ev) (void)hipEventRecord(ev[5 + 2 * sub], f->stream);
+    launch_sweep<S>(f, f->sweeps % 2 == 0, false, ev ? ev[4 + 2 * sub] : nullptr, ev ? ev[5 + 2 * sub] : nullptr);
     f->sweeps++;
     WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
     double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;

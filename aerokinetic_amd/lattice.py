@@ -134,6 +134,13 @@ class Lattice:
         _lib.check(self.lib.ak_diagnostics(self.h, _lib.dptr(out)), self.h, "ak_diagnostics")
         return out
 
+    def macroscopic(self):
+        """rho[x][y][z] and u[x][y][z][3] of the interior (FP64 host arrays)."""
+        rho = np.zeros(self.shape)
+        u = np.zeros((*self.shape, 3))
+        _lib.check(self.lib.ak_get_macroscopic(self.h, _lib.dptr(rho), _lib.dptr(u)), self.h, "ak_get_macroscopic")
+        return rho, u
+
     def plane(self, z):
         nx, ny, _ = self.shape
         rho = np.zeros((nx, ny))

@@ -22,12 +22,12 @@ def _log(msg):
     print(f"[profile_step] {msg}", flush=True)
 
 
-def run_uniform(case, storage, steps, warmup, sample_every):
+def run_uniform(case, storage, steps, warmup, sample_every, graphs=True):
     import numpy as np
     import torch
     from .solver import Solver
     section, flow, grid, plan = case.build()
-    s = Solver(section, flow, grid, storage=storage, plan=plan)
+    s = Solver(section, flow, grid, storage=storage, plan=plan, graphs=graphs)
     fluid = int(s._fluid.sum())
     _log(f"{case.name} storage {storage}: {int(np.prod(plan.shape))} cells, {fluid} fluid")
     s.advance(warmup, sample_every)
@@ -39,15 +39,15 @@ def run_uniform(case, storage, steps, warmup, sample_every):
     sweep_ms, bnd_ms = s.lattice.run_profiled(steps, s.steps, sample_every)
     s.steps += steps
     s.lattice.read_history()
-    _log(f"steps {steps}: wall {dt / steps * 1e3:.4f} ms/step (graph replay); direct launches with events: "
+    _log(f"steps {steps}: wall {dt / steps * 1e3:.4f} ms/step (graphs {graphs}); direct launches with events: "
          f"sweep {sweep_ms / steps:.4f} ms, boundary {bnd_ms / steps:.4f} ms")
     s.lattice.close()
 
 
-def run_two_level(case, storage, steps, warmup, sample_every):
+def run_two_level(case, storage, steps, warmup, sample_every, graphs=True):
     from .refined import RefinedSolver
     project, settings, plan = case.build()
-    s = RefinedSolver(project, settings, plan=plan, storage=storage)
+    s = RefinedSolver(project, settings, plan=plan, storage=storage, graphs=graphs)
     _log(f"{case.name} storage {storage}")
     import torch
     s.advance(warmup, sample_every)
@@ -57,27 +57,40 @@ def run_two_level(case, storage, steps, warmup, sample_every):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     tot, cs, fs = s.run_profiled(steps, sample_every)
-    _log(f"steps {steps}: wall {dt / steps * 1e3:.4f} ms/step (graph replay); direct launches with events "
+    _log(f"steps {steps}: wall {dt / steps * 1e3:.4f} ms/step (graphs {graphs}); direct launches with events "
          f"{tot / steps:.4f} ms/step, coarse sweep {cs / steps:.4f} ms, fine sweep {fs / (2 * steps):.4f} ms "
          f"per substep")
     s.close()
 
 
-def run_copy(gib, steps):
+def copy_bandwidth(gib, steps):
+    """(ms per copy, GB/s read + write) of the library's streaming copy of `gib` GiB."""
     import torch
-    n = int(gib * 2**30)
-    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    from . import _lib
+    L = _lib.load()
+    n = int(gib * 2**30) // 8 * 8
+    src = torch.ones(n // 8, dtype=torch.int64, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
+    st = torch.cuda.current_stream()
+    copy = lambda: _lib.check(L.ak_device_copy(_lib._vp(dst.data_ptr()), _lib._vp(src.data_ptr()), n,
+                                               _lib._vp(st.cuda_stream)), None, "ak_device_copy")
+    copy()
     torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
+    a.record(st)
     for _ in range(steps):
-        dst.copy_(src)
-    b.record()
+        copy()
+    b.record(st)
     torch.cuda.synchronize()
+    if not torch.equal(dst[:1024], src[:1024]):
+        raise RuntimeError("device copy mismatch")
     ms = a.elapsed_time(b) / steps
-    _log(f"copy {n} bytes: {ms:.4f} ms per copy, {2 * n / ms / 1e6:.1f} GB/s (read + write)")
+    return ms, 2 * n / ms / 1e6
+
+
+def run_copy(gib, steps):
+    ms, gbs = copy_bandwidth(gib, steps)
+    _log(f"copy {int(gib * 2**30)} bytes: {ms:.4f} ms per copy, {gbs:.1f} GB/s (read + write)")
 
 
 def main(argv=None):
@@ -89,14 +102,15 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--sample-every", type=int, default=1)
     ap.add_argument("--copy-gib", type=float, default=4.0)
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly")
     a = ap.parse_args(argv)
     if a.workload == "copy":
         run_copy(a.copy_gib, a.steps)
     elif a.workload == "two-level":
-        run_two_level(cases.NACA0012_REFINED_256x192x8, a.storage, a.steps, a.warmup, a.sample_every)
+        run_two_level(cases.NACA0012_REFINED_256x192x8, a.storage, a.steps, a.warmup, a.sample_every, not a.no_graphs)
     else:
         case = cases.NACA0012_C32_384x256x8 if a.workload == "uniform-c32" else cases.NACA0012_C128_1536x1024x64
-        run_uniform(case, a.storage, a.steps, a.warmup, a.sample_every)
+        run_uniform(case, a.storage, a.steps, a.warmup, a.sample_every, not a.no_graphs)
     return 0
 
 

@@ -21,9 +21,18 @@ uniform_sweep: the same step on a uniform 1536x1024x64 lattice (100.7M cells,
 NACA0012 chord 128 at 8 deg, Re 1e5, characteristic farfield) in FP64, FP32 and
 FP16 shifted storage -- the HBM-roofline view of the fused AA sweep at full size.
 
+copy: the library's streaming device copy of 4 GiB (attainable HBM read+write
+bandwidth on this GPU), reported beside the vendor peak.
+
+traffic: HBM bytes per launch of the dominant kernel from the memory counters
+(FETCH_SIZE, WRITE_SIZE per dispatch), each scaled by what the counter reads for the
+streaming copy of a known size; measured separately (the counters serialise the
+GPU) and recorded below with the command that produced them.
+
 cpu_baseline: the FP64 oracle (C/OpenMP lattices + NumPy transfer, the
 reference's data flow) on the host, same two-level case, a bounded number of
-coarse steps.
+coarse steps, all host cores; beside it the uniform 384x256x8 oracle step on all
+cores and on one core.
 """
 from __future__ import annotations
 
@@ -39,6 +48,20 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
 BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
+
+# HBM bytes per launch from the memory counters (rocprofv3 --pmc FETCH_SIZE, then
+# WRITE_SIZE, KiB per dispatch; FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors that
+# map the counters onto the 8 GiB read + 8 GiB written by the streaming copy), for the
+# kernels whose roofline is reported: (workload, storage, kernel) -> bytes.
+MEASURED_TRAFFIC = {  # profiles/hbm_traffic.md (mean over both AA parities)
+    ('two-level', 'fp64', 'fine k_sweep'): 116681598,
+    ('two-level', 'fp64', 'coarse k_sweep'): 173193914,
+    ('two-level', 'fp32', 'fine k_sweep'): 62157757,
+    ('two-level', 'fp32', 'coarse k_sweep'): 87728733,
+    ('uniform-100m', 'fp16', 'k_sweep'): 11171226912,
+    ('uniform-100m', 'fp32', 'k_sweep'): 22250849507,
+    ('uniform-100m', 'fp64', 'k_sweep'): 44563787059,
+}
 METRIC = "MLUPS & %HBM roofline, FP64 vs FP16-store; cases/hour + p50/p99 latency"
 
 
@@ -46,13 +69,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def roofline(bytes_per_launch, seconds_per_launch):
+def roofline(bytes_per_launch, seconds_per_launch, traffic_key=None, copy_gbs=None):
     achieved = bytes_per_launch / seconds_per_launch / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "traffic": None, "algorithmic_bytes_per_launch": bytes_per_launch}
+    out = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+           "frac": achieved / PEAK_HBM_GBS, "traffic": MEASURED_TRAFFIC.get(traffic_key),
+           "algorithmic_bytes_per_launch": bytes_per_launch, "launch_ms": seconds_per_launch * 1e3}
+    if copy_gbs:
+        out["frac_of_copy"] = achieved / copy_gbs
+    return out
 
 
-def refined_gpu(case, storage, steps, warmup, profile_steps):
+def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
     import numpy as np
     import torch
     from aerokinetic_amd.refined import RefinedSolver
@@ -77,8 +104,10 @@ def refined_gpu(case, storage, steps, warmup, profile_steps):
            "updates_per_step": updates, "coarse_cells": coarse, "fine_owned_cells": fine_owned,
            "event_ms_per_step": tot / profile_steps, "coarse_sweep_ms": coarse_sweep * 1e3,
            "fine_sweep_ms": fine_sweep * 1e3,
-           "roofline": roofline(fine_cells * 2 * 27 * b, fine_sweep),
-           "coarse_sweep_roofline": roofline(coarse * 2 * 27 * b, coarse_sweep),
+           "fine_sweep_cells": fine_cells,
+           "roofline": roofline(fine_cells * 2 * 27 * b, fine_sweep, ("two-level", storage, "fine k_sweep"), copy_gbs),
+           "coarse_sweep_roofline": roofline(coarse * 2 * 27 * b, coarse_sweep, ("two-level", storage, "coarse k_sweep"),
+                                             copy_gbs),
            "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
            "diag": s.diagnostics()}
     s.close()
@@ -86,7 +115,7 @@ def refined_gpu(case, storage, steps, warmup, profile_steps):
     return out
 
 
-def uniform_gpu(case, storage, steps, warmup, profile_steps):
+def uniform_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
     import numpy as np
     import torch
     from aerokinetic_amd.solver import Solver
@@ -100,14 +129,17 @@ def uniform_gpu(case, storage, steps, warmup, profile_steps):
     recs = s.advance(steps, 1)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    rho, u = s.lattice.macroscopic()
     sweep_ms, bnd_ms = s.lattice.run_profiled(profile_steps, s.steps, 1)
     s.steps += profile_steps
     s.lattice.read_history()
     sweep = sweep_ms / profile_steps / 1e3
     out = {"storage": storage, "mlups": cells * steps / dt / 1e6, "ms_per_step": dt / steps * 1e3,
            "sweep_ms": sweep * 1e3, "boundary_ms": bnd_ms / profile_steps,
-           "roofline": roofline(fluid * 2 * 27 * BYTES[storage], sweep), "cells": cells,
-           "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs])}
+           "roofline": roofline(fluid * 2 * 27 * BYTES[storage], sweep, ("uniform-100m", storage, "k_sweep"), copy_gbs),
+           "cells": cells, "steps_compared": warmup + steps,
+           "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
+           "fields": (rho, u, s._fluid)}
     s.lattice.close()
     del s
     torch.cuda.empty_cache()
@@ -115,21 +147,51 @@ def uniform_gpu(case, storage, steps, warmup, profile_steps):
 
 
 def compare_loads(results, ref="fp64"):
+    """Loads (and, where recorded, rho/u fields) of each storage against the FP64-storage run."""
     import numpy as np
     out = {}
     for st, r in results.items():
         if st == ref or ref not in results:
             continue
         a, b = r["loads"], results[ref]["loads"]
+        ma, mb = a.mean(axis=0), b.mean(axis=0)
         out[st] = {"max_abs_dcoef_vs_fp64": dict(zip(("cl", "cd", "cm"), map(float, np.abs(a - b).max(axis=0)))),
-                   "mean_coef_vs_fp64": {k: [float(a[:, i].mean()), float(b[:, i].mean())]
-                                         for i, k in enumerate(("cl", "cd", "cm"))}}
+                   "mean_coef_vs_fp64": {k: [float(ma[i]), float(mb[i])] for i, k in enumerate(("cl", "cd", "cm"))},
+                   "mean_coef_rel_err": {k: float(abs(ma[i] - mb[i]) / abs(mb[i])) for i, k in enumerate(("cl", "cd"))}}
+        if "fields" in r and "fields" in results[ref]:
+            (ra, ua, fl), (rb, ub, _) = r["fields"], results[ref]["fields"]
+            dr, du = (ra - rb)[fl], (ua - ub)[fl]
+            out[st]["field_l2_vs_fp64"] = {
+                "rho_rel": float(np.linalg.norm(dr) / np.linalg.norm(rb[fl])),
+                "rho_minus_1_rel": float(np.linalg.norm(dr) / np.linalg.norm(rb[fl] - 1.0)),
+                "u_rel": float(np.linalg.norm(du) / np.linalg.norm(ub[fl])),
+                "rho_max_abs": float(np.abs(dr).max()), "u_max_abs": float(np.abs(du).max())}
     return out
 
 
-def cpu_baseline_refined(case, seconds):
+def cpu_uniform_oracle(seconds, threads):
+    """The uniform 384x256x8 oracle step (loads every step) on `threads` host cores."""
+    from oracle import oracle as O
+    from aerokinetic_amd import cases
+    section, flow, grid, plan = cases.NACA0012_C32_384x256x8.build()
+    O.Oracle.set_threads(threads)
+    o = O.from_case(section, flow, grid, plan)
+    cells = plan.shape[0] * plan.shape[1] * plan.shape[2]
+    for _ in range(2):
+        o.step(sample=True)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        o.step(sample=True)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": cells * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"uniform 384x256x8 FP64 oracle, {n} steps with loads ({dt:.1f} s)"}
+
+
+def cpu_baseline_refined(case, seconds, threads):
     from oracle import oracle as O
     from oracle.refined_oracle import RefinedOracle
+    O.Oracle.set_threads(threads)
     project, settings, plan = case.build()
     ro = RefinedOracle(project, settings, plan)
     updates = plan.shape[0] * plan.shape[1] * plan.shape[2] + 2 * 8 * plan.box.shape[0] * plan.box.shape[1] * plan.box.shape[2]
@@ -141,7 +203,7 @@ def cpu_baseline_refined(case, seconds):
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": updates * n / dt / 1e6, "unit": "MLUPS", "cores": O.Oracle.threads(), "kind": "port",
+    return {"value": updates * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle (C/OpenMP lattices, NumPy transfer) on the same two-level case, "
                       f"1 warm-up + {n} timed coarse steps ({dt:.1f} s)"}
 
@@ -149,11 +211,11 @@ def cpu_baseline_refined(case, seconds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--storages", default="fp64,fp32")
     ap.add_argument("--profile-steps", type=int, default=40)
-    ap.add_argument("--uniform-storages", default="fp16,fp64,fp32", help="empty to skip the 100M-cell sweep")
+    ap.add_argument("--uniform-storages", default="fp64,fp32,fp16", help="empty to skip the 100M-cell sweep")
     ap.add_argument("--uniform-steps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -162,12 +224,15 @@ def main():
               "multi-GPU runs are not supported (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     from aerokinetic_amd import cases
+    from aerokinetic_amd.profile_step import copy_bandwidth
+    copy_ms, copy_gbs = copy_bandwidth(4.0, 10)
+    log(f"streaming copy 4 GiB: {copy_ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
     case = cases.NACA0012_REFINED_256x192x8
     storages = [s for s in args.storages.split(",") if s]
     results = {}
     for st in storages:
         log(f"two-level {st}")
-        results[st] = refined_gpu(case, st, args.steps, args.warmup, args.profile_steps)
+        results[st] = refined_gpu(case, st, args.steps, args.warmup, args.profile_steps, copy_gbs)
         r = results[st]
         log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, fine sweep {r['fine_sweep_ms']:.4f} ms "
             f"({100 * r['roofline']['frac']:.1f}% HBM), coarse sweep {r['coarse_sweep_ms']:.4f} ms")
@@ -175,25 +240,33 @@ def main():
     ucase = cases.NACA0012_C128_1536x1024x64
     for st in [s for s in args.uniform_storages.split(",") if s]:
         log(f"uniform 100M {st}")
-        uni[st] = uniform_gpu(ucase, st, args.uniform_steps, 10, 20)
+        uni[st] = uniform_gpu(ucase, st, args.uniform_steps, 10, 20, copy_gbs)
         r = uni[st]
         log(f"  {r['mlups']:.0f} MLUPS, sweep {r['sweep_ms']:.3f} ms ({100 * r['roofline']['frac']:.1f}% HBM)")
-    cpu = cpu_baseline_refined(case, args.cpu_seconds) if args.cpu_seconds > 0 else None
+    uni_cmp = compare_loads(uni)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu = None
+    if args.cpu_seconds > 0:
+        log(f"cpu baseline on {cores} cores")
+        cpu = cpu_baseline_refined(case, args.cpu_seconds, cores)
+        cpu["uniform_384x256x8"] = {"all_cores": cpu_uniform_oracle(args.cpu_seconds / 3, cores),
+                                    "one_core": cpu_uniform_oracle(args.cpu_seconds / 3, 1)}
     head = results[storages[0]]
-    strip = lambda r: {k: v for k, v in r.items() if k not in ("loads", "diag")}
+    strip = lambda r: {k: v for k, v in r.items() if k not in ("loads", "diag", "fields")}
     line = {"metric": METRIC, "value": head["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None,
             "dtype": "f64" if storages[0] == "fp64" else "f32", "data": "synthetic",
             "config": {"workload": case.name + ", legacy coarse farfield, fine Bouzidi walls, loads every coarse step",
                        "storage": storages[0], "updates_per_step": head["updates_per_step"],
-                       "parallelism": "single GPU"},
+                       "parallelism": "single GPU", "step_graphs": True},
             "roofline": head["roofline"], "cpu_baseline": cpu,
+            "copy": {"bytes": 4 * 2**30, "ms": copy_ms, "gbs_read_write": copy_gbs},
             "by_storage": {st: strip(r) for st, r in results.items()},
             "loads_vs_fp64": compare_loads(results),
             "uniform_sweep": {"workload": ucase.name + ", characteristic farfield, loads every step",
                               "by_storage": {st: strip(r) for st, r in uni.items()},
-                              "loads_vs_fp64": compare_loads(uni)}}
+                              "vs_fp64": uni_cmp}}
     print(json.dumps(line))
 
 

@@ -105,8 +105,9 @@ int ak_run(void *handle, int nsteps, long long first_step, int sample_every);
  * dispatch of Solver.step (solver.py:252-279). */
 int ak_set_graph_mode(void *handle, int enable);
 
-/* ak_run with HIP events on the lattice's stream around every sweep and every
- * boundary phase (measurement only; synchronises at the end).
+/* ak_run with HIP events (measurement only; direct launches; synchronises at the end):
+ * each sweep launch records its own kernel start and end, each boundary phase is
+ * bracketed on the lattice's stream.
  * out = {total sweep milliseconds, total boundary milliseconds, steps}. */
 int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_every, double *out);
 
@@ -182,6 +183,12 @@ int ak_refined_transfer_errors(void *handle);
 
 /* Pre-collision populations of one level (0 coarse, 1 fine padded), exchange layout. */
 int ak_refined_get_populations(void *handle, int level, double *host_out);
+
+/* Device-to-device streaming copy of `bytes` (8-byte aligned buffers and size) on
+ * `stream`.  No counterpart in the reference: it measures the attainable HBM
+ * bandwidth beside the stencil kernels and is the known-size kernel that calibrates
+ * the memory traffic counters. */
+int ak_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 /* Last error message of this handle (or of the library when handle is NULL). */
 int ak_last_error(void *handle, char *buf, size_t len);

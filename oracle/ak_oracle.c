@@ -826,3 +826,11 @@ int ako_threads(void) {
     return 1;
 #endif
 }
+
+void ako_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -57,6 +57,7 @@ def lib():
         L.ako_step.argtypes = [ctypes.c_void_p, _c_double_p, _c_double_p]
         L.ako_diagnostics.argtypes = [ctypes.c_void_p, _c_double_p]
         L.ako_threads.restype = ctypes.c_int
+        L.ako_set_threads.argtypes = [ctypes.c_int]
         for name in ("ako_collide", "ako_boundaries", "ako_stream"):
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.ako_measure.argtypes = [ctypes.c_void_p, _c_int_p, _c_int_p, ctypes.c_double, ctypes.c_double,
@@ -172,6 +173,11 @@ class Oracle:
     @staticmethod
     def threads():
         return lib().ako_threads()
+
+    @staticmethod
+    def set_threads(n):
+        """OpenMP threads of the oracle's lattice kernels (process-wide)."""
+        lib().ako_set_threads(int(n))
 
 
 def from_case(section, flow, grid, plan, farfield=None):

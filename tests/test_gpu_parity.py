@@ -133,3 +133,17 @@ def test_periodic_shear_wave_decay_and_parity():
     harmonic = abs(np.fft.fft(vy.mean(axis=(1, 2)))[1]) * 2 / n
     assert harmonic / amp == pytest.approx(np.exp(-nu * (2 * np.pi / n) ** 2 * steps), rel=0.012)
     np.testing.assert_allclose(rho, 1.0, atol=2e-12)
+
+
+def test_macroscopic_export_matches_population_moments():
+    from aerokinetic_amd.d3q27 import CX, CY, CZ
+    section, flow, grid, plan = small_case("characteristic_transport_v1")
+    s = gpu_solver(section, flow, grid, plan)
+    for n in (3, 4):  # both AA parities
+        s.advance(n, 0)
+        g = s.populations()
+        rho, u = s.lattice.macroscopic()
+        r_ref = 1.0 + g.sum(axis=-1)
+        j = np.stack([g @ CX, g @ CY, g @ CZ], axis=-1)
+        np.testing.assert_allclose(rho, r_ref, rtol=0, atol=1e-14)
+        np.testing.assert_allclose(u, j / r_ref[..., None], rtol=0, atol=1e-14)
