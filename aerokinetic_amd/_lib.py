@@ -61,6 +61,14 @@ _SIGS = {
     "ak_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
     "ak_version": (ctypes.c_char_p, []),
     "ak_device_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "ak_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp]),
+    "ak_batch_destroy": (ctypes.c_int, [_vp]),
+    "ak_batch_add": (ctypes.c_int, [_vp, _vp]),
+    "ak_batch_remove": (ctypes.c_int, [_vp, _vp]),
+    "ak_batch_size": (ctypes.c_int, [_vp]),
+    "ak_batch_set_graph_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ak_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "ak_batch_synchronize": (ctypes.c_int, [_vp]),
     "ak_refined_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _ip, _ip, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          _vp]),

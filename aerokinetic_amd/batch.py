@@ -1,0 +1,87 @@
+"""Several two-level cases stepped together on one GPU.
+
+`RefinedBatch` wraps the library's batch entry points (`ak_batch_*`).  The members are
+ordinary `RefinedSolver` objects, each with its own populations, transfer buffers, history,
+step count and stream.  Per coarse step of the batch:
+
+* the members' coarse sweeps run as one launch over their concatenated tile lists, and
+  likewise each fine substep's sweeps;
+* the members' transfer, farfield and wall kernels run on the members' own streams,
+  forked from and joined back into the batch stream, so they overlap on the device;
+* the whole step is a captured HIP graph (re-captured when the membership changes).
+
+A member gives the same history alone and in any mix: the same kernels do the same
+arithmetic per member, and the load reductions have a fixed order.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _lib
+
+_vp = _lib._vp
+
+
+class RefinedBatch:
+    def __init__(self, storage="fp32", device=0, graphs=True, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise _lib.NativeLibraryError("no HIP device visible to PyTorch-ROCm")
+        if storage not in _lib.STORE:
+            raise ValueError("storage must be fp64, fp32 or fp16")
+        self.L = _lib.load()
+        self.storage = storage
+        h = _vp()
+        st = stream.cuda_stream if stream is not None else None
+        _lib.check(self.L.ak_batch_create(ctypes.byref(h), int(device), _lib.STORE[storage], _vp(st)), None,
+                   "ak_batch_create")
+        self.h = h
+        _lib.check(self.L.ak_batch_set_graph_mode(h, 1 if graphs else 0), None, "ak_batch_set_graph_mode")
+        self.members = []
+
+    def add(self, solver):
+        """Admit a RefinedSolver (same storage; at a coarse-step boundary)."""
+        if solver.storage != self.storage:
+            raise ValueError(f"a {solver.storage} case cannot join a {self.storage} batch")
+        _lib.check(self.L.ak_batch_add(self.h, solver.h), None, "ak_batch_add")
+        self.members.append(solver)
+
+    def remove(self, solver):
+        _lib.check(self.L.ak_batch_remove(self.h, solver.h), None, "ak_batch_remove")
+        self.members.remove(solver)
+
+    def __len__(self):
+        return len(self.members)
+
+    def advance(self, nsteps, sample=True):
+        """nsteps coarse steps of every member.
+
+        Returns ({member: [load records]}, {member: TransferError}); a member whose transfer
+        guards fired is reported in the second mapping and stays in the batch until removed.
+        """
+        from .refined import TransferError
+        if nsteps <= 0 or not self.members:
+            return {m: [] for m in self.members}, {}
+        _lib.check(self.L.ak_batch_run(self.h, int(nsteps), 1 if sample else 0), None, "ak_batch_run")
+        _lib.check(self.L.ak_batch_synchronize(self.h), None, "ak_batch_synchronize")
+        out, failed = {}, {}
+        for m in self.members:
+            m.steps += nsteps
+            out[m] = [m._record(r) for r in m._read()] if sample else []
+            try:
+                m._check_transfer()
+            except TransferError as e:
+                failed[m] = e
+        return out, failed
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ak_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -213,9 +213,10 @@ void one_step(Lattice *s, bool sample) {
   s->sweeps++;
 }
 
-void set_ctl(Lattice *s, long long step, int row) {
-  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, s->stream, s->d_ctl, step, row, s->hist_width);
+void set_ctl_on(Lattice *s, hipStream_t st, long long step, int row) {
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, st, s->d_ctl, step, row, s->hist_width);
 }
+void set_ctl(Lattice *s, long long step, int row) { set_ctl_on(s, s->stream, step, row); }
 
 // Enqueue one step on `st` through the graph in slot `slot` of `gc`: captured from
 // `launch` (which also advances the host-side counters) on first use or after a
@@ -881,3 +882,4 @@ int ak_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
 }  // extern "C"
 
 #include "ak_refined_capi.hpp"
+#include "ak_batch_capi.hpp"

@@ -65,13 +65,14 @@ __device__ __forceinline__ void st_g(gchar *p, typename St<S>::C v, float scale)
   else *(gT *)p = v;
 }
 
+// One tile (block `lb` of the lattice's sweep grid) of the fused AA stream + collide.
+// `slot` picks the diagnostics accumulator slot of this block.
 template <int S, bool ODD, bool PXY, bool DIAG>
-__global__ __launch_bounds__(PXY ? 256 : 1024) void k_sweep(typename St<S>::T *__restrict__ A,
-                                               const uint8_t *__restrict__ solid, Desc L, SweepGeom G,
-                                               typename St<S>::C ws, typename St<S>::C wb, float scale,
-                                               float inv_scale, DiagAcc *__restrict__ diag) {
+__device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, const uint8_t *__restrict__ solid,
+                                            const Desc &L, const SweepGeom &G, typename St<S>::C ws,
+                                            typename St<S>::C wb, float scale, float inv_scale,
+                                            DiagAcc *__restrict__ diag, int lb, int slot) {
   using C = typename St<S>::C;
-  const int lb = xcd_remap(blockIdx.x, G.nblocks);
   const int tx = lb % G.gx, ty = (lb / G.gx) % G.gy, tz = lb / (G.gx * G.gy);
   const int x = tx * G.bx + (int)(threadIdx.x % G.bx);
   const int y = ty * G.by + (int)(threadIdx.x / G.bx);
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(PXY ? 256 : 1024) void k_sweep(typename St<S>::T *_
         vu = su[k] > vu ? su[k] : vu; vd = sd[k] > vd ? sd[k] : vd;
         vr = sr[k] < vr ? sr[k] : vr; bad |= sb[k];
       }
-      DiagSlot *sl = &diag->s[blockIdx.x % DIAG_SLOTS];
+      DiagSlot *sl = &diag->s[slot];
       atomicMax(&sl->u2max, (unsigned long long)__double_as_longlong((double)vu));
       atomicMax(&sl->drmax, (unsigned long long)__double_as_longlong((double)vd));
       if (vr > C(0)) atomicMin(&sl->rhomin, (unsigned long long)__double_as_longlong((double)vr >= 3.0e38 ? 1e300 : (double)vr));
@@ -167,6 +168,53 @@ __global__ __launch_bounds__(PXY ? 256 : 1024) void k_sweep(typename St<S>::T *_
       if (bad) atomicMax(&sl->nonfinite, 1ull);
     }
   }
+}
+
+template <int S, bool ODD, bool PXY, bool DIAG>
+__global__ __launch_bounds__(PXY ? 256 : 1024) void k_sweep(typename St<S>::T *__restrict__ A,
+                                                            const uint8_t *__restrict__ solid, Desc L, SweepGeom G,
+                                                            typename St<S>::C ws, typename St<S>::C wb, float scale,
+                                                            float inv_scale, DiagAcc *__restrict__ diag) {
+  sweep_block<S, ODD, PXY, DIAG>(A, solid, L, G, ws, wb, scale, inv_scale, diag, xcd_remap(blockIdx.x, G.nblocks),
+                                 (int)(blockIdx.x % DIAG_SLOTS));
+}
+
+// One lattice of a batched sweep: its arguments and the first block of its tiles in the
+// concatenated tile list.  `odd` selects the AA form of this member's sweep.
+struct BatchMember {
+  void *A;
+  const uint8_t *solid;
+  Desc L;
+  SweepGeom G;
+  double ws, wb;
+  float scale, inv_scale;
+  DiagAcc *diag;
+  int first_block, odd;
+};
+
+// The sweeps of several lattices of one storage type (uneven sizes) in one launch over
+// their concatenated tile lists (256-thread tiles; x/y ghosts kept, z periodic).  The
+// member of a block is found by bisection over the members' first blocks; each member
+// keeps its own parity, relaxation rates, scale and diagnostics.
+template <int S, bool DIAG>
+__global__ __launch_bounds__(256) void k_sweep_batch(const BatchMember *__restrict__ M, int nm, int nblocks) {
+  using T = typename St<S>::T;
+  using C = typename St<S>::C;
+  const int b = xcd_remap(blockIdx.x, nblocks);
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (M[mid].first_block <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchMember &m = M[lo];
+  const int lb = b - m.first_block;
+  if (m.odd)
+    sweep_block<S, true, false, DIAG>((T *)m.A, m.solid, m.L, m.G, (C)m.ws, (C)m.wb, m.scale, m.inv_scale, m.diag, lb,
+                                      lb % DIAG_SLOTS);
+  else
+    sweep_block<S, false, false, DIAG>((T *)m.A, m.solid, m.L, m.G, (C)m.ws, (C)m.wb, m.scale, m.inv_scale, m.diag, lb,
+                                       lb % DIAG_SLOTS);
 }
 
 // ------------------------------------------------------------------ copy

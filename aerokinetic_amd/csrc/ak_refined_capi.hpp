@@ -116,27 +116,32 @@ void launch_restrict(Refined *r) {
     hipLaunchKernelGGL((k_restrict<S, false>), dim3(blocks_for(nbox, 128)), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, (const double *)r->d_mf, r->B, r->V, r->F, c->scale, r->d_err);
 }
 
-// ev (optional, 8 events): step start/end, coarse sweep start/end, fine sweep 0 and 1 start/end
-// (the sweep events hold the kernels' own start and end times)
-// One coarse step; its number and history row come from the coarse lattice's d_ctl.
+// The coarse step in phases.  A single case runs them in order with its own sweeps in
+// between (refined_coarse_step); a batch runs each phase for every member and one
+// batched sweep launch in place of the members' sweeps (ak_batch_run).  Every launch
+// goes to the member's coarse-lattice stream, which both levels share.
+
+// donor moments at time t (pre-collision coarse state, before the coarse sweep)
 template <int S>
-void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
-  Lattice *c = r->c, *f = r->f;
-  if (ev) (void)hipEventRecord(ev[0], c->stream);
+void phase_before_coarse(Refined *r) {
+  Lattice *c = r->c;
   using T_ = typename St<S>::T;
   const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
-  const unsigned cwave_blocks = blocks_for(4LL * r->nt * c->nz * 64, 256);
-  // donor moments at time t (pre-collision coarse state)
   if (c->sweeps % 2 == 0)
     hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_before, r->d_err);
   else
     hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_before, r->d_err);
-  // coarse collide (AA sweep gathers the pre-collision state)
-  launch_sweep<S>(c, c->sweeps % 2 == 0, false, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
-  c->sweeps++;
-  const bool cpe = c->sweeps % 2 == 0;
-  const double volc = r->dxc * r->dxc * r->dxc, volf = r->dxf * r->dxf * r->dxf;
-  if (cpe) {
+}
+
+// after the coarse sweep: coarse face flux, coarse farfield, donor moments at t + dt
+template <int S>
+void phase_after_coarse(Refined *r) {
+  Lattice *c = r->c;
+  using T_ = typename St<S>::T;
+  const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
+  const unsigned cwave_blocks = blocks_for(4LL * r->nt * c->nz * 64, 256);
+  const double volc = r->dxc * r->dxc * r->dxc;
+  if (c->sweeps % 2 == 0) {
     hipLaunchKernelGGL((k_flux_coarse<S, true>), dim3(cwave_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
     launch_boundary<S, true>(c, false);
     hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
@@ -145,31 +150,50 @@ void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
     launch_boundary<S, false>(c, false);
     hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
   }
+}
+
+// fine substep `sub` before its sweep: time-interpolated prolongation into the donor layers
+template <int S>
+void phase_before_fine(Refined *r, int sub) {
+  Lattice *f = r->f;
+  using T_ = typename St<S>::T;
+  const double frac = sub * 0.5;
   const long long nf = (long long)f->nx * f->ny * f->nz;
+  const long long npx = (long long)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
+  const long long npy = (long long)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (r->B.nb[2] + 6);
+  hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(npx, 256)), dim3(256), 0, f->stream, r->d_before, r->d_after, frac, r->B, r->d_px);
+  hipLaunchKernelGGL(k_prolong_y, dim3(blocks_for(npy, 256)), dim3(256), 0, f->stream, r->d_px, r->B, r->d_py);
+  if (f->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_prolong_z_inject<S, true>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
+  else
+    hipLaunchKernelGGL((k_prolong_z_inject<S, false>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
+}
+
+// fine substep `sub` after its sweep: fine face flux, walls + load partials
+template <int S>
+void phase_after_fine(Refined *r, int sub) {
+  Lattice *f = r->f;
+  using T_ = typename St<S>::T;
+  const double volf = r->dxf * r->dxf * r->dxf;
   const unsigned fflux_blocks = blocks_for(4LL * (2 * r->nt) * f->nz * 64, 256);
-  for (int sub = 0; sub < 2; ++sub) {
-    const double frac = sub * 0.5;
-    const long long npx = (long long)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
-    const long long npy = (long long)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (r->B.nb[2] + 6);
-    hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(npx, 256)), dim3(256), 0, f->stream, r->d_before, r->d_after, frac, r->B, r->d_px);
-    hipLaunchKernelGGL(k_prolong_y, dim3(blocks_for(npy, 256)), dim3(256), 0, f->stream, r->d_px, r->B, r->d_py);
-    if (f->sweeps % 2 == 0)
-      hipLaunchKernelGGL((k_prolong_z_inject<S, true>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
-    else
-      hipLaunchKernelGGL((k_prolong_z_inject<S, false>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
-    launch_sweep<S>(f, f->sweeps % 2 == 0, false, ev ? ev[4 + 2 * sub] : nullptr, ev ? ev[5 + 2 * sub] : nullptr);
-    f->sweeps++;
-    WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
-    double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;
-    if (f->sweeps % 2 == 0) {
-      hipLaunchKernelGGL((k_flux_fine<S, true>), dim3(fflux_blocks), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, sub, r->d_regf);
-      if (f->nlinks) hipLaunchKernelGGL((k_walls<S, true, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
-    } else {
-      hipLaunchKernelGGL((k_flux_fine<S, false>), dim3(fflux_blocks), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, sub, r->d_regf);
-      if (f->nlinks) hipLaunchKernelGGL((k_walls<S, false, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
-    }
+  WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
+  double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;
+  if (f->sweeps % 2 == 0) {
+    hipLaunchKernelGGL((k_flux_fine<S, true>), dim3(fflux_blocks), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, sub, r->d_regf);
+    if (f->nlinks) hipLaunchKernelGGL((k_walls<S, true, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
+  } else {
+    hipLaunchKernelGGL((k_flux_fine<S, false>), dim3(fflux_blocks), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, sub, r->d_regf);
+    if (f->nlinks) hipLaunchKernelGGL((k_walls<S, false, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
   }
-  const double inv_volc = 1.0 / volc;
+}
+
+// end of the coarse step: reflux, restriction, loads (sampled) or the step tick
+template <int S>
+void phase_end(Refined *r, bool sample) {
+  Lattice *c = r->c, *f = r->f;
+  using T_ = typename St<S>::T;
+  const unsigned cwave_blocks = blocks_for(4LL * r->nt * c->nz * 64, 256);
+  const double inv_volc = 1.0 / (r->dxc * r->dxc * r->dxc), volf = r->dxf * r->dxf * r->dxf;
   if (c->sweeps % 2 == 0)
     hipLaunchKernelGGL((k_reflux<S, true>), dim3(cwave_blocks), dim3(256), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
   else
@@ -184,6 +208,26 @@ void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
   } else {
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
   }
+}
+
+// One coarse step of one case; its number and history row come from the coarse lattice's
+// d_ctl.  ev (optional, 8 events): step start/end, coarse sweep start/end, fine sweep 0 and
+// 1 start/end (the sweep events hold the kernels' own start and end times).
+template <int S>
+void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
+  Lattice *c = r->c, *f = r->f;
+  if (ev) (void)hipEventRecord(ev[0], c->stream);
+  phase_before_coarse<S>(r);
+  launch_sweep<S>(c, c->sweeps % 2 == 0, false, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+  c->sweeps++;
+  phase_after_coarse<S>(r);
+  for (int sub = 0; sub < 2; ++sub) {
+    phase_before_fine<S>(r, sub);
+    launch_sweep<S>(f, f->sweeps % 2 == 0, false, ev ? ev[4 + 2 * sub] : nullptr, ev ? ev[5 + 2 * sub] : nullptr);
+    f->sweeps++;
+    phase_after_fine<S>(r, sub);
+  }
+  phase_end<S>(r, sample);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
 }
 

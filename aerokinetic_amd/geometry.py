@@ -46,6 +46,25 @@ def segments_intersect(a, b, c, d) -> bool:
     return (-1e-12 <= t <= 1 + 1e-12) and (-1e-12 <= s <= 1 + 1e-12)
 
 
+def _segment_hits(a, b, C, D):
+    """segments_intersect(a, b, C[k], D[k]) for every k at once (same tolerances)."""
+    ab, cd, ca = b - a, D - C, C - a
+    den = _cross(ab, cd)
+    tol = 1e-12 * np.maximum(np.linalg.norm(ab) * np.linalg.norm(cd, axis=1), 1e-30)
+    par = np.abs(den) <= tol
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = _cross(ca, cd) / den
+        s = _cross(ca, ab) / den
+    hit = ~par & (t >= -1e-12) & (t <= 1 + 1e-12) & (s >= -1e-12) & (s <= 1 + 1e-12)
+    col = par & (np.abs(_cross(ca, ab)) <= tol)
+    if col.any():
+        k = int(np.argmax(np.abs(ab)))
+        lo = np.maximum(min(a[k], b[k]), np.minimum(C[:, k], D[:, k]))
+        hi = np.minimum(max(a[k], b[k]), np.maximum(C[:, k], D[:, k]))
+        hit |= col & (lo <= hi + 1e-12)
+    return hit
+
+
 @dataclass(frozen=True)
 class Contour:
     """A closed polygon in a common frame (any length unit)."""
@@ -68,12 +87,10 @@ class Contour:
         if area <= 1e-12 * float(np.ptp(pts, axis=0).max()) ** 2:
             raise ValueError("Contour has zero enclosed area")
         n = len(pts)
-        for i in range(n):
-            for j in range(i + 2, n):
-                if i == 0 and j == n - 1:
-                    continue
-                if segments_intersect(pts[i], nxt[i], pts[j], nxt[j]):
-                    raise ValueError(f"Self-intersection in contour {self.name}")
+        for i in range(n - 2):
+            j0, j1 = i + 2, (n - 1 if i == 0 else n)  # non-adjacent edges only
+            if j1 > j0 and _segment_hits(pts[i], nxt[i], pts[j0:j1], nxt[j0:j1]).any():
+                raise ValueError(f"Self-intersection in contour {self.name}")
         pts.setflags(write=False)
         object.__setattr__(self, "xy", pts)
 
@@ -165,8 +182,7 @@ class Section:
                 if not touching:
                     ea, eb = e.xy, np.roll(e.xy, -1, axis=0)
                     fa, fb = f.xy, np.roll(f.xy, -1, axis=0)
-                    touching = any(segments_intersect(a, b, c, d)
-                                   for a, b in zip(ea, eb) for c, d in zip(fa, fb))
+                    touching = any(_segment_hits(a, b, fa, fb).any() for a, b in zip(ea, eb))
                 if touching:
                     raise ValueError(f"Elements touch or overlap: {e.name}, {f.name}")
 

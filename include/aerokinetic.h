@@ -184,6 +184,28 @@ int ak_refined_transfer_errors(void *handle);
 /* Pre-collision populations of one level (0 coarse, 1 fine padded), exchange layout. */
 int ak_refined_get_populations(void *handle, int level, double *host_out);
 
+/* ---- Batches of two-level cases (serving) --------------------------------------------------
+ * No counterpart in the reference, which runs one case per process (Solver.run /
+ * refined_run.run, solver.py:353-537, refined_run.py:19-157; polar sweeps loop over such
+ * runs).  A batch steps several refined cases of one storage type together: the members'
+ * coarse sweeps run as ONE launch over their concatenated tile lists, likewise each fine
+ * substep's sweeps, while the members' transfer kernels run on their own streams and
+ * overlap.  Each batch step is a captured HIP graph (per launch variant).  A member keeps
+ * its own step count, history rows and buffers, and gives the same result alone or in any
+ * mix (same kernels, same arithmetic, fixed-order reductions).  Errors: ak_last_error(NULL). */
+int ak_batch_create(void **handle, int device, int storage, void *stream);
+int ak_batch_destroy(void *handle);
+/* Add / remove a refined case (ak_refined_create handle; each member needs a stream of its
+ * own).  Membership changes re-capture the batch graph on the next run. */
+int ak_batch_add(void *handle, void *refined);
+int ak_batch_remove(void *handle, void *refined);
+int ak_batch_size(void *handle);
+int ak_batch_set_graph_mode(void *handle, int enable);
+/* nsteps coarse steps of every member; sample_every 1 (loads of every step into each member's
+ * history, read with ak_read_history on ak_refined_level(member, 0)) or 0. */
+int ak_batch_run(void *handle, int nsteps, int sample_every);
+int ak_batch_synchronize(void *handle);
+
 /* Device-to-device streaming copy of `bytes` (8-byte aligned buffers and size) on
  * `stream`.  No counterpart in the reference: it measures the attainable HBM
  * bandwidth beside the stencil kernels and is the known-size kernel that calibrates

@@ -1,0 +1,109 @@
+"""Batched two-level cases: a member gives the same result alone and in any mix, a batch
+of one equals the single-case solver, and the continuous-batching service completes a
+stream of requests."""
+import numpy as np
+import pytest
+
+from aerokinetic_amd import serving as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(req, storage, graphs=True):
+    from aerokinetic_amd.refined import RefinedSolver
+    project, settings, plan = S.project_of(req)
+    return RefinedSolver(project, settings, plan=plan, storage=storage, graphs=graphs)
+
+
+def _reqs():
+    return S.polar_requests(3, seed=5, chord=(16, 20), span=(8, 8), reynolds=(1e4, 2e4), alpha=(2.0, 8.0))
+
+
+def _loads(recs):
+    return np.array([[r["step"], r["cl"], r["cd"], r["cm"]] for r in recs])
+
+
+def test_batch_of_one_matches_single_case_solver():
+    from aerokinetic_amd.batch import RefinedBatch
+    req = _reqs()[0]
+    print("request", req, flush=True)
+    single = _solver(req, "fp64")
+    member = _solver(req, "fp64")
+    batch = RefinedBatch("fp64")
+    batch.add(member)
+    a = single.advance(9, 1)
+    recs, failed = batch.advance(9)
+    assert not failed
+    b = recs[member]
+    np.testing.assert_allclose(_loads(a), _loads(b), rtol=0, atol=1e-12)
+    for level in (0, 1):
+        ga, gb = single.populations(level), member.populations(level)
+        np.testing.assert_allclose(ga, gb, rtol=0, atol=1e-13)
+    batch.close()
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp64"])
+def test_member_is_identical_alone_and_in_a_mix(storage):
+    """Case A alone vs A sharing every batched launch with B (joined from the start) and C
+    (joined after 4 steps, different size and age): A's loads and populations are
+    bit-identical."""
+    from aerokinetic_amd.batch import RefinedBatch
+    ra, rb, rc = _reqs()
+    print("requests", ra.id, rb.id, rc.id, storage, flush=True)
+    alone_a = _solver(ra, storage)
+    alone = RefinedBatch(storage)
+    alone.add(alone_a)
+    mix_a, mix_b, mix_c = _solver(ra, storage), _solver(rb, storage), _solver(rc, storage)
+    mix_c.advance(3, 0)  # C is older: a different AA parity when it joins
+    mix = RefinedBatch(storage)
+    mix.add(mix_b)
+    mix.add(mix_a)
+    r1, _ = alone.advance(4)
+    r2, _ = mix.advance(4)
+    mix.add(mix_c)
+    r3, _ = alone.advance(6)
+    r4, f4 = mix.advance(6)
+    assert not f4
+    np.testing.assert_array_equal(_loads(r1[alone_a] + r3[alone_a]), _loads(r2[mix_a] + r4[mix_a]))
+    assert [r["step"] for r in r4[mix_c]] == list(range(4, 10))
+    mix.remove(mix_b)
+    r5, _ = alone.advance(3)
+    r6, _ = mix.advance(3)
+    np.testing.assert_array_equal(_loads(r5[alone_a]), _loads(r6[mix_a]))
+    for level in (0, 1):
+        np.testing.assert_array_equal(alone_a.populations(level), mix_a.populations(level))
+    # C after 3 own steps + 9 in the mix equals C stepped alone 12 times
+    ref_c = _solver(rc, storage)
+    ref_c.advance(12, 0)
+    np.testing.assert_array_equal(ref_c.populations(1), mix_c.populations(1))
+    alone.close()
+    mix.close()
+
+
+def test_batch_graph_replay_matches_direct_launches():
+    from aerokinetic_amd.batch import RefinedBatch
+    reqs = _reqs()[:2]
+    out = []
+    for graphs in (False, True):
+        b = RefinedBatch("fp32", graphs=graphs)
+        members = [_solver(r, "fp32") for r in reqs]
+        for m in members:
+            b.add(m)
+        recs, _ = b.advance(5)
+        out.append(([_loads(recs[m]) for m in members], [m.populations(1) for m in members]))
+        b.close()
+    for x, y in zip(out[0][0] + out[0][1], out[1][0] + out[1][1]):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_service_admits_steps_and_retires_a_request_stream():
+    reqs = S.polar_requests(4, seed=9, rate=20.0, chord=(16, 18), span=(8, 8), reynolds=(1e4, 2e4))
+    print("arrivals", [round(r.arrival, 3) for r in reqs], flush=True)
+    srv = S.Server(storage="fp32", chunk=40, window=80, tol=0.005, max_steps=160, builders=2)
+    rep = srv.serve(reqs)
+    s = rep.summary()
+    assert s["cases"] == 4 and sum(s["reasons"].values()) == 4
+    assert s["reasons"]["failed"] == 0
+    assert s["peak_members"] >= 2
+    for r in rep.results:
+        assert r.latency > 0 and 80 <= r.steps <= 160 and np.isfinite(r.cl)
