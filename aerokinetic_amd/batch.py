@@ -53,26 +53,38 @@ class RefinedBatch:
     def __len__(self):
         return len(self.members)
 
-    def advance(self, nsteps, sample=True):
-        """nsteps coarse steps of every member.
-
-        Returns ({member: [load records]}, {member: TransferError}); a member whose transfer
-        guards fired is reported in the second mapping and stays in the batch until removed.
-        """
-        from .refined import TransferError
+    def launch(self, nsteps, sample=True):
+        """Enqueue nsteps coarse steps of every member and return at once (the GPU runs them
+        while the host prepares other work); `collect` waits and reads the results."""
         if nsteps <= 0 or not self.members:
-            return {m: [] for m in self.members}, {}
+            return
         _lib.check(self.L.ak_batch_run(self.h, int(nsteps), 1 if sample else 0), None, "ak_batch_run")
+        for m in self.members:
+            m.steps += nsteps
+
+    def collect(self, raw=False):
+        """Wait for the enqueued steps.  Returns ({member: records}, {member: TransferError});
+        records are dicts, or with raw=True the history rows as an array (step, t*, cl, cd, cm
+        in columns 0-3 as recorded).  A member whose transfer guards fired is reported in the
+        second mapping and stays in the batch until removed."""
+        from .refined import TransferError
         _lib.check(self.L.ak_batch_synchronize(self.h), None, "ak_batch_synchronize")
         out, failed = {}, {}
         for m in self.members:
-            m.steps += nsteps
-            out[m] = [m._record(r) for r in m._read()] if sample else []
+            rows = m._read()
+            out[m] = rows if raw else [m._record(r) for r in rows]
             try:
                 m._check_transfer()
             except TransferError as e:
                 failed[m] = e
         return out, failed
+
+    def advance(self, nsteps, sample=True):
+        """nsteps coarse steps of every member: launch + collect."""
+        if nsteps <= 0 or not self.members:
+            return {m: [] for m in self.members}, {}
+        self.launch(nsteps, sample)
+        return self.collect()
 
     def close(self):
         if getattr(self, "h", None):

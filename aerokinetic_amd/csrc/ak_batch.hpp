@@ -1,0 +1,176 @@
+// ak_batch.hpp -- device side of a batch of two-level cases.
+//
+// Every kernel of the coarse step runs for all members of a batch in ONE launch over the
+// members' concatenated work lists: block b belongs to the member whose first block is
+// the last one <= b (bisection over a per-phase table), and that member's arguments come
+// from its CaseDev record.  The per-item work is the same device function the single-case
+// kernels call (the *_body functions), so a member computes bit-identically alone, in a
+// batch of one, or in any mix.
+#pragma once
+
+#include "ak_kernels.hpp"
+#include "ak_refined.hpp"
+
+namespace ak {
+
+// Everything the step kernels of one two-level case read.
+struct CaseDev {
+  void *Ac, *Af;  // coarse / fine populations
+  Desc Lc, Lf;
+  float c_scale, c_inv, f_scale, f_inv;
+  BoxGeom B;
+  Levels V;
+  FreeState F;
+  const FluxTable *T;
+  double *before, *after, *regc, *regf, *part, *px, *py, *mf;
+  int *err;
+  double volc, volf, inv_volc, loads_scale;
+  WallLinks W;  // fine-level walls (lever arm dx_f)
+  int wall_blocks;
+  LegacyLinks K;  // coarse-level legacy farfield
+  double U;
+  StepCtl *ctl;
+  double *hist;
+  int cpar;  // coarse sweep-count parity at the start of a step when the lead member's is even
+  int fpar;  // fine sweep-count parity at the start of a step
+};
+
+// The step's kernels, each with a table of first blocks per member.
+enum BatchPhase {
+  BP_DONOR, BP_FLUXC, BP_LEGACY, BP_PX, BP_PY, BP_PZ, BP_FLUXF, BP_WALLS, BP_REFLUX, BP_FMOM, BP_RESTRICT,
+  BP_COUNT
+};
+
+// Member owning block b: the last m with first[m] <= b (members with no blocks are skipped).
+__device__ __forceinline__ int batch_owner(const int *__restrict__ first, int nm, int b) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define AK_BATCH_MEMBER                                                          \
+  const int m_ = batch_owner(first, nm, (int)blockIdx.x);                        \
+  const CaseDev &c = C[m_];                                                      \
+  const int lb_ = (int)blockIdx.x - first[m_];                                   \
+  const long long t = (long long)lb_ * blockDim.x + threadIdx.x;                 \
+  (void)lb_;
+// AA form of a coarse / fine kernel: the sweep count (at step start + delta) is even
+#define AK_PE_COARSE(delta) (((((c.cpar ^ lead) + (delta)) & 1)) == 0)
+#define AK_PE_FINE(delta) ((((c.fpar + (delta)) & 1)) == 0)
+
+template <int S>
+__global__ __launch_bounds__(128) void kb_donor(const CaseDev *__restrict__ C, const int *__restrict__ first, int nm,
+                                                int lead, int delta, int after) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  double *out = after ? c.after : c.before;
+  if (AK_PE_COARSE(delta)) donor_moments_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, out, c.err, t);
+  else donor_moments_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, out, c.err, t);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_flux_coarse(const CaseDev *__restrict__ C, const int *__restrict__ first,
+                                                      int nm, int lead, int delta) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  if (AK_PE_COARSE(delta)) flux_coarse_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.T, c.volc, c.c_inv, c.regc, t);
+  else flux_coarse_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.T, c.volc, c.c_inv, c.regc, t);
+}
+
+template <int S>
+__global__ __launch_bounds__(128) void kb_legacy(const CaseDev *__restrict__ C, const int *__restrict__ first, int nm,
+                                                 int lead, int delta) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  if (AK_PE_COARSE(delta)) legacy_body<S, true>((T *)c.Ac, c.Lc, c.K, c.U, c.c_scale, c.c_inv, t);
+  else legacy_body<S, false>((T *)c.Ac, c.Lc, c.K, c.U, c.c_scale, c.c_inv, t);
+}
+
+__global__ __launch_bounds__(256) void kb_prolong_x(const CaseDev *__restrict__ C, const int *__restrict__ first,
+                                                    int nm, double frac) {
+  AK_BATCH_MEMBER
+  prolong_x_body(c.before, c.after, frac, c.B, c.px, t);
+}
+
+__global__ __launch_bounds__(256) void kb_prolong_y(const CaseDev *__restrict__ C, const int *__restrict__ first,
+                                                    int nm) {
+  AK_BATCH_MEMBER
+  prolong_y_body(c.px, c.B, c.py, t);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_prolong_z(const CaseDev *__restrict__ C, const int *__restrict__ first,
+                                                    int nm, int delta) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  if (AK_PE_FINE(delta)) prolong_z_inject_body<S, true>((T *)c.Af, c.Lf, c.B, c.V, c.py, c.f_scale, c.err, t);
+  else prolong_z_inject_body<S, false>((T *)c.Af, c.Lf, c.B, c.V, c.py, c.f_scale, c.err, t);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_flux_fine(const CaseDev *__restrict__ C, const int *__restrict__ first,
+                                                    int nm, int delta, int sub) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  if (AK_PE_FINE(delta)) flux_fine_body<S, true>((const T *)c.Af, c.Lf, c.B, c.T, c.volf, c.f_inv, sub, c.regf, t);
+  else flux_fine_body<S, false>((const T *)c.Af, c.Lf, c.B, c.T, c.volf, c.f_inv, sub, c.regf, t);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_walls(const CaseDev *__restrict__ C, const int *__restrict__ first, int nm,
+                                                int delta, int sub) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  double *part = c.part + (size_t)sub * c.wall_blocks * c.W.nelem * 5;
+  if (AK_PE_FINE(delta)) walls_body<S, true, true>((T *)c.Af, c.Lf, c.W, c.f_scale, c.f_inv, part, t, lb_);
+  else walls_body<S, false, true>((T *)c.Af, c.Lf, c.W, c.f_scale, c.f_inv, part, t, lb_);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_reflux(const CaseDev *__restrict__ C, const int *__restrict__ first, int nm,
+                                                 int lead, int delta) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  if (AK_PE_COARSE(delta))
+    reflux_body<S, true>((T *)c.Ac, c.Lc, c.B, c.regc, c.regf, c.inv_volc, c.c_scale, c.c_inv, c.err, t);
+  else
+    reflux_body<S, false>((T *)c.Ac, c.Lc, c.B, c.regc, c.regf, c.inv_volc, c.c_scale, c.c_inv, c.err, t);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_fine_moments(const CaseDev *__restrict__ C, const int *__restrict__ first,
+                                                       int nm, int delta) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  if (AK_PE_FINE(delta)) fine_moments_body<S, true>((const T *)c.Af, c.Lf, c.B, c.f_inv, c.mf, c.err, t);
+  else fine_moments_body<S, false>((const T *)c.Af, c.Lf, c.B, c.f_inv, c.mf, c.err, t);
+}
+
+template <int S>
+__global__ __launch_bounds__(128) void kb_restrict(const CaseDev *__restrict__ C, const int *__restrict__ first,
+                                                   int nm, int lead, int delta) {
+  using T = typename St<S>::T;
+  AK_BATCH_MEMBER
+  if (AK_PE_COARSE(delta)) restrict_body<S, true>((T *)c.Ac, c.Lc, c.mf, c.B, c.V, c.F, c.c_scale, c.err, t);
+  else restrict_body<S, false>((T *)c.Ac, c.Lc, c.mf, c.B, c.V, c.F, c.c_scale, c.err, t);
+}
+
+// One 64-thread block per member: the loads of a sampled step, or the step tick.
+__global__ __launch_bounds__(64) void kb_loads(const CaseDev *__restrict__ C, int sample) {
+  const CaseDev &c = C[blockIdx.x];
+  if (sample)
+    loads_final_body(c.part, c.W.n ? 2 * c.wall_blocks : 0, c.W.nelem, c.loads_scale, c.loads_scale, c.volf, c.ctl,
+                     c.hist, nullptr);
+  else if (threadIdx.x == 0)
+    c.ctl->step += 1;
+}
+
+#undef AK_BATCH_MEMBER
+#undef AK_PE_COARSE
+#undef AK_PE_FINE
+
+}  // namespace ak

@@ -1,16 +1,17 @@
 // ak_batch_capi.hpp -- C ABI of a batch of two-level cases stepped together (included by
 // ak_capi.hip after the refined ABI).
 //
-// A batch holds refined cases of one storage type, each with its own buffers, step count,
-// history and stream.  One batch step = one coarse step of every member:
-//   members' donor moments -> ONE batched coarse sweep over all members' tiles ->
-//   members' coarse flux / farfield / donor moments / prolongation -> ONE batched fine
-//   sweep -> members' fine flux / walls / prolongation -> ONE batched fine sweep ->
-//   members' fine flux / walls / reflux / restriction / loads.
-// The per-member phases are forked onto the members' own streams and joined before each
-// batched sweep, so small transfer kernels of different cases overlap on the device.  The
-// whole batch step is captured once per launch variant as a HIP graph and replayed while
-// the membership (and every member's kernel arguments) stays the same.
+// A batch holds refined cases of one storage type, each with its own buffers, step count
+// and history.  One batch step = one coarse step of every member, as the same kernel
+// sequence as a single case, where every kernel runs for ALL members in one launch over
+// their concatenated work lists (ak_batch.hpp):
+//   donor moments -> coarse sweeps -> coarse face flux -> legacy farfield -> donor moments
+//   -> 2 x [prolongation x, y, z -> fine sweeps -> fine face flux -> walls + load partials]
+//   -> reflux -> fine moments -> restriction -> loads.
+// The launch count per step is independent of the number of members (about 20), so small
+// cases that are latency-bound alone fill the GPU together.  The step is captured on the
+// batch stream as a HIP graph per launch variant (lead member's coarse parity x sampled)
+// and replayed while the membership stays the same.
 
 namespace {
 
@@ -21,7 +22,10 @@ struct Batch {
   std::vector<Refined *> members;
   unsigned long long gen = 1;       // membership generation
   BatchMember *d_desc = nullptr;    // [4][cap]: coarse sweep (lead parity 0, 1), fine sweep (substep 0, 1)
+  CaseDev *d_cases = nullptr;       // [cap]
+  int *d_first = nullptr;           // [BP_COUNT][cap + 1] first block of each member per kernel
   int cap = 0, nb_coarse = 0, nb_fine = 0;
+  int nblk[BP_COUNT] = {};
   unsigned long long desc_key = 0;  // state the descriptors were built for
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> join;
@@ -89,10 +93,15 @@ int build_descriptors(Batch *b) {
   if (key == b->desc_key && b->d_desc) return 0;
   const int n = (int)b->members.size();
   if (n > b->cap) {
-    if (b->d_desc) (void)hipFree(b->d_desc);
+    for (void *p : {(void *)b->d_desc, (void *)b->d_cases, (void *)b->d_first})
+      if (p) (void)hipFree(p);
     b->d_desc = nullptr;
+    b->d_cases = nullptr;
+    b->d_first = nullptr;
     b->cap = std::max(n, 2 * b->cap);
     BCHK(hipMalloc((void **)&b->d_desc, sizeof(BatchMember) * 4 * (size_t)b->cap));
+    BCHK(hipMalloc((void **)&b->d_cases, sizeof(CaseDev) * (size_t)b->cap));
+    BCHK(hipMalloc((void **)&b->d_first, sizeof(int) * BP_COUNT * (size_t)(b->cap + 1)));
   }
   std::vector<BatchMember> h(4 * (size_t)b->cap);
   const long long lead = b->members[0]->c->sweeps;
@@ -117,6 +126,50 @@ int build_descriptors(Batch *b) {
     b->nb_fine = first;
   }
   BCHK(hipMemcpy(b->d_desc, h.data(), sizeof(BatchMember) * h.size(), hipMemcpyHostToDevice));
+  // per-member arguments of the step kernels and their work-list tables
+  std::vector<CaseDev> cd((size_t)b->cap);
+  std::vector<int> first((size_t)BP_COUNT * (b->cap + 1), 0);
+  for (int i = 0; i < n; ++i) {
+    const Refined *r = b->members[i];
+    const Lattice *c = r->c, *f = r->f;
+    CaseDev &d = cd[i];
+    d.Ac = c->pop; d.Af = f->pop; d.Lc = c->L; d.Lf = f->L;
+    d.c_scale = c->scale; d.c_inv = c->inv_scale; d.f_scale = f->scale; d.f_inv = f->inv_scale;
+    d.B = r->B; d.V = r->V; d.F = r->F; d.T = r->d_T;
+    d.before = r->d_before; d.after = r->d_after; d.regc = r->d_regc; d.regf = r->d_regf; d.part = r->d_part;
+    d.px = r->d_px; d.py = r->d_py; d.mf = r->d_mf; d.err = r->d_err;
+    d.volc = r->dxc * r->dxc * r->dxc;
+    d.volf = r->dxf * r->dxf * r->dxf;
+    d.inv_volc = 1.0 / d.volc;
+    d.loads_scale = d.volf * (r->dxf / r->dtf) / r->dtc / (0.5 * r->span);
+    d.W = WallLinks{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
+    d.wall_blocks = f->nlinks ? r->wall_blocks : 0;
+    d.K = LegacyLinks{c->d_legx, c->d_legz, c->d_legdir, c->d_legkind, c->d_pdf, c->d_pdfnd, c->nleg};
+    d.U = c->U;
+    d.ctl = c->d_ctl;
+    d.hist = c->d_hist;
+    d.cpar = (int)(((c->sweeps - lead) % 2 + 2) % 2);
+    d.fpar = (int)(f->sweeps % 2);
+    const long long nb0 = r->B.nb[0], nb1 = r->B.nb[1], nb2 = r->B.nb[2];
+    const long long items[BP_COUNT] = {
+        (nb0 + 6) * (nb1 + 6) * (nb2 + 6),                 // donor moments (128)
+        4LL * r->nt * c->nz * FLUX_LANES,                  // coarse face flux (256)
+        (long long)c->nleg,                                // legacy farfield (128)
+        (2 * nb0 + 4) * (nb1 + 6) * (nb2 + 6),             // prolongation x (256)
+        (2 * nb0 + 4) * (2 * nb1 + 4) * (nb2 + 6),         // prolongation y (256)
+        (long long)f->nx * f->ny * f->nz,                  // prolongation z + injection (256)
+        4LL * (2 * r->nt) * f->nz * FLUX_LANES,            // fine face flux (256)
+        (long long)d.wall_blocks * 256,                    // walls (256)
+        4LL * r->nt * c->nz * 64,                          // reflux, a wave per cell (256)
+        8 * nb0 * nb1 * nb2,                               // fine moments (256)
+        nb0 * nb1 * nb2};                                  // restriction (128)
+    const int threads[BP_COUNT] = {128, 256, 128, 256, 256, 256, 256, 256, 256, 256, 128};
+    for (int k = 0; k < BP_COUNT; ++k)
+      first[(size_t)k * (b->cap + 1) + i + 1] = first[(size_t)k * (b->cap + 1) + i] + (int)blocks_for(items[k], threads[k]);
+  }
+  for (int k = 0; k < BP_COUNT; ++k) b->nblk[k] = first[(size_t)k * (b->cap + 1) + n];
+  BCHK(hipMemcpy(b->d_cases, cd.data(), sizeof(CaseDev) * cd.size(), hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(b->d_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice));
   b->desc_key = key;
   return 0;
 }
@@ -127,44 +180,34 @@ void batched_sweep(Batch *b, const BatchMember *d, int nblocks) {
                      nblocks);
 }
 
-// Run `phase(member)` for every member on the member's own stream, forked from and joined
-// back into the batch stream.
-template <class F>
-void fork_join(Batch *b, F &&phase) {
-  (void)hipEventRecord(b->fork, b->stream);
-  for (size_t i = 0; i < b->members.size(); ++i) {
-    Refined *r = b->members[i];
-    (void)hipStreamWaitEvent(r->c->stream, b->fork, 0);
-    phase(r);
-    (void)hipEventRecord(b->join[i], r->c->stream);
-  }
-  for (size_t i = 0; i < b->members.size(); ++i) (void)hipStreamWaitEvent(b->stream, b->join[i], 0);
-}
-
-// One coarse step of every member (host counters advance as in refined_coarse_step).
+// One coarse step of every member: the single-case kernel sequence, each kernel one launch
+// over all members (host counters advance as in refined_coarse_step).
 template <int S>
 void batch_step(Batch *b, bool sample) {
-  const int lead_parity = (int)(b->members[0]->c->sweeps % 2);
-  const int nc = b->cap;
-  fork_join(b, [](Refined *r) { phase_before_coarse<S>(r); });
-  batched_sweep<S>(b, b->d_desc + (size_t)lead_parity * nc, b->nb_coarse);
+  const int nm = (int)b->members.size(), lead = (int)(b->members[0]->c->sweeps % 2), nc = b->cap;
+  hipStream_t st = b->stream;
+  const CaseDev *C = b->d_cases;
+  auto F = [b](int k) { return b->d_first + (size_t)k * (b->cap + 1); };
+  auto go = [&](int k) { return b->nblk[k] > 0; };
+  if (go(BP_DONOR)) hipLaunchKernelGGL((kb_donor<S>), dim3(b->nblk[BP_DONOR]), dim3(128), 0, st, C, F(BP_DONOR), nm, lead, 0, 0);
+  batched_sweep<S>(b, b->d_desc + (size_t)lead * nc, b->nb_coarse);
   for (Refined *r : b->members) r->c->sweeps++;
-  fork_join(b, [](Refined *r) {
-    phase_after_coarse<S>(r);
-    phase_before_fine<S>(r, 0);
-  });
-  batched_sweep<S>(b, b->d_desc + (size_t)2 * nc, b->nb_fine);
-  for (Refined *r : b->members) r->f->sweeps++;
-  fork_join(b, [](Refined *r) {
-    phase_after_fine<S>(r, 0);
-    phase_before_fine<S>(r, 1);
-  });
-  batched_sweep<S>(b, b->d_desc + (size_t)3 * nc, b->nb_fine);
-  for (Refined *r : b->members) r->f->sweeps++;
-  fork_join(b, [sample](Refined *r) {
-    phase_after_fine<S>(r, 1);
-    phase_end<S>(r, sample);
-  });
+  if (go(BP_FLUXC)) hipLaunchKernelGGL((kb_flux_coarse<S>), dim3(b->nblk[BP_FLUXC]), dim3(256), 0, st, C, F(BP_FLUXC), nm, lead, 1);
+  if (go(BP_LEGACY)) hipLaunchKernelGGL((kb_legacy<S>), dim3(b->nblk[BP_LEGACY]), dim3(128), 0, st, C, F(BP_LEGACY), nm, lead, 1);
+  if (go(BP_DONOR)) hipLaunchKernelGGL((kb_donor<S>), dim3(b->nblk[BP_DONOR]), dim3(128), 0, st, C, F(BP_DONOR), nm, lead, 1, 1);
+  for (int sub = 0; sub < 2; ++sub) {
+    if (go(BP_PX)) hipLaunchKernelGGL(kb_prolong_x, dim3(b->nblk[BP_PX]), dim3(256), 0, st, C, F(BP_PX), nm, sub * 0.5);
+    if (go(BP_PY)) hipLaunchKernelGGL(kb_prolong_y, dim3(b->nblk[BP_PY]), dim3(256), 0, st, C, F(BP_PY), nm);
+    if (go(BP_PZ)) hipLaunchKernelGGL((kb_prolong_z<S>), dim3(b->nblk[BP_PZ]), dim3(256), 0, st, C, F(BP_PZ), nm, sub);
+    batched_sweep<S>(b, b->d_desc + (size_t)(2 + sub) * nc, b->nb_fine);
+    for (Refined *r : b->members) r->f->sweeps++;
+    if (go(BP_FLUXF)) hipLaunchKernelGGL((kb_flux_fine<S>), dim3(b->nblk[BP_FLUXF]), dim3(256), 0, st, C, F(BP_FLUXF), nm, sub + 1, sub);
+    if (go(BP_WALLS)) hipLaunchKernelGGL((kb_walls<S>), dim3(b->nblk[BP_WALLS]), dim3(256), 0, st, C, F(BP_WALLS), nm, sub + 1, sub);
+  }
+  if (go(BP_REFLUX)) hipLaunchKernelGGL((kb_reflux<S>), dim3(b->nblk[BP_REFLUX]), dim3(256), 0, st, C, F(BP_REFLUX), nm, lead, 1);
+  if (go(BP_FMOM)) hipLaunchKernelGGL((kb_fine_moments<S>), dim3(b->nblk[BP_FMOM]), dim3(256), 0, st, C, F(BP_FMOM), nm, 2);
+  if (go(BP_RESTRICT)) hipLaunchKernelGGL((kb_restrict<S>), dim3(b->nblk[BP_RESTRICT]), dim3(128), 0, st, C, F(BP_RESTRICT), nm, lead, 1);
+  hipLaunchKernelGGL(kb_loads, dim3(nm), dim3(64), 0, st, C, sample ? 1 : 0);
 }
 
 // Order the batch stream after all work queued on the members' streams (or the reverse).
@@ -195,7 +238,8 @@ int ak_batch_create(void **handle, int device, int storage, void *stream) {
   b->storage = storage;
   b->stream = (hipStream_t)stream;
   if (!b->stream) {
-    if (hipStreamCreate(&b->stream) != hipSuccess) {
+    // non-blocking: cases being set up on the null stream while a chunk runs do not wait for it
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
       delete b;
       return bfail("hipStreamCreate failed");
     }
@@ -218,7 +262,8 @@ int ak_batch_destroy(void *handle) {
   b->graphs.clear();
   for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
   (void)hipEventDestroy(b->fork);
-  if (b->d_desc) (void)hipFree(b->d_desc);
+  for (void *p : {(void *)b->d_desc, (void *)b->d_cases, (void *)b->d_first})
+    if (p) (void)hipFree(p);
   if (b->own_stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return 0;
@@ -231,14 +276,13 @@ int ak_batch_add(void *handle, void *refined) {
   if (r->c->storage != b->storage) return bfail("member storage %d differs from the batch's %d", r->c->storage, b->storage);
   if (r->c->dev != b->dev) return bfail("member on device %d, batch on device %d", r->c->dev, b->dev);
   if (check_ready(r->c) || check_ready(r->f)) return bfail("member populations not bound");
-  if (r->c->stream == b->stream) return bfail("a member must have a stream of its own");
-  for (Refined *m : b->members) {
+  for (Refined *m : b->members)
     if (m == r) return bfail("case already in the batch");
-    if (m->c->stream == r->c->stream) return bfail("two members share a stream");
-  }
   if (r->f->sweeps % 2 != 1) return bfail("member not at a coarse-step boundary");
   hipEvent_t e = nullptr;
   BCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  BCHK(hipStreamSynchronize(b->stream));
+  b->graphs.clear();
   b->join.push_back(e);
   b->members.push_back(r);
   b->gen++;
@@ -250,7 +294,10 @@ int ak_batch_remove(void *handle, void *refined) {
   if (!b) return bfail("null batch handle");
   auto it = std::find(b->members.begin(), b->members.end(), R(refined));
   if (it == b->members.end()) return bfail("case not in the batch");
-  // the member's stream must see the batch's work before it is used alone again
+  // the batch's work is finished and its graphs are gone before the member can be used
+  // alone again or destroyed
+  BCHK(hipStreamSynchronize(b->stream));
+  b->graphs.clear();
   if (order_members_after(b)) return -1;
   b->members.erase(it);
   (void)hipEventDestroy(b->join.back());
@@ -268,8 +315,10 @@ int ak_batch_set_graph_mode(void *handle, int enable) {
   return 0;
 }
 
-// nsteps coarse steps of every member.  sample_every is 1 (loads every step, to each
-// member's history) or 0 (none).  Step numbers continue each member's own count.
+// nsteps coarse steps of every member, enqueued on the batch stream (returns before they
+// run).  sample_every is 1 (loads every step, to each member's history) or 0 (none).  Step
+// numbers continue each member's own count.  Synchronise the batch (ak_batch_synchronize)
+// before reading a member's history or state.
 int ak_batch_run(void *handle, int nsteps, int sample_every) {
   Batch *b = BT(handle);
   if (!b) return bfail("null batch handle");
@@ -318,7 +367,6 @@ int ak_batch_run(void *handle, int nsteps, int sample_every) {
       r->steps++;
     }
   }
-  if (order_members_after(b)) return -1;
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail("batch run: %s", hipGetErrorString(e));
   return 0;

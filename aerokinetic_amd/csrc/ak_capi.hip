@@ -882,4 +882,5 @@ int ak_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
 }  // extern "C"
 
 #include "ak_refined_capi.hpp"
+#include "ak_batch.hpp"
 #include "ak_batch_capi.hpp"

@@ -248,10 +248,8 @@ struct WallLinks {
 constexpr int MAX_ELEM = 8;
 
 template <int S, bool PE, bool LOADS>
-__global__ __launch_bounds__(256) void k_walls(typename St<S>::T *__restrict__ A, Desc L, WallLinks W,
-                                               float scale, float inv_scale, double *__restrict__ partials) {
+__device__ __forceinline__ void walls_body(typename St<S>::T *__restrict__ A, Desc L, WallLinks W, float scale, float inv_scale, double *__restrict__ partials, long long t, int blk) {
   using C = typename St<S>::C;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long total = (long long)W.n * L.nz;
   double f[5] = {0, 0, 0, 0, 0};
   int el = -1;
@@ -305,7 +303,7 @@ __global__ __launch_bounds__(256) void k_walls(typename St<S>::T *__restrict__ A
         for (int c = 0; c < 5; ++c) {
           double s = red[0][c];
           for (int k = 1; k < nw; ++k) s += red[k][c];
-          partials[((long long)blockIdx.x * W.nelem + e) * 5 + c] = s;
+          partials[((long long)blk * W.nelem + e) * 5 + c] = s;
         }
       }
       __syncthreads();
@@ -313,9 +311,12 @@ __global__ __launch_bounds__(256) void k_walls(typename St<S>::T *__restrict__ A
   }
 }
 
-// Sum the per-block wall partials (fixed order), convert to coefficients and write
-// the loads part of a history row; clear the diagnostics accumulator.
-// cl = Fy fscale, cd = Fx fscale, cm = -Mz mscale, mass exchange = mass * vscale.
+template <int S, bool PE, bool LOADS>
+__global__ __launch_bounds__(256) void k_walls(typename St<S>::T *__restrict__ A, Desc L, WallLinks W,
+                                               float scale, float inv_scale, double *__restrict__ partials) {
+  walls_body<S, PE, LOADS>(A, L, W, scale, inv_scale, partials, (long long)blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.x);
+}
+
 // Step number and history-row cursor of a lattice, kept in device memory so that a
 // captured step (HIP graph) replays with the right row and step number.  At the
 // start of a step `step` is that step's number; the kernel that ends a step
@@ -338,10 +339,12 @@ __global__ void k_tick(StepCtl *ctl) {
   if (threadIdx.x == 0) ctl->step += 1;
 }
 
-// Fold the wall-load partials into the current history row.  diag != nullptr:
+// Fold the wall-load partials (per-block sums, added in a fixed order) into the current
+// history row: cl = Fy fscale, cd = Fx fscale, cm = -Mz mscale, mass exchange = mass
+// vscale.  diag != nullptr:
 // also clear the diagnostic slots for the sweep that follows (k_diag_final ends
 // the step); diag == nullptr: this kernel ends the sampled step.
-__global__ void k_loads_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
+__device__ __forceinline__ void loads_final_body(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
                               double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
   __shared__ double acc[MAX_ELEM * 5][64];
   const int lane = threadIdx.x;  // 64 threads
@@ -381,6 +384,11 @@ __global__ void k_loads_final(const double *__restrict__ partials, int nblocks, 
     diag->s[k].rhomin = (unsigned long long)__double_as_longlong(1e300);
     diag->s[k].nonfinite = 0ull;
   }
+}
+
+__global__ void k_loads_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
+                              double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
+  loads_final_body(partials, nblocks, nelem, fscale, mscale, vscale, ctl, hist, diag);
 }
 
 __global__ void k_diag_final(const DiagAcc *diag, StepCtl *ctl, double *__restrict__ hist) {
@@ -631,10 +639,9 @@ __device__ __forceinline__ typename St<S>::C ubb_value(const typename St<S>::T *
 }
 
 template <int S, bool PE>
-__global__ __launch_bounds__(128) void k_legacy(typename St<S>::T *__restrict__ A, Desc L, LegacyLinks K,
-                                                double U, float scale, float inv_scale) {
+__device__ __forceinline__ void legacy_body(typename St<S>::T *__restrict__ A, Desc L, LegacyLinks K, double U, float scale, float inv_scale, long long t) {
   using C = typename St<S>::C;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = (int)t;
   if (k >= K.n) return;
   int x, y;
   ff_unpack(K.x[k], x, y);
@@ -661,6 +668,12 @@ __global__ __launch_bounds__(128) void k_legacy(typename St<S>::T *__restrict__ 
     K.pdf_nd[k] = v;
   }
   stv<S>(A + loc_in<PE>(L, x, y, z, id), val, scale);
+}
+
+template <int S, bool PE>
+__global__ __launch_bounds__(128) void k_legacy(typename St<S>::T *__restrict__ A, Desc L, LegacyLinks K,
+                                                double U, float scale, float inv_scale) {
+  legacy_body<S, PE>(A, L, K, U, scale, inv_scale, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------------ init / exchange

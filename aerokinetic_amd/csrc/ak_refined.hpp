@@ -111,10 +111,8 @@ struct FreeState {
 
 // Pre-collision moments of the coarse donor block (box +- 3 cells; z wraps).
 template <int S, bool PE>
-__global__ void k_donor_moments(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F,
-                                float inv_scale, double *__restrict__ M, int *__restrict__ err) {
+__device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F, float inv_scale, double *__restrict__ M, int *__restrict__ err, long long t) {
   const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)dx * dy * dz) return;
   const int i = (int)(t % dx), j = (int)((t / dx) % dy), k = (int)(t / ((long long)dx * dy));
   const int x = B.lo[0] - 3 + i, y = B.lo[1] - 3 + j;
@@ -133,6 +131,12 @@ __global__ void k_donor_moments(const typename St<S>::T *__restrict__ A, Desc L,
   if (e) atomicOr(err, e);
 }
 
+template <int S, bool PE>
+__global__ void k_donor_moments(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F,
+                                float inv_scale, double *__restrict__ M, int *__restrict__ err) {
+  donor_moments_body<S, PE>(A, L, B, F, inv_scale, M, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Fine donor cells (the two pad layers in x and y) from the time-interpolated,
 // prolonged, rescaled coarse moments.  The 5-point conservative prolongation is
 // separable, so it runs as three one-axis passes over the donor block:
@@ -148,10 +152,8 @@ __device__ __forceinline__ double prolong_weight(int side, int k) {
   return side ? -lo[k + 2] + (k == 0 ? 2.0 : 0.0) : lo[k + 2];
 }
 
-__global__ void k_prolong_x(const double *__restrict__ M0, const double *__restrict__ M1, double frac, BoxGeom B,
-                            double *__restrict__ X) {
+__device__ __forceinline__ void prolong_x_body(const double *__restrict__ M0, const double *__restrict__ M1, double frac, BoxGeom B, double *__restrict__ X, long long t) {
   const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)cxn * dy * dz) return;
   const int cx = (int)(t % cxn), qy = (int)((t / cxn) % dy), qz = (int)(t / ((long long)cxn * dy));
   // the y pass only reads columns of donor cells: every row next to an x pad, and only
@@ -168,9 +170,13 @@ __global__ void k_prolong_x(const double *__restrict__ M0, const double *__restr
   for (int k = 0; k < 10; ++k) o[k] = m[k];
 }
 
-__global__ void k_prolong_y(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y) {
+__global__ void k_prolong_x(const double *__restrict__ M0, const double *__restrict__ M1, double frac, BoxGeom B,
+                            double *__restrict__ X) {
+  prolong_x_body(M0, M1, frac, B, X, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__device__ __forceinline__ void prolong_y_body(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y, long long t) {
   const int dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4, cyn = 2 * B.nb[1] + 4;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)cxn * cyn * dz) return;
   const int cx = (int)(t % cxn), cy = (int)((t / cxn) % cyn), qz = (int)(t / ((long long)cxn * cyn));
   const int p = B.pad;
@@ -186,10 +192,12 @@ __global__ void k_prolong_y(const double *__restrict__ X, BoxGeom B, double *__r
   for (int k = 0; k < 10; ++k) o[k] = m[k];
 }
 
+__global__ void k_prolong_y(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y) {
+  prolong_y_body(X, B, Y, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 template <int S, bool PE>
-__global__ void k_prolong_z_inject(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V,
-                                   const double *__restrict__ Y, float scale, int *__restrict__ err) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prolong_z_inject_body(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V, const double *__restrict__ Y, float scale, int *__restrict__ err, long long t) {
   if (t >= (long long)L.nx * L.ny * L.nz) return;
   const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
   const int p = B.pad;
@@ -210,6 +218,12 @@ __global__ void k_prolong_z_inject(typename St<S>::T *__restrict__ A, Desc L, Bo
   if (e) atomicOr(err, e);
 }
 
+template <int S, bool PE>
+__global__ void k_prolong_z_inject(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V,
+                                   const double *__restrict__ Y, float scale, int *__restrict__ err) {
+  prolong_z_inject_body<S, PE>(A, L, B, V, Y, scale, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Face flux registers.  Face f = 2*axis + (side > 0), axis in {0 (x faces), 1 (y faces)}.
 struct FluxTerm {
   int8_t d, s0, s1;  // direction, tangential shifts (in-plane axis, z)
@@ -220,72 +234,87 @@ struct FluxTable {  // host-built, copied once into a device buffer
   int n[2];
 };
 
-// One wavefront per face element: lane k evaluates term k (< 64), then a fixed-order
-// butterfly sums the wave, so the register value does not depend on scheduling.
+// FLUX_LANES lanes per face element: lane j sums terms j, j + FLUX_LANES, ... in that
+// order, then a fixed-order butterfly over the element's lanes adds the partial sums, so
+// the register value does not depend on scheduling.
+constexpr int FLUX_LANES = 8;
+
 template <int S, bool PE>
-__device__ __forceinline__ void flux_wave(const typename St<S>::T *A, const Desc &L, const FluxTable *__restrict__ T,
-                                          int axis, int face_cell, int tpos, int zpos, float inv, int lane,
-                                          double (&out)[4]) {
+__device__ __forceinline__ void flux_terms(const typename St<S>::T *A, const Desc &L, const FluxTable *__restrict__ T,
+                                           int axis, int face_cell, int tpos, int zpos, float inv, int sub_lane,
+                                           bool active, double (&out)[4]) {
   out[0] = out[1] = out[2] = out[3] = 0.0;
-  if (lane < T->n[axis]) {
-    const FluxTerm &tm = T->term[axis][lane];
-    const int nrm = axis == 0 ? cxq(tm.d) : cyq(tm.d);
-    const int a = face_cell + (nrm < 0 ? 1 : 0);
-    const int tt = tpos - tm.s0;
-    const int zz = ((zpos - tm.s1) % L.nz + L.nz) % L.nz;
-    const int x = axis == 0 ? a : tt, y = axis == 0 ? tt : a;
-    const double v = (double)ld<S>(A + loc_out<PE>(L, x, y, zz, tm.d), inv);
-    for (int c = 0; c < 4; ++c) out[c] = v * tm.coef[c];
+  if (active) {
+    for (int k = sub_lane; k < T->n[axis]; k += FLUX_LANES) {
+      const FluxTerm &tm = T->term[axis][k];
+      const int nrm = axis == 0 ? cxq(tm.d) : cyq(tm.d);
+      const int a = face_cell + (nrm < 0 ? 1 : 0);
+      const int tt = tpos - tm.s0;
+      const int zz = ((zpos - tm.s1) % L.nz + L.nz) % L.nz;
+      const int x = axis == 0 ? a : tt, y = axis == 0 ? tt : a;
+      const double v = (double)ld<S>(A + loc_out<PE>(L, x, y, zz, tm.d), inv);
+      for (int c = 0; c < 4; ++c) out[c] += v * tm.coef[c];
+    }
   }
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1)
+  for (int o = FLUX_LANES / 2; o >= 1; o >>= 1)
     for (int c = 0; c < 4; ++c) out[c] += __shfl_xor(out[c], o, 64);
 }
 
 // Coarse face registers reg[f][t][z][4] (integrated over one coarse step: x dx_c^3).
 template <int S, bool PE>
-__global__ __launch_bounds__(256) void k_flux_coarse(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
-                                                     const FluxTable *__restrict__ T, double vol, float inv,
-                                                     double *__restrict__ reg) {
+__device__ __forceinline__ void flux_coarse_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T, double vol, float inv, double *__restrict__ reg, long long t) {
   const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
-  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-  const int lane = threadIdx.x % 64;
-  if (w >= 4LL * nt * L.nz) return;
+  const long long w = t / FLUX_LANES;  // face element; its lanes are consecutive within a wave
+  const int sub_lane = (int)(t % FLUX_LANES);
+  const long long ne = 4LL * nt * L.nz;
   const int zpos = (int)(w % L.nz), ti = (int)((w / L.nz) % nt), f = (int)(w / ((long long)L.nz * nt));
-  const int axis = f / 2, side = f % 2;
+  const int axis = (f / 2) & 1, side = f % 2;
   const int other = 1 - axis;
-  if (ti >= B.nb[other]) return;  // whole wave leaves together
+  const bool active = w < ne && ti < B.nb[other];
   const int face_cell = side ? B.hi[axis] - 1 : B.lo[axis] - 1;
   double o[4];
-  flux_wave<S, PE>(A, L, T, axis, face_cell, B.lo[other] + ti, zpos, inv, lane, o);
-  if (lane == 0) {
+  flux_terms<S, PE>(A, L, T, axis, face_cell, B.lo[other] + ti, zpos, inv, sub_lane, active, o);
+  if (active && sub_lane == 0) {
     double *r = reg + (((long long)f * nt + ti) * L.nz + zpos) * 4;
     for (int c = 0; c < 4; ++c) r[c] = o[c] * vol;
   }
 }
 
+template <int S, bool PE>
+__global__ __launch_bounds__(256) void k_flux_coarse(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
+                                                     const FluxTable *__restrict__ T, double vol, float inv,
+                                                     double *__restrict__ reg) {
+  flux_coarse_body<S, PE>(A, L, B, T, vol, inv, reg, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Fine face registers per fine face element regf[f][t_f][z_f][4] (x dx_f^3), summed over
 // the two substeps (accumulate = 1 on the second); the reflux folds the 2x2 children.
+template <int S, bool PE>
+__device__ __forceinline__ void flux_fine_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T, double vol, float inv, int accumulate, double *__restrict__ regf, long long t) {
+  const int nt = 2 * (B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1]);
+  const long long w = t / FLUX_LANES;  // face element; its lanes are consecutive within a wave
+  const int sub_lane = (int)(t % FLUX_LANES);
+  const long long ne = 4LL * nt * L.nz;
+  const int zf = (int)(w % L.nz), tf = (int)((w / L.nz) % nt), f = (int)(w / ((long long)L.nz * nt));
+  const int axis = (f / 2) & 1, side = f % 2;
+  const int other = 1 - axis;
+  const bool active = w < ne && tf < 2 * B.nb[other];
+  const int n_axis = 2 * B.nb[axis];
+  const int face_cell = side ? B.pad + n_axis - 1 : B.pad - 1;
+  double o[4];
+  flux_terms<S, PE>(A, L, T, axis, face_cell, B.pad + tf, zf, inv, sub_lane, active, o);
+  if (active && sub_lane == 0) {
+    double *r = regf + (((long long)f * nt + tf) * L.nz + zf) * 4;
+    for (int c = 0; c < 4; ++c) r[c] = (accumulate ? r[c] : 0.0) + o[c] * vol;
+  }
+}
+
 template <int S, bool PE>
 __global__ __launch_bounds__(256) void k_flux_fine(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
                                                    const FluxTable *__restrict__ T, double vol, float inv,
                                                    int accumulate, double *__restrict__ regf) {
-  const int nt = 2 * (B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1]);
-  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-  const int lane = threadIdx.x % 64;
-  if (w >= 4LL * nt * L.nz) return;
-  const int zf = (int)(w % L.nz), tf = (int)((w / L.nz) % nt), f = (int)(w / ((long long)L.nz * nt));
-  const int axis = f / 2, side = f % 2;
-  const int other = 1 - axis;
-  if (tf >= 2 * B.nb[other]) return;
-  const int n_axis = 2 * B.nb[axis];
-  const int face_cell = side ? B.pad + n_axis - 1 : B.pad - 1;
-  double o[4];
-  flux_wave<S, PE>(A, L, T, axis, face_cell, B.pad + tf, zf, inv, lane, o);
-  if (lane == 0) {
-    double *r = regf + (((long long)f * nt + tf) * L.nz + zf) * 4;
-    for (int c = 0; c < 4; ++c) r[c] = (accumulate ? r[c] : 0.0) + o[c] * vol;
-  }
+  flux_fine_body<S, PE>(A, L, B, T, vol, inv, accumulate, regf, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Equilibrium reflux on the coarse cells just outside each face: one wavefront per
@@ -305,12 +334,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <int S, bool PE>
-__global__ __launch_bounds__(256) void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
-                                                const double *__restrict__ rc, const double *__restrict__ rf,
-                                                double inv_vol, float scale, float inv, int *__restrict__ err) {
+__device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const double *__restrict__ rc, const double *__restrict__ rf, double inv_vol, float scale, float inv, int *__restrict__ err, long long t) {
   // rf holds fine face elements [f][2 nt][2 nz][4]
   const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
-  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const long long w = t / 64;
   const int lane = threadIdx.x % 64;
   if (w >= 4LL * nt * L.nz) return;
   const int zpos = (int)(w % L.nz), ti = (int)((w / L.nz) % nt), f = (int)(w / ((long long)L.nz * nt));
@@ -353,12 +380,17 @@ __global__ __launch_bounds__(256) void k_reflux(typename St<S>::T *__restrict__ 
   if (e) atomicOr(err, e);
 }
 
+template <int S, bool PE>
+__global__ __launch_bounds__(256) void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
+                                                const double *__restrict__ rc, const double *__restrict__ rf,
+                                                double inv_vol, float scale, float inv, int *__restrict__ err) {
+  reflux_body<S, PE>(A, L, B, rc, rf, inv_vol, scale, inv, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Moments of every owned fine cell outside the fine virtual core (first restriction stage).
 template <int S, bool PE>
-__global__ void k_fine_moments(const typename St<S>::T *__restrict__ Af, Desc Lf, BoxGeom B, float inv,
-                               double *__restrict__ Mf, int *__restrict__ err) {
+__device__ __forceinline__ void fine_moments_body(const typename St<S>::T *__restrict__ Af, Desc Lf, BoxGeom B, float inv, double *__restrict__ Mf, int *__restrict__ err, long long t) {
   const int fx = 2 * B.nb[0], fy = 2 * B.nb[1], fz = 2 * B.nb[2];
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)fx * fy * fz) return;
   const int x = (int)(t % fx), y = (int)((t / fx) % fy), z = (int)(t / ((long long)fx * fy));
   if (in_core(B, x / 2, y / 2)) return;
@@ -370,13 +402,17 @@ __global__ void k_fine_moments(const typename St<S>::T *__restrict__ Af, Desc Lf
   if (e) atomicOr(err, e);
 }
 
+template <int S, bool PE>
+__global__ void k_fine_moments(const typename St<S>::T *__restrict__ Af, Desc Lf, BoxGeom B, float inv,
+                               double *__restrict__ Mf, int *__restrict__ err) {
+  fine_moments_body<S, PE>(Af, Lf, B, inv, Mf, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Covered coarse cells from the fine state: free stream in the virtual core, otherwise
 // the 2x2x2 mean of the children's moments (from k_fine_moments), rescaled fine->coarse
 // and regularised.
 template <int S, bool PEC>
-__global__ void k_restrict(typename St<S>::T *__restrict__ Ac, Desc Lc, const double *__restrict__ Mf, BoxGeom B,
-                           Levels V, FreeState F, float scale, int *__restrict__ err) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void restrict_body(typename St<S>::T *__restrict__ Ac, Desc Lc, const double *__restrict__ Mf, BoxGeom B, Levels V, FreeState F, float scale, int *__restrict__ err, long long t) {
   if (t >= (long long)B.nb[0] * B.nb[1] * B.nb[2]) return;
   const int bx = (int)(t % B.nb[0]), by = (int)((t / B.nb[0]) % B.nb[1]), bz = (int)(t / ((long long)B.nb[0] * B.nb[1]));
   const int x = B.lo[0] + bx, y = B.lo[1] + by, z = B.lo[2] + bz;
@@ -399,6 +435,12 @@ __global__ void k_restrict(typename St<S>::T *__restrict__ Ac, Desc Lc, const do
   }
   scatter_in<S, PEC>(Ac, Lc, x, y, z, scale, g);
   if (e) atomicOr(err, e);
+}
+
+template <int S, bool PEC>
+__global__ void k_restrict(typename St<S>::T *__restrict__ Ac, Desc Lc, const double *__restrict__ Mf, BoxGeom B,
+                           Levels V, FreeState F, float scale, int *__restrict__ err) {
+  restrict_body<S, PEC>(Ac, Lc, Mf, B, V, F, scale, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Initial pre-collision state of one level: rho = 1, seeded momentum, regularised

@@ -139,14 +139,14 @@ void phase_after_coarse(Refined *r) {
   Lattice *c = r->c;
   using T_ = typename St<S>::T;
   const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
-  const unsigned cwave_blocks = blocks_for(4LL * r->nt * c->nz * 64, 256);
+  const unsigned cflux_blocks = blocks_for(4LL * r->nt * c->nz * FLUX_LANES, 256);
   const double volc = r->dxc * r->dxc * r->dxc;
   if (c->sweeps % 2 == 0) {
-    hipLaunchKernelGGL((k_flux_coarse<S, true>), dim3(cwave_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
+    hipLaunchKernelGGL((k_flux_coarse<S, true>), dim3(cflux_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
     launch_boundary<S, true>(c, false);
     hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
   } else {
-    hipLaunchKernelGGL((k_flux_coarse<S, false>), dim3(cwave_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
+    hipLaunchKernelGGL((k_flux_coarse<S, false>), dim3(cflux_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
     launch_boundary<S, false>(c, false);
     hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
   }
@@ -175,7 +175,7 @@ void phase_after_fine(Refined *r, int sub) {
   Lattice *f = r->f;
   using T_ = typename St<S>::T;
   const double volf = r->dxf * r->dxf * r->dxf;
-  const unsigned fflux_blocks = blocks_for(4LL * (2 * r->nt) * f->nz * 64, 256);
+  const unsigned fflux_blocks = blocks_for(4LL * (2 * r->nt) * f->nz * FLUX_LANES, 256);
   WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
   double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;
   if (f->sweeps % 2 == 0) {
@@ -298,7 +298,7 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
     delete r;
     return -1;
   }
-  (void)hipMemset(r->d_err, 0, sizeof(int));
+  (void)hipMemsetAsync(r->d_err, 0, sizeof(int), r->c->stream);
   if (hipMemcpy(r->d_T, &r->T, sizeof(FluxTable), hipMemcpyHostToDevice) != hipSuccess) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
     if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
@@ -360,7 +360,7 @@ int ak_refined_init_seeded(void *handle, const double *p) {
   sf.dx = r->dxf;
   sf.ox = r->B.lo[0] * r->dxc - r->B.pad * r->dxf;
   sf.oy = r->B.lo[1] * r->dxc - r->B.pad * r->dxf;
-  (void)hipMemset(r->d_err, 0, sizeof(int));
+  (void)hipMemsetAsync(r->d_err, 0, sizeof(int), r->c->stream);
   const long long ncv = (long long)c->nx * c->ny * c->nz, nfv = (long long)f->nx * f->ny * f->nz;
   switch (c->storage) {
     case 0:
@@ -474,7 +474,7 @@ int ak_refined_transfer_errors(void *handle) {
   int e = 0;
   HIPCHK(r->c, hipStreamSynchronize(r->c->stream));
   HIPCHK(r->c, hipMemcpy(&e, r->d_err, sizeof(int), hipMemcpyDeviceToHost));
-  (void)hipMemset(r->d_err, 0, sizeof(int));
+  (void)hipMemsetAsync(r->d_err, 0, sizeof(int), r->c->stream);
   return e;
 }
 

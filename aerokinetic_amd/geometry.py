@@ -47,21 +47,24 @@ def segments_intersect(a, b, c, d) -> bool:
 
 
 def _segment_hits(a, b, C, D):
-    """segments_intersect(a, b, C[k], D[k]) for every k at once (same tolerances)."""
+    """segments_intersect(a, b, c, d) elementwise over broadcast segment arrays (same tolerances)."""
     ab, cd, ca = b - a, D - C, C - a
     den = _cross(ab, cd)
-    tol = 1e-12 * np.maximum(np.linalg.norm(ab) * np.linalg.norm(cd, axis=1), 1e-30)
+    tol = 1e-12 * np.maximum(np.linalg.norm(ab, axis=-1) * np.linalg.norm(cd, axis=-1), 1e-30)
     par = np.abs(den) <= tol
     with np.errstate(divide="ignore", invalid="ignore"):
         t = _cross(ca, cd) / den
         s = _cross(ca, ab) / den
     hit = ~par & (t >= -1e-12) & (t <= 1 + 1e-12) & (s >= -1e-12) & (s <= 1 + 1e-12)
     col = par & (np.abs(_cross(ca, ab)) <= tol)
-    if col.any():
-        k = int(np.argmax(np.abs(ab)))
-        lo = np.maximum(min(a[k], b[k]), np.minimum(C[:, k], D[:, k]))
-        hi = np.minimum(max(a[k], b[k]), np.maximum(C[:, k], D[:, k]))
-        hit |= col & (lo <= hi + 1e-12)
+    if np.any(col):
+        shape = np.broadcast_shapes(np.shape(a), np.shape(b), np.shape(C), np.shape(D))
+        a_, b_, c_, d_ = (np.broadcast_to(v, shape) for v in (a, b, C, D))
+        k = np.argmax(np.abs(np.broadcast_to(ab, shape)), axis=-1)[..., None]
+        pick = lambda v: np.take_along_axis(v, k, axis=-1)[..., 0]
+        lo = np.maximum(np.minimum(pick(a_), pick(b_)), np.minimum(pick(c_), pick(d_)))
+        hi = np.minimum(np.maximum(pick(a_), pick(b_)), np.maximum(pick(c_), pick(d_)))
+        hit = hit | (col & (lo <= hi + 1e-12))
     return hit
 
 
@@ -87,9 +90,12 @@ class Contour:
         if area <= 1e-12 * float(np.ptp(pts, axis=0).max()) ** 2:
             raise ValueError("Contour has zero enclosed area")
         n = len(pts)
-        for i in range(n - 2):
-            j0, j1 = i + 2, (n - 1 if i == 0 else n)  # non-adjacent edges only
-            if j1 > j0 and _segment_hits(pts[i], nxt[i], pts[j0:j1], nxt[j0:j1]).any():
+        ii, jj = np.triu_indices(n, k=2)  # non-adjacent edge pairs (first and last edges meet)
+        keep = ~((ii == 0) & (jj == n - 1))
+        ii, jj = ii[keep], jj[keep]
+        for s0 in range(0, len(ii), 1 << 20):
+            i, j = ii[s0:s0 + (1 << 20)], jj[s0:s0 + (1 << 20)]
+            if _segment_hits(pts[i], nxt[i], pts[j], nxt[j]).any():
                 raise ValueError(f"Self-intersection in contour {self.name}")
         pts.setflags(write=False)
         object.__setattr__(self, "xy", pts)
@@ -131,16 +137,19 @@ class Contour:
         points = np.asarray(points, dtype=float)
         offsets = np.asarray(offsets, dtype=float)
         best = np.full(len(points), np.inf)
-        p = self.xy
-        for a, b in zip(p, np.roll(p, -1, axis=0)):
-            edge = b - a
-            den = _cross(offsets, edge)
+        a = self.xy
+        edge = np.roll(a, -1, axis=0) - a
+        step = max(1, (1 << 22) // max(len(a), 1))  # points per block of the (point, edge) table
+        for i in range(0, len(points), step):
+            pt, off = points[i:i + step, None, :], offsets[i:i + step, None, :]
+            den = _cross(off, edge[None])
             ok = np.abs(den) > 1e-14
-            rel = a - points
-            t = np.divide(_cross(rel, edge), den, out=np.full_like(best, np.inf), where=ok)
-            s = np.divide(_cross(rel, offsets), den, out=np.full_like(best, np.inf), where=ok)
+            rel = a[None] - pt
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t = np.where(ok, _cross(rel, edge[None]) / den, np.inf)
+                s = np.where(ok, _cross(rel, off) / den, np.inf)
             hit = ok & (t >= -1e-10) & (t <= 1 + 1e-10) & (s >= -1e-10) & (s <= 1 + 1e-10)
-            best = np.minimum(best, np.where(hit, t, np.inf))
+            best[i:i + step] = np.where(hit, t, np.inf).min(axis=1)
         if not np.isfinite(best).all():
             raise ValueError("A fluid-to-solid link has no geometric intersection")
         return np.clip(best, 1e-10, 1.0)
@@ -182,7 +191,7 @@ class Section:
                 if not touching:
                     ea, eb = e.xy, np.roll(e.xy, -1, axis=0)
                     fa, fb = f.xy, np.roll(f.xy, -1, axis=0)
-                    touching = any(_segment_hits(a, b, fa, fb).any() for a, b in zip(ea, eb))
+                    touching = bool(_segment_hits(ea[:, None], eb[:, None], fa[None], fb[None]).any())
                 if touching:
                     raise ValueError(f"Elements touch or overlap: {e.name}, {f.name}")
 

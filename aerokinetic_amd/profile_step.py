@@ -6,8 +6,9 @@
 
 Workloads: `uniform-c32` (384x256x8), `uniform-100m` (1536x1024x64) and
 `two-level` (coarse 256x192x8 + fine 160x96x16), all NACA0012 (see
-aerokinetic_amd.cases), and `copy`, a device-to-device copy of a buffer of known
-size (the calibration kernel for memory counters).  Each workload prints one
+aerokinetic_amd.cases); `copy`, a device-to-device copy of a buffer of known size
+(the calibration kernel for memory counters); `serve`, the batched service over
+seeded two-level requests (--steps = the per-case step cap).  Each workload prints one
 line per phase with HIP-event timings of the lattice's stream, so a profiler's
 per-kernel averages can be set beside them.
 """
@@ -93,19 +94,35 @@ def run_copy(gib, steps):
     _log(f"copy {int(gib * 2**30)} bytes: {ms:.4f} ms per copy, {gbs:.1f} GB/s (read + write)")
 
 
+def run_serve(storage, cases, rate, chunk, window, max_steps):
+    from . import serving as S
+    reqs = S.polar_requests(cases, seed=2026, rate=rate, chord=(16, 24), span=(8, 8), reynolds=(1e4, 3e4))
+    t0 = time.perf_counter()
+    rep = S.Server(storage=storage, chunk=chunk, window=window, max_steps=max_steps).serve(reqs)
+    s = rep.summary()
+    _log(f"serve {cases} requests {storage}: {time.perf_counter() - t0:.2f} s, {s['cases_per_hour']:.0f} cases/h, "
+         f"p50 {s['latency_p50_s']:.2f} s, p99 {s['latency_p99_s']:.2f} s, peak members {s['peak_members']}, "
+         f"batch steps {rep.gpu_steps}, serving-thread seconds {s['host_s']}")
+
+
 def main(argv=None):
     from . import cases
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--workload", choices=("uniform-c32", "uniform-100m", "two-level", "copy"), required=True)
+    ap.add_argument("--workload", choices=("uniform-c32", "uniform-100m", "two-level", "copy", "serve"), required=True)
     ap.add_argument("--storage", choices=("fp64", "fp32", "fp16"), default="fp64")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--sample-every", type=int, default=1)
     ap.add_argument("--copy-gib", type=float, default=4.0)
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly")
+    ap.add_argument("--cases", type=int, default=24, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=16.0, help="serve: Poisson arrival rate (1/s)")
+    ap.add_argument("--chunk", type=int, default=100, help="serve: coarse steps per batch call")
     a = ap.parse_args(argv)
     if a.workload == "copy":
         run_copy(a.copy_gib, a.steps)
+    elif a.workload == "serve":
+        run_serve(a.storage, a.cases, a.rate, a.chunk, 400, a.steps)
     elif a.workload == "two-level":
         run_two_level(cases.NACA0012_REFINED_256x192x8, a.storage, a.steps, a.warmup, a.sample_every, not a.no_graphs)
     else:

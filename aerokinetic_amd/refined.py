@@ -41,12 +41,50 @@ class TransferError(ValueError):
     pass
 
 
+def host_setup(project: Project, plan, pad=2):
+    """Host-side preparation of a two-level case (no GPU work): contours, level masks, the
+    fine wall links and the seeded-field parameters.  `RefinedSolver(..., host=...)` takes it,
+    so it can be computed on a worker thread while the GPU steps other cases."""
+    section, flow = project.section, project.flow
+    contours = section.normalized(flow.alpha_deg)
+    box = plan.box
+    nx, ny, _ = plan.shape
+    fine_shape = tuple(2 * n for n in box.shape)
+    fnx, fny = fine_shape[0] + 2 * pad, fine_shape[1] + 2 * pad
+    # cell masks: coarse covered cells are computed donors (2); fine solid (1), donor layers (2)
+    cmask = np.zeros((nx, ny), dtype=np.uint8)
+    cmask[box.lower[0]:box.upper[0], box.lower[1]:box.upper[1]] = 2
+    dxf = plan.fine_level.dx
+    origin = (plan.origin[0] + box.lower[0] * plan.coarse_level.dx - pad * dxf,
+              plan.origin[1] + box.lower[1] * plan.coarse_level.dx - pad * dxf)
+    lab = labels(contours, (fnx, fny), origin, dxf)[1:-1, 1:-1]
+    solid = lab > 0
+    core = np.repeat(np.repeat(virtual_core(box.shape)[:, :, 0], 2, 0), 2, 1)
+    owned_solid = solid[pad:-pad, pad:-pad]
+    if np.any(solid & ~np.pad(core, pad, constant_values=False)):
+        raise ValueError("A solid intersects the fluid donor band")
+    for k, e in enumerate(contours):
+        if not (lab == k + 1).any():
+            raise ValueError(f"Element {e.name} has no fine solid cells; increase resolution")
+    fmask = np.full((fnx, fny), 2, dtype=np.uint8)
+    fmask[pad:-pad, pad:-pad] = np.where(owned_solid, 1, 0)
+    links, fallback = wall_links(contours, (fnx, fny), origin, dxf)
+
+    class _P:  # uniform-plan view (shape, dx, span) used by the seeded field
+        pass
+    q = _P()
+    q.shape, q.dx, q.span = plan.shape, plan.coarse_level.dx, plan.span
+    return {"contours": contours, "fine_shape": fine_shape, "cmask": cmask, "fmask": fmask,
+            "owned_solid": owned_solid, "fine_origin_padded": origin, "links": links, "fallback": fallback,
+            "seed": seed_parameters(section, flow, q)}
+
+
 class RefinedSolver:
     HISTORY_COLUMNS = ["step", "t_star", "cl", "cd", "cm", "time_s"]
     pad = 2
 
     def __init__(self, project: Project, refinement=RefinementSettings(), progress=None, *, storage="fp64",
-                 device=0, base_plan=None, plan=None, fp16_scale=0.0, graphs=True):
+                 device=0, base_plan=None, plan=None, fp16_scale=0.0, graphs=True, host=None):
         import torch
         if not torch.cuda.is_available():
             raise _lib.NativeLibraryError("no HIP device visible to PyTorch-ROCm")
@@ -56,13 +94,15 @@ class RefinedSolver:
         if self.plan.estimated_memory_gib > self.grid.memory_limit_gib:
             raise MemoryError(f"Estimated {self.plan.estimated_memory_gib:.2f} GiB exceeds the explicit limit "
                               f"{self.grid.memory_limit_gib:.2f} GiB")
+        if storage not in _lib.STORE:
+            raise ValueError("storage must be fp64, fp32 or fp16")
         self.storage = storage
-        self.contours = self.section.normalized(self.flow.alpha_deg)
+        hs = host or host_setup(project, self.plan, self.pad)
+        self.contours = hs["contours"]
         self.L = _bind()
         p, box = self.plan, self.plan.box
         nx, ny, nz = p.shape
-        self.fine_shape = tuple(2 * n for n in box.shape)
-        fnx, fny = self.fine_shape[0] + 2 * self.pad, self.fine_shape[1] + 2 * self.pad
+        self.fine_shape = hs["fine_shape"]
         h = _vp()
         lo = np.array(box.lower, dtype=np.int32)
         hi = np.array(box.upper, dtype=np.int32)
@@ -81,29 +121,13 @@ class RefinedSolver:
             off = (-buf.data_ptr()) % 128
             _lib.check(self.L.ak_bind_populations(lvl, _vp(buf.data_ptr() + off), n), lvl, "ak_bind_populations")
             self._bufs.append(buf)
-        # cell masks: coarse covered cells are computed donors (2); fine solid (1), donor layers (2)
-        cmask = np.zeros((nx, ny), dtype=np.uint8)
-        cmask[box.lower[0]:box.upper[0], box.lower[1]:box.upper[1]] = 2
-        _lib.check(self.L.ak_set_solid(self._levels[0], _lib.u8ptr(cmask)), self._levels[0], "ak_set_solid")
-        dxf = p.fine_level.dx
-        self.fine_origin_padded = (p.origin[0] + box.lower[0] * p.coarse_level.dx - self.pad * dxf,
-                                   p.origin[1] + box.lower[1] * p.coarse_level.dx - self.pad * dxf)
-        lab = labels(self.contours, (fnx, fny), self.fine_origin_padded, dxf)[1:-1, 1:-1]
-        solid = lab > 0
-        core = np.repeat(np.repeat(virtual_core(box.shape)[:, :, 0], 2, 0), 2, 1)
-        owned_solid = solid[self.pad:-self.pad, self.pad:-self.pad]
-        if np.any(solid & ~np.pad(core, self.pad, constant_values=False)):
-            raise ValueError("A solid intersects the fluid donor band")
-        for k, e in enumerate(self.contours):
-            if not (lab == k + 1).any():
-                raise ValueError(f"Element {e.name} has no fine solid cells; increase resolution")
-        fmask = np.full((fnx, fny), 2, dtype=np.uint8)
-        fmask[self.pad:-self.pad, self.pad:-self.pad] = np.where(owned_solid, 1, 0)
-        _lib.check(self.L.ak_set_solid(self._levels[1], _lib.u8ptr(fmask)), self._levels[1], "ak_set_solid")
-        self._fine_solid_xy = owned_solid
-        links, fallback = wall_links(self.contours, (fnx, fny), self.fine_origin_padded, dxf)
+        _lib.check(self.L.ak_set_solid(self._levels[0], _lib.u8ptr(hs["cmask"])), self._levels[0], "ak_set_solid")
+        self.fine_origin_padded = hs["fine_origin_padded"]
+        _lib.check(self.L.ak_set_solid(self._levels[1], _lib.u8ptr(hs["fmask"])), self._levels[1], "ak_set_solid")
+        self._fine_solid_xy = hs["owned_solid"]
+        links = hs["links"]
         self.links = links
-        self.wall_fallback_links = fallback * 2 * nz
+        self.wall_fallback_links = hs["fallback"] * 2 * nz
         args = [_lib.iptr(np.ascontiguousarray(links[k], dtype=np.int32)) for k in ("x", "y", "d")]
         q = np.ascontiguousarray(links["q"])
         el = np.ascontiguousarray(links["element"], dtype=np.int32)
@@ -112,6 +136,7 @@ class RefinedSolver:
                    "ak_refined_set_walls")
         self._keep = (links, q, el)
         self.hist_width = int(self.L.ak_history_width(self._levels[0]))
+        self._seed = hs["seed"]
         self.initialize_flow()
 
     def close(self):
@@ -133,7 +158,7 @@ class RefinedSolver:
         return hashlib.sha256(json.dumps(data, sort_keys=True, separators=(",", ":")).encode()).hexdigest()
 
     def initialize_flow(self):
-        sp = seed_parameters(self.section, self.flow, self.plan_uniform())
+        sp = self._seed
         arr = np.array([sp["U"], sp["amp"], sp["phase_y"], sp["phase_z"], sp["ky"], sp["kz"], sp["lex"], sp["ley"],
                         self.plan.origin[0], self.plan.origin[1]])
         _lib.check(self.L.ak_refined_init_seeded(self.h, _lib.dptr(arr)), None, "ak_refined_init_seeded")
@@ -167,7 +192,7 @@ class RefinedSolver:
         return rec
 
     def _read(self):
-        out = np.zeros((4096, self.hist_width))
+        out = np.empty((4096, self.hist_width))
         n = _lib.check(self.L.ak_read_history(self._levels[0], _lib.dptr(out), 4096), self._levels[0], "ak_read_history")
         return out[:n]
 

@@ -6,9 +6,10 @@ one `RefinedBatch`:
 
 * **Admission.** A request is admitted once it has arrived, a slot is free and its
   footprint (population bytes per cell x cells on both levels, plus the transfer
-  buffers) fits the memory budget.  Cases are built (geometry, wall links, initial
-  field) on host worker threads while the GPU keeps stepping the others, and join the
-  batch at the next chunk boundary.
+  buffers) fits the memory budget.  The host part of a case (geometry, masks, wall
+  links) is prepared on worker threads while the GPU keeps stepping the others; the
+  device part (buffers, tables, initial field) is created on the serving thread, which
+  makes every GPU call, and the case joins the batch at the next chunk boundary.
 * **Stepping.** The batch advances in chunks of coarse steps, with loads recorded every
   coarse step.  Members of different sizes and ages share each batched sweep launch,
   and their transfer kernels overlap.
@@ -27,7 +28,6 @@ from collections import deque
 from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import asdict, dataclass, field
 import math
-import threading
 import time
 
 import numpy as np
@@ -151,6 +151,7 @@ class ServeReport:
     makespan_s: float = 0.0
     peak_members: int = 0
     gpu_steps: int = 0
+    host_s: dict = field(default_factory=dict)  # serving-thread time by activity
 
     def summary(self):
         lat = np.array([r.latency for r in self.results]) if self.results else np.zeros(1)
@@ -160,6 +161,7 @@ class ServeReport:
                 "makespan_s": self.makespan_s, "latency_p50_s": float(np.percentile(lat, 50)),
                 "latency_p99_s": float(np.percentile(lat, 99)), "latency_mean_s": float(lat.mean()),
                 "peak_members": self.peak_members, "coarse_steps": int(sum(r.steps for r in self.results)),
+                "host_s": {k: round(v, 3) for k, v in self.host_s.items()},
                 "reasons": {k: sum(r.reason == k for r in self.results) for k in ("converged", "max_steps", "failed")}}
 
 
@@ -172,7 +174,7 @@ class Server:
     """Continuous-batching service (see the module docstring)."""
 
     def __init__(self, storage="fp16", device=0, memory_bytes=None, max_members=64, chunk=200, window=2000,
-                 tol=0.005, cl_floor=0.1, max_steps=20000, builders=2, graphs=True):
+                 tol=0.005, cl_floor=0.1, max_steps=20000, builders=4, graphs=True):
         if chunk < 4:
             raise ValueError("chunk must be at least 4 coarse steps")
         self.storage, self.device = storage, device
@@ -180,9 +182,6 @@ class Server:
         self.max_members, self.chunk = max_members, chunk
         self.window, self.tol, self.cl_floor, self.max_steps = window, tol, cl_floor, max_steps
         self.builders, self.graphs = builders, graphs
-        # Graph capture on the batch stream must not overlap the null-stream copies and
-        # memsets of a case being built on a worker thread: both take this lock.
-        self._setup_lock = threading.Lock()
 
     def _budget(self):
         if self.memory_bytes:
@@ -191,12 +190,18 @@ class Server:
         free, _ = torch.cuda.mem_get_info(self.device)
         return int(free * 0.9)
 
-    def _build(self, req, plan_bundle):
+    def _prepare(self, req):
+        """Host-only part of a case (worker thread): project, plan, footprint, geometry."""
+        from .refined import host_setup
+        bundle = project_of(req)
+        return bundle, footprint_bytes(bundle[2], self.storage), host_setup(bundle[0], bundle[2])
+
+    def _build(self, bundle, host):
+        """Device part of a case (serving thread)."""
         from .refined import RefinedSolver
-        project, settings, plan = plan_bundle
-        with self._setup_lock:
-            return RefinedSolver(project, settings, plan=plan, storage=self.storage, device=self.device,
-                                 graphs=self.graphs)
+        project, settings, plan = bundle
+        return RefinedSolver(project, settings, plan=plan, storage=self.storage, device=self.device,
+                             graphs=self.graphs, host=host)
 
     def _retire_reason(self, st):
         steps = len(st["cl"])
@@ -209,7 +214,7 @@ class Server:
     def _result(self, st, now, reason):
         req, cl = st["req"], st["cl"]
         w = min(len(cl), self.window) or 1
-        loads = np.array(st["loads"][-w:]) if st["loads"] else np.zeros((1, 3))
+        loads = np.concatenate(st["loads"])[-w:] if st["loads"] else np.zeros((1, 3))
         return CaseResult(req.id, req.arrival, st["admitted"], now, len(cl), reason, *map(float, loads.mean(axis=0)),
                           _window_variation(cl, w, self.cl_floor) if cl else float("nan"), st["cells"])
 
@@ -221,68 +226,74 @@ class Server:
         budget = self._budget()
         queue = deque(sorted(requests, key=lambda r: r.arrival))
         batch = RefinedBatch(self.storage, self.device, graphs=self.graphs)
-        building, ready, active = {}, deque(), {}
+        building, ready, active = {}, deque(), {}  # prepare futures; prepared; in the batch
         reserved = 0
-        changed = True  # membership changed since the last chunk: the next steps capture graphs
         t0 = time.perf_counter()
+        staged = []  # cases whose device part was built while the last chunk ran
+        clock = {"build": 0.0, "launch": 0.0, "collect": 0.0, "retire": 0.0, "admit": 0.0, "idle": 0.0}
+
+        def timed(key, fn, *a, **k):
+            t = time.perf_counter()
+            out = fn(*a, **k)
+            clock[key] += time.perf_counter() - t
+            return out
+
         with ThreadPoolExecutor(self.builders) as pool:
-            while queue or building or ready or active:
+            while queue or building or ready or active or staged:
+                if ready and not (active or staged or building) and reserved + ready[0][2] > budget:
+                    raise MemoryError(f"request {ready[0][0].id} needs {ready[0][2]} bytes, over the budget {budget}")
                 now = time.perf_counter() - t0
-                while (queue and queue[0].arrival <= now
-                       and len(active) + len(ready) + len(building) < self.max_members):
-                    req = queue[0]
-                    bundle = project_of(req)
-                    need = footprint_bytes(bundle[2], self.storage)
-                    if reserved + need > budget:
-                        break
-                    queue.popleft()
-                    reserved += need
-                    building[pool.submit(self._build, req, bundle)] = (req, need, bundle[2])
+                # arrived requests are prepared on the host ahead of admission
+                while queue and queue[0].arrival <= now and len(building) + len(ready) < 2 * self.builders:
+                    req = queue.popleft()
+                    building[pool.submit(self._prepare, req)] = req
                 for fut in [f for f in building if f.done()]:
-                    req, need, plan = building.pop(fut)
-                    ready.append((req, need, plan, fut.result()))
-                while ready:
-                    req, need, plan, solver = ready.popleft()
-                    batch.add(solver)
-                    changed = True
-                    cells = int(np.prod(plan.shape)) + 8 * int(np.prod(plan.box.shape))
-                    active[solver] = {"req": req, "need": need, "admitted": time.perf_counter() - t0, "cl": [],
-                                      "loads": [], "cells": cells}
-                    log(f"admit {req.id} ({cells} cells) t={active[solver]['admitted']:.2f}s members={len(batch)}")
+                    req = building.pop(fut)
+                    ready.append((req, *fut.result()))
+                # the batch is idle here: admitted cases join it
+                for solver, st in staged:
+                    timed("admit", batch.add, solver)
+                    st["admitted"] = time.perf_counter() - t0
+                    active[solver] = st
+                    log(f"admit {st['req'].id} ({st['cells']} cells) t={st['admitted']:.2f}s members={len(batch)}")
+                staged = []
                 report.peak_members = max(report.peak_members, len(batch))
+                if active:
+                    timed("launch", batch.launch, self.chunk, sample=True)
+                # admission: a free slot and the footprint within the budget; the device part is
+                # built while the chunk runs
+                while ready and len(active) + len(staged) < self.max_members and reserved + ready[0][2] <= budget:
+                    req, bundle, need, host = ready.popleft()
+                    reserved += need
+                    plan = bundle[2]
+                    cells = int(np.prod(plan.shape)) + 8 * int(np.prod(plan.box.shape))
+                    staged.append((timed("build", self._build, bundle, host),
+                                   {"req": req, "need": need, "admitted": None, "cl": [], "loads": [], "cells": cells}))
                 if not active:
+                    if staged:
+                        continue
                     if building:
-                        wait(list(building), timeout=1.0, return_when=FIRST_COMPLETED)
+                        timed("idle", wait, list(building), timeout=1.0, return_when=FIRST_COMPLETED)
                     elif queue:
-                        time.sleep(max(0.0, min(queue[0].arrival - (time.perf_counter() - t0), 1.0)))
+                        timed("idle", time.sleep, max(0.0, min(queue[0].arrival - (time.perf_counter() - t0), 1.0)))
                     continue
-                if changed:
-                    # the first two steps capture both parity variants of the batch graph
-                    with self._setup_lock:
-                        recs, failed = batch.advance(2, sample=True)
-                    more, failed2 = batch.advance(self.chunk - 2, sample=True)
-                    for m, rs in more.items():
-                        recs[m].extend(rs)
-                    failed.update(failed2)
-                    changed = False
-                else:
-                    recs, failed = batch.advance(self.chunk, sample=True)
+                rows, failed = timed("collect", batch.collect, raw=True)
                 report.gpu_steps += self.chunk
                 now = time.perf_counter() - t0
-                for solver, rs in recs.items():
+                for solver, r in rows.items():
                     st = active[solver]
-                    st["cl"].extend(r["cl"] for r in rs)
-                    st["loads"].extend((r["cl"], r["cd"], r["cm"]) for r in rs)
+                    st["cl"].extend(r[:, 1].tolist())
+                    st["loads"].append(r[:, 1:4].copy())
                     reason = "failed" if solver in failed else self._retire_reason(st)
                     if reason:
-                        batch.remove(solver)
-                        changed = True
-                        solver.close()
+                        timed("retire", batch.remove, solver)
+                        timed("retire", solver.close)
                         del active[solver]
                         reserved -= st["need"]
                         report.results.append(self._result(st, now, reason))
                         log(f"retire {st['req'].id} {reason} after {len(st['cl'])} steps, t={now:.2f}s")
         report.makespan_s = time.perf_counter() - t0
+        report.host_s = clock
         batch.close()
         return report
 
@@ -313,7 +324,7 @@ def serve_sequential(requests, storage="fp64", device=0, chunk=200, window=2000,
                 rs, reason = [], "failed"
             report.gpu_steps += chunk
             st["cl"].extend(r["cl"] for r in rs)
-            st["loads"].extend((r["cl"], r["cd"], r["cm"]) for r in rs)
+            st["loads"].append(np.array([[r["cl"], r["cd"], r["cm"]] for r in rs]).reshape(-1, 3))
             reason = reason or rule._retire_reason(st)
             if reason:
                 break

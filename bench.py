@@ -37,6 +37,7 @@ cores and on one core.
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
 import sys
@@ -208,7 +209,35 @@ def cpu_baseline_refined(case, seconds, threads):
                       f"1 warm-up + {n} timed coarse steps ({dt:.1f} s)"}
 
 
+def serving_gpu(args):
+    """Continuous batching (FP16 storage) vs one-at-a-time FP64 solves of the same requests."""
+    from aerokinetic_amd import serving as S
+    reqs = S.polar_requests(args.serve_cases, seed=2026, rate=args.serve_rate, chord=(16, 24), span=(8, 8),
+                            reynolds=(1e4, 3e4))
+    rule = dict(chunk=args.serve_chunk, window=args.serve_window, tol=0.005, max_steps=args.serve_max_steps)
+    out = {"workload": (f"{len(reqs)} two-level NACA 4-digit requests (seed 2026: camber 0-4 %, thickness 9-15 %, "
+                        f"alpha -4..12 deg, Re 1e4-3e4 log-uniform, coarse chord 16-24 cells, 12c x 8c x 8 cells, "
+                        f"fine patch 2:1), Poisson arrivals {args.serve_rate}/s; retire when the "
+                        f"{args.serve_window}-step Cl window varies < 0.5 % or at {args.serve_max_steps} steps"),
+           "rule": rule}
+    for st in [s for s in args.serve_storages.split(",") if s]:
+        log(f"serving batched {st}")
+        rep = S.Server(storage=st, **rule).serve(reqs)
+        out[f"batched_{st}"] = rep.summary()
+        log(f"  {out[f'batched_{st}']['cases_per_hour']:.0f} cases/h, p50 {out[f'batched_{st}']['latency_p50_s']:.2f} s, "
+            f"p99 {out[f'batched_{st}']['latency_p99_s']:.2f} s, peak members {rep.peak_members}")
+    log("serving sequential fp64")
+    seq = S.serve_sequential(reqs, storage="fp64", **rule)
+    out["sequential_fp64"] = seq.summary()
+    log(f"  {out['sequential_fp64']['cases_per_hour']:.0f} cases/h, p50 {out['sequential_fp64']['latency_p50_s']:.2f} s")
+    for k in list(out):
+        if k.startswith("batched_"):
+            out[f"{k}_vs_sequential_fp64"] = out[k]["cases_per_hour"] / out["sequential_fp64"]["cases_per_hour"]
+    return out
+
+
 def main():
+    faulthandler.enable()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
@@ -218,6 +247,12 @@ def main():
     ap.add_argument("--uniform-storages", default="fp64,fp32,fp16", help="empty to skip the 100M-cell sweep")
     ap.add_argument("--uniform-steps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-cases", type=int, default=48, help="0 skips the serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=64.0)
+    ap.add_argument("--serve-storages", default="fp16,fp32")
+    ap.add_argument("--serve-chunk", type=int, default=200)
+    ap.add_argument("--serve-window", type=int, default=400)
+    ap.add_argument("--serve-max-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: a case runs on a single MI355X here; "
@@ -244,6 +279,7 @@ def main():
         r = uni[st]
         log(f"  {r['mlups']:.0f} MLUPS, sweep {r['sweep_ms']:.3f} ms ({100 * r['roofline']['frac']:.1f}% HBM)")
     uni_cmp = compare_loads(uni)
+    serving = serving_gpu(args) if args.serve_cases > 0 else None
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cpu = None
     if args.cpu_seconds > 0:
@@ -266,7 +302,8 @@ def main():
             "loads_vs_fp64": compare_loads(results),
             "uniform_sweep": {"workload": ucase.name + ", characteristic farfield, loads every step",
                               "by_storage": {st: strip(r) for st, r in uni.items()},
-                              "vs_fp64": uni_cmp}}
+                              "vs_fp64": uni_cmp},
+            "serving": serving}
     print(json.dumps(line))
 
 

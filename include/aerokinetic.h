@@ -201,8 +201,10 @@ int ak_batch_add(void *handle, void *refined);
 int ak_batch_remove(void *handle, void *refined);
 int ak_batch_size(void *handle);
 int ak_batch_set_graph_mode(void *handle, int enable);
-/* nsteps coarse steps of every member; sample_every 1 (loads of every step into each member's
- * history, read with ak_read_history on ak_refined_level(member, 0)) or 0. */
+/* Enqueue nsteps coarse steps of every member on the batch stream (returns before they run);
+ * sample_every 1 (loads of every step into each member's history) or 0.  Call
+ * ak_batch_synchronize before reading a member's history (ak_read_history on
+ * ak_refined_level(member, 0)) or state; cases may be created meanwhile. */
 int ak_batch_run(void *handle, int nsteps, int sample_every);
 int ak_batch_synchronize(void *handle);
 

@@ -42,7 +42,7 @@ def test_batch_of_one_matches_single_case_solver():
     batch.close()
 
 
-@pytest.mark.parametrize("storage", ["fp32", "fp64"])
+@pytest.mark.parametrize("storage", ["fp32", "fp64", "fp16"])
 def test_member_is_identical_alone_and_in_a_mix(storage):
     """Case A alone vs A sharing every batched launch with B (joined from the start) and C
     (joined after 4 steps, different size and age): A's loads and populations are

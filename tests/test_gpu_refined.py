@@ -65,3 +65,23 @@ def test_refined_gpu_matches_recorded_history(name, steps):
     print("worst |dC|", name, worst)
     assert len(recs) == steps // every
     assert worst < 5e-12
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_reduced_storage_loads_track_fp64(storage):
+    """A two-level case in FP32 / FP16-shifted storage (FP32 arithmetic) against FP64 storage:
+    no transfer guard fires, and the mean Cl and Cd over steps 101-300 agree within 1 %."""
+    from aerokinetic_amd import serving as S
+    from aerokinetic_amd.refined import RefinedSolver
+    req = S.CaseRequest(0, 0.0, 0.02, 0.4, 0.12, 6.0, 2e4, 20, 8)
+    project, settings, plan = S.project_of(req)
+    print("case", storage, plan.shape, plan.box, flush=True)
+    out = {}
+    for st in ("fp64", storage):
+        s = RefinedSolver(project, settings, plan=plan, storage=st)
+        recs = s.advance(300, 1)
+        out[st] = np.array([[r["cl"], r["cd"]] for r in recs[100:]]).mean(axis=0)
+        s.close()
+    ref, got = out["fp64"], out[storage]
+    assert np.all(np.isfinite(got))
+    np.testing.assert_allclose(got, ref, rtol=0.01)
