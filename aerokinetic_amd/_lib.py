@@ -61,6 +61,12 @@ _SIGS = {
     "ak_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
     "ak_version": (ctypes.c_char_p, []),
     "ak_device_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "ak_state_bytes": (ctypes.c_size_t, [_vp]),
+    "ak_save_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "ak_load_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "ak_refined_state_bytes": (ctypes.c_size_t, [_vp]),
+    "ak_refined_save_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "ak_refined_load_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "ak_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp]),
     "ak_batch_destroy": (ctypes.c_int, [_vp]),
     "ak_batch_add": (ctypes.c_int, [_vp, _vp]),

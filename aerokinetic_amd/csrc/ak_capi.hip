@@ -864,6 +864,104 @@ int ak_synchronize(void *handle) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Restartable state of a lattice: a fixed header, the raw population buffer (every storage
+// precision bit-exact), then the boundary histories: characteristic farfield incoming and
+// derivative values, clocks and glancing trace rows; legacy outflow pdf and pdf_nd.
+struct StateHeader {
+  char magic[8];
+  int version, storage, nx, ny, nz, ff_kind, nff, nleg;
+  long long sweeps;
+  unsigned long long pop_bytes;
+};
+constexpr int STATE_VERSION = 1;
+
+struct StatePart {
+  void *dev;
+  size_t bytes;
+};
+
+std::vector<StatePart> state_parts(Lattice *s) {
+  std::vector<StatePart> p{{s->pop, s->pop_bytes}};
+  if (s->ff_kind == AK_FARFIELD_CHARACTERISTIC && s->nff) {
+    p.push_back({s->d_inc, (size_t)s->nff * sizeof(double)});
+    p.push_back({s->d_der, (size_t)s->nff * sizeof(double)});
+    p.push_back({s->d_clock, (size_t)s->nff * sizeof(long long)});
+    p.push_back({s->d_rows, (size_t)2 * s->nz * s->nx * sizeof(double)});
+  }
+  if (s->ff_kind == AK_FARFIELD_LEGACY && s->nleg) {
+    p.push_back({s->d_pdf, (size_t)s->nleg * sizeof(double)});
+    p.push_back({s->d_pdfnd, (size_t)s->nleg * sizeof(double)});
+  }
+  return p;
+}
+
+StateHeader state_header(const Lattice *s) {
+  StateHeader h{};
+  std::memcpy(h.magic, "AKSTATE", 8);
+  h.version = STATE_VERSION;
+  h.storage = s->storage; h.nx = s->nx; h.ny = s->ny; h.nz = s->nz;
+  h.ff_kind = s->ff_kind; h.nff = s->nff; h.nleg = s->nleg;
+  h.sweeps = s->sweeps;
+  h.pop_bytes = s->pop_bytes;
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ak_state_bytes(void *handle) {
+  Lattice *s = H(handle);
+  if (!s) return 0;
+  size_t n = sizeof(StateHeader);
+  for (const StatePart &p : state_parts(s)) n += p.bytes;
+  return n;
+}
+
+int ak_save_state(void *handle, void *host, size_t bytes) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  if (!host || bytes < ak_state_bytes(handle)) return fail(s, "state buffer of %zu bytes < %zu", bytes, ak_state_bytes(handle));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const StateHeader h = state_header(s);
+  char *o = (char *)host;
+  std::memcpy(o, &h, sizeof h);
+  o += sizeof h;
+  for (const StatePart &p : state_parts(s)) {
+    HIPCHK(s, hipMemcpy(o, p.dev, p.bytes, hipMemcpyDeviceToHost));
+    o += p.bytes;
+  }
+  return 0;
+}
+
+int ak_load_state(void *handle, const void *host, size_t bytes) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  s->gen++;  // invalidates captured step graphs
+  StateHeader h{};
+  if (!host || bytes < sizeof h) return fail(s, "state buffer too short");
+  std::memcpy(&h, host, sizeof h);
+  const StateHeader c = state_header(s);
+  if (std::memcmp(h.magic, c.magic, 8) || h.version != STATE_VERSION)
+    return fail(s, "not a lattice state (or another state version)");
+  if (h.storage != c.storage || h.nx != c.nx || h.ny != c.ny || h.nz != c.nz || h.ff_kind != c.ff_kind ||
+      h.nff != c.nff || h.nleg != c.nleg || h.pop_bytes != c.pop_bytes)
+    return fail(s, "state of another lattice (storage, shape or boundary set differ)");
+  if (bytes != ak_state_bytes(handle)) return fail(s, "state size %zu, expected %zu", bytes, ak_state_bytes(handle));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const char *i = (const char *)host + sizeof h;
+  for (const StatePart &p : state_parts(s)) {
+    HIPCHK(s, hipMemcpy(p.dev, i, p.bytes, hipMemcpyHostToDevice));
+    i += p.bytes;
+  }
+  s->sweeps = h.sweeps;
+  return 0;
+}
+
 int ak_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
   if (!dst || !src) return fail(nullptr, "null copy buffer");
   if (((uintptr_t)dst | (uintptr_t)src | bytes) % 8) return fail(nullptr, "copy needs 8-byte aligned buffers and size");

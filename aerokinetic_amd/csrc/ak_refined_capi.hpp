@@ -467,6 +467,38 @@ int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int 
   return 0;
 }
 
+// Restartable state of a two-level case: the coarse-step count, then the coarse and fine
+// lattice states (ak_save_state layout).  The transfer buffers are rebuilt every step.
+size_t ak_refined_state_bytes(void *handle) {
+  Refined *r = R(handle);
+  return r ? sizeof(long long) + ak_state_bytes(r->c) + ak_state_bytes(r->f) : 0;
+}
+
+int ak_refined_save_state(void *handle, void *host, size_t bytes) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  if (!host || bytes < ak_refined_state_bytes(handle)) return fail(nullptr, "refined state buffer too short");
+  char *o = (char *)host;
+  std::memcpy(o, &r->steps, sizeof(long long));
+  o += sizeof(long long);
+  if (ak_save_state(r->c, o, ak_state_bytes(r->c))) return -1;
+  return ak_save_state(r->f, o + ak_state_bytes(r->c), ak_state_bytes(r->f));
+}
+
+int ak_refined_load_state(void *handle, const void *host, size_t bytes) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  if (!host || bytes != ak_refined_state_bytes(handle)) return fail(nullptr, "refined state of another size");
+  const char *i = (const char *)host;
+  long long steps = 0;
+  std::memcpy(&steps, i, sizeof steps);
+  i += sizeof(long long);
+  if (ak_load_state(r->c, i, ak_state_bytes(r->c)) || ak_load_state(r->f, i + ak_state_bytes(r->c), ak_state_bytes(r->f)))
+    return -1;
+  r->steps = steps;
+  return 0;
+}
+
 // Transfer error flags raised so far (bit 1 density, 2 negative population, 4 Mach, 8 non-finite); clears them.
 int ak_refined_transfer_errors(void *handle) {
   Refined *r = R(handle);

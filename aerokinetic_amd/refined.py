@@ -249,34 +249,78 @@ class RefinedSolver:
         return {"local_mach_max": float(mach), "density_error_max": float(dens),
                 "wall_fallback_links_including_donors": self.wall_fallback_links}
 
+    def save_checkpoint(self, path, context=None):
+        """Restartable device state of both levels + run context (refined_checkpoint.py:54-142)."""
+        from .checkpoint import save_checkpoint
+        return save_checkpoint(self, path, context)
+
     def run(self, output, callback=None, stop_event=None, resume=False, residual_callback=None):
-        """Run project.run.duration_convective coarse time (refined_run.py:19-157)."""
+        """Run project.run.duration_convective coarse time (refined_run.py:19-157); with resume=True
+        continue an interrupted run in `output` from the checkpoint this solver was restored from."""
         from .timing import duration_steps
+        from ._io import file_hash
         from . import __version__
-        if resume:
-            raise NotImplementedError("resume requires device checkpoints, which this package does not write yet")
+        import os
         settings = self.project.run
         output = Path(output)
-        if output.exists() and any(output.iterdir()):
+        history = output / "history.csv"
+        if not resume and output.exists() and any(output.iterdir()):
             raise FileExistsError("Use a new/empty output directory to avoid mixing histories")
         output.mkdir(parents=True, exist_ok=True)
-        write_json(output / "project.json", refined_project_dict(self.project, self.refinement))
         end_step = self.steps + duration_steps(settings.duration_convective, self.plan.coarse_level.dt)
+        rows = []
+        if resume:
+            old = json.loads((output / "metadata.json").read_text(encoding="utf-8"))
+            context = getattr(self, "_checkpoint_context", {})
+            if (old.get("case_fingerprint") != self.case_fingerprint
+                    or context.get("case_fingerprint") != self.case_fingerprint):
+                raise ValueError("Resume requires the original project, settings and checkpoint")
+            if old["status"] == "completed":
+                raise ValueError("This run has already completed")
+            end_step, prefix = context["end_step"], context["history_bytes"]
+            if file_hash(history, prefix) != context["history_sha256"]:
+                raise ValueError("History before the checkpoint has changed; refusing to mix results")
+            if history.stat().st_size > prefix:
+                import shutil
+                import uuid
+                recovery = output / "recovery" / uuid.uuid4().hex
+                recovery.mkdir(parents=True)
+                shutil.copy2(history, recovery / "history_before_resume.csv")
+                with history.open("r+b") as handle:
+                    handle.truncate(prefix)
+            with history.open(newline="", encoding="utf-8") as handle:
+                for row in csv.DictReader(handle):
+                    rows.append({"step": int(row["step"]), "t_star": float(row["t_star"]),
+                                 **{k: float(row[k]) for k in ("cl", "cd", "cm")}})
+        else:
+            write_json(output / "project.json", refined_project_dict(self.project, self.refinement))
         metadata = {"version": __version__, "refined_plan": json.loads(json.dumps(self.plan.as_dict(), default=list)),
                     "case_fingerprint": self.case_fingerprint, "status": "running", "end_step": end_step,
-                    "device": {"backend": "hip-gfx950", "storage": self.storage}}
+                    "resume_supported": True, "device": {"backend": "hip-gfx950", "storage": self.storage}}
         write_json(output / "metadata.json", metadata)
-        rows, state, start = [], "completed", time.monotonic()
+        state, start = "completed", time.monotonic()
         try:
-            with (output / "history.csv").open("w", newline="", encoding="utf-8") as handle:
+            with history.open("a" if resume else "w", newline="", encoding="utf-8") as handle:
                 writer = csv.DictWriter(handle, self.HISTORY_COLUMNS)
-                writer.writeheader()
-                se = settings.sample_every
+                if not resume:
+                    writer.writeheader()
+
+                def checkpoint():
+                    handle.flush()
+                    os.fsync(handle.fileno())
+                    self.save_checkpoint(output / "checkpoint.npz",
+                                         {"case_fingerprint": self.case_fingerprint, "end_step": end_step,
+                                          "history_bytes": history.stat().st_size,
+                                          "history_sha256": file_hash(history)})
+
+                se, ce = settings.sample_every, settings.checkpoint_every
                 while self.steps < end_step:
                     if stop_event is not None and stop_event.is_set():
                         state = "stopped"
                         break
                     nxt = min(end_step, (self.steps // se + 1) * se if callback else self.steps + 64 * se)
+                    if ce:
+                        nxt = min(nxt, (self.steps // ce + 1) * ce)
                     recs = self.advance(nxt - self.steps, se)
                     stop = False
                     for rec in recs:
@@ -292,7 +336,10 @@ class RefinedSolver:
                         self.diagnostics()
                     if stop:
                         break
+                    if ce and self.steps % ce == 0 and self.steps < end_step:
+                        checkpoint()
                 self.diagnostics()
+                checkpoint()
         except BaseException as error:
             state = "failed"
             metadata["failure"] = f"{type(error).__name__}: {error}"

@@ -156,6 +156,11 @@ class Solver:
         self.lattice.set_populations(g)
 
     # ------------------------------------------------------------------ outputs
+    def save_checkpoint(self, path, context=None):
+        """Restartable device state + run context (checkpoint.py:32-70)."""
+        from .checkpoint import save_checkpoint
+        return save_checkpoint(self, path, context)
+
     def save_frame(self, path, max_pixels=800):
         """Mid-span speed, vorticity and Cp planes (solver.py:312-334)."""
         nx, ny, nz = self.plan.shape
@@ -185,21 +190,60 @@ class Solver:
         """
         from .convergence import StopCriteria
         from .timing import duration_steps
+        from ._io import file_hash
         from . import __version__
-        if resume:
-            raise NotImplementedError("resume requires device checkpoints, which this package does not write yet")
+        import os
+        import shutil
+        import uuid
         termination = termination or StopCriteria()
         settings = RunSettings(duration_convective, sample_every, frame_every, check_every, average_after,
                                checkpoint_every, termination)
         project = Project(self.section, self.flow, self.grid, settings, "+".join(e.name for e in self.section.elements))
         output = Path(output)
-        if output.exists() and any(output.iterdir()):
+        if not resume and output.exists() and any(output.iterdir()):
             raise FileExistsError("Use a new/empty output directory to avoid mixing histories")
         output.mkdir(parents=True, exist_ok=True)
-        (output / "frames").mkdir(exist_ok=True)
+        (output / "frames").mkdir(exist_ok=resume)
         end_step = self.steps + duration_steps(duration_convective, self.plan.dt_convective)
         if termination.max_steps:
             end_step = min(end_step, self.steps + termination.max_steps)
+        history = output / "history.csv"
+        rows = []
+        start_step = self.steps
+        if resume:
+            # checkpoint.py / solver.py:389-425: same project, settings and durable history prefix
+            old = json.loads((output / "metadata.json").read_text(encoding="utf-8"))
+            context = getattr(self, "_checkpoint_context", {})
+            saved = Project.load(output / "project.json")
+            if (saved.case_fingerprint != project.case_fingerprint or asdict(saved.run) != asdict(settings)
+                    or old.get("case_fingerprint") != project.case_fingerprint
+                    or context.get("case_fingerprint") != project.case_fingerprint):
+                raise ValueError("Resume requires the original project, settings and checkpoint")
+            if old["status"] == "completed":
+                raise ValueError("This run has already completed")
+            start_step = old.get("start_step", 0)
+            end_step = context["end_step"]
+            prefix = context["history_bytes"]
+            if file_hash(history, prefix) != context["history_sha256"]:
+                raise ValueError("History before the checkpoint has changed; refusing to mix results")
+            # samples and frames written after the last durable checkpoint are kept aside
+            future = [f for f in (output / "frames").glob("frame_*.npz") if int(f.stem.split("_")[-1]) > self.steps]
+            if history.stat().st_size > prefix or future:
+                recovery = output / "recovery" / uuid.uuid4().hex
+                recovery.mkdir(parents=True)
+                shutil.copy2(history, recovery / "history_before_resume.csv")
+                with history.open("r+b") as handle:
+                    handle.truncate(prefix)
+                for f in future:
+                    shutil.move(str(f), str(recovery / f.name))
+            with history.open(newline="", encoding="utf-8") as handle:
+                for row in csv.DictReader(handle):
+                    rows.append({"step": int(row["step"]), "t_star": float(row["t_star"]),
+                                 **{k: float(row[k]) for k in ("cl", "cd", "cm")}})
+            if rows and rows[-1]["step"] > self.steps:
+                raise ValueError("History extends beyond the checkpoint")
+        if self.steps > end_step:
+            raise ValueError("Checkpoint is past the requested end time")
         metadata = {"version": __version__, "flow": asdict(self.flow), "grid": asdict(self.grid),
                     "plan": self.plan.as_dict(), "reference_chord_input_units": self.section.chord,
                     "moment_reference_input_units": self.section.moment_reference,
@@ -208,21 +252,31 @@ class Solver:
                     "force_time": "midpoint of boundary momentum-transfer interval",
                     "status": "running", "experimental_validation": "not_performed",
                     "case_fingerprint": project.case_fingerprint, "run": asdict(settings),
-                    "start_step": self.steps, "start_t_star": self.steps * self.plan.dt_convective,
-                    "resume_supported": False, "end_step": end_step,
+                    "start_step": start_step, "start_t_star": start_step * self.plan.dt_convective,
+                    "resume_supported": True, "end_step": end_step,
                     "device": {"backend": "hip-gfx950", "storage": self.storage,
                                "library": _lib.load().ak_version().decode()}}
-        project.save(output / "project.json")
+        if not resume:
+            project.save(output / "project.json")
         write_json(output / "metadata.json", metadata)
-        rows, state, reason = [], "completed", "duration_reached"
+        state, reason = "completed", "duration_reached"
         start = time.monotonic()
-        history = output / "history.csv"
         try:
             self.diagnostics()
             self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
-            with history.open("w", newline="", encoding="utf-8") as handle:
+            with history.open("a" if resume else "w", newline="", encoding="utf-8") as handle:
                 writer = csv.DictWriter(handle, self.HISTORY_COLUMNS)
-                writer.writeheader()
+                if not resume:
+                    writer.writeheader()
+
+                def checkpoint():
+                    handle.flush()
+                    os.fsync(handle.fileno())
+                    self.save_checkpoint(output / "checkpoint.npz",
+                                         {"case_fingerprint": project.case_fingerprint, "end_step": end_step,
+                                          "history_bytes": history.stat().st_size,
+                                          "history_sha256": file_hash(history)})
+
                 stop = False
                 while self.steps < end_step and not stop:
                     if stop_event is not None and stop_event.is_set():
@@ -230,7 +284,7 @@ class Solver:
                         break
                     # next event: a sample (when a callback must see it), a check, a frame, the end
                     nxt = end_step
-                    for every in (check_every, frame_every) + ((sample_every,) if callback else ()):
+                    for every in (check_every, frame_every, checkpoint_every) + ((sample_every,) if callback else ()):
                         if every:
                             nxt = min(nxt, (self.steps // every + 1) * every)
                     if not callback:
@@ -257,8 +311,11 @@ class Solver:
                     if termination.wall_time_limit_s and time.monotonic() - start > termination.wall_time_limit_s:
                         state, reason = "stopped", "wall_time_limit"
                         break
+                    if checkpoint_every and self.steps % checkpoint_every == 0 and self.steps < end_step:
+                        checkpoint()
                 self.diagnostics()
                 self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
+                checkpoint()
         except BaseException as error:
             state, reason = "failed", "failed"
             metadata["failure"] = f"{type(error).__name__}: {error}"

@@ -130,6 +130,15 @@ int ak_diagnostics(void *handle, double *out);
 /* rho[x][y] and u[x][y][3] of one z plane (host, FP64; synchronises). */
 int ak_get_plane(void *handle, int z, double *rho, double *u);
 
+/* Restartable device state (checkpoint.py:32-135, refined_checkpoint.py:54-142): the raw
+ * population buffer (bit-exact in every storage precision), the AA parity and the boundary
+ * histories (characteristic incoming/derivative/clock/glancing trace; legacy outflow
+ * pdf/pdf_nd).  ak_load_state accepts only a state of a lattice with the same storage,
+ * shape and boundary set. */
+size_t ak_state_bytes(void *handle);
+int ak_save_state(void *handle, void *host, size_t bytes);
+int ak_load_state(void *handle, const void *host, size_t bytes);
+
 /* Wait for all work of this lattice. */
 int ak_synchronize(void *handle);
 
@@ -180,6 +189,11 @@ int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int 
 /* Transfer guard flags raised since the last call, then cleared (refinement.py:159-163,187-202):
  * 1 density <= 0, 2 negative population, 4 local Mach > 0.3, 8 non-finite. */
 int ak_refined_transfer_errors(void *handle);
+
+/* Restartable state of a two-level case: coarse-step count plus both lattice states. */
+size_t ak_refined_state_bytes(void *handle);
+int ak_refined_save_state(void *handle, void *host, size_t bytes);
+int ak_refined_load_state(void *handle, const void *host, size_t bytes);
 
 /* Pre-collision populations of one level (0 coarse, 1 fine padded), exchange layout. */
 int ak_refined_get_populations(void *handle, int level, double *host_out);
