@@ -62,6 +62,7 @@ _SIGS = {
     "ak_version": (ctypes.c_char_p, []),
     "ak_device_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     "ak_state_bytes": (ctypes.c_size_t, [_vp]),
+    "ak_residual": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]),
     "ak_save_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "ak_load_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "ak_refined_state_bytes": (ctypes.c_size_t, [_vp]),

@@ -115,5 +115,6 @@ def restore_checkpoint(solver_type, path, progress=None, device=0, plan=None):
         raise ValueError("Checkpoint wall links differ from the rebuilt case")
     solver.steps = meta["steps"]
     solver._checkpoint_context = meta["context"]
+    solver._restored_control = meta["context"].get("run_control")
     solver.diagnostics()
     return solver

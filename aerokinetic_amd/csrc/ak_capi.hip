@@ -75,6 +75,9 @@ struct Lattice {
   std::vector<double> host_hist;
   long long sweeps = 0;  // sweeps since the initial state: parity of the AA form
   StepCtl *d_ctl = nullptr;
+  // stationarity residual: previous (rho, j / U) of every cell, block partials, result
+  double *d_prev = nullptr, *d_rpart = nullptr, *d_rout = nullptr;
+  bool prev_valid = false;
   // step graphs: replayed by ak_run while no setter has changed a kernel argument
   bool use_graphs = true;
   unsigned long long gen = 1;
@@ -873,7 +876,7 @@ namespace {
 // derivative values, clocks and glancing trace rows; legacy outflow pdf and pdf_nd.
 struct StateHeader {
   char magic[8];
-  int version, storage, nx, ny, nz, ff_kind, nff, nleg;
+  int version, storage, nx, ny, nz, ff_kind, nff, nleg, has_prev;
   long long sweeps;
   unsigned long long pop_bytes;
 };
@@ -884,7 +887,7 @@ struct StatePart {
   size_t bytes;
 };
 
-std::vector<StatePart> state_parts(Lattice *s) {
+std::vector<StatePart> state_parts(Lattice *s, bool with_prev) {
   std::vector<StatePart> p{{s->pop, s->pop_bytes}};
   if (s->ff_kind == AK_FARFIELD_CHARACTERISTIC && s->nff) {
     p.push_back({s->d_inc, (size_t)s->nff * sizeof(double)});
@@ -896,7 +899,15 @@ std::vector<StatePart> state_parts(Lattice *s) {
     p.push_back({s->d_pdf, (size_t)s->nleg * sizeof(double)});
     p.push_back({s->d_pdfnd, (size_t)s->nleg * sizeof(double)});
   }
+  if (with_prev) p.push_back({s->d_prev, (size_t)4 * s->nx * s->ny * s->nz * sizeof(double)});
   return p;
+}
+
+// Bytes of a state of this lattice with or without the residual snapshot.
+size_t state_size(Lattice *s, bool with_prev) {
+  size_t n = sizeof(StateHeader);
+  for (const StatePart &p : state_parts(s, with_prev)) n += p.bytes;
+  return n;
 }
 
 StateHeader state_header(const Lattice *s) {
@@ -905,9 +916,19 @@ StateHeader state_header(const Lattice *s) {
   h.version = STATE_VERSION;
   h.storage = s->storage; h.nx = s->nx; h.ny = s->ny; h.nz = s->nz;
   h.ff_kind = s->ff_kind; h.nff = s->nff; h.nleg = s->nleg;
+  h.has_prev = s->prev_valid ? 1 : 0;
   h.sweeps = s->sweeps;
   h.pop_bytes = s->pop_bytes;
   return h;
+}
+
+// Bytes of the lattice state blob at `host` (read from its header), 0 if it is not one.
+size_t state_blob_size(Lattice *s, const void *host, size_t avail) {
+  StateHeader h{};
+  if (avail < sizeof h) return 0;
+  std::memcpy(&h, host, sizeof h);
+  if (std::memcmp(h.magic, "AKSTATE", 8)) return 0;
+  return state_size(s, h.has_prev != 0);
 }
 
 }  // namespace
@@ -916,10 +937,7 @@ extern "C" {
 
 size_t ak_state_bytes(void *handle) {
   Lattice *s = H(handle);
-  if (!s) return 0;
-  size_t n = sizeof(StateHeader);
-  for (const StatePart &p : state_parts(s)) n += p.bytes;
-  return n;
+  return s ? state_size(s, s->prev_valid) : 0;
 }
 
 int ak_save_state(void *handle, void *host, size_t bytes) {
@@ -931,7 +949,7 @@ int ak_save_state(void *handle, void *host, size_t bytes) {
   char *o = (char *)host;
   std::memcpy(o, &h, sizeof h);
   o += sizeof h;
-  for (const StatePart &p : state_parts(s)) {
+  for (const StatePart &p : state_parts(s, s->prev_valid)) {
     HIPCHK(s, hipMemcpy(o, p.dev, p.bytes, hipMemcpyDeviceToHost));
     o += p.bytes;
   }
@@ -951,14 +969,58 @@ int ak_load_state(void *handle, const void *host, size_t bytes) {
   if (h.storage != c.storage || h.nx != c.nx || h.ny != c.ny || h.nz != c.nz || h.ff_kind != c.ff_kind ||
       h.nff != c.nff || h.nleg != c.nleg || h.pop_bytes != c.pop_bytes)
     return fail(s, "state of another lattice (storage, shape or boundary set differ)");
-  if (bytes != ak_state_bytes(handle)) return fail(s, "state size %zu, expected %zu", bytes, ak_state_bytes(handle));
+  // the residual snapshot travels with the state when it was taken
+  const bool with_prev = h.has_prev != 0;
+  if (bytes != state_size(s, with_prev)) return fail(s, "state size %zu, expected %zu", bytes, state_size(s, with_prev));
+  if (with_prev && !s->d_prev && dalloc(s, &s->d_prev, (size_t)4 * s->nx * s->ny * s->nz)) return -1;
+  s->prev_valid = with_prev;
   HIPCHK(s, hipStreamSynchronize(s->stream));
   const char *i = (const char *)host + sizeof h;
-  for (const StatePart &p : state_parts(s)) {
+  for (const StatePart &p : state_parts(s, with_prev)) {
     HIPCHK(s, hipMemcpy(p.dev, i, p.bytes, hipMemcpyHostToDevice));
     i += p.bytes;
   }
   s->sweeps = h.sweeps;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+template <int S, bool PE>
+void launch_residual(Lattice *s, bool incoming, double inv_u, double inv_dt, int reset, unsigned nb) {
+  using T = typename St<S>::T;
+  const T *A = (const T *)s->pop;
+  if (incoming)
+    hipLaunchKernelGGL((k_residual<S, PE, false, true>), dim3(nb), dim3(256), 0, s->stream, A, s->d_solid, s->L, s->inv_scale, inv_u, inv_dt, reset, s->d_prev, s->d_rpart);
+  else if (s->ff_kind == AK_FARFIELD_PERIODIC)
+    hipLaunchKernelGGL((k_residual<S, PE, true, false>), dim3(nb), dim3(256), 0, s->stream, A, s->d_solid, s->L, s->inv_scale, inv_u, inv_dt, reset, s->d_prev, s->d_rpart);
+  else
+    hipLaunchKernelGGL((k_residual<S, PE, false, false>), dim3(nb), dim3(256), 0, s->stream, A, s->d_solid, s->L, s->inv_scale, inv_u, inv_dt, reset, s->d_prev, s->d_rpart);
+}
+template <int S>
+void residual_any(Lattice *s, bool incoming, double inv_u, double inv_dt, int reset, unsigned nb) {
+  if (s->sweeps % 2 == 0) launch_residual<S, true>(s, incoming, inv_u, inv_dt, reset, nb);
+  else launch_residual<S, false>(s, incoming, inv_u, inv_dt, reset, nb);
+}
+}  // namespace
+
+extern "C" {
+
+int ak_residual(void *handle, int incoming, double lattice_speed, double dt, int reset, double *out) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  if (!(lattice_speed > 0) || !(dt > 0)) return fail(s, "residual needs a positive lattice speed and interval");
+  const long long n = (long long)s->nx * s->ny * s->nz;
+  const unsigned nb = blocks_for(n, 256);
+  if (!s->d_prev && dalloc(s, &s->d_prev, (size_t)4 * n)) return -1;
+  if (!s->d_rpart && (dalloc(s, &s->d_rpart, (size_t)9 * nb) || dalloc(s, &s->d_rout, 9))) return -1;
+  if (!s->prev_valid) reset = 1;
+  AK_DISPATCH(s, residual_any, s, incoming != 0, 1.0 / lattice_speed, 1.0 / dt, reset, nb);
+  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s->stream, (const double *)s->d_rpart, (int)nb, s->d_rout);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(out, s->d_rout, 9 * sizeof(double), hipMemcpyDeviceToHost));
+  s->prev_valid = true;
   return 0;
 }
 

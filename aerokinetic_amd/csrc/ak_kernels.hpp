@@ -824,6 +824,83 @@ __global__ void k_plane(const typename St<S>::T *__restrict__ A, Desc L, int z, 
   u_out[3 * t] = (double)jx / rho; u_out[3 * t + 1] = (double)jy / rho; u_out[3 * t + 2] = (double)jz / rho;
 }
 
+// Stationarity residual of the conserved fields q = (rho, j / U) over owned fluid cells
+// (mask 0): d = (q - q_prev) / dt.  Per-block sums of d^2 and maxima of |d| per channel and
+// the cell count (fixed-order reductions), then q_prev = q.  IN selects the incoming
+// (pre-collision) populations of a refined level, otherwise the stored post-collision ones.
+template <int S, bool PE, bool PXY, bool IN>
+__global__ __launch_bounds__(256) void k_residual(const typename St<S>::T *__restrict__ A,
+                                                  const uint8_t *__restrict__ solid, Desc L, float inv_scale,
+                                                  double inv_u, double inv_dt, int reset, double *__restrict__ prev,
+                                                  double *__restrict__ part) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // sum d^2 [4], max |d| [4], cells
+  if (t < (long long)L.nx * L.ny * L.nz) {
+    const int z = (int)(t % L.nz), y = (int)((t / L.nz) % L.ny), x = (int)(t / ((long long)L.nz * L.ny));
+    if (solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)] == 0) {
+      const long long c = L.base + cell_of(L, x, y, z);
+      double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+      for (int j = 0; j < NQ; ++j) {
+        long long src;
+        if constexpr (IN) src = loc_in<PE>(L, x, y, z, j);
+        else if constexpr (PE) src = c + (long long)opp(j) * L.QS;
+        else src = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
+        const double g = (double)ld<S>(A + src, inv_scale);
+        r += g; jx += cxq(j) * g; jy += cyq(j) * g; jz += czq(j) * g;
+      }
+      const double q[4] = {1.0 + r, jx * inv_u, jy * inv_u, jz * inv_u};
+      double *p = prev + t * 4;
+      for (int k = 0; k < 4; ++k) {
+        if (!reset) {
+          const double d = (q[k] - p[k]) * inv_dt;
+          v[k] = d * d;
+          v[4 + k] = fabs(d);
+        }
+        p[k] = q[k];
+      }
+      v[8] = 1.0;
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1)
+    for (int k = 0; k < 9; ++k) {
+      const double w = __shfl_xor(v[k], o, 64);
+      v[k] = (k >= 4 && k < 8) ? (w > v[k] ? w : v[k]) : v[k] + w;
+    }
+  __shared__ double red[4][9];
+  const int w = threadIdx.x / 64;
+  if (threadIdx.x % 64 == 0)
+    for (int k = 0; k < 9; ++k) red[w][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 9; ++k) {
+      double a = red[0][k];
+      for (int i = 1; i < (int)(blockDim.x / 64); ++i) a = (k >= 4 && k < 8) ? (red[i][k] > a ? red[i][k] : a) : a + red[i][k];
+      part[(long long)blockIdx.x * 9 + k] = a;
+    }
+  }
+}
+
+// Fixed-order fold of the residual block partials: out = {sum d^2 [4], max |d| [4], cells}.
+__global__ void k_residual_final(const double *__restrict__ part, int nblocks, double *__restrict__ out) {
+  __shared__ double acc[9][64];
+  const int lane = threadIdx.x;  // 64 threads
+  for (int k = 0; k < 9; ++k) {
+    double a = 0.0;
+    for (int b = lane; b < nblocks; b += 64) {
+      const double x = part[(long long)b * 9 + k];
+      a = (k >= 4 && k < 8) ? (x > a ? x : a) : a + x;
+    }
+    acc[k][lane] = a;
+  }
+  __syncthreads();
+  if (lane == 0)
+    for (int k = 0; k < 9; ++k) {
+      double a = acc[k][0];
+      for (int i = 1; i < 64; ++i) a = (k >= 4 && k < 8) ? (acc[k][i] > a ? acc[k][i] : a) : a + acc[k][i];
+      out[k] = a;
+    }
+}
+
 template <typename T>
 __global__ void k_fill(T *p, long long n, T v) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;

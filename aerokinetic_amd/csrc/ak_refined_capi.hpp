@@ -488,13 +488,15 @@ int ak_refined_save_state(void *handle, void *host, size_t bytes) {
 int ak_refined_load_state(void *handle, const void *host, size_t bytes) {
   Refined *r = R(handle);
   if (!r) return fail(nullptr, "null refined handle");
-  if (!host || bytes != ak_refined_state_bytes(handle)) return fail(nullptr, "refined state of another size");
+  if (!host || bytes < sizeof(long long)) return fail(nullptr, "refined state too short");
   const char *i = (const char *)host;
   long long steps = 0;
   std::memcpy(&steps, i, sizeof steps);
   i += sizeof(long long);
-  if (ak_load_state(r->c, i, ak_state_bytes(r->c)) || ak_load_state(r->f, i + ak_state_bytes(r->c), ak_state_bytes(r->f)))
-    return -1;
+  const size_t nc = state_blob_size(r->c, i, bytes - sizeof(long long));
+  const size_t nf = nc ? state_blob_size(r->f, i + nc, bytes - sizeof(long long) - std::min(nc, bytes - sizeof(long long))) : 0;
+  if (!nc || !nf || bytes != sizeof(long long) + nc + nf) return fail(nullptr, "refined state of another size or layout");
+  if (ak_load_state(r->c, i, nc) || ak_load_state(r->f, i + nc, nf)) return -1;
   r->steps = steps;
   return 0;
 }

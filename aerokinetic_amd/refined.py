@@ -249,6 +249,23 @@ class RefinedSolver:
         return {"local_mach_max": float(mach), "density_error_max": float(dens),
                 "wall_fallback_links_including_donors": self.wall_fallback_links}
 
+    @property
+    def residual_dt(self):
+        """Coarse convective time per coarse step (the residual's time unit)."""
+        return self.plan.coarse_level.dt
+
+    def residual_norms(self, reset=False, dt=1.0):
+        """Volume-weighted L2 and L-infinity of d(rho, j/U)/dt over the owned fluid cells of both
+        levels (pre-collision state; convergence.py:55-96)."""
+        from .convergence import combine_levels
+        parts = []
+        for lvl, dx in zip(self._levels, (self.plan.coarse_level.dx, self.plan.fine_level.dx)):
+            out = np.zeros(9)
+            _lib.check(self.L.ak_residual(lvl, 1, self.flow.lattice_speed, float(dt), 1 if reset else 0,
+                                          _lib.dptr(out)), lvl, "ak_residual")
+            parts.append((out, dx ** 3))
+        return (np.zeros(4), np.zeros(4)) if reset else combine_levels(parts)
+
     def save_checkpoint(self, path, context=None):
         """Restartable device state of both levels + run context (refined_checkpoint.py:54-142)."""
         from .checkpoint import save_checkpoint
@@ -257,6 +274,7 @@ class RefinedSolver:
     def run(self, output, callback=None, stop_event=None, resume=False, residual_callback=None):
         """Run project.run.duration_convective coarse time (refined_run.py:19-157); with resume=True
         continue an interrupted run in `output` from the checkpoint this solver was restored from."""
+        from .convergence import RunControl
         from .timing import duration_steps
         from ._io import file_hash
         from . import __version__
@@ -294,11 +312,12 @@ class RefinedSolver:
                                  **{k: float(row[k]) for k in ("cl", "cd", "cm")}})
         else:
             write_json(output / "project.json", refined_project_dict(self.project, self.refinement))
+        control = RunControl(self, settings, output, resume=resume, callback=residual_callback)
         metadata = {"version": __version__, "refined_plan": json.loads(json.dumps(self.plan.as_dict(), default=list)),
                     "case_fingerprint": self.case_fingerprint, "status": "running", "end_step": end_step,
                     "resume_supported": True, "device": {"backend": "hip-gfx950", "storage": self.storage}}
         write_json(output / "metadata.json", metadata)
-        state, start = "completed", time.monotonic()
+        state, reason, start = "completed", "duration_reached", time.monotonic()
         try:
             with history.open("a" if resume else "w", newline="", encoding="utf-8") as handle:
                 writer = csv.DictWriter(handle, self.HISTORY_COLUMNS)
@@ -308,19 +327,20 @@ class RefinedSolver:
                 def checkpoint():
                     handle.flush()
                     os.fsync(handle.fileno())
-                    self.save_checkpoint(output / "checkpoint.npz",
-                                         {"case_fingerprint": self.case_fingerprint, "end_step": end_step,
-                                          "history_bytes": history.stat().st_size,
-                                          "history_sha256": file_hash(history)})
+                    context = {"case_fingerprint": self.case_fingerprint, "end_step": end_step,
+                               "history_bytes": history.stat().st_size, "history_sha256": file_hash(history)}
+                    context.update(control.checkpoint_context())
+                    self.save_checkpoint(output / "checkpoint.npz", context)
 
-                se, ce = settings.sample_every, settings.checkpoint_every
+                se, ce, ke = settings.sample_every, settings.checkpoint_every, settings.check_every
                 while self.steps < end_step:
                     if stop_event is not None and stop_event.is_set():
-                        state = "stopped"
+                        state, reason = "stopped", "user_stopped"
                         break
                     nxt = min(end_step, (self.steps // se + 1) * se if callback else self.steps + 64 * se)
-                    if ce:
-                        nxt = min(nxt, (self.steps // ce + 1) * ce)
+                    for every in (ce, ke):
+                        if every:
+                            nxt = min(nxt, (self.steps // every + 1) * every)
                     recs = self.advance(nxt - self.steps, se)
                     stop = False
                     for rec in recs:
@@ -329,19 +349,26 @@ class RefinedSolver:
                         writer.writerow({k: rec.get(k, "") for k in self.HISTORY_COLUMNS})
                         rows.append(rec)
                         if callback is not None and callback(self, rec) is False:
-                            state, stop = "stopped", True
+                            state, reason, stop = "stopped", "user_stopped", True
                             break
                     handle.flush()
-                    if settings.check_every and self.steps % settings.check_every == 0:
+                    if ke and self.steps % ke == 0:
                         self.diagnostics()
+                        control.observe()
                     if stop:
+                        break
+                    selected = control.stop_reason()
+                    if selected:
+                        reason = selected
+                        if reason == "wall_time_limit":
+                            state = "stopped"
                         break
                     if ce and self.steps % ce == 0 and self.steps < end_step:
                         checkpoint()
                 self.diagnostics()
                 checkpoint()
         except BaseException as error:
-            state = "failed"
+            state, reason = "failed", "failed"
             metadata["failure"] = f"{type(error).__name__}: {error}"
             raise
         finally:
@@ -351,6 +378,9 @@ class RefinedSolver:
                        "wall_clock_seconds": time.monotonic() - start, "samples": len(rows),
                        "mean": dict(zip(("cl", "cd", "cm"), map(float, sel.mean(axis=0)))) if ok else None,
                        "rms_fluctuation": dict(zip(("cl", "cd", "cm"), map(float, sel.std(axis=0)))) if ok else None}
+            summary.update(control.summary(reason))
+            control.close()
+            metadata["termination_reason"] = reason
             metadata["status"] = state
             write_json(output / "metadata.json", metadata)
             write_json(output / "summary.json", summary)

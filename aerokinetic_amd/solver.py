@@ -156,6 +156,21 @@ class Solver:
         self.lattice.set_populations(g)
 
     # ------------------------------------------------------------------ outputs
+    @property
+    def residual_dt(self):
+        """Convective time per step (the residual's time unit)."""
+        return self.plan.dt_convective
+
+    def residual_norms(self, reset=False, dt=1.0):
+        """L2 and L-infinity of d(rho, j/U)/dt over owned fluid cells since the last call
+        (convergence.py:55-96); reset only takes the snapshot."""
+        from .convergence import combine_levels
+        out = np.zeros(9)
+        h = self.lattice.h
+        _lib.check(_lib.load().ak_residual(h, 0, self.flow.lattice_speed, float(dt), 1 if reset else 0,
+                                           _lib.dptr(out)), h, "ak_residual")
+        return (np.zeros(4), np.zeros(4)) if reset else combine_levels([(out, self.plan.dx ** 3)])
+
     def save_checkpoint(self, path, context=None):
         """Restartable device state + run context (checkpoint.py:32-70)."""
         from .checkpoint import save_checkpoint
@@ -188,7 +203,7 @@ class Solver:
         callback(solver, record) may return False to stop.  Mean/RMS coefficients
         are plain sample statistics over t_star >= average_after.
         """
-        from .convergence import StopCriteria
+        from .convergence import RunControl, StopCriteria
         from .timing import duration_steps
         from ._io import file_hash
         from . import __version__
@@ -205,8 +220,6 @@ class Solver:
         output.mkdir(parents=True, exist_ok=True)
         (output / "frames").mkdir(exist_ok=resume)
         end_step = self.steps + duration_steps(duration_convective, self.plan.dt_convective)
-        if termination.max_steps:
-            end_step = min(end_step, self.steps + termination.max_steps)
         history = output / "history.csv"
         rows = []
         start_step = self.steps
@@ -258,6 +271,9 @@ class Solver:
                                "library": _lib.load().ak_version().decode()}}
         if not resume:
             project.save(output / "project.json")
+        control = RunControl(self, settings, output, resume=resume, callback=residual_callback)
+        metadata["residual_definition"] = ("Volume-weighted L2/Linf of d(rho,j_common)/dt*, owned fluid only; "
+                                           "stationary-flow residual, not a turbulence-accuracy measure")
         write_json(output / "metadata.json", metadata)
         state, reason = "completed", "duration_reached"
         start = time.monotonic()
@@ -272,10 +288,10 @@ class Solver:
                 def checkpoint():
                     handle.flush()
                     os.fsync(handle.fileno())
-                    self.save_checkpoint(output / "checkpoint.npz",
-                                         {"case_fingerprint": project.case_fingerprint, "end_step": end_step,
-                                          "history_bytes": history.stat().st_size,
-                                          "history_sha256": file_hash(history)})
+                    context = {"case_fingerprint": project.case_fingerprint, "end_step": end_step,
+                               "history_bytes": history.stat().st_size, "history_sha256": file_hash(history)}
+                    context.update(control.checkpoint_context())
+                    self.save_checkpoint(output / "checkpoint.npz", context)
 
                 stop = False
                 while self.steps < end_step and not stop:
@@ -306,10 +322,16 @@ class Solver:
                     handle.flush()
                     if check_every and self.steps % check_every == 0:
                         self.diagnostics()
+                        control.observe()
                     if (frame_every and self.steps % frame_every == 0) or self.steps == end_step:
                         self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
-                    if termination.wall_time_limit_s and time.monotonic() - start > termination.wall_time_limit_s:
-                        state, reason = "stopped", "wall_time_limit"
+                    if stop:
+                        break
+                    selected = control.stop_reason()
+                    if selected:
+                        reason = selected
+                        if reason == "wall_time_limit":
+                            state = "stopped"
                         break
                     if checkpoint_every and self.steps % checkpoint_every == 0 and self.steps < end_step:
                         checkpoint()
@@ -329,8 +351,9 @@ class Solver:
                        "mean": dict(zip(("cl", "cd", "cm"), map(float, sel.mean(axis=0)))) if ok else None,
                        "rms_fluctuation": dict(zip(("cl", "cd", "cm"), map(float, sel.std(axis=0)))) if ok else None,
                        "wall_fallback_links": self.wall_fallback_links,
-                       "experimental_validation": "not_performed", "termination_reason": reason,
-                       "termination": asdict(termination)}
+                       "experimental_validation": "not_performed"}
+            summary.update(control.summary(reason))
+            control.close()
             metadata["termination_reason"] = reason
             metadata["status"] = state
             write_json(output / "metadata.json", metadata)

@@ -139,6 +139,14 @@ size_t ak_state_bytes(void *handle);
 int ak_save_state(void *handle, void *host, size_t bytes);
 int ak_load_state(void *handle, const void *host, size_t bytes);
 
+/* Stationarity residual (convergence.py:55-96, 138-158): d = (q - q_prev) / dt with
+ * q = (rho, jx / U, jy / U, jz / U) on owned fluid cells (mask 0), from the stored
+ * post-collision state (incoming = 0, uniform lattices) or the incoming pre-collision state
+ * (incoming = 1, refined levels); then q_prev = q.  out = {sum d^2 [4], max |d| [4], cells}.
+ * reset = 1 (or no snapshot yet) only records q_prev (out = zeros).  The snapshot is part of
+ * the restartable state once taken. */
+int ak_residual(void *handle, int incoming, double lattice_speed, double dt, int reset, double *out);
+
 /* Wait for all work of this lattice. */
 int ak_synchronize(void *handle);
 

@@ -76,8 +76,9 @@ def test_interrupted_run_resumes_to_the_same_history(tmp_path):
     resumed = restore_checkpoint(Solver, tmp_path / "part" / "checkpoint.npz")
     s3 = resumed.run(dur, tmp_path / "part", resume=True, **kw)
     assert s3["status"] == "completed" and s3["steps"] == s1["steps"]
-    read = lambda p: list(csv.reader(open(p / "history.csv")))
+    read = lambda p, f="history.csv": list(csv.reader(open(p / f)))
     assert read(tmp_path / "full") == read(tmp_path / "part")
+    assert read(tmp_path / "full", "residuals.csv") == read(tmp_path / "part", "residuals.csv")
     assert s3["mean"] == s1["mean"]
 
 
@@ -99,5 +100,46 @@ def test_interrupted_refined_run_resumes_to_the_same_history(tmp_path):
     assert resumed.steps == 30
     s3 = resumed.run(tmp_path / "part", resume=True)
     assert s3["status"] == "completed" and s3["steps"] == s1["steps"] == 60
-    read = lambda p: list(csv.reader(open(p / "history.csv")))
+    read = lambda p, f="history.csv": list(csv.reader(open(p / f)))
     assert read(tmp_path / "full") == read(tmp_path / "part")
+    assert read(tmp_path / "full", "residuals.csv") == read(tmp_path / "part", "residuals.csv")
+
+
+def test_residuals_match_host_restatement():
+    """Device residual norms = the reference's definition on the exported state
+    (convergence.py:55-96): d = (q1 - q0)/dt, q = (rho, j/U) on owned fluid cells."""
+    from aerokinetic_amd.d3q27 import CX, CY, CZ
+    from aerokinetic_amd.solver import Solver
+    section, flow, grid = small_case("characteristic_transport_v1")
+    s = Solver(section, flow, grid)
+    s.advance(5, 0)
+    fluid = ~s._solid_xy
+
+    def q(g):
+        out = np.stack([1.0 + g.sum(-1), g @ CX, g @ CY, g @ CZ], axis=-1)
+        out[..., 1:] /= flow.lattice_speed
+        return out[fluid]
+
+    q0 = q(s.populations())
+    s.residual_norms(reset=True)
+    s.advance(7, 0)
+    dt = 7 * s.residual_dt
+    l2, linf = s.residual_norms(dt=dt)
+    d = (q(s.populations()) - q0) / dt
+    np.testing.assert_allclose(l2, np.sqrt((d * d).mean(axis=(0, 1))), rtol=1e-9)
+    np.testing.assert_allclose(linf, np.abs(d).max(axis=(0, 1)), rtol=1e-9)
+    assert np.all(l2 > 0)
+
+
+def test_steady_mode_stops_on_residual_targets(tmp_path):
+    from aerokinetic_amd.convergence import StopCriteria
+    from aerokinetic_amd.solver import Solver
+    section, flow, grid = small_case("legacy")
+    s = Solver(section, flow, grid)
+    crit = StopCriteria(mode="steady", mass_tolerance=1e3, momentum_x_tolerance=1e3, momentum_y_tolerance=1e3,
+                        momentum_z_tolerance=1e3, minimum_t_star=0.0, consecutive_checks=3)
+    summary = s.run(400 * s.plan.dt_convective, tmp_path / "r", sample_every=10, check_every=20, termination=crit)
+    assert summary["termination_reason"] == "residual_targets_met" and summary["steps"] == 60
+    rows = list(csv.DictReader(open(tmp_path / "r" / "residuals.csv")))
+    assert [int(r["step"]) for r in rows] == [20, 40, 60] and rows[-1]["streak"] == "3"
+    assert float(rows[0]["l2_jx"]) > 0
