@@ -31,6 +31,9 @@
  *   - characteristic farfield with NEEP reconstruction (farfield.py:34-224);
  *   - legacy farfield: velocity bounce-back on x-, y-, y+ and the
  *     history-carrying extrapolation outflow on x+ (solver.py:130-135);
+ *   - exterior reservoir: the same velocity bounce-back on every x/y side of
+ *     the padded storage of the exterior farfield (exterior.py:29-34); the
+ *     exterior ring itself is restated in exterior.py next to this file;
  *   - loads: F = sum c_d (f_out(x,d) + f_in(x+c_d, dbar)), z-moment about the
  *     moment reference (solver.py:204-250) and the coefficient normalisation
  *     (solver.py:27-30);
@@ -101,7 +104,7 @@ typedef struct {
     wall_link *links;
     long fallback_links;
     double rx_off, ry_off, chord_cells;
-    /* farfield: 0 none (periodic x/y), 1 legacy, 2 characteristic */
+    /* farfield: 0 none (periodic x/y), 1 legacy, 2 characteristic, 3 exterior reservoir */
     int ff_kind;
     int nff;
     ff_link *ff;
@@ -512,7 +515,9 @@ static void build_characteristic(ako *s) {
     s->nff = n;
 }
 
-static void build_legacy(ako *s) {
+/* velocity bounce-back links of every boundary cell; with `outflow` the x+ side is the
+ * extrapolation outflow instead (legacy), without it every side is bounce-back (reservoir) */
+static void build_legacy(ako *s, int outflow) {
     int cu = 4096, co = 1024, nu = 0, no = 0;
     wall_link *Lu = (wall_link *)malloc(sizeof(wall_link) * cu);
     out_link *Lo = (out_link *)malloc(sizeof(out_link) * co);
@@ -522,7 +527,7 @@ static void build_legacy(ako *s) {
             for (int z = 0; z < s->nz; ++z)
                 for (int d = 0; d < NQ; ++d) {
                     const int gx = x + CX[d], gy = y + CY[d];
-                    if (gx >= s->nx) {
+                    if (gx >= s->nx && outflow) {
                         if (no == co) { co *= 2; Lo = (out_link *)realloc(Lo, sizeof(out_link) * co); }
                         out_link *l = &Lo[no++];
                         l->x = x; l->y = y; l->z = z; l->d = d;
@@ -534,7 +539,7 @@ static void build_legacy(ako *s) {
                         const double uinf[3] = {s->U, 0.0, 0.0};
                         l->pdf_nd = equilibrium(id, 1.0, uinf) - WT[id];
                         l->pdf = l->pdf_nd;
-                    } else if (gx < 0 || gy < 0 || gy >= s->ny) {
+                    } else if (gx < 0 || gx >= s->nx || gy < 0 || gy >= s->ny) {
                         if (nu == cu) { cu *= 2; Lu = (wall_link *)realloc(Lu, sizeof(wall_link) * cu); }
                         wall_link *l = &Lu[nu++];
                         l->x = x; l->y = y; l->z = z; l->d = d; l->q = 0; l->elem = 0;
@@ -546,14 +551,15 @@ static void build_legacy(ako *s) {
     s->out = Lo; s->nout = no;
 }
 
-/* kind: 0 = none (periodic x/y), 1 = legacy, 2 = characteristic.  Reads the
- * current populations, so call after the initial state is set. */
+/* kind: 0 = none (periodic x/y), 1 = legacy, 2 = characteristic, 3 = exterior
+ * reservoir (bounce-back on every side).  Reads the current populations, so call
+ * after the initial state is set. */
 int ako_set_farfield(void *h, int kind) {
     ako *s = (ako *)h;
     s->ff_kind = kind;
     if (kind == 2) build_characteristic(s);
-    else if (kind == 1) build_legacy(s);
-    return kind == 2 ? s->nff : kind == 1 ? s->nubb + s->nout : 0;
+    else if (kind == 1 || kind == 3) build_legacy(s, kind == 1);
+    return kind == 2 ? s->nff : (kind == 1 || kind == 3) ? s->nubb + s->nout : 0;
 }
 
 /* ---------------------------------------------------------------- step */
@@ -729,7 +735,7 @@ void ako_boundaries(void *h) {
     ako *s = (ako *)h;
     sync_periodic(s);
     if (s->ff_kind == 2) apply_characteristic(s);
-    else if (s->ff_kind == 1) apply_legacy(s);
+    else if (s->ff_kind == 1 || s->ff_kind == 3) apply_legacy(s);
     apply_walls(s);
 }
 
@@ -776,7 +782,7 @@ void ako_step(void *h, double *loads, double *coef) {
     ako *s = (ako *)h;
     sync_periodic(s);
     if (s->ff_kind == 2) apply_characteristic(s);
-    else if (s->ff_kind == 1) apply_legacy(s);
+    else if (s->ff_kind == 1 || s->ff_kind == 3) apply_legacy(s);
     apply_walls(s);
     if (loads) {
         measure_loads(s, loads);
@@ -817,6 +823,49 @@ void ako_diagnostics(void *h, double *out) {
     out[1] = derr;
     out[2] = rmin;
     out[3] = finite;
+}
+
+/* populations of a list of interior cells (xyz: n x 3), current state */
+void ako_get_cells(void *h, int n, const int *xyz, double *g) {
+    ako *s = (ako *)h;
+    for (int k = 0; k < n; ++k)
+        memcpy(g + (long)k * NQ, s->src + cid(s, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]) * NQ, sizeof(double) * NQ);
+}
+
+void ako_set_cells(void *h, int n, const int *xyz, const double *g) {
+    ako *s = (ako *)h;
+    for (int k = 0; k < n; ++k)
+        memcpy(s->src + cid(s, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]) * NQ, g + (long)k * NQ, sizeof(double) * NQ);
+}
+
+/* the state after one collision, without changing it: (nx, ny, nz, 27); solid cells as stored */
+void ako_collided(void *h, double *g) {
+    ako *s = (ako *)h;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < s->nx; ++x)
+        for (int y = 0; y < s->ny; ++y)
+            for (int z = 0; z < s->nz; ++z) {
+                const double *src = s->src + cid(s, x, y, z) * NQ;
+                double *o = g + (((long)x * s->ny + y) * s->nz + z) * NQ;
+                if (s->solid[(x + 1) * s->Y + (y + 1)]) memcpy(o, src, sizeof(double) * NQ);
+                else collide(src, o, s->omega_s, s->omega_b);
+            }
+}
+
+/* wall loads of the current state after the boundary objects: per element [Fx, Fy, Fz, Mz]
+ * and the wall mass exchange sum(incoming - outgoing) (solver.py:204-250) */
+void ako_wall_loads(void *h, double *per_elem, double *mass) {
+    ako *s = (ako *)h;
+    measure_loads(s, per_elem);
+    double m = 0.0;
+    for (int k = 0; k < s->nlinks; ++k) {
+        const wall_link *l = &s->links[k];
+        const int d = l->d, id = NQ - 1 - d;
+        const double out = s->src[cid(s, l->x, l->y, l->z) * NQ + d];
+        const double in = s->src[cid(s, l->x + CX[d], l->y + CY[d], l->z + CZ[d]) * NQ + id];
+        m += in - out;
+    }
+    *mass = m;
 }
 
 int ako_threads(void) {

@@ -62,6 +62,10 @@ def lib():
             getattr(L, name).argtypes = [ctypes.c_void_p]
         L.ako_measure.argtypes = [ctypes.c_void_p, _c_int_p, _c_int_p, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, _c_double_p]
+        L.ako_get_cells.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
+        L.ako_set_cells.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
+        L.ako_collided.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.ako_wall_loads.argtypes = [ctypes.c_void_p, _c_double_p, _c_double_p]
         _lib = L
     return _lib
 
@@ -74,7 +78,7 @@ def _ip(a):
     return a.ctypes.data_as(_c_int_p)
 
 
-FARFIELD_KIND = {"periodic": 0, "legacy": 1, "characteristic_transport_v1": 2}
+FARFIELD_KIND = {"periodic": 0, "legacy": 1, "characteristic_transport_v1": 2, "reservoir": 3}
 
 
 def seed_parameters(section, flow, plan):
@@ -163,6 +167,41 @@ class Oracle:
         coef = np.zeros(3)
         self.L.ako_step(self.h, _dp(loads), _dp(coef))
         return {"cl": coef[0], "cd": coef[1], "cm": coef[2], "element_lattice_loads": loads[: self.nelem]}
+
+    # collide_stream building blocks (exterior farfield, refined levels)
+    def collide(self):
+        self.L.ako_collide(self.h)
+
+    def boundaries(self):
+        self.L.ako_boundaries(self.h)
+
+    def stream(self):
+        self.L.ako_stream(self.h)
+
+    def cells(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32)
+        g = np.zeros((len(xyz), 27))
+        self.L.ako_get_cells(self.h, len(xyz), _ip(xyz), _dp(g))
+        return g
+
+    def set_cells(self, xyz, g):
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        assert g.shape == (len(xyz), 27)
+        self.L.ako_set_cells(self.h, len(xyz), _ip(xyz), _dp(g))
+
+    def collided(self):
+        """The state after one more collision (the stored state is left as it is)."""
+        g = np.zeros((*self.shape, 27))
+        self.L.ako_collided(self.h, _dp(g))
+        return g
+
+    def wall_loads(self):
+        """Per-element [Fx, Fy, Fz, Mz] and the wall mass exchange of the current state."""
+        loads = np.zeros((max(self.nelem, 1), 4))
+        mass = np.zeros(1)
+        self.L.ako_wall_loads(self.h, _dp(loads), _dp(mass))
+        return loads[: self.nelem], float(mass[0])
 
     def diagnostics(self):
         out = np.zeros(4)

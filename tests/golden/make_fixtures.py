@@ -8,6 +8,12 @@ so the tests never need it.  Sources (reference paths):
   tests/fixtures/legacy051_project.json (fingerprint fixture)
   docs/verification_040/{startup_project.json,startup_history.csv} (two-level startup, every 4 coarse steps)
   docs/verification_multielement_050/fowler/{project.json,history.csv} (two elements, every coarse step)
+  docs/verification_user_tests_20261108/raw/final_smoke/study/cases/N16/{project.json,history.csv}
+      (exterior farfield, 136x51x8, every step from step 1)
+  docs/verification_exterior_library_20261108/raw/cold_offline_final_v2/runtime/parent/{project.json,history.csv}
+      (exterior farfield, 88x51x8, every 4 steps)
+  docs/verification_exterior_library_20261108/raw/matched_airfoils/airfoil_down3/{project.json,history.csv}
+      (exterior farfield, 176x101x16, every 8 steps)
 """
 import csv
 import json
@@ -21,6 +27,9 @@ OUT = Path(__file__).resolve().parent
 CASES = {
     "n32_legacy_down3": ("docs/verification_stage1_20261106/corrected32_t12", 240),
     "n32_characteristic_down3": ("docs/verification_farfield_20261107/airfoil/characteristic_down3", 400),
+    "exterior_n16_smoke": ("docs/verification_user_tests_20261108/raw/final_smoke/study/cases/N16", 48),
+    "exterior_n16_short": ("docs/verification_exterior_library_20261108/raw/cold_offline_final_v2/runtime/parent", 32),
+    "exterior_n32_down3": ("docs/verification_exterior_library_20261108/raw/matched_airfoils/airfoil_down3", 400),
 }
 
 for name, (rel, last_step) in CASES.items():

@@ -98,3 +98,116 @@ def test_refined_oracle_pins_transfer_interface_and_fine_walls(name, steps):
         if rec["step"] in ref:
             for k in ("cl", "cd", "cm"):
                 assert rec[k] == pytest.approx(ref[rec["step"]][k], abs=1e-12)
+
+
+# ---------------------------------------------------------------- exterior farfield
+EXT_COLUMNS = ("wall_mass_exchange_lattice", "wall_mass_exchange_accumulated",
+               "exterior_source_mass_lattice", "exterior_source_mass_accumulated")
+
+
+def _exterior_history(name):
+    with open(GOLDEN / name / "history_head.csv", newline="") as fh:
+        return {int(r["step"]): {k: float(v) for k, v in r.items() if k != "time_s" and v != ""}
+                for r in csv.DictReader(fh)}
+
+
+@pytest.mark.parametrize("name, shape, last_step", [
+    ("exterior_n16_smoke", (136, 51, 8), 48),   # every step from step 1
+    ("exterior_n16_short", (88, 51, 8), 32),
+    ("exterior_n32_down3", (176, 101, 16), 160),  # departs from the legacy history from step 64
+])
+def test_exterior_oracle_pins_ring_predictor_reconstruction_and_sources(name, shape, last_step):
+    """The exterior restatement (oracle/exterior.py) against the reference's recorded
+    exterior runs: loads, guards, wall mass exchange and the exterior source of mass."""
+    from oracle.exterior import ExteriorOracle
+    project = Project.load(GOLDEN / name / "project.json")
+    plan = project.plan()
+    assert plan.shape == shape and project.grid.farfield == "exterior_regularized_fd_v1"
+    ref = _exterior_history(name)
+    o = ExteriorOracle(project.section, project.flow, project.grid, plan)
+    assert o.storage_shape == (shape[0] + 2, shape[1] + 2, shape[2])
+    for step in range(1, last_step + 1):
+        rec = o.step(sample=step in ref)
+        if step not in ref:
+            continue
+        r = ref[step]
+        for k in ("cl", "cd", "cm"):
+            assert abs(rec[k] - r[k]) < 5e-12, (step, k, rec[k], r[k])
+        d = o.diagnostics()
+        assert d["local_mach_max"] == pytest.approx(r["local_mach_max"], rel=1e-12)
+        assert d["density_error_max"] == pytest.approx(r["density_error_max"], rel=1e-11)
+        for k in EXT_COLUMNS:
+            # a step's source is a sum of ~1e5 ring populations' changes (1e-13 absolute
+            # rounding floor per step, which the accumulated column adds up)
+            floor = 1e-12 + (2e-13 * step if k.endswith("accumulated") else 0.0)
+            assert abs(rec[k] - r[k]) <= floor + 1e-9 * abs(r[k]), (step, k, rec[k], r[k])
+
+
+def _band_oracle():
+    from oracle.exterior import ExteriorOracle
+    project = Project.load(GOLDEN / "exterior_n16_short" / "project.json")
+    return ExteriorOracle(project.section, project.flow, project.grid, project.plan())
+
+
+def test_exterior_reconstruction_moments_closed_form():
+    """For a linear velocity field the three-point differences are exact, so the
+    reconstructed ring populations must carry rho, rho u and the second moment
+    rho (u u + I/3) - rho/3 [(A + A^T)/omega + 2/3 tr(A) (1 - 1/omega) I] exactly."""
+    from oracle.exterior import _C
+    o = _band_oracle()
+    X, Y, Z = o.storage_shape
+    A = np.array([[1e-3, -2e-3, 0.0], [5e-4, -7e-4, 0.0], [0.0, 0.0, 0.0]])  # A[a][b] = d_a u_b
+    xyz = o.band_xyz.reshape(len(o.band), Z, 3).astype(float)
+    u = np.array([0.04, 0.01, -0.002]) + xyz[..., 0, None] * A[0] + xyz[..., 1, None] * A[1]
+    lnrho = np.full(u.shape[:2], np.log(1.003))
+    V = np.concatenate([lnrho[..., None], u], axis=-1)
+    pdf, err = o.regularize(V)
+    assert err < 1e-15
+    rho = 1.003
+    w = o.omega
+    S = -rho / 3 * ((A + A.T) / w + np.eye(3) * (2 / 3 * np.trace(A) * (1 - 1 / w)))
+    f = pdf + np.array([{0: 8 / 27, 1: 2 / 27, 2: 1 / 54, 3: 1 / 216}[int(np.abs(c).sum())] for c in _C])
+    second = np.einsum("rzi,ia,ib->rzab", f, _C, _C)
+    ur = u[: o.R]
+    expect = rho * (ur[..., :, None] * ur[..., None, :] + np.eye(3) / 3) + S
+    assert np.max(np.abs(second - expect)) < 1e-15
+
+
+def test_exterior_predictor_keeps_free_stream():
+    """The uniform free stream is a fixed point of the ring predictor (rates vanish, the
+    imposed inflow and glancing invariants are the free stream's)."""
+    o = _band_oracle()
+    X, Y, Z = o.storage_shape
+    V = np.zeros((len(o.band), Z, 4))
+    V[..., 1] = o.U
+    o.rows = np.stack([-V[o.g_row_index[0], :, 2] - V[o.g_row_index[0], :, 0] * np.sqrt(1 / 3),
+                       V[o.g_row_index[1], :, 2] - V[o.g_row_index[1], :, 0] * np.sqrt(1 / 3)])
+    y = o.predict(V, 7)
+    assert np.max(np.abs(y - V[: o.R])) < 1e-17
+
+
+def test_exterior_predictor_signs_fail_on_planted_mistake(monkeypatch):
+    """A flipped sign of the density rate of the predictor breaks the pin above within
+    48 steps (through the exterior source column; the loads feel it later)."""
+    import oracle.exterior as E
+    project = Project.load(GOLDEN / "exterior_n16_smoke" / "project.json")
+    ref = _exterior_history("exterior_n16_smoke")
+    good = E.ExteriorOracle(project.section, project.flow, project.grid, project.plan())
+    bad = E.ExteriorOracle(project.section, project.flow, project.grid, project.plan())
+    rhs = bad._rhs
+
+    def flipped(y, cur):
+        out = rhs(y, cur)
+        out[..., 0] = -out[..., 0]
+        return out
+    bad._rhs = flipped
+
+    def violation(rec, r):
+        k = "exterior_source_mass_lattice"
+        return abs(rec[k] - r[k]) - (1e-12 + 1e-9 * abs(r[k]))
+    worst_good = worst_bad = -1.0
+    for step in range(1, 49):
+        a, b = good.step(sample=True), bad.step(sample=True)
+        worst_good = max(worst_good, violation(a, ref[step]))
+        worst_bad = max(worst_bad, violation(b, ref[step]))
+    assert worst_good <= 0 < worst_bad
