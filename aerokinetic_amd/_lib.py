@@ -16,8 +16,9 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "lib" / "libaerokinetic_hip.so"
 
 STORE = {"fp64": 0, "fp32": 1, "fp16": 2}
-FARFIELD = {"periodic": 0, "legacy": 1, "characteristic_transport_v1": 2}
+FARFIELD = {"periodic": 0, "legacy": 1, "characteristic_transport_v1": 2, "exterior_regularized_fd_v1": 4}
 HIST_FIXED = 9
+HIST_EXT = 41  # exterior columns: wall mass accumulated, source mass, source mass accumulated
 MAX_ELEM = 8
 
 
@@ -63,6 +64,7 @@ _SIGS = {
     "ak_device_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     "ak_state_bytes": (ctypes.c_size_t, [_vp]),
     "ak_residual": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]),
+    "ak_exterior_state": (ctypes.c_int, [_vp, _dp]),
     "ak_save_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "ak_load_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "ak_refined_state_bytes": (ctypes.c_size_t, [_vp]),

@@ -19,6 +19,7 @@
 
 #include "../../include/aerokinetic.h"
 #include "ak_kernels.hpp"
+#include "ak_exterior.hpp"
 
 using namespace ak;
 
@@ -68,6 +69,13 @@ struct Lattice {
   int32_t *d_legx = nullptr, *d_legz = nullptr;
   int8_t *d_legdir = nullptr, *d_legkind = nullptr;
   double *d_pdf = nullptr, *d_pdfnd = nullptr;
+  // exterior farfield: band tables, band / ring states, source partials and accumulators
+  int ext_R2 = 0, ext_B2 = 0, ext_ring_blocks = 0;
+  int32_t *d_ext_bxy = nullptr, *d_ext_map = nullptr;
+  double *d_ext_M = nullptr, *d_ext_Y1 = nullptr, *d_ext_Y2 = nullptr, *d_ext_pred = nullptr;
+  double *d_ext_rows = nullptr, *d_ext_part = nullptr;
+  ExtAcc *d_ext_acc = nullptr;
+  int *d_ext_err = nullptr;
   // diagnostics / history
   DiagAcc *d_diag = nullptr;
   double *d_hist = nullptr;
@@ -181,7 +189,8 @@ void launch_boundary(Lattice *s, bool loads) {
     FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff};
     hipLaunchKernelGGL((k_farfield<S, PE>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, A, s->L, I,
                        s->ffp, s->scale, s->inv_scale);
-  } else if (s->ff_kind == AK_FARFIELD_LEGACY && s->nleg) {
+  } else if ((s->ff_kind == AK_FARFIELD_LEGACY || s->ff_kind == AK_FARFIELD_EXTERIOR) && s->nleg) {
+    // the exterior reservoir is the same velocity bounce-back on every side
     LegacyLinks K{s->d_legx, s->d_legz, s->d_legdir, s->d_legkind, s->d_pdf, s->d_pdfnd, s->nleg};
     hipLaunchKernelGGL((k_legacy<S, PE>), dim3(blocks_for(s->nleg, 128)), dim3(128), 0, s->stream, A, s->L, K,
                        s->U, s->scale, s->inv_scale);
@@ -203,14 +212,64 @@ void launch_loads_final(Lattice *s) {
                      s->nelem, 1.0 / qa, 1.0 / (qa * s->chord), 1.0, s->d_ctl, s->d_hist, s->d_diag);
 }
 
-// One uniform step; the step number and history row come from s->d_ctl.
+ExtDev ext_dev(Lattice *s) {
+  ExtDev E{};
+  E.band_xy = s->d_ext_bxy; E.map = s->d_ext_map;
+  E.R2 = s->ext_R2; E.B2 = s->ext_B2; E.X = s->nx; E.Y = s->ny; E.Z = s->nz;
+  E.U = s->U; E.omega = s->ws;
+  E.M = s->d_ext_M; E.Y1 = s->d_ext_Y1; E.Y2 = s->d_ext_Y2; E.PRED = s->d_ext_pred;
+  E.rows = s->d_ext_rows; E.part = s->d_ext_part; E.acc = s->d_ext_acc; E.err = s->d_ext_err;
+  return E;
+}
+
+// The three predictor stages of the ring; t from the step cursor (ctl) or t0.
+void launch_ext_predict(Lattice *s, const StepCtl *ctl, double t0) {
+  const ExtDev E = ext_dev(s);
+  const dim3 g(s->ext_ring_blocks), b(128);
+  hipLaunchKernelGGL(k_ext_stage<1>, g, b, 0, s->stream, E, ctl, t0);
+  hipLaunchKernelGGL(k_ext_stage<2>, g, b, 0, s->stream, E, ctl, t0);
+  hipLaunchKernelGGL(k_ext_stage<3>, g, b, 0, s->stream, E, ctl, t0);
+}
+
+// Exterior phases after the sweep of form `odd` (the ring was skipped by it).
 template <int S>
-void one_step(Lattice *s, bool sample) {
+void launch_exterior(Lattice *s, bool odd, bool sample) {
+  using T = typename St<S>::T;
+  using C = typename St<S>::C;
+  T *A = (T *)s->pop;
+  const ExtDev E = ext_dev(s);
+  const long long nb = (long long)(s->ext_B2 - s->ext_R2) * s->nz;
+  // after an odd sweep the state is in streamed form, after an even one in post-even form
+  if (odd)
+    hipLaunchKernelGGL((k_ext_band<S, false>), dim3(blocks_for(nb, 128)), dim3(128), 0, s->stream, (const T *)A, s->L, E, s->ext_R2, s->ext_B2, s->inv_scale);
+  else
+    hipLaunchKernelGGL((k_ext_band<S, true>), dim3(blocks_for(nb, 128)), dim3(128), 0, s->stream, (const T *)A, s->L, E, s->ext_R2, s->ext_B2, s->inv_scale);
+  const dim3 g(s->ext_ring_blocks), b(128);
+  const C ws = (C)s->ws, wb = (C)s->wb;
+#define AK_RING(O, D) hipLaunchKernelGGL((k_ext_ring<S, O, D>), g, b, 0, s->stream, A, s->L, E, ws, wb, s->scale, s->inv_scale, s->d_diag)
+  if (odd) { if (sample) AK_RING(true, true); else AK_RING(true, false); }
+  else { if (sample) AK_RING(false, true); else AK_RING(false, false); }
+#undef AK_RING
+  launch_ext_predict(s, s->d_ctl, 0.0);
+  hipLaunchKernelGGL(k_ext_final, dim3(1), dim3(64), 0, s->stream, (const double *)s->d_partials,
+                     s->nlinks ? s->wall_blocks : 0, s->nelem, E, s->ext_ring_blocks, (const StepCtl *)s->d_ctl,
+                     s->d_hist, sample ? 1 : 0);
+}
+
+// One uniform step; the step number and history row come from s->d_ctl.  With `ev`, four
+// events: around the boundary phase (ev[0], ev[1]) and the sweep kernel itself (ev[2], ev[3]).
+template <int S>
+void one_step(Lattice *s, bool sample, hipEvent_t *ev = nullptr) {
   const bool pe = (s->sweeps % 2) == 0;
-  if (pe) launch_boundary<S, true>(s, sample);
-  else launch_boundary<S, false>(s, sample);
+  const bool ext = s->ff_kind == AK_FARFIELD_EXTERIOR;
+  if (ev) (void)hipEventRecord(ev[0], s->stream);
+  // the exterior farfield accumulates the wall mass exchange on every step
+  if (pe) launch_boundary<S, true>(s, sample || ext);
+  else launch_boundary<S, false>(s, sample || ext);
   if (sample) launch_loads_final(s);
-  launch_sweep<S>(s, pe, sample);
+  if (ev) (void)hipEventRecord(ev[1], s->stream);
+  launch_sweep<S>(s, pe, sample, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+  if (ext) launch_exterior<S>(s, pe, sample);
   if (sample) hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, s->d_ctl, s->d_hist);
   else hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s->stream, s->d_ctl);
   s->sweeps++;
@@ -311,6 +370,82 @@ int check_ready(Lattice *s) {
   return 0;
 }
 
+// Exterior farfield set-up on the initial (post-even) state: ring mask, reservoir links,
+// band tables and buffers; the ring at the free stream; the first step's band state, trace
+// rows and prediction; then one collision of every cell (collide_stream order).
+template <int S>
+void ext_initial_launches(Lattice *s) {
+  using T = typename St<S>::T;
+  using C = typename St<S>::C;
+  T *A = (T *)s->pop;
+  const ExtDev E = ext_dev(s);
+  const long long nr = (long long)s->ext_R2 * s->nz, nb = (long long)s->ext_B2 * s->nz;
+  hipLaunchKernelGGL((k_ext_fill_ring<S>), dim3(blocks_for(nr, 128)), dim3(128), 0, s->stream, A, s->L, E, s->scale);
+  hipLaunchKernelGGL((k_ext_band<S, true>), dim3(blocks_for(nb, 128)), dim3(128), 0, s->stream, (const T *)A, s->L, E, 0, s->ext_B2, s->inv_scale);
+  hipLaunchKernelGGL(k_ext_rows, dim3(blocks_for(2LL * s->nz * s->nx, 128)), dim3(128), 0, s->stream, E, s->d_ext_rows);
+  launch_ext_predict(s, nullptr, 0.0);
+  const long long n = (long long)s->nx * s->ny * s->nz;
+  hipLaunchKernelGGL((k_collide_inplace<S>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, A, s->d_solid, s->L,
+                     (C)s->ws, (C)s->wb, s->scale, s->inv_scale);
+}
+
+int set_exterior(Lattice *s) {
+  const int nx = s->nx, ny = s->ny, nz = s->nz;
+  std::vector<uint8_t> m((size_t)(nx + 2) * (ny + 2));
+  HIPCHK(s, hipMemcpy(m.data(), s->d_solid, m.size(), hipMemcpyDeviceToHost));
+  // band: ring cells first, then the cells one and two inside it
+  std::vector<int32_t> bxy, map((size_t)nx * ny, -1);
+  for (int pass = 0; pass < 2; ++pass)
+    for (int x = 0; x < nx; ++x)
+      for (int y = 0; y < ny; ++y) {
+        const int depth = std::min(std::min(x, nx - 1 - x), std::min(y, ny - 1 - y));
+        if (depth > 2 || (pass == 0) != (depth == 0)) continue;
+        map[(size_t)x * ny + y] = (int32_t)bxy.size();
+        bxy.push_back(x | (y << 16));
+        if (depth == 0) {
+          uint8_t &mk = m[(size_t)(y + 1) * (nx + 2) + (x + 1)];
+          if (mk == 1) return fail(s, "solid cell (%d, %d) on the exterior ring", x, y);
+          mk = 3;
+        }
+      }
+  s->ext_R2 = 2 * nx + 2 * (ny - 2);
+  s->ext_B2 = (int)bxy.size();
+  s->ext_ring_blocks = (int)blocks_for((long long)s->ext_R2 * nz, 128);
+  HIPCHK(s, hipMemcpy(s->d_solid, m.data(), m.size(), hipMemcpyHostToDevice));
+  // reservoir: velocity bounce-back of every ring link leaving the storage in x or y
+  std::vector<int32_t> xs, zs;
+  std::vector<int8_t> dir, knd;
+  for (int r = 0; r < s->ext_R2; ++r) {
+    const int x = bxy[r] & 0xffff, y = bxy[r] >> 16;
+    for (int z = 0; z < nz; ++z)
+      for (int d = 0; d < 27; ++d) {
+        const int gx = x + cxq(d), gy = y + cyq(d);
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) continue;
+        xs.push_back((x + 1) | ((y + 1) << 16));
+        zs.push_back(z);
+        dir.push_back((int8_t)d);
+        knd.push_back(0);
+      }
+  }
+  s->nleg = (int)xs.size();
+  const std::vector<double> none(xs.size(), 0.0);
+  const size_t rz = (size_t)s->ext_R2 * nz * 4, bz = (size_t)s->ext_B2 * nz * 4;
+  const std::vector<double> zb(bz, 0.0), zr(rz, 0.0), zrows((size_t)2 * nz * nx, 0.0),
+      zpart((size_t)s->ext_ring_blocks * 5, 0.0);
+  const std::vector<ExtAcc> acc(1, ExtAcc{});
+  const std::vector<int> err(1, 0);
+  if (upload(s, &s->d_legx, xs) || upload(s, &s->d_legz, zs) || upload(s, &s->d_legdir, dir) ||
+      upload(s, &s->d_legkind, knd) || upload(s, &s->d_pdf, none) || upload(s, &s->d_pdfnd, none) ||
+      upload(s, &s->d_ext_bxy, bxy) || upload(s, &s->d_ext_map, map) || upload(s, &s->d_ext_M, zb) ||
+      upload(s, &s->d_ext_Y1, zr) || upload(s, &s->d_ext_Y2, zr) || upload(s, &s->d_ext_pred, zr) ||
+      upload(s, &s->d_ext_rows, zrows) || upload(s, &s->d_ext_part, zpart) || upload(s, &s->d_ext_acc, acc) ||
+      upload(s, &s->d_ext_err, err))
+    return -1;
+  AK_DISPATCH(s, ext_initial_launches, s);
+  HIPCHK(s, hipGetLastError());
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -378,7 +513,7 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
   set_geom(s, 64, 4);
   s->geom_small = s->geom;
   s->hist_cap = history_capacity > 0 ? history_capacity : 4096;
-  s->hist_width = AK_HIST_FIXED + 4 * MAX_ELEM;
+  s->hist_width = AK_HIST_WIDTH;
   if (dalloc(s, &s->d_solid, (size_t)(nx + 2) * (ny + 2)) || dalloc(s, &s->d_diag, 1) ||
       dalloc(s, &s->d_hist, (size_t)s->hist_cap * s->hist_width) || dalloc(s, &s->d_ctl, 1)) {
     for (void *p : s->owned) (void)hipFree(p);
@@ -491,9 +626,11 @@ int ak_set_farfield(void *handle, int kind, double U) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
   s->gen++;  // invalidates captured step graphs
-  if (kind < 0 || kind > 3) return fail(s, "unknown farfield kind %d", kind);
-  if (kind == AK_FARFIELD_CHARACTERISTIC && s->sweeps % 2 != 0)
-    return fail(s, "set the characteristic farfield on the initial state");
+  if (kind < 0 || kind > 4) return fail(s, "unknown farfield kind %d", kind);
+  if ((kind == AK_FARFIELD_CHARACTERISTIC || kind == AK_FARFIELD_EXTERIOR) && s->sweeps % 2 != 0)
+    return fail(s, "set the characteristic or exterior farfield on the initial state");
+  if (kind == AK_FARFIELD_EXTERIOR && (s->nx < 6 || s->ny < 6 || s->nz < 4))
+    return fail(s, "the exterior farfield needs a padded lattice of at least 6 x 6 x 4");
   s->ff_kind = kind;
   s->U = U;
   const int nx = s->nx, ny = s->ny, nz = s->nz;
@@ -610,6 +747,8 @@ int ak_set_farfield(void *handle, int kind, double U) {
     if (upload(s, &s->d_legx, xs) || upload(s, &s->d_legz, zs) || upload(s, &s->d_legdir, dir) ||
         upload(s, &s->d_legkind, knd) || upload(s, &s->d_pdf, pdf) || upload(s, &s->d_pdfnd, pdfnd))
       return -1;
+  } else if (kind == AK_FARFIELD_EXTERIOR) {
+    if (set_exterior(s)) return -1;
   }
   return 0;
 }
@@ -677,27 +816,12 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
       if (flush_history(s)) return -1;
       set_ctl(s, number, 0);
     }
-    const bool pe = (s->sweeps % 2) == 0;
-    HIPCHK(s, hipEventRecord(ev[4 * k], s->stream));
     switch (s->storage) {
-      case 0: if (pe) launch_boundary<0, true>(s, sample); else launch_boundary<0, false>(s, sample); break;
-      case 1: if (pe) launch_boundary<1, true>(s, sample); else launch_boundary<1, false>(s, sample); break;
-      default: if (pe) launch_boundary<2, true>(s, sample); else launch_boundary<2, false>(s, sample); break;
+      case 0: one_step<0>(s, sample, &ev[4 * k]); break;
+      case 1: one_step<1>(s, sample, &ev[4 * k]); break;
+      default: one_step<2>(s, sample, &ev[4 * k]); break;
     }
-    if (sample) launch_loads_final(s);
-    HIPCHK(s, hipEventRecord(ev[4 * k + 1], s->stream));
-    switch (s->storage) {
-      case 0: launch_sweep<0>(s, pe, sample, ev[4 * k + 2], ev[4 * k + 3]); break;
-      case 1: launch_sweep<1>(s, pe, sample, ev[4 * k + 2], ev[4 * k + 3]); break;
-      default: launch_sweep<2>(s, pe, sample, ev[4 * k + 2], ev[4 * k + 3]); break;
-    }
-    if (sample) {
-      hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, s->d_ctl, s->d_hist);
-      s->hist_rows++;
-    } else {
-      hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s->stream, s->d_ctl);
-    }
-    s->sweeps++;
+    if (sample) s->hist_rows++;
   }
   HIPCHK(s, hipStreamSynchronize(s->stream));
   double sweep = 0.0, bnd = 0.0;
@@ -873,7 +997,8 @@ namespace {
 
 // Restartable state of a lattice: a fixed header, the raw population buffer (every storage
 // precision bit-exact), then the boundary histories: characteristic farfield incoming and
-// derivative values, clocks and glancing trace rows; legacy outflow pdf and pdf_nd.
+// derivative values, clocks and glancing trace rows; legacy outflow pdf and pdf_nd; the
+// exterior ring's prediction, band state, trace rows and accumulators.
 struct StateHeader {
   char magic[8];
   int version, storage, nx, ny, nz, ff_kind, nff, nleg, has_prev;
@@ -898,6 +1023,14 @@ std::vector<StatePart> state_parts(Lattice *s, bool with_prev) {
   if (s->ff_kind == AK_FARFIELD_LEGACY && s->nleg) {
     p.push_back({s->d_pdf, (size_t)s->nleg * sizeof(double)});
     p.push_back({s->d_pdfnd, (size_t)s->nleg * sizeof(double)});
+  }
+  if (s->ff_kind == AK_FARFIELD_EXTERIOR && s->ext_R2) {
+    // predicted ring state, band state, initial trace rows, accumulators, error flag
+    p.push_back({s->d_ext_pred, (size_t)s->ext_R2 * s->nz * 4 * sizeof(double)});
+    p.push_back({s->d_ext_M, (size_t)s->ext_B2 * s->nz * 4 * sizeof(double)});
+    p.push_back({s->d_ext_rows, (size_t)2 * s->nz * s->nx * sizeof(double)});
+    p.push_back({s->d_ext_acc, sizeof(ExtAcc)});
+    p.push_back({s->d_ext_err, sizeof(int)});
   }
   if (with_prev) p.push_back({s->d_prev, (size_t)4 * s->nx * s->ny * s->nz * sizeof(double)});
   return p;
@@ -1021,6 +1154,23 @@ int ak_residual(void *handle, int incoming, double lattice_speed, double dt, int
   HIPCHK(s, hipStreamSynchronize(s->stream));
   HIPCHK(s, hipMemcpy(out, s->d_rout, 9 * sizeof(double), hipMemcpyDeviceToHost));
   s->prev_valid = true;
+  return 0;
+}
+
+int ak_exterior_state(void *handle, double *out) {
+  Lattice *s = H(handle);
+  if (check_ready(s)) return -1;
+  if (s->ff_kind != AK_FARFIELD_EXTERIOR) return fail(s, "the lattice has no exterior farfield");
+  ExtAcc a{};
+  int err = 0;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(&a, s->d_ext_acc, sizeof a, hipMemcpyDeviceToHost));
+  HIPCHK(s, hipMemcpy(&err, s->d_ext_err, sizeof err, hipMemcpyDeviceToHost));
+  out[0] = a.wall_mass; out[1] = a.wall_total;
+  for (int c = 0; c < 4; ++c) { out[2 + c] = a.src[c]; out[6 + c] = a.src_total[c]; }
+  out[10] = a.moment_err;
+  out[11] = (double)a.steps;
+  out[12] = err;
   return 0;
 }
 

@@ -1,5 +1,6 @@
 // ak_kernels.hpp -- gfx950 kernels of one lattice step (sweep, walls, farfield, reductions).
 #pragma once
+#include "../../include/aerokinetic.h"
 #include "ak_device.hpp"
 
 namespace ak {
@@ -79,7 +80,8 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
   const int z = tz;
   bool active = x < L.nx && y < L.ny;
   uint8_t mk = 1;
-  if (active) { mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]; active = mk != 1; }
+  // mask: 0 fluid, 1 solid, 2 fluid outside the diagnostics, 3 exterior ring (its own kernel)
+  if (active) { mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]; active = mk != 1 && mk != 3; }
   C drho = 0, u2 = 0;
   if (active) {
     C g[NQ];
@@ -365,7 +367,7 @@ __device__ __forceinline__ void loads_final_body(const double *__restrict__ part
         v[c] = s;
         tot[c] += s;
       }
-      for (int c = 0; c < 4; ++c) row[9 + e * 4 + c] = v[c];
+      for (int c = 0; c < 4; ++c) row[AK_HIST_FIXED + e * 4 + c] = v[c];
     }
     row[0] = (double)ctl->step;
     row[1] = tot[1] * fscale;
@@ -697,7 +699,9 @@ __global__ void k_init_seeded(typename St<S>::T *__restrict__ A, const uint8_t *
   double ux = p.U;
   double uy = p.amp * sin(p.ky * ry + p.phy) * cos(p.kz * zz + p.phz) * env;
   double uz = -p.amp * p.ky / p.kz * cos(p.ky * ry + p.phy) * sin(p.kz * zz + p.phz) * env;
-  if (solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]) ux = uy = uz = 0.0;
+  const uint8_t mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)];
+  if (mk == 3) { ux = p.U; uy = uz = 0.0; }  // exterior ring: free stream
+  else if (mk) ux = uy = uz = 0.0;
   const long long c = L.base + cell_of(L, x, y, z);
 #pragma unroll
   for (int i = 0; i < NQ; ++i) {
@@ -746,7 +750,8 @@ __global__ void k_import(typename St<S>::T *__restrict__ A, Desc L, const double
 }
 
 // Read-only state diagnostics: per-block [sum rho, max |u|^2, max |rho-1|, min rho, bad]
-// over fluid cells; the host folds the blocks in a fixed order.
+// over fluid cells (the guards also over the exterior ring, the density sum over the
+// physical fluid only); the host folds the blocks in a fixed order.
 template <int S, bool PE, bool PXY>
 __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__restrict__ A,
                                                     const uint8_t *__restrict__ solid, Desc L,
@@ -758,7 +763,8 @@ __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__r
   int bad = 0;
   if (t < n) {
     const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
-    if (solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)] == 0) {
+    const uint8_t mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)];
+    if (mk == 0 || mk == 3) {
       const long long c = L.base + cell_of(L, x, y, z);
       C r = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
@@ -776,7 +782,7 @@ __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__r
       const double ux = (double)jx / rho, uy = (double)jy / rho, uz = (double)jz / rho;
       const double u2 = ux * ux + uy * uy + uz * uz;
       if (!(isfinite(rho) && isfinite(u2))) bad = 1;
-      else { sr = rho; vu = u2; vd = fabs((double)r); vr = rho; }
+      else { sr = mk == 0 ? rho : 0.0; vu = u2; vd = fabs((double)r); vr = rho; }
     }
   }
 #pragma unroll

@@ -7,8 +7,13 @@ Public surface of the reference (`src/aerokinetic/solver.py:27-537`):
   `diagnostics()`, `run(...)`, `save_frame(...)` and the `NumericalFailure`
   error (solver.py:23-24, 252-537);
 * outputs `history.csv` (step,t_star,time_s,cl,cd,cm,local_mach_max,
-  density_error_max), `metadata.json`, `summary.json`, `project.json` and
-  mid-span `frames/*.npz`.
+  density_error_max; with the exterior farfield also the wall mass exchange and
+  the exterior source of mass, per step and accumulated), `metadata.json`,
+  `summary.json`, `project.json` and mid-span `frames/*.npz`.
+
+The `exterior_regularized_fd_v1` farfield (solver.py:36-41, 81-90, 252-279,
+298-310) runs on a lattice padded by one cell in x and y (the exterior ring);
+loads, frames, residuals and the density mean cover the physical box only.
 
 All numerics run in the native HIP library: one AA-pattern lattice in HBM,
 boundary kernels before each sweep, loads and low-Mach guards reduced on the
@@ -58,15 +63,16 @@ def seed_parameters(section, flow, plan):
             "lex": float(le[0]), "ley": float(le[1])}
 
 
+EXTERIOR_COLUMNS = ["wall_mass_exchange_lattice", "wall_mass_exchange_accumulated",
+                    "exterior_source_mass_lattice", "exterior_source_mass_accumulated"]
+
+
 class Solver:
     HISTORY_COLUMNS = ["step", "t_star", "time_s", "cl", "cd", "cm", "local_mach_max", "density_error_max"]
 
     def __init__(self, section: Section, flow: Flow = Flow(), grid: Grid = Grid(), progress=None, *,
                  storage="fp64", device=0, plan: Plan | None = None, fp16_scale=0.0, tile=None, graphs=True):
         self.section, self.flow, self.grid = section, flow, grid
-        if grid.farfield == EXTERIOR_FARFIELD:
-            raise ValueError("exterior_regularized_fd_v1 is not provided by this package; "
-                             "use the characteristic or legacy farfield")
         if storage not in _lib.STORE:
             raise ValueError("storage must be fp64, fp32 or fp16")
         self.plan = plan or Plan.create(section, flow, grid)
@@ -77,29 +83,40 @@ class Solver:
                               f"limit {grid.memory_limit_gib:.2f} GiB. Inspect Plan before choosing a larger limit.")
         self._progress = progress or (lambda message: None)
         nx, ny, nz = self.plan.shape
-        lab = labels(self.contours, (nx, ny), self.plan.origin, self.plan.dx)
-        self._solid_xy = lab[1:-1, 1:-1] > 0
+        # the exterior farfield pads the physical box by one cell in x and y (solver.py:36-41)
+        self._exterior = grid.farfield == EXTERIOR_FARFIELD
+        pad = 1 if self._exterior else 0
+        self.storage_shape = (nx + 2 * pad, ny + 2 * pad, nz)
+        self.storage_origin = (self.plan.origin[0] - pad * self.plan.dx, self.plan.origin[1] - pad * self.plan.dx)
+        self._core = (slice(pad, pad + nx), slice(pad, pad + ny))
+        if self._exterior:
+            self.HISTORY_COLUMNS = Solver.HISTORY_COLUMNS + EXTERIOR_COLUMNS
+        X, Y = self.storage_shape[:2]
+        lab = labels(self.contours, (X, Y), self.storage_origin, self.plan.dx)
+        storage_solid = lab[1:-1, 1:-1] > 0
         for k, e in enumerate(self.contours):
             if not (lab == k + 1).any():
                 raise ValueError(f"Element {e.name} has no solid cells; increase resolution")
+        self._solid_xy = storage_solid[self._core]
         self._fluid = np.broadcast_to(~self._solid_xy[:, :, None], self.plan.shape)
         self._progress("Building the D3Q27 cumulant lattice on the GPU")
-        self.lattice = Lattice(nx, ny, nz, storage=storage, omega_shear=self.plan.relaxation_rate,
+        self.lattice = Lattice(X, Y, nz, storage=storage, omega_shear=self.plan.relaxation_rate,
                                omega_bulk=1.0, fp16_scale=fp16_scale, device=device)
         if tile:
             self.lattice.set_tile(*tile)
         self.lattice.set_graphs(graphs)
-        self.lattice.set_solid(self._solid_xy)
-        self.links, fallback = wall_links(self.contours, (nx, ny), self.plan.origin, self.plan.dx)
+        self.lattice.set_solid(storage_solid)
+        self.links, fallback = wall_links(self.contours, (X, Y), self.storage_origin, self.plan.dx)
         self.wall_fallback_links = fallback * nz
-        self.lattice.set_walls(self.links, len(self.contours), self.plan.origin[0] / self.plan.dx,
-                               self.plan.origin[1] / self.plan.dx, grid.cells_per_chord, flow.lattice_speed)
+        self.lattice.set_walls(self.links, len(self.contours), self.storage_origin[0] / self.plan.dx,
+                               self.storage_origin[1] / self.plan.dx, grid.cells_per_chord, flow.lattice_speed)
         self._initialize()
         self.last_loads = None
 
     # ------------------------------------------------------------------ state
     def _initialize(self):
-        self.lattice.init_seeded(seed_parameters(self.section, self.flow, self.plan), self.plan.origin, self.plan.dx)
+        self.lattice.init_seeded(seed_parameters(self.section, self.flow, self.plan), self.storage_origin,
+                                 self.plan.dx)
         self.lattice.set_farfield(self.grid.farfield, self.flow.lattice_speed)
         self.steps = 0
 
@@ -112,7 +129,37 @@ class Solver:
                "density_min": float(row[6]), "nonfinite": float(row[7])}
         if self.plan.dt_seconds is not None:
             rec["time_s"] = (int(row[0]) - 0.5) * self.plan.dt_seconds
+        if self._exterior:
+            e = _lib.HIST_EXT
+            rec.update(wall_mass_exchange_lattice=float(row[8]), wall_mass_exchange_accumulated=float(row[e]),
+                       exterior_source_mass_lattice=float(row[e + 1]),
+                       exterior_source_mass_accumulated=float(row[e + 2]))
         return rec
+
+    def _exterior_state(self):
+        out = np.zeros(13)
+        h = self.lattice.h
+        _lib.check(_lib.load().ak_exterior_state(h, _lib.dptr(out)), h, "ak_exterior_state")
+        return out
+
+    def boundary_diagnostics(self):
+        """Exterior-farfield bookkeeping in lattice mass per step (solver.py:298-310); None
+        for the other farfields."""
+        if not self._exterior:
+            return None
+        st = self._exterior_state()
+        last = None
+        if st[11] > 0:
+            last = {"step": int(st[11]), "reconstruction_moment_error": float(st[10]),
+                    "exterior_source_mass_momentum": [float(v) for v in st[2:6]]}
+        return {"model": EXTERIOR_FARFIELD, "pdf_stage": "post_collision",
+                "physical_shape": list(self.plan.shape), "storage_shape": list(self.storage_shape),
+                "storage_origin": list(self.storage_origin),
+                "wall_mass_exchange_lattice": float(st[0]), "wall_mass_exchange_accumulated": float(st[1]),
+                "exterior_source_mass_momentum_accumulated": [float(v) for v in st[6:10]],
+                "last_exterior_step": last,
+                "definition": "Wall incoming-minus-outgoing mass and explicit auxiliary PDF source; a closed "
+                              "bookkeeping budget does not prove zero wall mass flux or accuracy."}
 
     def advance(self, nsteps, sample_every=0):
         """Advance nsteps; returns the records of the sampled steps (device-batched)."""
@@ -120,6 +167,8 @@ class Solver:
             return []
         self.lattice.run(nsteps, self.steps, sample_every)
         self.steps += nsteps
+        if self._exterior and self._exterior_state()[12]:
+            raise NumericalFailure("Non-finite exterior macro predictor")
         if sample_every <= 0:
             return []
         return [self._record(r) for r in self.lattice.read_history()]
@@ -131,8 +180,9 @@ class Solver:
             return None
         rec = recs[-1]
         loads = {k: rec[k] for k in ("cl", "cd", "cm", "element_lattice_loads", "step", "t_star")}
-        if "time_s" in rec:
-            loads["time_s"] = rec["time_s"]
+        for k in ["time_s"] + EXTERIOR_COLUMNS:
+            if k in rec:
+                loads[k] = rec[k]
         self.last_loads = loads
         self._last_diag = rec
         return loads
@@ -171,6 +221,28 @@ class Solver:
                                            _lib.dptr(out)), h, "ak_residual")
         return (np.zeros(4), np.zeros(4)) if reset else combine_levels([(out, self.plan.dx ** 3)])
 
+    @staticmethod
+    def _failed_safe(report, failed):
+        """The boundary report of a run; after a failure non-finite numbers become null
+        (and are flagged) so that the failed run's JSON stays valid (solver.py:519-530)."""
+        try:
+            value = report()
+        except Exception:
+            if not failed:
+                raise
+            return None
+        if not failed:
+            return value
+        replaced = []
+
+        def invalid(constant):
+            replaced.append(constant)
+            return None
+        value = json.loads(json.dumps(value, allow_nan=True), parse_constant=invalid)
+        if replaced:
+            value["nonfinite_diagnostics_replaced_with_null"] = True
+        return value
+
     def save_checkpoint(self, path, context=None):
         """Restartable device state + run context (checkpoint.py:32-70)."""
         from .checkpoint import save_checkpoint
@@ -181,7 +253,7 @@ class Solver:
         nx, ny, nz = self.plan.shape
         stride = max(1, math.ceil(max(nx, ny) / max_pixels))
         rho, vel = self.lattice.plane(nz // 2)
-        rho, vel = rho[::stride, ::stride], vel[::stride, ::stride]
+        rho, vel = rho[self._core][::stride, ::stride], vel[self._core][::stride, ::stride]
         U = self.flow.lattice_speed
         speed = np.linalg.norm(vel, axis=-1) / U
         h = stride * self.plan.dx
@@ -274,6 +346,8 @@ class Solver:
         control = RunControl(self, settings, output, resume=resume, callback=residual_callback)
         metadata["residual_definition"] = ("Volume-weighted L2/Linf of d(rho,j_common)/dt*, owned fluid only; "
                                            "stationary-flow residual, not a turbulence-accuracy measure")
+        if self._exterior:
+            metadata["boundary_model"] = self.boundary_diagnostics()
         write_json(output / "metadata.json", metadata)
         state, reason = "completed", "duration_reached"
         start = time.monotonic()
@@ -354,6 +428,9 @@ class Solver:
                        "experimental_validation": "not_performed"}
             summary.update(control.summary(reason))
             control.close()
+            if self._exterior:
+                boundary = self._failed_safe(self.boundary_diagnostics, state == "failed")
+                summary["boundary_diagnostics"] = metadata["boundary_model"] = boundary
             metadata["termination_reason"] = reason
             metadata["status"] = state
             write_json(output / "metadata.json", metadata)

@@ -48,12 +48,17 @@ extern "C" {
 enum { AK_STORE_F64 = 0, AK_STORE_F32 = 1, AK_STORE_F16 = 2 };
 /* outer boundary */
 enum { AK_FARFIELD_PERIODIC = 0, AK_FARFIELD_LEGACY = 1, AK_FARFIELD_CHARACTERISTIC = 2,
-       AK_FARFIELD_NONE = 3 /* x/y ghosts left to the caller (refined fine level) */ };
+       AK_FARFIELD_NONE = 3 /* x/y ghosts left to the caller (refined fine level) */,
+       AK_FARFIELD_EXTERIOR = 4 /* exterior_regularized_fd_v1: the lattice is the physical box
+                                   padded by one cell in x and y (src/aerokinetic/exterior.py:18-200) */ };
 /* one history row per sampled step (doubles) */
 enum {
   AK_HIST_STEP = 0, AK_HIST_CL = 1, AK_HIST_CD = 2, AK_HIST_CM = 3,
   AK_HIST_MACH = 4, AK_HIST_DENSITY_ERR = 5, AK_HIST_RHO_MIN = 6, AK_HIST_NONFINITE = 7,
-  AK_HIST_MASS_EXCHANGE = 8, AK_HIST_FIXED = 9 /* then nelem*4 per-element loads Fx,Fy,Fz,Mz */
+  AK_HIST_MASS_EXCHANGE = 8, AK_HIST_FIXED = 9 /* then nelem*4 per-element loads Fx,Fy,Fz,Mz */,
+  /* exterior farfield only, after room for 8 elements' loads (solver.py:443-445, 265-271) */
+  AK_HIST_EXT_WALL_MASS_TOTAL = 41, AK_HIST_EXT_SOURCE = 42, AK_HIST_EXT_SOURCE_TOTAL = 43,
+  AK_HIST_WIDTH = 44
 };
 
 /* Create a lattice of nx*ny*nz fluid cells (z periodic) on `device`.
@@ -83,7 +88,14 @@ int ak_set_walls(void *handle, int n, const int32_t *x, const int32_t *y, const 
                  const double *q, const int32_t *element, int nelem, double rx_off, double ry_off,
                  double chord_cells, double lattice_speed);
 
-/* Outer boundary; reads the current populations, so call after the initial state. */
+/* Outer boundary; reads the current populations, so call after the initial state.
+ * AK_FARFIELD_EXTERIOR (ExteriorBoundary, exterior.py:18-50; solver.py:81-90): the lattice is
+ * the physical box padded by one cell in x and y (at least 6 x 6 x 4).  The outermost x/y cells
+ * are the exterior ring (mask value 3, no solid allowed there): they are set to the free
+ * stream, the first step's ring prediction is taken from the initial state, and every cell is
+ * collided once (the reference's collide -> boundary -> stream order); the storage edge's
+ * ghosts are a velocity bounce-back reservoir.  History rows then also carry
+ * AK_HIST_EXT_* columns and ak_exterior_state reports the accumulators. */
 int ak_set_farfield(void *handle, int kind, double lattice_speed);
 
 /* Seeded initial field (u_x = U plus the spanwise-periodic Gaussian-envelope
@@ -133,8 +145,9 @@ int ak_get_plane(void *handle, int z, double *rho, double *u);
 /* Restartable device state (checkpoint.py:32-135, refined_checkpoint.py:54-142): the raw
  * population buffer (bit-exact in every storage precision), the AA parity and the boundary
  * histories (characteristic incoming/derivative/clock/glancing trace; legacy outflow
- * pdf/pdf_nd).  ak_load_state accepts only a state of a lattice with the same storage,
- * shape and boundary set. */
+ * pdf/pdf_nd; exterior ring prediction, band state, trace rows and accumulators).
+ * ak_load_state accepts only a state of a lattice with the same storage, shape and
+ * boundary set. */
 size_t ak_state_bytes(void *handle);
 int ak_save_state(void *handle, void *host, size_t bytes);
 int ak_load_state(void *handle, const void *host, size_t bytes);
@@ -146,6 +159,13 @@ int ak_load_state(void *handle, const void *host, size_t bytes);
  * reset = 1 (or no snapshot yet) only records q_prev (out = zeros).  The snapshot is part of
  * the restartable state once taken. */
 int ak_residual(void *handle, int incoming, double lattice_speed, double dt, int reset, double *out);
+
+/* Exterior farfield bookkeeping (ExteriorBoundary.step report and source_total,
+ * exterior.py:187-200; Solver._wall_mass_exchange/_wall_mass_total, solver.py:262-271):
+ * out[13] = {wall mass exchange of the last step, accumulated; exterior source of the last
+ * step (mass, momentum x/y/z), accumulated [4]; largest rebuild mass/momentum mismatch of the
+ * last step; exterior steps taken; 1 if a ring prediction was non-finite}. */
+int ak_exterior_state(void *handle, double *out);
 
 /* Wait for all work of this lattice. */
 int ak_synchronize(void *handle);
