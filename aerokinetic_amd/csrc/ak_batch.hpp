@@ -23,6 +23,8 @@ struct CaseDev {
   FreeState F;
   const FluxTable *T;
   double *before, *after, *regc, *regf, *part, *px, *py, *mf;
+  const int32_t *lpx, *lpy, *lpz;  // prolongation work items
+  int npx, npy, npz;
   int *err;
   double volc, volf, inv_volc, loads_scale;
   WallLinks W;  // fine-level walls (lever arm dx_f)
@@ -63,7 +65,7 @@ __device__ __forceinline__ int batch_owner(const int *__restrict__ first, int nm
 #define AK_PE_FINE(delta) ((((c.fpar + (delta)) & 1)) == 0)
 
 template <int S>
-__global__ __launch_bounds__(128) void kb_donor(const CaseDev *__restrict__ C, const int *__restrict__ first, int nm,
+__global__ __launch_bounds__(256) void kb_donor(const CaseDev *__restrict__ C, const int *__restrict__ first, int nm,
                                                 int lead, int delta, int after) {
   using T = typename St<S>::T;
   AK_BATCH_MEMBER
@@ -72,34 +74,16 @@ __global__ __launch_bounds__(128) void kb_donor(const CaseDev *__restrict__ C, c
   else donor_moments_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, out, c.err, t);
 }
 
-template <int S>
-__global__ __launch_bounds__(256) void kb_flux_coarse(const CaseDev *__restrict__ C, const int *__restrict__ first,
-                                                      int nm, int lead, int delta) {
-  using T = typename St<S>::T;
-  AK_BATCH_MEMBER
-  if (AK_PE_COARSE(delta)) flux_coarse_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.T, c.volc, c.c_inv, c.regc, t);
-  else flux_coarse_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.T, c.volc, c.c_inv, c.regc, t);
-}
-
-template <int S>
-__global__ __launch_bounds__(128) void kb_legacy(const CaseDev *__restrict__ C, const int *__restrict__ first, int nm,
-                                                 int lead, int delta) {
-  using T = typename St<S>::T;
-  AK_BATCH_MEMBER
-  if (AK_PE_COARSE(delta)) legacy_body<S, true>((T *)c.Ac, c.Lc, c.K, c.U, c.c_scale, c.c_inv, t);
-  else legacy_body<S, false>((T *)c.Ac, c.Lc, c.K, c.U, c.c_scale, c.c_inv, t);
-}
-
 __global__ __launch_bounds__(256) void kb_prolong_x(const CaseDev *__restrict__ C, const int *__restrict__ first,
                                                     int nm, double frac) {
   AK_BATCH_MEMBER
-  prolong_x_body(c.before, c.after, frac, c.B, c.px, t);
+  if (t < c.npx) prolong_x_body(c.before, c.after, frac, c.B, c.px, c.lpx[t]);
 }
 
 __global__ __launch_bounds__(256) void kb_prolong_y(const CaseDev *__restrict__ C, const int *__restrict__ first,
                                                     int nm) {
   AK_BATCH_MEMBER
-  prolong_y_body(c.px, c.B, c.py, t);
+  if (t < c.npy) prolong_y_body(c.px, c.B, c.py, c.lpy[t]);
 }
 
 template <int S>
@@ -107,27 +91,9 @@ __global__ __launch_bounds__(256) void kb_prolong_z(const CaseDev *__restrict__ 
                                                     int nm, int delta) {
   using T = typename St<S>::T;
   AK_BATCH_MEMBER
-  if (AK_PE_FINE(delta)) prolong_z_inject_body<S, true>((T *)c.Af, c.Lf, c.B, c.V, c.py, c.f_scale, c.err, t);
-  else prolong_z_inject_body<S, false>((T *)c.Af, c.Lf, c.B, c.V, c.py, c.f_scale, c.err, t);
-}
-
-template <int S>
-__global__ __launch_bounds__(256) void kb_flux_fine(const CaseDev *__restrict__ C, const int *__restrict__ first,
-                                                    int nm, int delta, int sub) {
-  using T = typename St<S>::T;
-  AK_BATCH_MEMBER
-  if (AK_PE_FINE(delta)) flux_fine_body<S, true>((const T *)c.Af, c.Lf, c.B, c.T, c.volf, c.f_inv, sub, c.regf, t);
-  else flux_fine_body<S, false>((const T *)c.Af, c.Lf, c.B, c.T, c.volf, c.f_inv, sub, c.regf, t);
-}
-
-template <int S>
-__global__ __launch_bounds__(256) void kb_walls(const CaseDev *__restrict__ C, const int *__restrict__ first, int nm,
-                                                int delta, int sub) {
-  using T = typename St<S>::T;
-  AK_BATCH_MEMBER
-  double *part = c.part + (size_t)sub * c.wall_blocks * c.W.nelem * 5;
-  if (AK_PE_FINE(delta)) walls_body<S, true, true>((T *)c.Af, c.Lf, c.W, c.f_scale, c.f_inv, part, t, lb_);
-  else walls_body<S, false, true>((T *)c.Af, c.Lf, c.W, c.f_scale, c.f_inv, part, t, lb_);
+  if (t >= c.npz) return;
+  if (AK_PE_FINE(delta)) prolong_z_inject_body<S, true>((T *)c.Af, c.Lf, c.B, c.V, c.py, c.f_scale, c.err, c.lpz[t]);
+  else prolong_z_inject_body<S, false>((T *)c.Af, c.Lf, c.B, c.V, c.py, c.f_scale, c.err, c.lpz[t]);
 }
 
 template <int S>
@@ -168,6 +134,57 @@ __global__ __launch_bounds__(64) void kb_loads(const CaseDev *__restrict__ C, in
   else if (threadIdx.x == 0)
     c.ctl->step += 1;
 }
+
+// Fused phases (as k_after_coarse / k_after_fine of a single case): the block ranges of
+// several phases, each over all members, in one launch; the member of a block comes from
+// that phase's table.
+#define AK_BATCH_MEMBER_IN(tbl, blk)                                             \
+  const int m_ = batch_owner(tbl, nm, blk);                                      \
+  const CaseDev &c = C[m_];                                                      \
+  const int lb_ = (blk) - (tbl)[m_];                                             \
+  const long long t = (long long)lb_ * blockDim.x + threadIdx.x;                 \
+  (void)lb_;
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_after_coarse(const CaseDev *__restrict__ C, const int *__restrict__ f_flux,
+                                                       const int *__restrict__ f_leg, const int *__restrict__ f_don,
+                                                       int nm, int lead, int nb_flux, int nb_leg) {
+  using T = typename St<S>::T;
+  const int delta = 1;
+  const int b = (int)blockIdx.x;
+  if (b < nb_flux) {
+    AK_BATCH_MEMBER_IN(f_flux, b)
+    if (AK_PE_COARSE(delta)) flux_coarse_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.T, c.volc, c.c_inv, c.regc, t);
+    else flux_coarse_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.T, c.volc, c.c_inv, c.regc, t);
+  } else if (b < nb_flux + nb_leg) {
+    AK_BATCH_MEMBER_IN(f_leg, b - nb_flux)
+    if (AK_PE_COARSE(delta)) legacy_body<S, true>((T *)c.Ac, c.Lc, c.K, c.U, c.c_scale, c.c_inv, t);
+    else legacy_body<S, false>((T *)c.Ac, c.Lc, c.K, c.U, c.c_scale, c.c_inv, t);
+  } else {
+    AK_BATCH_MEMBER_IN(f_don, b - nb_flux - nb_leg)
+    if (AK_PE_COARSE(delta)) donor_moments_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.after, c.err, t);
+    else donor_moments_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.after, c.err, t);
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void kb_after_fine(const CaseDev *__restrict__ C, const int *__restrict__ f_flux,
+                                                     const int *__restrict__ f_walls, int nm, int delta, int sub,
+                                                     int nb_flux) {
+  using T = typename St<S>::T;
+  const int b = (int)blockIdx.x;
+  if (b < nb_flux) {
+    AK_BATCH_MEMBER_IN(f_flux, b)
+    if (AK_PE_FINE(delta)) flux_fine_body<S, true>((const T *)c.Af, c.Lf, c.B, c.T, c.volf, c.f_inv, sub, c.regf, t);
+    else flux_fine_body<S, false>((const T *)c.Af, c.Lf, c.B, c.T, c.volf, c.f_inv, sub, c.regf, t);
+  } else {
+    AK_BATCH_MEMBER_IN(f_walls, b - nb_flux)
+    double *part = c.part + (size_t)sub * c.wall_blocks * c.W.nelem * 5;
+    if (AK_PE_FINE(delta)) walls_body<S, true, true>((T *)c.Af, c.Lf, c.W, c.f_scale, c.f_inv, part, t, lb_);
+    else walls_body<S, false, true>((T *)c.Af, c.Lf, c.W, c.f_scale, c.f_inv, part, t, lb_);
+  }
+}
+#undef AK_BATCH_MEMBER_IN
 
 #undef AK_BATCH_MEMBER
 #undef AK_PE_COARSE

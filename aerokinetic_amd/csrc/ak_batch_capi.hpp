@@ -138,6 +138,8 @@ int build_descriptors(Batch *b) {
     d.B = r->B; d.V = r->V; d.F = r->F; d.T = r->d_T;
     d.before = r->d_before; d.after = r->d_after; d.regc = r->d_regc; d.regf = r->d_regf; d.part = r->d_part;
     d.px = r->d_px; d.py = r->d_py; d.mf = r->d_mf; d.err = r->d_err;
+    d.lpx = r->d_lpx; d.lpy = r->d_lpy; d.lpz = r->d_lpz;
+    d.npx = r->n_lpx; d.npy = r->n_lpy; d.npz = r->n_lpz;
     d.volc = r->dxc * r->dxc * r->dxc;
     d.volf = r->dxf * r->dxf * r->dxf;
     d.inv_volc = 1.0 / d.volc;
@@ -152,18 +154,18 @@ int build_descriptors(Batch *b) {
     d.fpar = (int)(f->sweeps % 2);
     const long long nb0 = r->B.nb[0], nb1 = r->B.nb[1], nb2 = r->B.nb[2];
     const long long items[BP_COUNT] = {
-        (nb0 + 6) * (nb1 + 6) * (nb2 + 6),                 // donor moments (128)
+        (nb0 + 6) * (nb1 + 6) * (nb2 + 6),                 // donor moments (256)
         4LL * r->nt * c->nz * FLUX_LANES,                  // coarse face flux (256)
-        (long long)c->nleg,                                // legacy farfield (128)
-        (2 * nb0 + 4) * (nb1 + 6) * (nb2 + 6),             // prolongation x (256)
-        (2 * nb0 + 4) * (2 * nb1 + 4) * (nb2 + 6),         // prolongation y (256)
-        (long long)f->nx * f->ny * f->nz,                  // prolongation z + injection (256)
+        (long long)c->nleg,                                // legacy farfield (256)
+        (long long)r->n_lpx,                               // prolongation x work items (64)
+        (long long)r->n_lpy,                               // prolongation y work items (64)
+        (long long)r->n_lpz,                               // prolongation z + injection items (64)
         4LL * (2 * r->nt) * f->nz * FLUX_LANES,            // fine face flux (256)
         (long long)d.wall_blocks * 256,                    // walls (256)
         4LL * r->nt * c->nz * 64,                          // reflux, a wave per cell (256)
         8 * nb0 * nb1 * nb2,                               // fine moments (256)
         nb0 * nb1 * nb2};                                  // restriction (128)
-    const int threads[BP_COUNT] = {128, 256, 128, 256, 256, 256, 256, 256, 256, 256, 128};
+    const int threads[BP_COUNT] = {256, 256, 256, 64, 64, 64, 256, 256, 256, 256, 128};
     for (int k = 0; k < BP_COUNT; ++k)
       first[(size_t)k * (b->cap + 1) + i + 1] = first[(size_t)k * (b->cap + 1) + i] + (int)blocks_for(items[k], threads[k]);
   }
@@ -189,20 +191,26 @@ void batch_step(Batch *b, bool sample) {
   const CaseDev *C = b->d_cases;
   auto F = [b](int k) { return b->d_first + (size_t)k * (b->cap + 1); };
   auto go = [&](int k) { return b->nblk[k] > 0; };
-  if (go(BP_DONOR)) hipLaunchKernelGGL((kb_donor<S>), dim3(b->nblk[BP_DONOR]), dim3(128), 0, st, C, F(BP_DONOR), nm, lead, 0, 0);
+  if (go(BP_DONOR)) hipLaunchKernelGGL((kb_donor<S>), dim3(b->nblk[BP_DONOR]), dim3(256), 0, st, C, F(BP_DONOR), nm, lead, 0, 0);
   batched_sweep<S>(b, b->d_desc + (size_t)lead * nc, b->nb_coarse);
   for (Refined *r : b->members) r->c->sweeps++;
-  if (go(BP_FLUXC)) hipLaunchKernelGGL((kb_flux_coarse<S>), dim3(b->nblk[BP_FLUXC]), dim3(256), 0, st, C, F(BP_FLUXC), nm, lead, 1);
-  if (go(BP_LEGACY)) hipLaunchKernelGGL((kb_legacy<S>), dim3(b->nblk[BP_LEGACY]), dim3(128), 0, st, C, F(BP_LEGACY), nm, lead, 1);
-  if (go(BP_DONOR)) hipLaunchKernelGGL((kb_donor<S>), dim3(b->nblk[BP_DONOR]), dim3(128), 0, st, C, F(BP_DONOR), nm, lead, 1, 1);
+  // face flux, farfield and donor moments of the new coarse state: one launch (disjoint slots)
+  {
+    const int nfl = b->nblk[BP_FLUXC], nlg = b->nblk[BP_LEGACY], ndn = b->nblk[BP_DONOR];
+    if (nfl + nlg + ndn > 0)
+      hipLaunchKernelGGL((kb_after_coarse<S>), dim3(nfl + nlg + ndn), dim3(256), 0, st, C, F(BP_FLUXC), F(BP_LEGACY),
+                         F(BP_DONOR), nm, lead, nfl, nlg);
+  }
   for (int sub = 0; sub < 2; ++sub) {
-    if (go(BP_PX)) hipLaunchKernelGGL(kb_prolong_x, dim3(b->nblk[BP_PX]), dim3(256), 0, st, C, F(BP_PX), nm, sub * 0.5);
-    if (go(BP_PY)) hipLaunchKernelGGL(kb_prolong_y, dim3(b->nblk[BP_PY]), dim3(256), 0, st, C, F(BP_PY), nm);
-    if (go(BP_PZ)) hipLaunchKernelGGL((kb_prolong_z<S>), dim3(b->nblk[BP_PZ]), dim3(256), 0, st, C, F(BP_PZ), nm, sub);
+    if (go(BP_PX)) hipLaunchKernelGGL(kb_prolong_x, dim3(b->nblk[BP_PX]), dim3(64), 0, st, C, F(BP_PX), nm, sub * 0.5);
+    if (go(BP_PY)) hipLaunchKernelGGL(kb_prolong_y, dim3(b->nblk[BP_PY]), dim3(64), 0, st, C, F(BP_PY), nm);
+    if (go(BP_PZ)) hipLaunchKernelGGL((kb_prolong_z<S>), dim3(b->nblk[BP_PZ]), dim3(64), 0, st, C, F(BP_PZ), nm, sub);
     batched_sweep<S>(b, b->d_desc + (size_t)(2 + sub) * nc, b->nb_fine);
     for (Refined *r : b->members) r->f->sweeps++;
-    if (go(BP_FLUXF)) hipLaunchKernelGGL((kb_flux_fine<S>), dim3(b->nblk[BP_FLUXF]), dim3(256), 0, st, C, F(BP_FLUXF), nm, sub + 1, sub);
-    if (go(BP_WALLS)) hipLaunchKernelGGL((kb_walls<S>), dim3(b->nblk[BP_WALLS]), dim3(256), 0, st, C, F(BP_WALLS), nm, sub + 1, sub);
+    // fine face flux and walls + loads: one launch
+    if (b->nblk[BP_FLUXF] + b->nblk[BP_WALLS] > 0)
+      hipLaunchKernelGGL((kb_after_fine<S>), dim3(b->nblk[BP_FLUXF] + b->nblk[BP_WALLS]), dim3(256), 0, st, C,
+                         F(BP_FLUXF), F(BP_WALLS), nm, sub + 1, sub, b->nblk[BP_FLUXF]);
   }
   if (go(BP_REFLUX)) hipLaunchKernelGGL((kb_reflux<S>), dim3(b->nblk[BP_REFLUX]), dim3(256), 0, st, C, F(BP_REFLUX), nm, lead, 1);
   if (go(BP_FMOM)) hipLaunchKernelGGL((kb_fine_moments<S>), dim3(b->nblk[BP_FMOM]), dim3(256), 0, st, C, F(BP_FMOM), nm, 2);

@@ -170,9 +170,13 @@ __device__ __forceinline__ void prolong_x_body(const double *__restrict__ M0, co
   for (int k = 0; k < 10; ++k) o[k] = m[k];
 }
 
+// The prolongation kernels run over compact lists of the items that do work (the host
+// builds them with the bodies' own conditions): the fine interface band is a few percent of
+// the box volume.
 __global__ void k_prolong_x(const double *__restrict__ M0, const double *__restrict__ M1, double frac, BoxGeom B,
-                            double *__restrict__ X) {
-  prolong_x_body(M0, M1, frac, B, X, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+                            double *__restrict__ X, const int32_t *__restrict__ list, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) prolong_x_body(M0, M1, frac, B, X, list[i]);
 }
 
 __device__ __forceinline__ void prolong_y_body(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y, long long t) {
@@ -192,8 +196,10 @@ __device__ __forceinline__ void prolong_y_body(const double *__restrict__ X, Box
   for (int k = 0; k < 10; ++k) o[k] = m[k];
 }
 
-__global__ void k_prolong_y(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y) {
-  prolong_y_body(X, B, Y, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void k_prolong_y(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y,
+                            const int32_t *__restrict__ list, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) prolong_y_body(X, B, Y, list[i]);
 }
 
 template <int S, bool PE>
@@ -220,8 +226,10 @@ __device__ __forceinline__ void prolong_z_inject_body(typename St<S>::T *__restr
 
 template <int S, bool PE>
 __global__ void k_prolong_z_inject(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V,
-                                   const double *__restrict__ Y, float scale, int *__restrict__ err) {
-  prolong_z_inject_body<S, PE>(A, L, B, V, Y, scale, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+                                   const double *__restrict__ Y, float scale, int *__restrict__ err,
+                                   const int32_t *__restrict__ list, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) prolong_z_inject_body<S, PE>(A, L, B, V, Y, scale, err, list[i]);
 }
 
 // Face flux registers.  Face f = 2*axis + (side > 0), axis in {0 (x faces), 1 (y faces)}.
@@ -472,6 +480,44 @@ __global__ void k_init_level(typename St<S>::T *__restrict__ A, const uint8_t *_
   populations_of(m, g, er);
   scatter_in<S, false>(A, L, x, y, z, scale, g);
   if (er) atomicOr(err, er);
+}
+
+// ------------------------------------------------------------------ fused phases
+// After the coarse sweep, the coarse face flux (cells next to the box faces), the coarse
+// farfield (ghost slots of the domain edge and the edge cells' own slots) and the donor
+// moments of the new state (box +- 3, inside the domain) read and write disjoint slots, so
+// they run as one launch over consecutive block ranges: their latencies overlap.
+template <int S, bool PE>
+__global__ __launch_bounds__(256) void k_after_coarse(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
+                                                      const FluxTable *__restrict__ T, double volc, float scale,
+                                                      float inv, double *__restrict__ regc, LegacyLinks K, double U,
+                                                      FreeState F, double *__restrict__ after, int *__restrict__ err,
+                                                      int nb_flux, int nb_leg) {
+  const int b = blockIdx.x;
+  if (b < nb_flux) {
+    flux_coarse_body<S, PE>(A, L, B, T, volc, inv, regc, (long long)b * 256 + threadIdx.x);
+  } else if (b < nb_flux + nb_leg) {
+    legacy_body<S, PE>(A, L, K, U, scale, inv, (long long)(b - nb_flux) * 256 + threadIdx.x);
+  } else {
+    donor_moments_body<S, PE>(A, L, B, F, inv, after, err, (long long)(b - nb_flux - nb_leg) * 256 + threadIdx.x);
+  }
+}
+
+// After a fine sweep, the fine face flux (box faces) and the walls with their loads
+// (around the body, which keeps clear of the faces) likewise share one launch.  The wall
+// partials keep their per-block layout (block index within the wall range).
+template <int S, bool PE>
+__global__ __launch_bounds__(256) void k_after_fine(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
+                                                    const FluxTable *__restrict__ T, double volf, float scale, float inv,
+                                                    int sub, double *__restrict__ regf, WallLinks W,
+                                                    double *__restrict__ part, int nb_flux) {
+  const int b = blockIdx.x;
+  if (b < nb_flux) {
+    flux_fine_body<S, PE>(A, L, B, T, volf, inv, sub, regf, (long long)b * 256 + threadIdx.x);
+  } else {
+    const int wb = b - nb_flux;
+    walls_body<S, PE, true>(A, L, W, scale, inv, part, (long long)wb * 256 + threadIdx.x, wb);
+  }
 }
 
 }  // namespace ak

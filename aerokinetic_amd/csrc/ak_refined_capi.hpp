@@ -18,6 +18,8 @@ struct Refined {
   double dxc = 0, dtc = 0, dxf = 0, dtf = 0, span = 0, U = 0;
   double *d_before = nullptr, *d_after = nullptr, *d_regc = nullptr, *d_regf = nullptr, *d_part = nullptr;
   double *d_px = nullptr, *d_py = nullptr;  // prolongation passes
+  int32_t *d_lpx = nullptr, *d_lpy = nullptr, *d_lpz = nullptr;  // their work items
+  int n_lpx = 0, n_lpy = 0, n_lpz = 0;
   double *d_mf = nullptr;                   // fine moments for the restriction
   int *d_err = nullptr;
   FluxTable T{};
@@ -100,6 +102,34 @@ void build_flux_table(FluxTable &T) {
   }
 }
 
+// Work items of the three prolongation passes: the indices for which the bodies
+// (prolong_x_body, prolong_y_body, prolong_z_inject_body) do work, same conditions.
+int build_prolong_lists(Refined *r) {
+  const BoxGeom &B = r->B;
+  const int dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4, cyn = 2 * B.nb[1] + 4, p = B.pad;
+  const Desc &L = r->f->L;
+  if ((long long)cxn * cyn * dz >= INT32_MAX || (long long)L.nx * L.ny * L.nz >= INT32_MAX)
+    return fail(nullptr, "refined box too large for 32-bit prolongation work items");
+  std::vector<int32_t> lx, ly, lz;
+  for (int qz = 0; qz < dz; ++qz)
+    for (int qy = 0; qy < dy; ++qy)
+      for (int cx = 0; cx < cxn; ++cx)
+        if (!(cx >= p && cx < cxn - p && qy > 4 && qy < B.nb[1] + 1))
+          lx.push_back((int32_t)(((long long)qz * dy + qy) * cxn + cx));
+  for (int qz = 0; qz < dz; ++qz)
+    for (int cy = 0; cy < cyn; ++cy)
+      for (int cx = 0; cx < cxn; ++cx)
+        if (!(cx >= p && cx < cxn - p && cy >= p && cy < cyn - p))
+          ly.push_back((int32_t)(((long long)qz * cyn + cy) * cxn + cx));
+  for (int z = 0; z < L.nz; ++z)
+    for (int y = 0; y < L.ny; ++y)
+      for (int x = 0; x < L.nx; ++x)
+        if (!(x >= p && x < L.nx - p && y >= p && y < L.ny - p))
+          lz.push_back((int32_t)(((long long)z * L.ny + y) * L.nx + x));
+  r->n_lpx = (int)lx.size(); r->n_lpy = (int)ly.size(); r->n_lpz = (int)lz.size();
+  return (upload(r->c, &r->d_lpx, lx) || upload(r->c, &r->d_lpy, ly) || upload(r->c, &r->d_lpz, lz)) ? -1 : 0;
+}
+
 template <int S>
 void launch_restrict(Refined *r) {
   Lattice *c = r->c, *f = r->f;
@@ -141,15 +171,16 @@ void phase_after_coarse(Refined *r) {
   const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
   const unsigned cflux_blocks = blocks_for(4LL * r->nt * c->nz * FLUX_LANES, 256);
   const double volc = r->dxc * r->dxc * r->dxc;
-  if (c->sweeps % 2 == 0) {
-    hipLaunchKernelGGL((k_flux_coarse<S, true>), dim3(cflux_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
-    launch_boundary<S, true>(c, false);
-    hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
-  } else {
-    hipLaunchKernelGGL((k_flux_coarse<S, false>), dim3(cflux_blocks), dim3(256), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->d_T, volc, c->inv_scale, r->d_regc);
-    launch_boundary<S, false>(c, false);
-    hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_after, r->d_err);
-  }
+  // one launch: face flux, legacy farfield and donor moments (disjoint slots)
+  const LegacyLinks K{c->d_legx, c->d_legz, c->d_legdir, c->d_legkind, c->d_pdf, c->d_pdfnd, c->nleg};
+  const int nb_leg = (int)blocks_for(c->nleg, 256);
+  const dim3 grid(cflux_blocks + nb_leg + blocks_for(ndon, 256)), block(256);
+  if (c->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_after_coarse<S, true>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_T, volc,
+                       c->scale, c->inv_scale, r->d_regc, K, c->U, r->F, r->d_after, r->d_err, (int)cflux_blocks, nb_leg);
+  else
+    hipLaunchKernelGGL((k_after_coarse<S, false>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_T, volc,
+                       c->scale, c->inv_scale, r->d_regc, K, c->U, r->F, r->d_after, r->d_err, (int)cflux_blocks, nb_leg);
 }
 
 // fine substep `sub` before its sweep: time-interpolated prolongation into the donor layers
@@ -158,15 +189,14 @@ void phase_before_fine(Refined *r, int sub) {
   Lattice *f = r->f;
   using T_ = typename St<S>::T;
   const double frac = sub * 0.5;
-  const long long nf = (long long)f->nx * f->ny * f->nz;
-  const long long npx = (long long)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
-  const long long npy = (long long)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (r->B.nb[2] + 6);
-  hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(npx, 256)), dim3(256), 0, f->stream, r->d_before, r->d_after, frac, r->B, r->d_px);
-  hipLaunchKernelGGL(k_prolong_y, dim3(blocks_for(npy, 256)), dim3(256), 0, f->stream, r->d_px, r->B, r->d_py);
+  // 64-thread blocks spread the few thousand items over every CU
+  hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(r->n_lpx, 64)), dim3(64), 0, f->stream, r->d_before, r->d_after, frac, r->B, r->d_px, r->d_lpx, r->n_lpx);
+  hipLaunchKernelGGL(k_prolong_y, dim3(blocks_for(r->n_lpy, 64)), dim3(64), 0, f->stream, r->d_px, r->B, r->d_py, r->d_lpy, r->n_lpy);
+  const unsigned nzb = blocks_for(r->n_lpz, 64);
   if (f->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_prolong_z_inject<S, true>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
+    hipLaunchKernelGGL((k_prolong_z_inject<S, true>), dim3(nzb), dim3(64), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err, r->d_lpz, r->n_lpz);
   else
-    hipLaunchKernelGGL((k_prolong_z_inject<S, false>), dim3(blocks_for(nf, 256)), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err);
+    hipLaunchKernelGGL((k_prolong_z_inject<S, false>), dim3(nzb), dim3(64), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err, r->d_lpz, r->n_lpz);
 }
 
 // fine substep `sub` after its sweep: fine face flux, walls + load partials
@@ -178,13 +208,14 @@ void phase_after_fine(Refined *r, int sub) {
   const unsigned fflux_blocks = blocks_for(4LL * (2 * r->nt) * f->nz * FLUX_LANES, 256);
   WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
   double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;
-  if (f->sweeps % 2 == 0) {
-    hipLaunchKernelGGL((k_flux_fine<S, true>), dim3(fflux_blocks), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, sub, r->d_regf);
-    if (f->nlinks) hipLaunchKernelGGL((k_walls<S, true, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
-  } else {
-    hipLaunchKernelGGL((k_flux_fine<S, false>), dim3(fflux_blocks), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, r->d_T, volf, f->inv_scale, sub, r->d_regf);
-    if (f->nlinks) hipLaunchKernelGGL((k_walls<S, false, true>), dim3(r->wall_blocks), dim3(256), 0, f->stream, (T_ *)f->pop, f->L, W, f->scale, f->inv_scale, part);
-  }
+  // one launch: fine face flux and the walls with their loads (disjoint slots)
+  const dim3 grid(fflux_blocks + (f->nlinks ? r->wall_blocks : 0)), block(256);
+  if (f->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_after_fine<S, true>), grid, block, 0, f->stream, (T_ *)f->pop, f->L, r->B, r->d_T, volf,
+                       f->scale, f->inv_scale, sub, r->d_regf, W, part, (int)fflux_blocks);
+  else
+    hipLaunchKernelGGL((k_after_fine<S, false>), grid, block, 0, f->stream, (T_ *)f->pop, f->L, r->B, r->d_T, volf,
+                       f->scale, f->inv_scale, sub, r->d_regf, W, part, (int)fflux_blocks);
 }
 
 // end of the coarse step: reflux, restriction, loads (sampled) or the step tick
@@ -299,6 +330,12 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
     return -1;
   }
   (void)hipMemsetAsync(r->d_err, 0, sizeof(int), r->c->stream);
+  if (build_prolong_lists(r)) {
+    ak_lattice_destroy(hf); ak_lattice_destroy(hc);
+    if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
+    delete r;
+    return -1;
+  }
   if (hipMemcpy(r->d_T, &r->T, sizeof(FluxTable), hipMemcpyHostToDevice) != hipSuccess) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
     if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
