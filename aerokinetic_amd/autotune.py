@@ -1,7 +1,8 @@
 """Sweep tile autotuner.
 
 Times the fused AA sweep for each candidate thread tile (bx threads along x, by rows
-along y; bx a multiple of the 64-lane wavefront, bx*by <= 1024) with HIP events on
+along y; bx a multiple of the 64-lane wavefront, bx*by <= 1024; bx = 0: row-linear
+blocks of `by` consecutive cells of a plane) with HIP events on
 the lattice's own stream and keeps the fastest.  A tile is tried on both sweep
 parities (odd sweeps gather from neighbours, even sweeps read in place) by timing an
 even number of steps.  The lattice state advances while tuning, so tune on a
@@ -11,7 +12,7 @@ scratch copy (or before the initial state is set).  Results are cached per
 from __future__ import annotations
 
 CANDIDATES = ((64, 1), (64, 2), (64, 4), (128, 1), (128, 2), (256, 1), (64, 8), (128, 4), (256, 2), (512, 1),
-              (64, 16), (128, 8), (256, 4), (512, 2), (1024, 1))
+              (64, 16), (128, 8), (256, 4), (512, 2), (1024, 1), (0, 256), (0, 512))
 _cache: dict = {}
 
 

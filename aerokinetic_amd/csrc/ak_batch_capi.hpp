@@ -58,16 +58,8 @@ unsigned long long batch_key(const Batch *b) {
   return k;
 }
 
-SweepGeom tile_geom(const Lattice *s) {
-  SweepGeom G{};
-  G.bx = 64;
-  G.by = 4;
-  G.gx = (s->nx + 63) / 64;
-  G.gy = (s->ny + 3) / 4;
-  G.gz = s->nz;
-  G.nblocks = G.gx * G.gy * G.gz;
-  return G;
-}
+// batched sweeps run 256-thread blocks: the default geometry of each member
+SweepGeom tile_geom(const Lattice *s) { return default_geom(s->nx, s->ny, s->nz); }
 
 BatchMember member_of(const Lattice *s, bool odd, int first) {
   BatchMember m{};

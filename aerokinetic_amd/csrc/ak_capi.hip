@@ -139,14 +139,30 @@ int upload(Lattice *s, T **dst, const std::vector<T> &v) {
 
 inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
-void set_geom(Lattice *s, int bx, int by) {
-  s->geom.bx = bx;
-  s->geom.by = by;
-  s->geom.gx = (s->nx + bx - 1) / bx;
-  s->geom.gy = (s->ny + by - 1) / by;
-  s->geom.gz = s->nz;
-  s->geom.nblocks = s->geom.gx * s->geom.gy * s->geom.gz;
+// Sweep geometry: bx x by tiles, or with bx == 0 the row-linear mode with `by` threads.
+SweepGeom make_geom(int nx, int ny, int nz, int bx, int by) {
+  SweepGeom G{};
+  if (bx == 0) {
+    G.lin = 1;
+    G.bx = by;
+    G.by = 1;
+    G.gx = (int)(((long long)nx * ny + by - 1) / by);
+    G.gy = 1;
+  } else {
+    G.bx = bx;
+    G.by = by;
+    G.gx = (nx + bx - 1) / bx;
+    G.gy = (ny + by - 1) / by;
+  }
+  G.gz = nz;
+  G.nblocks = G.gx * G.gy * G.gz;
+  return G;
 }
+// default: 64 x 4 tiles when rows are whole waves, row-linear blocks of 256 otherwise
+SweepGeom default_geom(int nx, int ny, int nz) {
+  return nx % 64 == 0 ? make_geom(nx, ny, nz, 64, 4) : make_geom(nx, ny, nz, 0, 256);
+}
+void set_geom(Lattice *s, int bx, int by) { s->geom = make_geom(s->nx, s->ny, s->nz, bx, by); }
 
 // ---------------------------------------------------------------- launches
 
@@ -527,7 +543,7 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
     return fail(nullptr, "lattice plane of %lld elements too large for 32-bit in-plane offsets", (long long)(P * (ny + 2)));
   }
   s->pop_bytes = s->pop_elems * s->elem_size;
-  set_geom(s, 64, 4);
+  s->geom = default_geom(nx, ny, nz);
   s->geom_small = s->geom;
   s->hist_cap = history_capacity > 0 ? history_capacity : 4096;
   s->hist_width = AK_HIST_WIDTH;
@@ -774,8 +790,11 @@ int ak_set_sweep_tile(void *handle, int bx, int by) {
   Lattice *s = H(handle);
   if (!s) return fail(nullptr, "null lattice handle");
   s->gen++;  // invalidates captured step graphs
-  if (bx < 64 || bx % 64 || by < 1 || bx * by > 1024)
+  if (bx == 0) {
+    if (by < 64 || by % 64 || by > 1024) return fail(s, "row-linear sweep blocks of %d threads: a multiple of 64, <= 1024", by);
+  } else if (bx < 64 || bx % 64 || by < 1 || bx * by > 1024) {
     return fail(s, "tile %dx%d: bx multiple of 64, bx*by <= 1024", bx, by);
+  }
   set_geom(s, bx, by);
   return 0;
 }

@@ -6,13 +6,16 @@
 namespace ak {
 
 // ------------------------------------------------------------------ sweep
-// One thread per interior cell; blocks tile (x, y) of one z plane.  The 1-D grid is
-// remapped so that each XCD (blocks are dealt round-robin over 8 XCDs) owns a
-// contiguous slab of tiles and the neighbour rows it gathers stay in its own L2.
+// One thread per interior cell; blocks tile (x, y) of one z plane, or (row-linear mode)
+// take bx consecutive cells of a plane in (y, x) order, so that a row width that is not a
+// multiple of 64 leaves idle lanes only in a plane's last block.  The 1-D grid is remapped
+// so that each XCD (blocks are dealt round-robin over 8 XCDs) owns a contiguous slab of
+// tiles and the neighbour rows it gathers stay in its own L2.
 struct SweepGeom {
-  int bx, by;      // block tile (threads) in x, y
-  int gx, gy, gz;  // tiles per axis
+  int bx, by;      // block tile (threads) in x, y; row-linear: bx threads, by = 1
+  int gx, gy, gz;  // tiles per axis; row-linear: gx blocks per plane, gy = 1
   int nblocks;
+  int lin;         // 1: row-linear mode
 };
 
 // Order-independent maxima/minima as bit patterns of non-negative doubles.  Blocks
@@ -74,10 +77,18 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
                                             typename St<S>::C wb, float scale, float inv_scale,
                                             DiagAcc *__restrict__ diag, int lb, int slot) {
   using C = typename St<S>::C;
-  const int tx = lb % G.gx, ty = (lb / G.gx) % G.gy, tz = lb / (G.gx * G.gy);
-  const int x = tx * G.bx + (int)(threadIdx.x % G.bx);
-  const int y = ty * G.by + (int)(threadIdx.x / G.bx);
-  const int z = tz;
+  int x, y, z;
+  if (G.lin) {
+    const int j = (lb % G.gx) * G.bx + (int)threadIdx.x;
+    z = lb / G.gx;
+    y = j / L.nx;
+    x = j - y * L.nx;
+  } else {
+    const int tx = lb % G.gx, ty = (lb / G.gx) % G.gy;
+    x = tx * G.bx + (int)(threadIdx.x % G.bx);
+    y = ty * G.by + (int)(threadIdx.x / G.bx);
+    z = lb / (G.gx * G.gy);
+  }
   bool active = x < L.nx && y < L.ny;
   uint8_t mk = 1;
   // mask: 0 fluid, 1 solid, 2 fluid outside the diagnostics, 3 exterior ring (its own kernel)

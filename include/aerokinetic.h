@@ -103,7 +103,9 @@ int ak_set_farfield(void *handle, int kind, double lattice_speed);
  * p = {U, amp, phase_y, phase_z, ky, kz, lex, ley, ox, oy, dx}. */
 int ak_init_seeded(void *handle, const double *p);
 
-/* Thread tile of the sweep (bx a multiple of 64, bx*by <= 256); set by the tile autotuner. */
+/* Thread tile of the sweep (bx a multiple of 64, bx*by <= 1024), or with bx = 0 row-linear
+ * blocks of `by` threads (a multiple of 64) over consecutive cells of a plane; set by the tile
+ * autotuner.  Default: 64 x 4 when nx is a multiple of 64, row-linear 256 otherwise. */
 int ak_set_sweep_tile(void *handle, int bx, int by);
 
 /* Advance `nsteps` reference steps.  Steps with (first_step + k + 1) % sample_every == 0

@@ -97,13 +97,15 @@ def test_batch_graph_replay_matches_direct_launches():
 
 
 def test_service_admits_steps_and_retires_a_request_stream():
-    reqs = S.polar_requests(4, seed=9, rate=20.0, chord=(16, 18), span=(8, 8), reynolds=(1e4, 2e4))
+    # every request is available at t = 0 and the two builders prepare two of them side by
+    # side, so the second joins while the first still runs (a case lives >= one window)
+    reqs = S.polar_requests(4, seed=9, rate=None, chord=(16, 18), span=(8, 8), reynolds=(1e4, 2e4))
     print("arrivals", [round(r.arrival, 3) for r in reqs], flush=True)
-    srv = S.Server(storage="fp32", chunk=40, window=80, tol=0.005, max_steps=160, builders=2)
+    srv = S.Server(storage="fp32", chunk=40, window=400, tol=0.005, max_steps=800, builders=2)
     rep = srv.serve(reqs)
     s = rep.summary()
     assert s["cases"] == 4 and sum(s["reasons"].values()) == 4
     assert s["reasons"]["failed"] == 0
     assert s["peak_members"] >= 2
     for r in rep.results:
-        assert r.latency > 0 and 80 <= r.steps <= 160 and np.isfinite(r.cl)
+        assert r.latency > 0 and 400 <= r.steps <= 800 and np.isfinite(r.cl)
