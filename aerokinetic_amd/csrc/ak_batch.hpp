@@ -22,7 +22,7 @@ struct CaseDev {
   Levels V;
   FreeState F;
   const FluxTable *T;
-  double *before, *after, *regc, *regf, *part, *px, *py, *mf;
+  double *before, *after, *regc, *regf, *part, *px, *py;
   const int32_t *lpx, *lpy, *lpz;  // prolongation work items
   int npx, npy, npz;
   int *err;
@@ -39,7 +39,7 @@ struct CaseDev {
 
 // The step's kernels, each with a table of first blocks per member.
 enum BatchPhase {
-  BP_DONOR, BP_FLUXC, BP_LEGACY, BP_PX, BP_PY, BP_PZ, BP_FLUXF, BP_WALLS, BP_REFLUX, BP_FMOM, BP_RESTRICT,
+  BP_DONOR, BP_FLUXC, BP_LEGACY, BP_PX, BP_PY, BP_PZ, BP_FLUXF, BP_WALLS, BP_REFLUX, BP_RESTRICT,
   BP_COUNT
 };
 
@@ -107,22 +107,20 @@ __global__ __launch_bounds__(256) void kb_reflux(const CaseDev *__restrict__ C, 
     reflux_body<S, false>((T *)c.Ac, c.Lc, c.B, c.regc, c.regf, c.inv_volc, c.c_scale, c.c_inv, c.err, t);
 }
 
-template <int S>
-__global__ __launch_bounds__(256) void kb_fine_moments(const CaseDev *__restrict__ C, const int *__restrict__ first,
-                                                       int nm, int delta) {
-  using T = typename St<S>::T;
-  AK_BATCH_MEMBER
-  if (AK_PE_FINE(delta)) fine_moments_body<S, true>((const T *)c.Af, c.Lf, c.B, c.f_inv, c.mf, c.err, t);
-  else fine_moments_body<S, false>((const T *)c.Af, c.Lf, c.B, c.f_inv, c.mf, c.err, t);
-}
 
+// Fine moments + restriction, one block per restriction tile of a member (fine parity at
+// delta 2, coarse at delta 1 from the step start).
 template <int S>
-__global__ __launch_bounds__(128) void kb_restrict(const CaseDev *__restrict__ C, const int *__restrict__ first,
-                                                   int nm, int lead, int delta) {
+__global__ __launch_bounds__(256) void kb_restrict(const CaseDev *__restrict__ C, const int *__restrict__ first,
+                                                   int nm, int lead) {
   using T = typename St<S>::T;
   AK_BATCH_MEMBER
-  if (AK_PE_COARSE(delta)) restrict_body<S, true>((T *)c.Ac, c.Lc, c.mf, c.B, c.V, c.F, c.c_scale, c.err, t);
-  else restrict_body<S, false>((T *)c.Ac, c.Lc, c.mf, c.B, c.V, c.F, c.c_scale, c.err, t);
+  (void)t;
+  const bool pf = AK_PE_FINE(2), pc = AK_PE_COARSE(1);
+  if (pf && pc) restrict_tile_body<S, true, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_);
+  else if (pf) restrict_tile_body<S, true, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_);
+  else if (pc) restrict_tile_body<S, false, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_);
+  else restrict_tile_body<S, false, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_);
 }
 
 // One 64-thread block per member: the loads of a sampled step, or the step tick.

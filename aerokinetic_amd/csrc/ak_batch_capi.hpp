@@ -129,7 +129,7 @@ int build_descriptors(Batch *b) {
     d.c_scale = c->scale; d.c_inv = c->inv_scale; d.f_scale = f->scale; d.f_inv = f->inv_scale;
     d.B = r->B; d.V = r->V; d.F = r->F; d.T = r->d_T;
     d.before = r->d_before; d.after = r->d_after; d.regc = r->d_regc; d.regf = r->d_regf; d.part = r->d_part;
-    d.px = r->d_px; d.py = r->d_py; d.mf = r->d_mf; d.err = r->d_err;
+    d.px = r->d_px; d.py = r->d_py; d.err = r->d_err;
     d.lpx = r->d_lpx; d.lpy = r->d_lpy; d.lpz = r->d_lpz;
     d.npx = r->n_lpx; d.npy = r->n_lpy; d.npz = r->n_lpz;
     d.volc = r->dxc * r->dxc * r->dxc;
@@ -155,9 +155,8 @@ int build_descriptors(Batch *b) {
         4LL * (2 * r->nt) * f->nz * FLUX_LANES,            // fine face flux (256)
         (long long)d.wall_blocks * 256,                    // walls (256)
         4LL * r->nt * c->nz * 64,                          // reflux, a wave per cell (256)
-        8 * nb0 * nb1 * nb2,                               // fine moments (256)
-        nb0 * nb1 * nb2};                                  // restriction (128)
-    const int threads[BP_COUNT] = {256, 256, 256, 64, 64, 64, 256, 256, 256, 256, 128};
+        (long long)restrict_tiles(r->B) * 256};            // restriction, a block per tile (256)
+    const int threads[BP_COUNT] = {256, 256, 256, 64, 64, 64, 256, 256, 256, 256};
     for (int k = 0; k < BP_COUNT; ++k)
       first[(size_t)k * (b->cap + 1) + i + 1] = first[(size_t)k * (b->cap + 1) + i] + (int)blocks_for(items[k], threads[k]);
   }
@@ -205,8 +204,7 @@ void batch_step(Batch *b, bool sample) {
                          F(BP_FLUXF), F(BP_WALLS), nm, sub + 1, sub, b->nblk[BP_FLUXF]);
   }
   if (go(BP_REFLUX)) hipLaunchKernelGGL((kb_reflux<S>), dim3(b->nblk[BP_REFLUX]), dim3(256), 0, st, C, F(BP_REFLUX), nm, lead, 1);
-  if (go(BP_FMOM)) hipLaunchKernelGGL((kb_fine_moments<S>), dim3(b->nblk[BP_FMOM]), dim3(256), 0, st, C, F(BP_FMOM), nm, 2);
-  if (go(BP_RESTRICT)) hipLaunchKernelGGL((kb_restrict<S>), dim3(b->nblk[BP_RESTRICT]), dim3(128), 0, st, C, F(BP_RESTRICT), nm, lead, 1);
+  if (go(BP_RESTRICT)) hipLaunchKernelGGL((kb_restrict<S>), dim3(b->nblk[BP_RESTRICT]), dim3(256), 0, st, C, F(BP_RESTRICT), nm, lead);
   hipLaunchKernelGGL(kb_loads, dim3(nm), dim3(64), 0, st, C, sample ? 1 : 0);
 }
 

@@ -396,59 +396,67 @@ __global__ __launch_bounds__(256) void k_reflux(typename St<S>::T *__restrict__ 
 }
 
 // Moments of every owned fine cell outside the fine virtual core (first restriction stage).
-template <int S, bool PE>
-__device__ __forceinline__ void fine_moments_body(const typename St<S>::T *__restrict__ Af, Desc Lf, BoxGeom B, float inv, double *__restrict__ Mf, int *__restrict__ err, long long t) {
-  const int fx = 2 * B.nb[0], fy = 2 * B.nb[1], fz = 2 * B.nb[2];
-  if (t >= (long long)fx * fy * fz) return;
-  const int x = (int)(t % fx), y = (int)((t / fx) % fy), z = (int)(t / ((long long)fx * fy));
-  if (in_core(B, x / 2, y / 2)) return;
-  double g[NQ], m[10];
-  int e = 0;
-  gather_in<S, PE>(Af, Lf, B.pad + x, B.pad + y, z, inv, g);
-  moments_of(g, m, e);
-  for (int k = 0; k < 10; ++k) Mf[t * 10 + k] = m[k];
-  if (e) atomicOr(err, e);
+// Fine moments and restriction in one pass: a block takes a tile of RT_X x RT_Y covered
+// coarse cells of one coarse z layer; its 256 threads are their children, whose moments go
+// to LDS; then one thread per coarse cell averages its eight children (fixed order) as in
+// the reference's restriction (covered cells: free stream in the virtual core, otherwise the
+// 2x2x2 mean of the children's moments, rescaled fine->coarse and regularised); the moments
+// never leave the chip.
+constexpr int RT_X = 16, RT_Y = 2;
+__host__ __device__ inline int restrict_tiles(const BoxGeom &B) {
+  return ((B.nb[0] + RT_X - 1) / RT_X) * ((B.nb[1] + RT_Y - 1) / RT_Y) * B.nb[2];
 }
 
-template <int S, bool PE>
-__global__ void k_fine_moments(const typename St<S>::T *__restrict__ Af, Desc Lf, BoxGeom B, float inv,
-                               double *__restrict__ Mf, int *__restrict__ err) {
-  fine_moments_body<S, PE>(Af, Lf, B, inv, Mf, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-// Covered coarse cells from the fine state: free stream in the virtual core, otherwise
-// the 2x2x2 mean of the children's moments (from k_fine_moments), rescaled fine->coarse
-// and regularised.
-template <int S, bool PEC>
-__device__ __forceinline__ void restrict_body(typename St<S>::T *__restrict__ Ac, Desc Lc, const double *__restrict__ Mf, BoxGeom B, Levels V, FreeState F, float scale, int *__restrict__ err, long long t) {
-  if (t >= (long long)B.nb[0] * B.nb[1] * B.nb[2]) return;
-  const int bx = (int)(t % B.nb[0]), by = (int)((t / B.nb[0]) % B.nb[1]), bz = (int)(t / ((long long)B.nb[0] * B.nb[1]));
-  const int x = B.lo[0] + bx, y = B.lo[1] + by, z = B.lo[2] + bz;
-  const int fx = 2 * B.nb[0], fy = 2 * B.nb[1];
-  double g[NQ];
+template <int S, bool PE, bool PEC>
+__device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
+                                                   typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B, Levels V,
+                                                   FreeState F, float cscale, int *__restrict__ err, int tile) {
+  __shared__ double sm[10][256];
+  const int ntx = (B.nb[0] + RT_X - 1) / RT_X, nty = (B.nb[1] + RT_Y - 1) / RT_Y;
+  const int tbx = tile % ntx, tby = (tile / ntx) % nty, bz = tile / (ntx * nty);
+  const int tid = threadIdx.x;
+  const int cx = tid % (2 * RT_X), cy = (tid / (2 * RT_X)) % (2 * RT_Y), cz = tid / (4 * RT_X * RT_Y);
+  const int fx = 2 * RT_X * tbx + cx, fy = 2 * RT_Y * tby + cy, fz = 2 * bz + cz;
   int e = 0;
-  if (in_core(B, bx, by)) {
-    for (int i = 0; i < NQ; ++i) g[i] = F.g[i];
-  } else {
-    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < 2; ++c)
-      for (int b = 0; b < 2; ++b)
-        for (int a = 0; a < 2; ++a) {
-          const double *m = Mf + (((long long)(2 * bz + c) * fy + (2 * by + b)) * fx + (2 * bx + a)) * 10;
-          for (int k = 0; k < 10; ++k) acc[k] += m[k];
-        }
-    for (int k = 0; k < 10; ++k) acc[k] *= 0.125;
-    rescale(acc, V.shear_fc, V.bulk_fc);
-    populations_of(acc, g, e);
+  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (fx < 2 * B.nb[0] && fy < 2 * B.nb[1] && !in_core(B, fx / 2, fy / 2)) {
+    double g[NQ];
+    gather_in<S, PE>(Af, Lf, B.pad + fx, B.pad + fy, fz, finv, g);
+    moments_of(g, m, e);
   }
-  scatter_in<S, PEC>(Ac, Lc, x, y, z, scale, g);
+#pragma unroll
+  for (int k = 0; k < 10; ++k) sm[k][tid] = m[k];
+  __syncthreads();
+  if (tid < RT_X * RT_Y) {
+    const int ox = tid % RT_X, oy = tid / RT_X;
+    const int bx = RT_X * tbx + ox, by = RT_Y * tby + oy;
+    if (bx < B.nb[0] && by < B.nb[1]) {
+      double g[NQ];
+      if (in_core(B, bx, by)) {
+        for (int i = 0; i < NQ; ++i) g[i] = F.g[i];
+      } else {
+        double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < 2; ++c)
+          for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) {
+              const int ch = (2 * ox + a) + 2 * RT_X * ((2 * oy + b) + 2 * RT_Y * c);
+              for (int k = 0; k < 10; ++k) acc[k] += sm[k][ch];
+            }
+        for (int k = 0; k < 10; ++k) acc[k] *= 0.125;
+        rescale(acc, V.shear_fc, V.bulk_fc);
+        populations_of(acc, g, e);
+      }
+      scatter_in<S, PEC>(Ac, Lc, B.lo[0] + bx, B.lo[1] + by, B.lo[2] + bz, cscale, g);
+    }
+  }
   if (e) atomicOr(err, e);
 }
 
-template <int S, bool PEC>
-__global__ void k_restrict(typename St<S>::T *__restrict__ Ac, Desc Lc, const double *__restrict__ Mf, BoxGeom B,
-                           Levels V, FreeState F, float scale, int *__restrict__ err) {
-  restrict_body<S, PEC>(Ac, Lc, Mf, B, V, F, scale, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+template <int S, bool PE, bool PEC>
+__global__ __launch_bounds__(256) void k_restrict_tiles(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
+                                                        typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B,
+                                                        Levels V, FreeState F, float cscale, int *__restrict__ err) {
+  restrict_tile_body<S, PE, PEC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, (int)blockIdx.x);
 }
 
 // Initial pre-collision state of one level: rho = 1, seeded momentum, regularised

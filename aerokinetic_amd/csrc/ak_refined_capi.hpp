@@ -20,7 +20,6 @@ struct Refined {
   double *d_px = nullptr, *d_py = nullptr;  // prolongation passes
   int32_t *d_lpx = nullptr, *d_lpy = nullptr, *d_lpz = nullptr;  // their work items
   int n_lpx = 0, n_lpy = 0, n_lpz = 0;
-  double *d_mf = nullptr;                   // fine moments for the restriction
   int *d_err = nullptr;
   FluxTable T{};
   FluxTable *d_T = nullptr;
@@ -134,16 +133,14 @@ template <int S>
 void launch_restrict(Refined *r) {
   Lattice *c = r->c, *f = r->f;
   using T_ = typename St<S>::T;
-  const long long nfo = 8LL * r->B.nb[0] * r->B.nb[1] * r->B.nb[2];
-  const long long nbox = (long long)r->B.nb[0] * r->B.nb[1] * r->B.nb[2];
-  if (f->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_fine_moments<S, true>), dim3(blocks_for(nfo, 256)), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, f->inv_scale, r->d_mf, r->d_err);
-  else
-    hipLaunchKernelGGL((k_fine_moments<S, false>), dim3(blocks_for(nfo, 256)), dim3(256), 0, f->stream, (const T_ *)f->pop, f->L, r->B, f->inv_scale, r->d_mf, r->d_err);
-  if (c->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_restrict<S, true>), dim3(blocks_for(nbox, 128)), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, (const double *)r->d_mf, r->B, r->V, r->F, c->scale, r->d_err);
-  else
-    hipLaunchKernelGGL((k_restrict<S, false>), dim3(blocks_for(nbox, 128)), dim3(128), 0, c->stream, (T_ *)c->pop, c->L, (const double *)r->d_mf, r->B, r->V, r->F, c->scale, r->d_err);
+  const dim3 grid(restrict_tiles(r->B)), block(256);
+  const bool pf = f->sweeps % 2 == 0, pc = c->sweeps % 2 == 0;
+#define AK_RT(PF, PC)                                                                                            \
+  hipLaunchKernelGGL((k_restrict_tiles<S, PF, PC>), grid, block, 0, c->stream, (const T_ *)f->pop, f->L,         \
+                     f->inv_scale, (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, r->d_err)
+  if (pf) { if (pc) AK_RT(true, true); else AK_RT(true, false); }
+  else { if (pc) AK_RT(false, true); else AK_RT(false, false); }
+#undef AK_RT
 }
 
 // The coarse step in phases.  A single case runs them in order with its own sweeps in
@@ -323,7 +320,7 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
       dalloc(r->c, &r->d_regf, 4 * nreg) ||
       dalloc(r->c, &r->d_px, (size_t)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (nz + 6) * 10) ||
       dalloc(r->c, &r->d_py, (size_t)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (nz + 6) * 10) ||
-      dalloc(r->c, &r->d_mf, (size_t)8 * r->B.nb[0] * r->B.nb[1] * nz * 10) || dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
+      dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
     if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
     delete r;
