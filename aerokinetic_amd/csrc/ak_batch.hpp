@@ -22,9 +22,9 @@ struct CaseDev {
   Levels V;
   FreeState F;
   const FluxTable *T;
-  double *before, *after, *regc, *regf, *part, *px, *py;
-  const int32_t *lpx, *lpy, *lpz;  // prolongation work items
-  int npx, npy, npz;
+  double *before, *after, *regc, *regf, *part, *px;
+  const int32_t *lpx, *cols;  // prolongation x-pass items; fine interface-band columns
+  int npx, ncols;
   int *err;
   double volc, volf, inv_volc, loads_scale;
   WallLinks W;  // fine-level walls (lever arm dx_f)
@@ -39,7 +39,7 @@ struct CaseDev {
 
 // The step's kernels, each with a table of first blocks per member.
 enum BatchPhase {
-  BP_DONOR, BP_FLUXC, BP_LEGACY, BP_PX, BP_PY, BP_PZ, BP_FLUXF, BP_WALLS, BP_REFLUX, BP_RESTRICT,
+  BP_DONOR, BP_FLUXC, BP_LEGACY, BP_PX, BP_PYZ, BP_FLUXF, BP_WALLS, BP_REFLUX, BP_RESTRICT,
   BP_COUNT
 };
 
@@ -80,20 +80,15 @@ __global__ __launch_bounds__(256) void kb_prolong_x(const CaseDev *__restrict__ 
   if (t < c.npx) prolong_x_body(c.before, c.after, frac, c.B, c.px, c.lpx[t]);
 }
 
-__global__ __launch_bounds__(256) void kb_prolong_y(const CaseDev *__restrict__ C, const int *__restrict__ first,
-                                                    int nm) {
-  AK_BATCH_MEMBER
-  if (t < c.npy) prolong_y_body(c.px, c.B, c.py, c.lpy[t]);
-}
-
+// y and z passes + injection: one 64-thread block per group of PZ_COLS band columns
 template <int S>
-__global__ __launch_bounds__(256) void kb_prolong_z(const CaseDev *__restrict__ C, const int *__restrict__ first,
+__global__ __launch_bounds__(64) void kb_prolong_yz(const CaseDev *__restrict__ C, const int *__restrict__ first,
                                                     int nm, int delta) {
   using T = typename St<S>::T;
   AK_BATCH_MEMBER
-  if (t >= c.npz) return;
-  if (AK_PE_FINE(delta)) prolong_z_inject_body<S, true>((T *)c.Af, c.Lf, c.B, c.V, c.py, c.f_scale, c.err, c.lpz[t]);
-  else prolong_z_inject_body<S, false>((T *)c.Af, c.Lf, c.B, c.V, c.py, c.f_scale, c.err, c.lpz[t]);
+  (void)t;
+  if (AK_PE_FINE(delta)) prolong_yz_body<S, true>(c.px, (T *)c.Af, c.Lf, c.B, c.V, c.f_scale, c.err, c.cols, c.ncols, lb_);
+  else prolong_yz_body<S, false>(c.px, (T *)c.Af, c.Lf, c.B, c.V, c.f_scale, c.err, c.cols, c.ncols, lb_);
 }
 
 template <int S>

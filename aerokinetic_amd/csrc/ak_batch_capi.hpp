@@ -26,6 +26,7 @@ struct Batch {
   int *d_first = nullptr;           // [BP_COUNT][cap + 1] first block of each member per kernel
   int cap = 0, nb_coarse = 0, nb_fine = 0;
   int nblk[BP_COUNT] = {};
+  size_t yz_lds = 0;                // dynamic LDS of the y/z prolongation (largest member)
   unsigned long long desc_key = 0;  // state the descriptors were built for
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> join;
@@ -129,9 +130,9 @@ int build_descriptors(Batch *b) {
     d.c_scale = c->scale; d.c_inv = c->inv_scale; d.f_scale = f->scale; d.f_inv = f->inv_scale;
     d.B = r->B; d.V = r->V; d.F = r->F; d.T = r->d_T;
     d.before = r->d_before; d.after = r->d_after; d.regc = r->d_regc; d.regf = r->d_regf; d.part = r->d_part;
-    d.px = r->d_px; d.py = r->d_py; d.err = r->d_err;
-    d.lpx = r->d_lpx; d.lpy = r->d_lpy; d.lpz = r->d_lpz;
-    d.npx = r->n_lpx; d.npy = r->n_lpy; d.npz = r->n_lpz;
+    d.px = r->d_px; d.err = r->d_err;
+    d.lpx = r->d_lpx; d.cols = r->d_cols;
+    d.npx = r->n_lpx; d.ncols = r->n_cols;
     d.volc = r->dxc * r->dxc * r->dxc;
     d.volf = r->dxf * r->dxf * r->dxf;
     d.inv_volc = 1.0 / d.volc;
@@ -150,17 +151,18 @@ int build_descriptors(Batch *b) {
         4LL * r->nt * c->nz * FLUX_LANES,                  // coarse face flux (256)
         (long long)c->nleg,                                // legacy farfield (256)
         (long long)r->n_lpx,                               // prolongation x work items (64)
-        (long long)r->n_lpy,                               // prolongation y work items (64)
-        (long long)r->n_lpz,                               // prolongation z + injection items (64)
+        (long long)blocks_for(r->n_cols, PZ_COLS) * 64,    // prolongation y/z: a block per column group (64)
         4LL * (2 * r->nt) * f->nz * FLUX_LANES,            // fine face flux (256)
         (long long)d.wall_blocks * 256,                    // walls (256)
         4LL * r->nt * c->nz * 64,                          // reflux, a wave per cell (256)
         (long long)restrict_tiles(r->B) * 256};            // restriction, a block per tile (256)
-    const int threads[BP_COUNT] = {256, 256, 256, 64, 64, 64, 256, 256, 256, 256};
+    const int threads[BP_COUNT] = {256, 256, 256, 64, 64, 256, 256, 256, 256};
     for (int k = 0; k < BP_COUNT; ++k)
       first[(size_t)k * (b->cap + 1) + i + 1] = first[(size_t)k * (b->cap + 1) + i] + (int)blocks_for(items[k], threads[k]);
   }
   for (int k = 0; k < BP_COUNT; ++k) b->nblk[k] = first[(size_t)k * (b->cap + 1) + n];
+  b->yz_lds = 0;
+  for (const Refined *r : b->members) b->yz_lds = std::max(b->yz_lds, prolong_yz_lds(r->B));
   BCHK(hipMemcpy(b->d_cases, cd.data(), sizeof(CaseDev) * cd.size(), hipMemcpyHostToDevice));
   BCHK(hipMemcpy(b->d_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice));
   b->desc_key = key;
@@ -194,8 +196,7 @@ void batch_step(Batch *b, bool sample) {
   }
   for (int sub = 0; sub < 2; ++sub) {
     if (go(BP_PX)) hipLaunchKernelGGL(kb_prolong_x, dim3(b->nblk[BP_PX]), dim3(64), 0, st, C, F(BP_PX), nm, sub * 0.5);
-    if (go(BP_PY)) hipLaunchKernelGGL(kb_prolong_y, dim3(b->nblk[BP_PY]), dim3(64), 0, st, C, F(BP_PY), nm);
-    if (go(BP_PZ)) hipLaunchKernelGGL((kb_prolong_z<S>), dim3(b->nblk[BP_PZ]), dim3(64), 0, st, C, F(BP_PZ), nm, sub);
+    if (go(BP_PYZ)) hipLaunchKernelGGL((kb_prolong_yz<S>), dim3(b->nblk[BP_PYZ]), dim3(64), b->yz_lds, st, C, F(BP_PYZ), nm, sub);
     batched_sweep<S>(b, b->d_desc + (size_t)(2 + sub) * nc, b->nb_fine);
     for (Refined *r : b->members) r->f->sweeps++;
     // fine face flux and walls + loads: one launch

@@ -360,12 +360,13 @@ __global__ void k_tick(StepCtl *ctl) {
 __device__ __forceinline__ void loads_final_body(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
                               double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
   __shared__ double acc[MAX_ELEM * 5][64];
-  const int lane = threadIdx.x;  // 64 threads
-  for (int c = 0; c < nelem * 5; ++c) {
-    double s = 0.0;
-    for (int b = lane; b < nblocks; b += 64) s += partials[(long long)b * nelem * 5 + c];
-    acc[c][lane] = s;
-  }
+  const int lane = threadIdx.x;  // the first 64 threads of the block do the work
+  if (lane < 64)
+    for (int c = 0; c < nelem * 5; ++c) {
+      double s = 0.0;
+      for (int b = lane; b < nblocks; b += 64) s += partials[(long long)b * nelem * 5 + c];
+      acc[c][lane] = s;
+    }
   __syncthreads();
   if (lane == 0) {
     double *row = hist + (long long)ctl->row * ctl->width;
@@ -390,7 +391,7 @@ __device__ __forceinline__ void loads_final_body(const double *__restrict__ part
       ctl->row += 1;
     }
   }
-  if (!diag) return;
+  if (!diag || lane >= 64) return;
   for (int k = lane; k < DIAG_SLOTS; k += 64) {
     diag->s[k].u2max = 0ull;
     diag->s[k].drmax = 0ull;

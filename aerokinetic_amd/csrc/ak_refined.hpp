@@ -179,57 +179,65 @@ __global__ void k_prolong_x(const double *__restrict__ M0, const double *__restr
   if (i < n) prolong_x_body(M0, M1, frac, B, X, list[i]);
 }
 
-__device__ __forceinline__ void prolong_y_body(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y, long long t) {
-  const int dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4, cyn = 2 * B.nb[1] + 4;
-  if (t >= (long long)cxn * cyn * dz) return;
-  const int cx = (int)(t % cxn), cy = (int)((t / cxn) % cyn), qz = (int)(t / ((long long)cxn * cyn));
-  const int p = B.pad;
-  if (cx >= p && cx < cxn - p && cy >= p && cy < cyn - p) return;  // owned column
-  const int P = cy / 2 + 2, sd = cy & 1;
-  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = -2; b <= 2; ++b) {
-    const double w = prolong_weight(sd, b);
-    const double *q = X + (((long long)qz * dy + (P + b)) * cxn + cx) * 10;
-    for (int k = 0; k < 10; ++k) m[k] += w * q[k];
-  }
-  double *o = Y + t * 10;
-  for (int k = 0; k < 10; ++k) o[k] = m[k];
-}
-
-__global__ void k_prolong_y(const double *__restrict__ X, BoxGeom B, double *__restrict__ Y,
-                            const int32_t *__restrict__ list, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) prolong_y_body(X, B, Y, list[i]);
+// The y pass and the z pass + injection of the prolongation for a group of PZ_COLS fine
+// columns of the interface band (fine columns outside the owned box): the y pass
+// of a column at every donor z goes to LDS (dz * 10 doubles per column, dynamic shared
+// memory), then its fine cells (the fine cells outside the owned box, every z) interpolate
+// in z from there, are rescaled fine <- coarse, regularised and written into the slots the
+// next fine sweep reads.
+constexpr int PZ_COLS = 4;
+__host__ __device__ inline size_t prolong_yz_lds(const BoxGeom &B) {
+  return (size_t)PZ_COLS * (B.nb[2] + 6) * 10 * sizeof(double);
 }
 
 template <int S, bool PE>
-__device__ __forceinline__ void prolong_z_inject_body(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V, const double *__restrict__ Y, float scale, int *__restrict__ err, long long t) {
-  if (t >= (long long)L.nx * L.ny * L.nz) return;
-  const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
-  const int p = B.pad;
-  if (x >= p && x < L.nx - p && y >= p && y < L.ny - p) return;  // owned fine cell
-  const int pz = z + p;  // padded z coordinate of the periodic fine span
-  const int P = pz / 2 + 2, sd = pz & 1;
-  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int c = -2; c <= 2; ++c) {
-    const double w = prolong_weight(sd, c);
-    const double *q = Y + (((long long)(P + c) * L.ny + y) * L.nx + x) * 10;
-    for (int k = 0; k < 10; ++k) m[k] += w * q[k];
+__device__ __forceinline__ void prolong_yz_body(const double *__restrict__ X, typename St<S>::T *__restrict__ A, Desc L,
+                                                BoxGeom B, Levels V, float scale, int *__restrict__ err,
+                                                const int32_t *__restrict__ cols, int ncol, int group) {
+  extern __shared__ double yz[];  // [PZ_COLS][dz][10]
+  const int dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4;
+  const int c0 = group * PZ_COLS;
+  const int nc = ncol - c0 < PZ_COLS ? ncol - c0 : PZ_COLS;
+  for (int i = threadIdx.x; i < nc * dz; i += blockDim.x) {
+    const int k = i / dz, qz = i % dz;
+    const int col = cols[c0 + k], cx = col % cxn, cy = col / cxn;
+    const int P = cy / 2 + 2, sd = cy & 1;
+    double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = -2; b <= 2; ++b) {
+      const double w = prolong_weight(sd, b);
+      const double *q = X + (((long long)qz * dy + (P + b)) * cxn + cx) * 10;
+      for (int kk = 0; kk < 10; ++kk) m[kk] += w * q[kk];
+    }
+    double *o = yz + ((size_t)k * dz + qz) * 10;
+    for (int kk = 0; kk < 10; ++kk) o[kk] = m[kk];
   }
-  rescale(m, V.shear_cf, V.bulk_cf);
-  double g[NQ];
-  int e = 0;
-  populations_of(m, g, e);
-  scatter_in<S, PE>(A, L, x, y, z, scale, g);
-  if (e) atomicOr(err, e);
+  __syncthreads();
+  const int p = B.pad;
+  for (int i = threadIdx.x; i < nc * L.nz; i += blockDim.x) {
+    const int k = i / L.nz, z = i % L.nz;
+    const int col = cols[c0 + k], x = col % cxn, y = col / cxn;
+    const int pz = z + p;
+    const int P = pz / 2 + 2, sd = pz & 1;
+    double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = -2; c <= 2; ++c) {
+      const double w = prolong_weight(sd, c);
+      const double *q = yz + ((size_t)k * dz + (P + c)) * 10;
+      for (int kk = 0; kk < 10; ++kk) m[kk] += w * q[kk];
+    }
+    rescale(m, V.shear_cf, V.bulk_cf);
+    double g[NQ];
+    int e = 0;
+    populations_of(m, g, e);
+    scatter_in<S, PE>(A, L, x, y, z, scale, g);
+    if (e) atomicOr(err, e);
+  }
 }
 
 template <int S, bool PE>
-__global__ void k_prolong_z_inject(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, Levels V,
-                                   const double *__restrict__ Y, float scale, int *__restrict__ err,
-                                   const int32_t *__restrict__ list, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) prolong_z_inject_body<S, PE>(A, L, B, V, Y, scale, err, list[i]);
+__global__ __launch_bounds__(64) void k_prolong_yz(const double *__restrict__ X, typename St<S>::T *__restrict__ A,
+                                                   Desc L, BoxGeom B, Levels V, float scale, int *__restrict__ err,
+                                                   const int32_t *__restrict__ cols, int ncol) {
+  prolong_yz_body<S, PE>(X, A, L, B, V, scale, err, cols, ncol, (int)blockIdx.x);
 }
 
 // Face flux registers.  Face f = 2*axis + (side > 0), axis in {0 (x faces), 1 (y faces)}.
@@ -391,8 +399,18 @@ __device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, D
 template <int S, bool PE>
 __global__ __launch_bounds__(256) void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
                                                 const double *__restrict__ rc, const double *__restrict__ rf,
-                                                double inv_vol, float scale, float inv, int *__restrict__ err) {
-  reflux_body<S, PE>(A, L, B, rc, rf, inv_vol, scale, inv, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+                                                double inv_vol, float scale, float inv, int *__restrict__ err,
+                                                int nb_reflux, const double *__restrict__ part, int nparts, int nelem,
+                                                double fscale, double vscale, StepCtl *ctl, double *__restrict__ hist,
+                                                int sample) {
+  // the blocks after the reflux range: one block ends the step (its loads on sampled steps)
+  if ((int)blockIdx.x < nb_reflux) {
+    reflux_body<S, PE>(A, L, B, rc, rf, inv_vol, scale, inv, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  } else if (sample) {
+    loads_final_body(part, nparts, nelem, fscale, fscale, vscale, ctl, hist, nullptr);
+  } else if (threadIdx.x == 0) {
+    ctl->step += 1;
+  }
 }
 
 // Moments of every owned fine cell outside the fine virtual core (first restriction stage).

@@ -17,9 +17,9 @@ struct Refined {
   FreeState F{};
   double dxc = 0, dtc = 0, dxf = 0, dtf = 0, span = 0, U = 0;
   double *d_before = nullptr, *d_after = nullptr, *d_regc = nullptr, *d_regf = nullptr, *d_part = nullptr;
-  double *d_px = nullptr, *d_py = nullptr;  // prolongation passes
-  int32_t *d_lpx = nullptr, *d_lpy = nullptr, *d_lpz = nullptr;  // their work items
-  int n_lpx = 0, n_lpy = 0, n_lpz = 0;
+  double *d_px = nullptr;                   // prolongation x pass
+  int32_t *d_lpx = nullptr, *d_cols = nullptr;  // x-pass work items; fine interface-band columns
+  int n_lpx = 0, n_cols = 0;
   int *d_err = nullptr;
   FluxTable T{};
   FluxTable *d_T = nullptr;
@@ -101,32 +101,27 @@ void build_flux_table(FluxTable &T) {
   }
 }
 
-// Work items of the three prolongation passes: the indices for which the bodies
-// (prolong_x_body, prolong_y_body, prolong_z_inject_body) do work, same conditions.
+// Work items of the prolongation: the x-pass indices for which prolong_x_body does work
+// (same condition), and the fine columns outside the owned box (y/z passes + injection).
 int build_prolong_lists(Refined *r) {
   const BoxGeom &B = r->B;
-  const int dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4, cyn = 2 * B.nb[1] + 4, p = B.pad;
+  const int dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4, p = B.pad;
   const Desc &L = r->f->L;
-  if ((long long)cxn * cyn * dz >= INT32_MAX || (long long)L.nx * L.ny * L.nz >= INT32_MAX)
+  if ((long long)cxn * dy * dz >= INT32_MAX || (long long)L.nx * L.ny >= INT32_MAX)
     return fail(nullptr, "refined box too large for 32-bit prolongation work items");
-  std::vector<int32_t> lx, ly, lz;
+  if (prolong_yz_lds(B) > 65536) return fail(nullptr, "periodic span too long for the prolongation tiles");
+  std::vector<int32_t> lx, cols;
   for (int qz = 0; qz < dz; ++qz)
     for (int qy = 0; qy < dy; ++qy)
       for (int cx = 0; cx < cxn; ++cx)
         if (!(cx >= p && cx < cxn - p && qy > 4 && qy < B.nb[1] + 1))
           lx.push_back((int32_t)(((long long)qz * dy + qy) * cxn + cx));
-  for (int qz = 0; qz < dz; ++qz)
-    for (int cy = 0; cy < cyn; ++cy)
-      for (int cx = 0; cx < cxn; ++cx)
-        if (!(cx >= p && cx < cxn - p && cy >= p && cy < cyn - p))
-          ly.push_back((int32_t)(((long long)qz * cyn + cy) * cxn + cx));
-  for (int z = 0; z < L.nz; ++z)
-    for (int y = 0; y < L.ny; ++y)
-      for (int x = 0; x < L.nx; ++x)
-        if (!(x >= p && x < L.nx - p && y >= p && y < L.ny - p))
-          lz.push_back((int32_t)(((long long)z * L.ny + y) * L.nx + x));
-  r->n_lpx = (int)lx.size(); r->n_lpy = (int)ly.size(); r->n_lpz = (int)lz.size();
-  return (upload(r->c, &r->d_lpx, lx) || upload(r->c, &r->d_lpy, ly) || upload(r->c, &r->d_lpz, lz)) ? -1 : 0;
+  for (int y = 0; y < L.ny; ++y)
+    for (int x = 0; x < L.nx; ++x)
+      if (!(x >= p && x < L.nx - p && y >= p && y < L.ny - p)) cols.push_back((int32_t)((long long)y * L.nx + x));
+  r->n_lpx = (int)lx.size();
+  r->n_cols = (int)cols.size();
+  return (upload(r->c, &r->d_lpx, lx) || upload(r->c, &r->d_cols, cols)) ? -1 : 0;
 }
 
 template <int S>
@@ -188,12 +183,12 @@ void phase_before_fine(Refined *r, int sub) {
   const double frac = sub * 0.5;
   // 64-thread blocks spread the few thousand items over every CU
   hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(r->n_lpx, 64)), dim3(64), 0, f->stream, r->d_before, r->d_after, frac, r->B, r->d_px, r->d_lpx, r->n_lpx);
-  hipLaunchKernelGGL(k_prolong_y, dim3(blocks_for(r->n_lpy, 64)), dim3(64), 0, f->stream, r->d_px, r->B, r->d_py, r->d_lpy, r->n_lpy);
-  const unsigned nzb = blocks_for(r->n_lpz, 64);
+  const dim3 gyz(blocks_for(r->n_cols, PZ_COLS));
+  const size_t lds = prolong_yz_lds(r->B);
   if (f->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_prolong_z_inject<S, true>), dim3(nzb), dim3(64), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err, r->d_lpz, r->n_lpz);
+    hipLaunchKernelGGL((k_prolong_yz<S, true>), gyz, dim3(64), lds, f->stream, (const double *)r->d_px, (T_ *)f->pop, f->L, r->B, r->V, f->scale, r->d_err, r->d_cols, r->n_cols);
   else
-    hipLaunchKernelGGL((k_prolong_z_inject<S, false>), dim3(nzb), dim3(64), 0, f->stream, (T_ *)f->pop, f->L, r->B, r->V, r->d_py, f->scale, r->d_err, r->d_lpz, r->n_lpz);
+    hipLaunchKernelGGL((k_prolong_yz<S, false>), gyz, dim3(64), lds, f->stream, (const double *)r->d_px, (T_ *)f->pop, f->L, r->B, r->V, f->scale, r->d_err, r->d_cols, r->n_cols);
 }
 
 // fine substep `sub` after its sweep: fine face flux, walls + load partials
@@ -222,20 +217,22 @@ void phase_end(Refined *r, bool sample) {
   using T_ = typename St<S>::T;
   const unsigned cwave_blocks = blocks_for(4LL * r->nt * c->nz * 64, 256);
   const double inv_volc = 1.0 / (r->dxc * r->dxc * r->dxc), volf = r->dxf * r->dxf * r->dxf;
+  // impulse over the two substeps: F dx_f^3 (dx_f/dt_f) / dt_c; q_area = span / 2.  The refined
+  // history carries loads only (field diagnostics are read on demand).  The reflux launch's
+  // last block folds the loads (sampled step) or ticks the step; the loads read only the
+  // wall partials, the reflux only the coarse populations.
+  const double fscale = volf * (r->dxf / r->dtf) / r->dtc / (0.5 * r->span);
+  const int nparts = f->nlinks ? 2 * r->wall_blocks : 0;
+  const dim3 grid(cwave_blocks + 1), block(256);
   if (c->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_reflux<S, true>), dim3(cwave_blocks), dim3(256), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
+    hipLaunchKernelGGL((k_reflux<S, true>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf,
+                       inv_volc, c->scale, c->inv_scale, r->d_err, (int)cwave_blocks, (const double *)r->d_part, nparts,
+                       f->nelem, fscale, volf, c->d_ctl, c->d_hist, sample ? 1 : 0);
   else
-    hipLaunchKernelGGL((k_reflux<S, false>), dim3(cwave_blocks), dim3(256), 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf, inv_volc, c->scale, c->inv_scale, r->d_err);
+    hipLaunchKernelGGL((k_reflux<S, false>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf,
+                       inv_volc, c->scale, c->inv_scale, r->d_err, (int)cwave_blocks, (const double *)r->d_part, nparts,
+                       f->nelem, fscale, volf, c->d_ctl, c->d_hist, sample ? 1 : 0);
   launch_restrict<S>(r);
-  if (sample) {
-    // impulse over the two substeps: F dx_f^3 (dx_f/dt_f) / dt_c; q_area = span / 2.  The
-    // refined history carries loads only (field diagnostics are read on demand).
-    const double fscale = volf * (r->dxf / r->dtf) / r->dtc / (0.5 * r->span);
-    hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, c->stream, r->d_part, f->nlinks ? 2 * r->wall_blocks : 0,
-                       f->nelem, fscale, fscale, volf, c->d_ctl, c->d_hist, (DiagAcc *)nullptr);
-  } else {
-    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->d_ctl);
-  }
 }
 
 // One coarse step of one case; its number and history row come from the coarse lattice's
@@ -319,7 +316,6 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
   if (dalloc(r->c, &r->d_before, ndon) || dalloc(r->c, &r->d_after, ndon) || dalloc(r->c, &r->d_regc, nreg) ||
       dalloc(r->c, &r->d_regf, 4 * nreg) ||
       dalloc(r->c, &r->d_px, (size_t)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (nz + 6) * 10) ||
-      dalloc(r->c, &r->d_py, (size_t)(2 * r->B.nb[0] + 4) * (2 * r->B.nb[1] + 4) * (nz + 6) * 10) ||
       dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
     if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
