@@ -156,6 +156,11 @@ class Plan:
         return math.prod(v + 2 for v in storage) * ((2 * 27 * 8 + 4 * 8 + 8 + extra) * 1.3 + 128) / 2**30
 
     @staticmethod
+    def _storage(shape):
+        """Lattice of the exterior farfield: the physical box padded by one cell in x and y."""
+        return (shape[0] + 2, shape[1] + 2, shape[2])
+
+    @staticmethod
     def _notes(flow, grid, n, tau, exterior_shape=None):
         notes = ["Uniform isotropic grid; no mesh refinement or wall model.",
                  "Accuracy is unvalidated until grid/domain/span/Mach/statistical convergence and tunnel comparison.",
@@ -166,7 +171,9 @@ class Plan:
             if grid.farfield == EXTERIOR_FARFIELD:
                 if flow.sgs != "none":
                     raise ValueError("Exterior FD currently supports molecular viscosity with sgs=none only")
-                notes.append(f"Research exterior_regularized_fd_v1; auxiliary storage shape={exterior_shape}.")
+                notes.append(f"Research exterior_regularized_fd_v1; physical shape unchanged, auxiliary storage "
+                             f"shape={exterior_shape}; explicit exterior source. Uniform grids only; no validated "
+                             "airfoil accuracy or full viscous NSCBC.")
             else:
                 notes.append("Experimental characteristic_transport_v1 farfield with linearized transverse transport.")
         if n < 256:
@@ -194,7 +201,7 @@ class Plan:
         # Rounding slack is split evenly on both transverse sides.
         origin = (xmin - grid.upstream_chords, (ymin + ymax) / 2 - shape[1] / (2 * n))
         nu, tau, dt, dt_s = cls._lattice(flow, n)
-        notes = cls._notes(flow, grid, n, tau)
+        notes = cls._notes(flow, grid, n, tau, cls._storage(shape))
         return cls(shape, origin, 1 / n, shape[2] / n, nu, 1 / tau, dt, dt_s,
                    cls._memory(shape, grid.farfield), notes)
 
@@ -211,7 +218,8 @@ class Plan:
         le = section.leading_edge_normalized(flow.alpha_deg)
         origin = (float(le[0] - leading_edge_cell[0] / n), float(le[1] - leading_edge_cell[1] / n))
         nu, tau, dt, dt_s = cls._lattice(flow, n)
-        notes = cls._notes(flow, grid, n, tau) + (f"Explicit lattice {shape} with leading edge at cell {tuple(leading_edge_cell)}.",)
+        notes = cls._notes(flow, grid, n, tau, cls._storage(shape)) + (
+            f"Explicit lattice {shape} with leading edge at cell {tuple(leading_edge_cell)}.",)
         return cls(shape, origin, 1 / n, shape[2] / n, nu, 1 / tau, dt, dt_s,
                    cls._memory(shape, grid.farfield), notes)
 

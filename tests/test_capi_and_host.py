@@ -108,3 +108,29 @@ def test_explicit_plan_puts_leading_edge_on_requested_cell():
     np.testing.assert_allclose((le[0] - plan.origin[0]) / plan.dx, 128)
     np.testing.assert_allclose((le[1] - plan.origin[1]) / plan.dx, 128)
     assert flow.lattice_speed == pytest.approx(0.1 / np.sqrt(3))
+
+
+def test_exterior_plan_keeps_the_physical_domain_and_rejects_unsupported_models():
+    """tests/test_exterior.py:34-48 of the reference: the exterior farfield pads the storage,
+    not the physical plan; GPU target, SGS and refined plans are rejected."""
+    from dataclasses import replace
+    from aerokinetic_amd.config import Flow, Grid, Plan
+    from aerokinetic_amd.geometry import Section, naca4
+    from aerokinetic_amd.project import Project
+    from aerokinetic_amd.refined_plan import RefinedPlan, RefinementSettings
+    mode = "exterior_regularized_fd_v1"
+    p = Project(Section((naca4("0012", 31),)), Flow(reynolds=30000),
+                Grid(cells_per_chord=16, upstream_chords=1, downstream_chords=2, transverse_chords=1))
+    legacy = p.plan()
+    explicit = replace(p, grid=replace(p.grid, farfield=mode))
+    new = explicit.plan()
+    for name in ("shape", "origin", "dx", "span", "viscosity", "dt_convective", "dt_seconds"):
+        assert getattr(new, name) == getattr(legacy, name)
+    assert new.estimated_memory_gib > legacy.estimated_memory_gib
+    assert any("storage shape=(" in n for n in new.notes)
+    with pytest.raises(ValueError, match="CPU"):
+        replace(p.grid, farfield=mode, target="gpu")
+    with pytest.raises(ValueError, match="sgs=none"):
+        Plan.create(p.section, replace(p.flow, sgs="smagorinsky"), explicit.grid)
+    with pytest.raises(ValueError, match="uniform"):
+        RefinedPlan.create(explicit, RefinementSettings())
