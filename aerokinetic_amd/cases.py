@@ -58,6 +58,10 @@ class UniformCase:
 # Re 1e4, Ma 0.1, closed trailing edge, characteristic farfield, seeded perturbation 1e-4.
 NACA0012_C32_384x256x8 = UniformCase("naca0012 c32 384x256x8 a4 Re1e4 Ma0.1", (384, 256, 8), 32, 4.0, 1e4, 0.1,
                                      (128, 128))
+# The same plain case with the exterior farfield (exterior_regularized_fd_v1): the lattice is the
+# 384x256x8 box padded by one cell in x and y.
+NACA0012_C32_384x256x8_EXTERIOR = UniformCase("naca0012 c32 384x256x8 a4 Re1e4 Ma0.1 exterior", (384, 256, 8), 32,
+                                              4.0, 1e4, 0.1, (128, 128), farfield="exterior_regularized_fd_v1")
 # Precision-study lattice: 1536x1024x64 (~100M cells), chord 128 at 8 deg, Re 1e5, Ma 0.1.
 NACA0012_C128_1536x1024x64 = UniformCase("naca0012 c128 1536x1024x64 a8 Re1e5 Ma0.1", (1536, 1024, 64), 128, 8.0,
                                          1e5, 0.1, (512, 512))

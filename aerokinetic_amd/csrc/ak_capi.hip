@@ -859,7 +859,13 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
     }
     if (sample) s->hist_rows++;
   }
+  hipEvent_t last = nullptr;
+  HIPCHK(s, hipEventCreate(&last));
+  HIPCHK(s, hipEventRecord(last, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));
+  float span = 0.f;
+  if (nsteps > 0) HIPCHK(s, hipEventElapsedTime(&span, ev[0], last));
+  (void)hipEventDestroy(last);
   double sweep = 0.0, bnd = 0.0;
   for (int k = 0; k < nsteps; ++k) {
     float a = 0.f, b = 0.f;
@@ -872,6 +878,7 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
   out[0] = sweep;
   out[1] = bnd;
   out[2] = nsteps;
+  out[3] = span;
   return 0;
 }
 

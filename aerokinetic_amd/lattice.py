@@ -94,12 +94,13 @@ class Lattice:
     def run(self, nsteps, first_step, sample_every):
         _lib.check(self.lib.ak_run(self.h, int(nsteps), int(first_step), int(sample_every)), self.h, "ak_run")
 
-    def run_profiled(self, nsteps, first_step, sample_every):
-        """Advance like run(); returns (sweep ms, boundary ms) summed over the steps."""
-        out = np.zeros(3)
+    def run_profiled(self, nsteps, first_step, sample_every, total=False):
+        """Advance like run(); returns (sweep ms, boundary ms) summed over the steps, and with
+        total=True also the milliseconds of all the steps (direct launches)."""
+        out = np.zeros(4)
         _lib.check(self.lib.ak_run_profiled(self.h, int(nsteps), int(first_step), int(sample_every), _lib.dptr(out)),
                    self.h, "ak_run_profiled")
-        return float(out[0]), float(out[1])
+        return (float(out[0]), float(out[1]), float(out[3])) if total else (float(out[0]), float(out[1]))
 
     def read_history(self, max_rows=1 << 20):
         out = np.zeros((max_rows, self.hist_width)) if max_rows <= 4096 else None

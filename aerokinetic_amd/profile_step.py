@@ -4,7 +4,8 @@
     python -m aerokinetic_amd.profile_step --workload two-level --storage fp64 --steps 40
     python -m aerokinetic_amd.profile_step --workload copy --copy-gib 4 --steps 10
 
-Workloads: `uniform-c32` (384x256x8), `uniform-100m` (1536x1024x64) and
+Workloads: `uniform-c32` (384x256x8), `exterior-c32` (the same box with the exterior
+farfield), `uniform-100m` (1536x1024x64) and
 `two-level` (coarse 256x192x8 + fine 160x96x16), all NACA0012 (see
 aerokinetic_amd.cases); `copy`, a device-to-device copy of a buffer of known size
 (the calibration kernel for memory counters); `serve`, the batched service over
@@ -37,11 +38,11 @@ def run_uniform(case, storage, steps, warmup, sample_every, graphs=True):
     s.advance(steps, sample_every)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    sweep_ms, bnd_ms = s.lattice.run_profiled(steps, s.steps, sample_every)
+    sweep_ms, bnd_ms, tot_ms = s.lattice.run_profiled(steps, s.steps, sample_every, total=True)
     s.steps += steps
     s.lattice.read_history()
     _log(f"steps {steps}: wall {dt / steps * 1e3:.4f} ms/step (graphs {graphs}); direct launches with events: "
-         f"sweep {sweep_ms / steps:.4f} ms, boundary {bnd_ms / steps:.4f} ms")
+         f"sweep {sweep_ms / steps:.4f} ms, boundary {bnd_ms / steps:.4f} ms, step {tot_ms / steps:.4f} ms")
     s.lattice.close()
 
 
@@ -108,7 +109,8 @@ def run_serve(storage, cases, rate, chunk, window, max_steps):
 def main(argv=None):
     from . import cases
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--workload", choices=("uniform-c32", "uniform-100m", "two-level", "copy", "serve"), required=True)
+    ap.add_argument("--workload", choices=("uniform-c32", "exterior-c32", "uniform-100m", "two-level", "copy", "serve"),
+                    required=True)
     ap.add_argument("--storage", choices=("fp64", "fp32", "fp16"), default="fp64")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -126,7 +128,8 @@ def main(argv=None):
     elif a.workload == "two-level":
         run_two_level(cases.NACA0012_REFINED_256x192x8, a.storage, a.steps, a.warmup, a.sample_every, not a.no_graphs)
     else:
-        case = cases.NACA0012_C32_384x256x8 if a.workload == "uniform-c32" else cases.NACA0012_C128_1536x1024x64
+        case = {"uniform-c32": cases.NACA0012_C32_384x256x8, "exterior-c32": cases.NACA0012_C32_384x256x8_EXTERIOR,
+                "uniform-100m": cases.NACA0012_C128_1536x1024x64}[a.workload]
         run_uniform(case, a.storage, a.steps, a.warmup, a.sample_every, not a.no_graphs)
     return 0
 

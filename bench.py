@@ -17,6 +17,10 @@ sweep; its algorithmic bytes per launch = updated fine cells x 2 x 27 x bytes pe
 population (one read + one write per population, AA pattern), divided by its mean
 duration from HIP events recorded on the stream it is launched on.
 
+exterior_farfield: the plain 384x256x8 NACA0012 case (chord 32, 4 deg, Re 1e4, Ma 0.1)
+with the exterior farfield (a 386x258x8 padded lattice whose ring is rebuilt every step):
+step rate and the split of a step into sweep, boundary and ring phases.
+
 uniform_sweep: the same step on a uniform 1536x1024x64 lattice (100.7M cells,
 NACA0012 chord 128 at 8 deg, Re 1e5, characteristic farfield) in FP64, FP32 and
 FP16 shifted storage -- the HBM-roofline view of the fused AA sweep at full size.
@@ -147,6 +151,43 @@ def uniform_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
     return out
 
 
+def exterior_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
+    """The plain uniform case with the exterior farfield: step rate over the physical cells,
+    and the split of a step into the sweep, the boundary phase and the rest (the exterior
+    ring's band, rebuild, predictor stages and accumulators, plus the step's bookkeeping)."""
+    import numpy as np
+    import torch
+    from aerokinetic_amd.solver import Solver
+    section, flow, grid, plan = case.build()
+    s = Solver(section, flow, grid, storage=storage, plan=plan)
+    cells = int(np.prod(plan.shape))
+    X, Y, Z = s.storage_shape
+    ring = (2 * X + 2 * (Y - 2)) * Z
+    swept = int((~s._solid_xy).sum()) * Z + (X * Y - plan.shape[0] * plan.shape[1]) * Z - ring
+    s.advance(warmup, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = s.advance(steps, 1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    sweep_ms, bnd_ms, tot_ms = s.lattice.run_profiled(profile_steps, s.steps, 1, total=True)
+    s.steps += profile_steps
+    s.lattice.read_history()
+    sweep = sweep_ms / profile_steps / 1e3
+    out = {"storage": storage, "mlups": cells * steps / dt / 1e6, "ms_per_step": dt / steps * 1e3,
+           "storage_shape": list(s.storage_shape), "ring_cells": ring,
+           "sweep_ms": sweep * 1e3, "boundary_ms": bnd_ms / profile_steps,
+           "ring_and_bookkeeping_ms": (tot_ms - sweep_ms - bnd_ms) / profile_steps,
+           "step_ms_direct": tot_ms / profile_steps,
+           "sweep_roofline": roofline(swept * 2 * 27 * BYTES[storage], sweep, None, copy_gbs),
+           "last_sample": {k: recs[-1][k] for k in ("step", "cl", "cd", "cm", "exterior_source_mass_lattice",
+                                                     "wall_mass_exchange_lattice")}}
+    s.lattice.close()
+    del s
+    torch.cuda.empty_cache()
+    return out
+
+
 def compare_loads(results, ref="fp64"):
     """Loads (and, where recorded, rho/u fields) of each storage against the FP64-storage run."""
     import numpy as np
@@ -246,6 +287,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=40)
     ap.add_argument("--uniform-storages", default="fp64,fp32,fp16", help="empty to skip the 100M-cell sweep")
     ap.add_argument("--uniform-steps", type=int, default=100)
+    ap.add_argument("--exterior-storages", default="fp64,fp16", help="empty to skip the exterior-farfield case")
+    ap.add_argument("--exterior-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve-cases", type=int, default=48, help="0 skips the serving measurement")
     ap.add_argument("--serve-rate", type=float, default=64.0)
@@ -279,6 +322,13 @@ def main():
         r = uni[st]
         log(f"  {r['mlups']:.0f} MLUPS, sweep {r['sweep_ms']:.3f} ms ({100 * r['roofline']['frac']:.1f}% HBM)")
     uni_cmp = compare_loads(uni)
+    ext = {}
+    for st in [s for s in args.exterior_storages.split(",") if s]:
+        log(f"exterior farfield 384x256x8 {st}")
+        ext[st] = exterior_gpu(cases.NACA0012_C32_384x256x8_EXTERIOR, st, args.exterior_steps, 100, 50, copy_gbs)
+        r = ext[st]
+        log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.4f} ms/step; sweep {r['sweep_ms']:.4f} ms, boundary "
+            f"{r['boundary_ms']:.4f} ms, ring + bookkeeping {r['ring_and_bookkeeping_ms']:.4f} ms")
     serving = serving_gpu(args) if args.serve_cases > 0 else None
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cpu = None
@@ -303,6 +353,9 @@ def main():
             "uniform_sweep": {"workload": ucase.name + ", characteristic farfield, loads every step",
                               "by_storage": {st: strip(r) for st, r in uni.items()},
                               "vs_fp64": uni_cmp},
+            "exterior_farfield": {"workload": cases.NACA0012_C32_384x256x8_EXTERIOR.name + " (exterior_regularized_fd_v1; "
+                                              "MLUPS over the physical cells)",
+                                  "by_storage": ext},
             "serving": serving}
     print(json.dumps(line))
 

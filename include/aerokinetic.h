@@ -122,7 +122,8 @@ int ak_set_graph_mode(void *handle, int enable);
 /* ak_run with HIP events (measurement only; direct launches; synchronises at the end):
  * each sweep launch records its own kernel start and end, each boundary phase is
  * bracketed on the lattice's stream.
- * out = {total sweep milliseconds, total boundary milliseconds, steps}. */
+ * out = {total sweep milliseconds, total boundary milliseconds, steps, milliseconds of all the
+ * steps from the first boundary phase to the end of the last step}. */
 int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_every, double *out);
 
 /* Copy up to max_rows recorded rows to host (row length ak_history_width) and
