@@ -45,6 +45,7 @@ struct Lattice {
   float scale = 1.0f, inv_scale = 1.0f;
   Desc L{};
   size_t pop_elems = 0, pop_bytes = 0, elem_size = 8;
+  bool row_groups = true;  // [z][y][q][x] storage; false: one plane per direction
   void *pop = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -521,16 +522,19 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
   long long P = ((long long)nx + 2 + align - 1) / align * align;
   s->L.nx = nx; s->L.ny = ny; s->L.nz = nz;
   s->L.base = align;
-  if (storage == 2) {
-    // FP16: the 27 direction rows of one y row are adjacent ([z][y][q][x]), so a wave's 27
-    // loads fall in one 27-row group instead of 27 planes far apart; 2-byte elements gain
-    // most from it (measured on the 27-stream access pattern: 4.8 -> 5.3 TB/s; FP64 loses)
+  // The 27 direction rows of one y row are adjacent ([z][y][q][x]) whenever a z plane of
+  // that layout fits the sweep's 32-bit in-plane byte offsets: a wave's 27 loads and 27
+  // stores then fall in one 27-row group instead of 27 planes far apart.  Measured on the
+  // sweep's exact in-place access pattern (even + odd, 1536x1024x64, 64x4 tiles), TB/s
+  // with one plane per direction -> interleaved rows: FP64 5.02 -> 5.31, FP32 4.87 ->
+  // 5.20, FP16 4.57 -> 5.01.  Larger planes keep one plane per direction ([q][z][y][x]).
+  s->row_groups = (unsigned long long)27 * P * (ny + 2) * s->elem_size < (1ull << 31);
+  if (s->row_groups) {
     s->L.QS = P;
     s->L.P = 27 * P;
     s->L.PS = s->L.P * (ny + 2);
     s->pop_elems = (size_t)(s->L.base + (long long)nz * s->L.PS + align);
   } else {
-    // FP32 / FP64: one plane per direction ([q][z][y][x])
     s->L.P = P;
     s->L.PS = P * (ny + 2);
     s->L.QS = s->L.PS * nz;
@@ -1048,7 +1052,7 @@ struct StateHeader {
   long long sweeps;
   unsigned long long pop_bytes;
 };
-constexpr int STATE_VERSION = 2;
+constexpr int STATE_VERSION = 3;
 
 struct StatePart {
   void *dev;

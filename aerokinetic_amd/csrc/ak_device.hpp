@@ -4,10 +4,11 @@
 //   element (q, x, y, z) at  base + q*QS + z*PS + (y+1)*P + x,
 //   x in [-1, nx], y in [-1, ny] (one ghost cell each side), z in [0, nz) periodic.
 // The x = -1 ghost of a row is the last slot of the previous memory row (a row holds
-// >= nx+2 slots), so interior rows start on a 128-byte boundary.  FP32/FP64 keep one plane
-// per direction (QS = nz*PS); FP16 interleaves the 27 direction rows of each y row
-// (QS = row pitch, P = 27 row pitches).  Kernels only use the strides.  Every wavefront
-// of the sweep reads and writes 64 consecutive x of one direction.
+// >= nx+2 slots), so interior rows start on a 128-byte boundary.  The 27 direction rows of
+// each y row are interleaved (QS = row pitch, P = 27 row pitches) unless a z plane of that
+// layout exceeds 2 GiB; such lattices keep one plane per direction (QS = nz*PS).  Kernels
+// only use the strides.  Every wavefront of the sweep reads and writes 64 consecutive x of
+// one direction.
 //
 // AA pattern.  After an even sweep (and initially) the state is in "post-even"
 // form: f_out(x, i) is stored at (x, opposite(i)).  An odd sweep gathers its
