@@ -406,6 +406,25 @@ void ext_initial_launches(Lattice *s) {
                      (C)s->ws, (C)s->wb, s->scale, s->inv_scale);
 }
 
+// Every link of the legacy farfield is independent (a corner's outflow recomputes the
+// bounce-back value it needs), so their order only serves the memory system: velocity
+// bounce-back links first, then by z, row and x, with the links of one cell adjacent (they
+// share its 27 loads), so that the lanes of a wave take a few neighbouring cells of one z
+// plane rather than cells 64 planes apart.
+void order_links(int nx, int ny, int nz, std::vector<int32_t> &xs, std::vector<int32_t> &zs,
+                 std::vector<int8_t> &dir, std::vector<int8_t> &knd, std::vector<double> &pdf,
+                 std::vector<double> &pdfnd) {
+  std::vector<size_t> order(xs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  auto key = [&](size_t i) {
+    const long long y = xs[i] >> 16, x = xs[i] & 0xffff;
+    return ((((long long)knd[i] * nz + zs[i]) * (ny + 2) + y) * (nx + 2) + x) * 27 + dir[i];
+  };
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key(a) < key(b); });
+  auto perm = [&](auto &v) { auto c = v; for (size_t i = 0; i < order.size(); ++i) v[i] = c[order[i]]; };
+  perm(xs); perm(zs); perm(dir); perm(knd); perm(pdf); perm(pdfnd);
+}
+
 int set_exterior(Lattice *s) {
   const int nx = s->nx, ny = s->ny, nz = s->nz;
   std::vector<uint8_t> m((size_t)(nx + 2) * (ny + 2));
@@ -444,8 +463,9 @@ int set_exterior(Lattice *s) {
         knd.push_back(0);
       }
   }
+  std::vector<double> none(xs.size(), 0.0), none2(xs.size(), 0.0);
+  order_links(nx, ny, nz, xs, zs, dir, knd, none, none2);
   s->nleg = (int)xs.size();
-  const std::vector<double> none(xs.size(), 0.0);
   const size_t rz = (size_t)s->ext_R2 * nz * 4, bz = (size_t)s->ext_B2 * nz * 4;
   const std::vector<double> zb(bz, 0.0), zr(rz, 0.0), zrows((size_t)2 * nz * nx, 0.0),
       zpart((size_t)s->ext_ring_blocks * 5, 0.0);
@@ -699,11 +719,15 @@ int ak_set_farfield(void *handle, int kind, double U) {
       }
     s->ffp.U = U;
     s->ffp.rho0 = 1.0;
-    // items: every exterior ghost cell of each object, with the links entering the fluid
+    // items: every exterior ghost cell of each object, with the links entering the fluid.
+    // Within an object the ghost cells of one z follow each other along the face, so the
+    // lanes of a wave read consecutive x of a y face (coalesced) or consecutive rows of an
+    // x face, never 64 z planes far apart.
     std::vector<int32_t> g, gz, mask;
     std::vector<int8_t> obj;
     for (int ob = 0; ob < 8; ++ob) {
       const FFObj &o = s->ffp.o[ob];
+      std::vector<int32_t> cg, cm;
       for (int gx = -1; gx <= nx; ++gx)
         for (int gy = -1; gy <= ny; ++gy) {
           const int tx = gx < 0 ? -1 : gx >= nx ? 1 : 0, ty = gy < 0 ? -1 : gy >= ny ? 1 : 0;
@@ -714,12 +738,15 @@ int ak_set_farfield(void *handle, int kind, double U) {
             if (fx >= 0 && fx < nx && fy >= 0 && fy < ny) m |= 1 << d;
           }
           if (!m) continue;
-          for (int z = 0; z < nz; ++z) {
-            g.push_back((gx + 1) | ((gy + 1) << 16));
-            gz.push_back(z);
-            mask.push_back(m);
-            obj.push_back((int8_t)ob);
-          }
+          cg.push_back((gx + 1) | ((gy + 1) << 16));
+          cm.push_back(m);
+        }
+      for (int z = 0; z < nz; ++z)
+        for (size_t c = 0; c < cg.size(); ++c) {
+          g.push_back(cg[c]);
+          gz.push_back(z);
+          mask.push_back(cm[c]);
+          obj.push_back((int8_t)ob);
         }
     }
     s->nff = (int)g.size();
@@ -774,12 +801,7 @@ int ak_set_farfield(void *handle, int kind, double U) {
             pdfnd.push_back(kind_ == 1 ? e : 0.0);
           }
       }
-    // velocity bounce-back links first: the outflow of a corner reads their values
-    std::vector<size_t> order(xs.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return knd[a] < knd[b]; });
-    auto perm = [&](auto &v) { auto c = v; for (size_t i = 0; i < order.size(); ++i) v[i] = c[order[i]]; };
-    perm(xs); perm(zs); perm(dir); perm(knd); perm(pdf); perm(pdfnd);
+    order_links(nx, ny, nz, xs, zs, dir, knd, pdf, pdfnd);
     s->nleg = (int)xs.size();
     if (upload(s, &s->d_legx, xs) || upload(s, &s->d_legz, zs) || upload(s, &s->d_legdir, dir) ||
         upload(s, &s->d_legkind, knd) || upload(s, &s->d_pdf, pdf) || upload(s, &s->d_pdfnd, pdfnd))
