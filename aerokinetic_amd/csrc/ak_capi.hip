@@ -65,6 +65,7 @@ struct Lattice {
   int32_t *d_ffg = nullptr, *d_ffz = nullptr, *d_ffmask = nullptr;
   int8_t *d_ffobj = nullptr;
   double *d_inc = nullptr, *d_der = nullptr, *d_rows = nullptr;
+  uint8_t *d_fc = nullptr;  // x-face cache of the characteristic farfield (storage type)
   long long *d_clock = nullptr;
   int nleg = 0;
   int32_t *d_legx = nullptr, *d_legz = nullptr;
@@ -203,7 +204,10 @@ void launch_boundary(Lattice *s, bool loads) {
   using T = typename St<S>::T;
   T *A = (T *)s->pop;
   if (s->ff_kind == AK_FARFIELD_CHARACTERISTIC && s->nff) {
-    FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff};
+    if (s->d_fc)
+      hipLaunchKernelGGL((k_face_fill<S, PE>), dim3(blocks_for(2LL * s->ny * s->nz, 256)), dim3(256), 0, s->stream,
+                         (const T *)A, s->L, (T *)s->d_fc);
+    FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff, s->d_fc};
     hipLaunchKernelGGL((k_farfield<S, PE>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, A, s->L, I,
                        s->ffp, s->scale, s->inv_scale);
   } else if ((s->ff_kind == AK_FARFIELD_LEGACY || s->ff_kind == AK_FARFIELD_EXTERIOR) && s->nleg) {
@@ -690,6 +694,8 @@ int ak_set_farfield(void *handle, int kind, double U) {
     return fail(s, "the exterior farfield needs a padded lattice of at least 6 x 6 x 4");
   s->ff_kind = kind;
   s->U = U;
+  dfree(s, s->d_fc);
+  s->d_fc = nullptr;
   const int nx = s->nx, ny = s->ny, nz = s->nz;
   if (kind == AK_FARFIELD_CHARACTERISTIC) {
     // objects in the order normals (-1,-1),(-1,0),(-1,1),(0,-1),(0,1),(1,-1),(1,0),(1,1)
@@ -757,7 +763,8 @@ int ak_set_farfield(void *handle, int kind, double U) {
     if (upload(s, &s->d_inc, zeros) || upload(s, &s->d_der, zeros) || upload(s, &s->d_clock, zc)) return -1;
     std::vector<double> rows((size_t)2 * nz * nx, 0.0);
     if (upload(s, &s->d_rows, rows)) return -1;
-    FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff};
+    if (nx >= 2 && dalloc(s, &s->d_fc, (size_t)2 * NQ * ny * nz * s->elem_size)) return -1;
+    FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff, nullptr};
     const long long nr = 2LL * nz * nx;
     switch (s->storage) {
       case 0:

@@ -445,7 +445,19 @@ struct FFItems {
   long long *clock;
   const double *initial;   // glancing trace rows: [side][z][x], side 0 = y low, 1 = y high
   int n;
+  const void *fc;          // x-face cache of the last sweep (storage type), or null
 };
+
+// x-face cache: the post-collision populations f_out(x, q) of the cells x = 0 (side 0) and
+// x = nx - 1 (side 1) of every row, stored [z][side][q][y] in the storage type.  In the
+// lattice each such cell's 27 values lie in 27 different 128-byte lines that no other
+// lane of a wave uses; the characteristic farfield of an x face reads every one of them
+// five times (its cell and four transverse neighbours), so one gather pass (thread =
+// (y, side, z), consecutive rows per wave) copies them out first and the farfield reads
+// the copy with coalesced loads.
+__device__ __forceinline__ long long face_index(const Desc &L, int side, int y, int z, int q) {
+  return ((long long)(z * 2 + side) * NQ + q) * L.ny + y;
+}
 
 struct FFObj {
   int n[3];
@@ -458,14 +470,14 @@ struct FFParams {
   double U, rho0;
 };
 
-template <int S, bool PE>
-__device__ __forceinline__ void macro_cell(const typename St<S>::T *A, const Desc &L, int x, int y, int z,
-                                           float inv_scale, typename St<S>::C &drho, typename St<S>::C (&u)[3]) {
-  using C = typename St<S>::C;
+// Density deviation and velocity of one cell from its f_out(x, j), j = 0..26 in order
+// (`load(j)`), so that the lattice and the x-face cache give bit-identical moments.
+template <typename C, class Load>
+__device__ __forceinline__ void macro_of(Load &&load, C &drho, C (&u)[3]) {
   C r = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
   for (int j = 0; j < NQ; ++j) {
-    const C v = ld<S>(A + loc_out<PE>(L, x, y, z, j), inv_scale);
+    const C v = load(j);
     r += v;
     if (cxq(j)) jx += cxq(j) * v;
     if (cyq(j)) jy += cyq(j) * v;
@@ -476,6 +488,12 @@ __device__ __forceinline__ void macro_cell(const typename St<S>::T *A, const Des
   u[0] = jx / rho; u[1] = jy / rho; u[2] = jz / rho;
 }
 
+template <int S, bool PE>
+__device__ __forceinline__ void macro_cell(const typename St<S>::T *A, const Desc &L, int x, int y, int z,
+                                           float inv_scale, typename St<S>::C &drho, typename St<S>::C (&u)[3]) {
+  macro_of([&](int j) { return ld<S>(A + loc_out<PE>(L, x, y, z, j), inv_scale); }, drho, u);
+}
+
 template <typename C>
 __device__ __forceinline__ C log_rho(C drho, C rho0) {
   // ln(rho / rho0) with rho = 1 + drho
@@ -483,10 +501,9 @@ __device__ __forceinline__ C log_rho(C drho, C rho0) {
   else return log1pf(drho) - logf(rho0);
 }
 
-template <int S, bool PE>
-__device__ double ff_transport(const typename St<S>::T *A, const Desc &L, const FFObj &o, const int foot[3],
-                               const int edge[2], typename St<S>::C rm, const typename St<S>::C (&u)[3],
-                               float inv_scale, double rho0) {
+template <int S, class Macro>
+__device__ double ff_transport(const Desc &L, const FFObj &o, const int foot[3], const int edge[2],
+                               typename St<S>::C rm, const typename St<S>::C (&u)[3], double rho0, Macro &&macro) {
   using C = typename St<S>::C;
   const C cs = (C)0.57735026918962576451;
   C F = 0;
@@ -503,8 +520,8 @@ __device__ double ff_transport(const typename St<S>::T *A, const Desc &L, const 
     p1[2] = ((p1[2] % L.nz) + L.nz) % L.nz;
     p2[2] = ((p2[2] % L.nz) + L.nz) % L.nz;
     C d1, d2, u1[3], u2[3];
-    macro_cell<S, PE>(A, L, p1[0], p1[1], p1[2], inv_scale, d1, u1);
-    macro_cell<S, PE>(A, L, p2[0], p2[1], p2[2], inv_scale, d2, u2);
+    macro(p1[0], p1[1], p1[2], d1, u1);
+    macro(p2[0], p2[1], p2[2], d2, u2);
     C pdu = 0;
     for (int b = 0; b < 3; ++b) pdu += (C)o.P[a][b] * (c0 * u[b] + c1 * u1[b] + c2 * u2[b]);
     const C rm1 = (C)o.nh[0] * u1[0] + (C)o.nh[1] * u1[1] + (C)o.nh[2] * u1[2] - cs * log_rho<C>(d1, (C)rho0);
@@ -532,15 +549,24 @@ __global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict_
   int foot[3] = {gx - o.n[0], gy - o.n[1], gz};
   const int edge[2] = {foot[0] == 0 ? -1 : (foot[0] == L.nx - 1 ? 1 : 0),
                        foot[1] == 0 ? -1 : (foot[1] == L.ny - 1 ? 1 : 0)};
+  // x faces (normal (+-1, 0, 0)) read their cells, and the transverse neighbours of the
+  // transport (same x), from the x-face cache
+  using T = typename St<S>::T;
+  const T *fc = (o.n[1] == 0 && I.fc != nullptr) ? (const T *)I.fc : nullptr;
+  const int side = o.n[0] > 0 ? 1 : 0;
+  auto macro = [&](int x, int y, int z, C &dr, C (&uu)[3]) {
+    if (fc) macro_of([&](int j) { return ld<S>(fc + face_index(L, side, y, z, j), inv_scale); }, dr, uu);
+    else macro_cell<S, PE>(A, L, x, y, z, inv_scale, dr, uu);
+  };
   C drho, u[3];
-  macro_cell<S, PE>(A, L, foot[0], foot[1], foot[2], inv_scale, drho, u);
+  macro(foot[0], foot[1], foot[2], drho, u);
   const C cs = (C)0.57735026918962576451;
   const C un = (C)o.nh[0] * u[0] + (C)o.nh[1] * u[1] + (C)o.nh[2] * u[2];
   const C lr = log_rho<C>(drho, (C)Pm.rho0);
   const C rm = un - cs * lr, rp = un + cs * lr;
   C incoming;
   if (o.evolve && !o.glancing) {
-    const double F = ff_transport<S, PE>(A, L, o, foot, edge, rm, u, inv_scale, Pm.rho0);
+    const double F = ff_transport<S>(L, o, foot, edge, rm, u, Pm.rho0, macro);
     const double inc = I.incoming[k] + (F + I.derivative[k]) / 2.0;
     I.incoming[k] = inc;
     I.derivative[k] = F;
@@ -576,12 +602,24 @@ __global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict_
   for (int d = 0; d < NQ; ++d) {
     if (!((mask >> d) & 1)) continue;
     const int id = opp(d);
-    const C fo = ld<S>(A + loc_out<PE>(L, foot[0], foot[1], foot[2], id), inv_scale);
+    const C fo = fc ? ld<S>(fc + face_index(L, side, foot[1], foot[2], id), inv_scale)
+                    : ld<S>(A + loc_out<PE>(L, foot[0], foot[1], foot[2], id), inv_scale);
     const C val = fo + (equilibrium<C>(id, rhob, ub[0], ub[1], ub[2]) - equilibrium<C>(id, rho, u[0], u[1], u[2]));
     // fluid cell x = g - c_d reads direction id
     const int xz = ((gz - czq(d)) % L.nz + L.nz) % L.nz;
     stv<S>(A + loc_in<PE>(L, gx - cxq(d), gy - cyq(d), xz, id), val, scale);
   }
+}
+
+// Gather pass of the x-face cache (state in form PE); thread = (y, side, z).
+template <int S, bool PE>
+__global__ void k_face_fill(const typename St<S>::T *__restrict__ A, Desc L, typename St<S>::T *__restrict__ fc) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2LL * L.ny * L.nz) return;
+  const int y = (int)(t % L.ny), side = (int)((t / L.ny) % 2), z = (int)(t / (2LL * L.ny));
+  const int x = side ? L.nx - 1 : 0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) fc[face_index(L, side, y, z, q)] = A[loc_out<PE>(L, x, y, z, q)];
 }
 
 // Initial farfield state from the initial populations (post-even form): incoming =
@@ -606,7 +644,9 @@ __global__ void k_farfield_init(const typename St<S>::T *__restrict__ A, Desc L,
   macro_cell<S, true>(A, L, foot[0], foot[1], foot[2], inv_scale, drho, u);
   const C cs = (C)0.57735026918962576451;
   const C rm = (C)o.nh[0] * u[0] + (C)o.nh[1] * u[1] + (C)o.nh[2] * u[2] - cs * log_rho<C>(drho, (C)Pm.rho0);
-  const double F = ff_transport<S, true>(A, L, o, foot, edge, rm, u, inv_scale, Pm.rho0);
+  const double F = ff_transport<S>(L, o, foot, edge, rm, u, Pm.rho0, [&](int x, int y, int z, C &dr, C (&uu)[3]) {
+    macro_cell<S, true>(A, L, x, y, z, inv_scale, dr, uu);
+  });
   I.incoming[k] = (double)rm - F;
   I.derivative[k] = F;
 }

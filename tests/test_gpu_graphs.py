@@ -96,3 +96,26 @@ def test_refined_graph_replay_matches_direct_launches():
                 assert ra[k] == rb[k], k
     for level in (0, 1):
         np.testing.assert_array_equal(direct.populations(level), graph.populations(level))
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp16"])
+def test_x_face_cache_single_steps_match_one_run(storage):
+    """The characteristic farfield reads the x-face cells from a copy gathered at the start
+    of its boundary phase.  Twelve one-step calls and one twelve-step call (graph replays)
+    must give bit-identical populations and loads, also across a tile change to row-linear
+    sweep blocks."""
+    from aerokinetic_amd.solver import Solver
+    section, flow, grid, plan = small_case("characteristic_transport_v1")
+    print("case x-face cache", storage, plan.shape, flush=True)
+    one = Solver(section, flow, grid, plan=plan, storage=storage, graphs=True)
+    many = Solver(section, flow, grid, plan=plan, storage=storage, graphs=True)
+    a = one.advance(12, 1)
+    b = [many.step(sample=True) for _ in range(12)]
+    np.testing.assert_array_equal(_rows(a)[:, 1:4], np.array([[r["cl"], r["cd"], r["cm"]] for r in b]))
+    np.testing.assert_array_equal(one.populations(), many.populations())
+    one.lattice.set_tile(0, 256)
+    many.lattice.set_tile(0, 256)
+    one.advance(7, 0)
+    for _ in range(7):
+        many.step()
+    np.testing.assert_array_equal(one.populations(), many.populations())
