@@ -45,7 +45,6 @@ struct Lattice {
   float scale = 1.0f, inv_scale = 1.0f;
   Desc L{};
   size_t pop_elems = 0, pop_bytes = 0, elem_size = 8;
-  bool row_groups = true;  // [z][y][q][x] storage; false: one plane per direction
   void *pop = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -552,8 +551,8 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
   // sweep's exact in-place access pattern (even + odd, 1536x1024x64, 64x4 tiles), TB/s
   // with one plane per direction -> interleaved rows: FP64 5.02 -> 5.31, FP32 4.87 ->
   // 5.20, FP16 4.57 -> 5.01.  Larger planes keep one plane per direction ([q][z][y][x]).
-  s->row_groups = (unsigned long long)27 * P * (ny + 2) * s->elem_size < (1ull << 31);
-  if (s->row_groups) {
+  const bool row_groups = (unsigned long long)27 * P * (ny + 2) * s->elem_size < (1ull << 31);
+  if (row_groups) {
     s->L.QS = P;
     s->L.P = 27 * P;
     s->L.PS = s->L.P * (ny + 2);
