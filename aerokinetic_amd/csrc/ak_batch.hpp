@@ -74,21 +74,24 @@ __global__ __launch_bounds__(256) void kb_donor(const CaseDev *__restrict__ C, c
   else donor_moments_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, out, c.err, t);
 }
 
+// x pass of both substeps (after the coarse step's donor moments at t + dt)
 __global__ __launch_bounds__(256) void kb_prolong_x(const CaseDev *__restrict__ C, const int *__restrict__ first,
-                                                    int nm, double frac) {
+                                                    int nm) {
   AK_BATCH_MEMBER
-  if (t < c.npx) prolong_x_body(c.before, c.after, frac, c.B, c.px, c.lpx[t]);
+  if (t < c.npx) prolong_x_body(c.before, c.after, c.B, c.px, c.lpx[t]);
 }
 
 // y and z passes + injection: one 64-thread block per group of PZ_COLS band columns
+// (substep `sub`: fine AA form at step start + sub, x pass of time fraction sub / 2)
 template <int S>
 __global__ __launch_bounds__(64) void kb_prolong_yz(const CaseDev *__restrict__ C, const int *__restrict__ first,
-                                                    int nm, int delta) {
+                                                    int nm, int sub) {
   using T = typename St<S>::T;
   AK_BATCH_MEMBER
   (void)t;
-  if (AK_PE_FINE(delta)) prolong_yz_body<S, true>(c.px, (T *)c.Af, c.Lf, c.B, c.V, c.f_scale, c.err, c.cols, c.ncols, lb_);
-  else prolong_yz_body<S, false>(c.px, (T *)c.Af, c.Lf, c.B, c.V, c.f_scale, c.err, c.cols, c.ncols, lb_);
+  const double *X = c.px + sub * prolong_x_half(c.B);
+  if (AK_PE_FINE(sub)) prolong_yz_body<S, true>(X, (T *)c.Af, c.Lf, c.B, c.V, c.f_scale, c.err, c.cols, c.ncols, lb_);
+  else prolong_yz_body<S, false>(X, (T *)c.Af, c.Lf, c.B, c.V, c.f_scale, c.err, c.cols, c.ncols, lb_);
 }
 
 template <int S>

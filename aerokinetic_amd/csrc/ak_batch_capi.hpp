@@ -194,8 +194,9 @@ void batch_step(Batch *b, bool sample) {
       hipLaunchKernelGGL((kb_after_coarse<S>), dim3(nfl + nlg + ndn), dim3(256), 0, st, C, F(BP_FLUXC), F(BP_LEGACY),
                          F(BP_DONOR), nm, lead, nfl, nlg);
   }
+  // x pass of the prolongation for both substeps
+  if (go(BP_PX)) hipLaunchKernelGGL(kb_prolong_x, dim3(b->nblk[BP_PX]), dim3(64), 0, st, C, F(BP_PX), nm);
   for (int sub = 0; sub < 2; ++sub) {
-    if (go(BP_PX)) hipLaunchKernelGGL(kb_prolong_x, dim3(b->nblk[BP_PX]), dim3(64), 0, st, C, F(BP_PX), nm, sub * 0.5);
     if (go(BP_PYZ)) hipLaunchKernelGGL((kb_prolong_yz<S>), dim3(b->nblk[BP_PYZ]), dim3(64), b->yz_lds, st, C, F(BP_PYZ), nm, sub);
     batched_sweep<S>(b, b->d_desc + (size_t)(2 + sub) * nc, b->nb_fine);
     for (Refined *r : b->members) r->f->sweeps++;

@@ -152,7 +152,15 @@ __device__ __forceinline__ double prolong_weight(int side, int k) {
   return side ? -lo[k + 2] + (k == 0 ? 2.0 : 0.0) : lo[k + 2];
 }
 
-__device__ __forceinline__ void prolong_x_body(const double *__restrict__ M0, const double *__restrict__ M1, double frac, BoxGeom B, double *__restrict__ X, long long t) {
+// Doubles of one substep's x-pass output (the x pass of both substeps is one launch: it
+// needs only the coarse donor moments at t and t + dt, which the fine substeps leave alone).
+__host__ __device__ inline size_t prolong_x_half(const BoxGeom &B) {
+  return (size_t)(2 * B.nb[0] + 4) * (B.nb[1] + 6) * (B.nb[2] + 6) * 10;
+}
+
+// x pass of both substeps: substep s (time fraction s / 2) goes to X + s * prolong_x_half.
+__device__ __forceinline__ void prolong_x_body(const double *__restrict__ M0, const double *__restrict__ M1, BoxGeom B,
+                                               double *__restrict__ X, long long t) {
   const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4;
   if (t >= (long long)cxn * dy * dz) return;
   const int cx = (int)(t % cxn), qy = (int)((t / cxn) % dy), qz = (int)(t / ((long long)cxn * dy));
@@ -160,23 +168,31 @@ __device__ __forceinline__ void prolong_x_body(const double *__restrict__ M0, co
   // the parent rows of the y pads (qy <= 4 or qy >= nby+1) elsewhere
   if (cx >= B.pad && cx < cxn - B.pad && qy > 4 && qy < B.nb[1] + 1) return;
   const int P = cx / 2 + 2, sd = cx & 1;
-  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // each donor value is loaded once for both time fractions (0 and 1/2); every term keeps
+  // the form w * ((1 - frac) * M0 + frac * M1)
+  double m0[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, m1[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int a = -2; a <= 2; ++a) {
     const double w = prolong_weight(sd, a);
     const long long idx = (((long long)qz * dy + qy) * dx + (P + a)) * 10;
-    for (int k = 0; k < 10; ++k) m[k] += w * ((1.0 - frac) * M0[idx + k] + frac * M1[idx + k]);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      const double v0 = M0[idx + k], v1 = M1[idx + k];
+      m0[k] += w * ((1.0 - 0.0) * v0 + 0.0 * v1);
+      m1[k] += w * ((1.0 - 0.5) * v0 + 0.5 * v1);
+    }
   }
-  double *o = X + t * 10;
-  for (int k = 0; k < 10; ++k) o[k] = m[k];
+  double *o0 = X + t * 10, *o1 = X + prolong_x_half(B) + t * 10;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) { o0[k] = m0[k]; o1[k] = m1[k]; }
 }
 
 // The prolongation kernels run over compact lists of the items that do work (the host
 // builds them with the bodies' own conditions): the fine interface band is a few percent of
 // the box volume.
-__global__ void k_prolong_x(const double *__restrict__ M0, const double *__restrict__ M1, double frac, BoxGeom B,
+__global__ void k_prolong_x(const double *__restrict__ M0, const double *__restrict__ M1, BoxGeom B,
                             double *__restrict__ X, const int32_t *__restrict__ list, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) prolong_x_body(M0, M1, frac, B, X, list[i]);
+  if (i < n) prolong_x_body(M0, M1, B, X, list[i]);
 }
 
 // The y pass and the z pass + injection of the prolongation for a group of PZ_COLS fine

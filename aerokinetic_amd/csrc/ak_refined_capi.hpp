@@ -17,7 +17,7 @@ struct Refined {
   FreeState F{};
   double dxc = 0, dtc = 0, dxf = 0, dtf = 0, span = 0, U = 0;
   double *d_before = nullptr, *d_after = nullptr, *d_regc = nullptr, *d_regf = nullptr, *d_part = nullptr;
-  double *d_px = nullptr;                   // prolongation x pass
+  double *d_px = nullptr;                   // prolongation x pass of both substeps
   int32_t *d_lpx = nullptr, *d_cols = nullptr;  // x-pass work items; fine interface-band columns
   int n_lpx = 0, n_cols = 0;
   int *d_err = nullptr;
@@ -173,22 +173,25 @@ void phase_after_coarse(Refined *r) {
   else
     hipLaunchKernelGGL((k_after_coarse<S, false>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_T, volc,
                        c->scale, c->inv_scale, r->d_regc, K, c->U, r->F, r->d_after, r->d_err, (int)cflux_blocks, nb_leg);
+  // x pass of the prolongation for both fine substeps (time fractions 0 and 1/2); 64-thread
+  // blocks spread the few thousand items over every CU
+  hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(r->n_lpx, 64)), dim3(64), 0, c->stream, r->d_before, r->d_after, r->B,
+                     r->d_px, r->d_lpx, r->n_lpx);
 }
 
-// fine substep `sub` before its sweep: time-interpolated prolongation into the donor layers
+// fine substep `sub` before its sweep: y and z passes of the time-interpolated prolongation
+// (x pass from phase_after_coarse) into the donor layers
 template <int S>
 void phase_before_fine(Refined *r, int sub) {
   Lattice *f = r->f;
   using T_ = typename St<S>::T;
-  const double frac = sub * 0.5;
-  // 64-thread blocks spread the few thousand items over every CU
-  hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(r->n_lpx, 64)), dim3(64), 0, f->stream, r->d_before, r->d_after, frac, r->B, r->d_px, r->d_lpx, r->n_lpx);
+  const double *X = r->d_px + sub * prolong_x_half(r->B);
   const dim3 gyz(blocks_for(r->n_cols, PZ_COLS));
   const size_t lds = prolong_yz_lds(r->B);
   if (f->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_prolong_yz<S, true>), gyz, dim3(64), lds, f->stream, (const double *)r->d_px, (T_ *)f->pop, f->L, r->B, r->V, f->scale, r->d_err, r->d_cols, r->n_cols);
+    hipLaunchKernelGGL((k_prolong_yz<S, true>), gyz, dim3(64), lds, f->stream, X, (T_ *)f->pop, f->L, r->B, r->V, f->scale, r->d_err, r->d_cols, r->n_cols);
   else
-    hipLaunchKernelGGL((k_prolong_yz<S, false>), gyz, dim3(64), lds, f->stream, (const double *)r->d_px, (T_ *)f->pop, f->L, r->B, r->V, f->scale, r->d_err, r->d_cols, r->n_cols);
+    hipLaunchKernelGGL((k_prolong_yz<S, false>), gyz, dim3(64), lds, f->stream, X, (T_ *)f->pop, f->L, r->B, r->V, f->scale, r->d_err, r->d_cols, r->n_cols);
 }
 
 // fine substep `sub` after its sweep: fine face flux, walls + load partials
@@ -315,7 +318,7 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
   const size_t nreg = (size_t)4 * r->nt * nz * 4;
   if (dalloc(r->c, &r->d_before, ndon) || dalloc(r->c, &r->d_after, ndon) || dalloc(r->c, &r->d_regc, nreg) ||
       dalloc(r->c, &r->d_regf, 4 * nreg) ||
-      dalloc(r->c, &r->d_px, (size_t)(2 * r->B.nb[0] + 4) * (r->B.nb[1] + 6) * (nz + 6) * 10) ||
+      dalloc(r->c, &r->d_px, 2 * prolong_x_half(r->B)) ||
       dalloc(r->c, &r->d_err, 1) || dalloc(r->c, &r->d_T, 1)) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
     if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
