@@ -60,6 +60,13 @@ __device__ __forceinline__ int batch_owner(const int *__restrict__ first, int nm
   const int lb_ = (int)blockIdx.x - first[m_];                                   \
   const long long t = (long long)lb_ * blockDim.x + threadIdx.x;                 \
   (void)lb_;
+// the same for a block `blk` of one range of a fused launch, with that range's table
+#define AK_BATCH_MEMBER_IN(tbl, blk)                                             \
+  const int m_ = batch_owner(tbl, nm, blk);                                      \
+  const CaseDev &c = C[m_];                                                      \
+  const int lb_ = (blk) - (tbl)[m_];                                             \
+  const long long t = (long long)lb_ * blockDim.x + threadIdx.x;                 \
+  (void)lb_;
 // AA form of a coarse / fine kernel: the sweep count (at step start + delta) is even
 #define AK_PE_COARSE(delta) (((((c.cpar ^ lead) + (delta)) & 1)) == 0)
 #define AK_PE_FINE(delta) ((((c.fpar + (delta)) & 1)) == 0)
@@ -107,18 +114,27 @@ __global__ __launch_bounds__(256) void kb_reflux(const CaseDev *__restrict__ C, 
 
 
 // Fine moments + restriction, one block per restriction tile of a member (fine parity at
-// delta 2, coarse at delta 1 from the step start).
+// delta 2, coarse at delta 1 from the step start), then (blocks from nb_rt on) the next
+// step's donor moments of the donor cells the restriction does not write (as
+// k_restrict_tiles of a single case).
 template <int S>
-__global__ __launch_bounds__(256) void kb_restrict(const CaseDev *__restrict__ C, const int *__restrict__ first,
-                                                   int nm, int lead) {
+__global__ __launch_bounds__(256) void kb_restrict(const CaseDev *__restrict__ C, const int *__restrict__ f_rt,
+                                                   const int *__restrict__ f_don, int nm, int lead, int nb_rt) {
   using T = typename St<S>::T;
-  AK_BATCH_MEMBER
-  (void)t;
-  const bool pf = AK_PE_FINE(2), pc = AK_PE_COARSE(1);
-  if (pf && pc) restrict_tile_body<S, true, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_);
-  else if (pf) restrict_tile_body<S, true, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_);
-  else if (pc) restrict_tile_body<S, false, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_);
-  else restrict_tile_body<S, false, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_);
+  const int b = (int)blockIdx.x;
+  if (b < nb_rt) {
+    AK_BATCH_MEMBER_IN(f_rt, b)
+    (void)t;
+    const bool pf = AK_PE_FINE(2), pc = AK_PE_COARSE(1);
+    if (pf && pc) restrict_tile_body<S, true, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, c.before, c.c_inv);
+    else if (pf) restrict_tile_body<S, true, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, c.before, c.c_inv);
+    else if (pc) restrict_tile_body<S, false, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, c.before, c.c_inv);
+    else restrict_tile_body<S, false, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, c.before, c.c_inv);
+  } else {
+    AK_BATCH_MEMBER_IN(f_don, b - nb_rt)
+    if (AK_PE_COARSE(1)) donor_moments_body<S, true, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.before, c.err, t);
+    else donor_moments_body<S, false, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.before, c.err, t);
+  }
 }
 
 // One 64-thread block per member: the loads of a sampled step, or the step tick.
@@ -134,12 +150,6 @@ __global__ __launch_bounds__(64) void kb_loads(const CaseDev *__restrict__ C, in
 // Fused phases (as k_after_coarse / k_after_fine of a single case): the block ranges of
 // several phases, each over all members, in one launch; the member of a block comes from
 // that phase's table.
-#define AK_BATCH_MEMBER_IN(tbl, blk)                                             \
-  const int m_ = batch_owner(tbl, nm, blk);                                      \
-  const CaseDev &c = C[m_];                                                      \
-  const int lb_ = (blk) - (tbl)[m_];                                             \
-  const long long t = (long long)lb_ * blockDim.x + threadIdx.x;                 \
-  (void)lb_;
 
 template <int S>
 __global__ __launch_bounds__(256) void kb_after_coarse(const CaseDev *__restrict__ C, const int *__restrict__ f_flux,

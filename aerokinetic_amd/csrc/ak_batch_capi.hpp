@@ -175,6 +175,16 @@ void batched_sweep(Batch *b, const BatchMember *d, int nblocks) {
                      nblocks);
 }
 
+// Donor moments of every member's current state at the start of a batch run (each step's
+// restriction launch then leaves those of the next step).
+template <int S>
+void batch_start(Batch *b) {
+  const int nm = (int)b->members.size(), lead = (int)(b->members[0]->c->sweeps % 2);
+  if (b->nblk[BP_DONOR] > 0)
+    hipLaunchKernelGGL((kb_donor<S>), dim3(b->nblk[BP_DONOR]), dim3(256), 0, b->stream, b->d_cases,
+                       b->d_first + (size_t)BP_DONOR * (b->cap + 1), nm, lead, 0, 0);
+}
+
 // One coarse step of every member: the single-case kernel sequence, each kernel one launch
 // over all members (host counters advance as in refined_coarse_step).
 template <int S>
@@ -184,7 +194,6 @@ void batch_step(Batch *b, bool sample) {
   const CaseDev *C = b->d_cases;
   auto F = [b](int k) { return b->d_first + (size_t)k * (b->cap + 1); };
   auto go = [&](int k) { return b->nblk[k] > 0; };
-  if (go(BP_DONOR)) hipLaunchKernelGGL((kb_donor<S>), dim3(b->nblk[BP_DONOR]), dim3(256), 0, st, C, F(BP_DONOR), nm, lead, 0, 0);
   batched_sweep<S>(b, b->d_desc + (size_t)lead * nc, b->nb_coarse);
   for (Refined *r : b->members) r->c->sweeps++;
   // face flux, farfield and donor moments of the new coarse state: one launch (disjoint slots)
@@ -206,7 +215,10 @@ void batch_step(Batch *b, bool sample) {
                          F(BP_FLUXF), F(BP_WALLS), nm, sub + 1, sub, b->nblk[BP_FLUXF]);
   }
   if (go(BP_REFLUX)) hipLaunchKernelGGL((kb_reflux<S>), dim3(b->nblk[BP_REFLUX]), dim3(256), 0, st, C, F(BP_REFLUX), nm, lead, 1);
-  if (go(BP_RESTRICT)) hipLaunchKernelGGL((kb_restrict<S>), dim3(b->nblk[BP_RESTRICT]), dim3(256), 0, st, C, F(BP_RESTRICT), nm, lead);
+  // restriction + the next step's donor moments: one launch
+  if (b->nblk[BP_RESTRICT] + b->nblk[BP_DONOR] > 0)
+    hipLaunchKernelGGL((kb_restrict<S>), dim3(b->nblk[BP_RESTRICT] + b->nblk[BP_DONOR]), dim3(256), 0, st, C,
+                       F(BP_RESTRICT), F(BP_DONOR), nm, lead, b->nblk[BP_RESTRICT]);
   hipLaunchKernelGGL(kb_loads, dim3(nm), dim3(64), 0, st, C, sample ? 1 : 0);
 }
 
@@ -329,6 +341,11 @@ int ak_batch_run(void *handle, int nsteps, int sample_every) {
   if (order_after_members(b)) return -1;
   const bool sample = sample_every == 1;
   for (Refined *r : b->members) set_ctl_on(r->c, b->stream, r->steps + 1, r->c->hist_rows);
+  switch (b->storage) {
+    case 0: batch_start<0>(b); break;
+    case 1: batch_start<1>(b); break;
+    default: batch_start<2>(b); break;
+  }
   for (int k = 0; k < nsteps; ++k) {
     if (sample) {
       bool full = false;

@@ -110,7 +110,9 @@ struct FreeState {
 };
 
 // Pre-collision moments of the coarse donor block (box +- 3 cells; z wraps).
-template <int S, bool PE>
+// SKIP: leave out the covered cells outside the core (the restriction that writes their
+// populations writes their moments too).
+template <int S, bool PE, bool SKIP = false>
 __device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F, float inv_scale, double *__restrict__ M, int *__restrict__ err, long long t) {
   const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
   if (t >= (long long)dx * dy * dz) return;
@@ -120,8 +122,11 @@ __device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__re
   double m[10];
   int e = 0;
   const int bx = x - B.lo[0], by = y - B.lo[1];
-  if (bx >= 0 && bx < B.nb[0] && by >= 0 && by < B.nb[1] && in_core(B, bx, by)) {
+  const bool covered = bx >= 0 && bx < B.nb[0] && by >= 0 && by < B.nb[1];
+  if (covered && in_core(B, bx, by)) {
     for (int c = 0; c < 10; ++c) m[c] = F.m[c];
+  } else if (SKIP && covered) {
+    return;
   } else {
     double g[NQ];
     gather_in<S, PE>(A, L, x, y, z, inv_scale, g);
@@ -441,10 +446,23 @@ __host__ __device__ inline int restrict_tiles(const BoxGeom &B) {
   return ((B.nb[0] + RT_X - 1) / RT_X) * ((B.nb[1] + RT_Y - 1) / RT_Y) * B.nb[2];
 }
 
+// A population as the next coarse sweep (and the donor moments) read it back after
+// scatter_in stored it: the storage rounding, applied in registers.
+template <int S>
+__device__ __forceinline__ double stored_value(double v, float scale, float inv) {
+  typename St<S>::T t;
+  stv<S>(&t, (typename St<S>::C)v, scale);
+  return (double)ld<S>(&t, inv);
+}
+
+// With M, each restricted cell outside the core also writes its donor moments for the next
+// coarse step (the moments of its populations as stored, at every donor-block z index
+// that maps to its z), so that no separate donor pass reads them back.
 template <int S, bool PE, bool PEC>
 __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
                                                    typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B, Levels V,
-                                                   FreeState F, float cscale, int *__restrict__ err, int tile) {
+                                                   FreeState F, float cscale, int *__restrict__ err, int tile,
+                                                   double *__restrict__ M = nullptr, float cinv = 1.0f) {
   __shared__ double sm[10][256];
   const int ntx = (B.nb[0] + RT_X - 1) / RT_X, nty = (B.nb[1] + RT_Y - 1) / RT_Y;
   const int tbx = tile % ntx, tby = (tile / ntx) % nty, bz = tile / (ntx * nty);
@@ -479,6 +497,16 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
         for (int k = 0; k < 10; ++k) acc[k] *= 0.125;
         rescale(acc, V.shear_fc, V.bulk_fc);
         populations_of(acc, g, e);
+        if (M != nullptr) {
+          double h[NQ], md[10];
+          for (int i = 0; i < NQ; ++i) h[i] = stored_value<S>(g[i], cscale, cinv);
+          moments_of(h, md, e);
+          const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
+          for (int k = ((bz + 3) % Lc.nz + Lc.nz) % Lc.nz; k < dz; k += Lc.nz) {
+            double *o = M + ((((long long)k * dy) + (by + 3)) * dx + (bx + 3)) * 10;
+            for (int c = 0; c < 10; ++c) o[c] = md[c];
+          }
+        }
       }
       scatter_in<S, PEC>(Ac, Lc, B.lo[0] + bx, B.lo[1] + by, B.lo[2] + bz, cscale, g);
     }
@@ -486,11 +514,19 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
   if (e) atomicOr(err, e);
 }
 
+// End of a coarse step: the restriction tiles, then (block range after `ntiles`) the donor
+// moments of the next step's pre-collision state for the donor cells it does not write
+// (outside the box, and the core's free-stream constants).  The two ranges touch disjoint
+// slots: every slot is the incoming population of exactly one cell.
 template <int S, bool PE, bool PEC>
 __global__ __launch_bounds__(256) void k_restrict_tiles(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
                                                         typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B,
-                                                        Levels V, FreeState F, float cscale, int *__restrict__ err) {
-  restrict_tile_body<S, PE, PEC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, (int)blockIdx.x);
+                                                        Levels V, FreeState F, float cscale, float cinv,
+                                                        double *__restrict__ M, int *__restrict__ err, int ntiles) {
+  if ((int)blockIdx.x < ntiles)
+    restrict_tile_body<S, PE, PEC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, (int)blockIdx.x, M, cinv);
+  else
+    donor_moments_body<S, PEC, true>(Ac, Lc, B, F, cinv, M, err, (long long)((int)blockIdx.x - ntiles) * 256 + threadIdx.x);
 }
 
 // Initial pre-collision state of one level: rho = 1, seeded momentum, regularised

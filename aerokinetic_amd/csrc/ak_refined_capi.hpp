@@ -124,15 +124,19 @@ int build_prolong_lists(Refined *r) {
   return (upload(r->c, &r->d_lpx, lx) || upload(r->c, &r->d_cols, cols)) ? -1 : 0;
 }
 
+// restriction + the next step's donor moments (at t) in one launch
 template <int S>
 void launch_restrict(Refined *r) {
   Lattice *c = r->c, *f = r->f;
   using T_ = typename St<S>::T;
-  const dim3 grid(restrict_tiles(r->B)), block(256);
+  const int ntiles = restrict_tiles(r->B);
+  const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
+  const dim3 grid(ntiles + blocks_for(ndon, 256)), block(256);
   const bool pf = f->sweeps % 2 == 0, pc = c->sweeps % 2 == 0;
 #define AK_RT(PF, PC)                                                                                            \
   hipLaunchKernelGGL((k_restrict_tiles<S, PF, PC>), grid, block, 0, c->stream, (const T_ *)f->pop, f->L,         \
-                     f->inv_scale, (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, r->d_err)
+                     f->inv_scale, (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, c->inv_scale, r->d_before,   \
+                     r->d_err, ntiles)
   if (pf) { if (pc) AK_RT(true, true); else AK_RT(true, false); }
   else { if (pc) AK_RT(false, true); else AK_RT(false, false); }
 #undef AK_RT
@@ -143,7 +147,8 @@ void launch_restrict(Refined *r) {
 // batched sweep launch in place of the members' sweeps (ak_batch_run).  Every launch
 // goes to the member's coarse-lattice stream, which both levels share.
 
-// donor moments at time t (pre-collision coarse state, before the coarse sweep)
+// donor moments at time t (pre-collision coarse state, before the coarse sweep): at the
+// start of a run; every step's restriction launch then leaves those of the next step
 template <int S>
 void phase_before_coarse(Refined *r) {
   Lattice *c = r->c;
@@ -238,6 +243,17 @@ void phase_end(Refined *r, bool sample) {
   launch_restrict<S>(r);
 }
 
+// Donor moments of the current state, launched directly (not in a step graph) at the start
+// of every run call: the state may have been initialised, loaded or set since the last
+// step's restriction wrote them.
+void run_start(Refined *r) {
+  switch (r->c->storage) {
+    case 0: phase_before_coarse<0>(r); break;
+    case 1: phase_before_coarse<1>(r); break;
+    default: phase_before_coarse<2>(r); break;
+  }
+}
+
 // One coarse step of one case; its number and history row come from the coarse lattice's
 // d_ctl.  ev (optional, 8 events): step start/end, coarse sweep start/end, fine sweep 0 and
 // 1 start/end (the sweep events hold the kernels' own start and end times).
@@ -245,7 +261,6 @@ template <int S>
 void refined_coarse_step(Refined *r, bool sample, hipEvent_t *ev = nullptr) {
   Lattice *c = r->c, *f = r->f;
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  phase_before_coarse<S>(r);
   launch_sweep<S>(c, c->sweeps % 2 == 0, false, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
   c->sweeps++;
   phase_after_coarse<S>(r);
@@ -423,7 +438,10 @@ int ak_refined_run(void *handle, int nsteps, long long first_step, int sample_ev
   if (!r) return fail(nullptr, "null refined handle");
   Lattice *c = r->c, *f = r->f;
   if (check_ready(c) || check_ready(f)) return -1;
-  if (nsteps > 0) set_ctl(c, first_step + 1, c->hist_rows);
+  if (nsteps > 0) {
+    set_ctl(c, first_step + 1, c->hist_rows);
+    run_start(r);
+  }
   for (int k = 0; k < nsteps; ++k) {
     const long long number = first_step + k + 1;
     const bool sample = sample_every > 0 && number % sample_every == 0;
@@ -468,7 +486,10 @@ int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int 
   if (check_ready(c) || check_ready(r->f)) return -1;
   std::vector<hipEvent_t> ev(8 * (size_t)nsteps);
   for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
-  if (nsteps > 0) set_ctl(c, first_step + 1, c->hist_rows);
+  if (nsteps > 0) {
+    set_ctl(c, first_step + 1, c->hist_rows);
+    run_start(r);
+  }
   for (int k = 0; k < nsteps; ++k) {
     const long long number = first_step + k + 1;
     const bool sample = sample_every > 0 && number % sample_every == 0;
