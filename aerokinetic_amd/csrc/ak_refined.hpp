@@ -99,6 +99,15 @@ __device__ __forceinline__ void scatter_in(typename St<S>::T *A, const Desc &L, 
   for (int i = 0; i < NQ; ++i) stv<S>(A + loc_in<PE>(L, x, y, z, i), (typename St<S>::C)g[i], scale);
 }
 
+// A population as the next coarse sweep (and the donor moments) read it back after
+// scatter_in stored it: the storage rounding, applied in registers.
+template <int S>
+__device__ __forceinline__ double stored_value(double v, float scale, float inv) {
+  typename St<S>::T t;
+  stv<S>(&t, (typename St<S>::C)v, scale);
+  return (double)ld<S>(&t, inv);
+}
+
 __device__ __forceinline__ bool in_core(const BoxGeom &B, int bx, int by) {
   // covered cell (box coordinates) more than three rows inside the box in x and y
   return bx >= 3 && bx < B.nb[0] - 3 && by >= 3 && by < B.nb[1] - 3;
@@ -110,9 +119,10 @@ struct FreeState {
 };
 
 // Pre-collision moments of the coarse donor block (box +- 3 cells; z wraps).
-// SKIP: leave out the covered cells outside the core (the restriction that writes their
-// populations writes their moments too).
-template <int S, bool PE, bool SKIP = false>
+// SKIP 1: leave out the covered cells outside the core (the restriction that writes their
+// populations writes their moments too); SKIP 2: also the reflux cells (the coarse cells
+// next to the box faces, whose reflux writes theirs).
+template <int S, bool PE, int SKIP = 0>
 __device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F, float inv_scale, double *__restrict__ M, int *__restrict__ err, long long t) {
   const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
   if (t >= (long long)dx * dy * dz) return;
@@ -122,10 +132,12 @@ __device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__re
   double m[10];
   int e = 0;
   const int bx = x - B.lo[0], by = y - B.lo[1];
-  const bool covered = bx >= 0 && bx < B.nb[0] && by >= 0 && by < B.nb[1];
+  const bool inx = bx >= 0 && bx < B.nb[0], iny = by >= 0 && by < B.nb[1];
+  const bool covered = inx && iny;
+  const bool refluxed = (inx && (by == -1 || by == B.nb[1])) || (iny && (bx == -1 || bx == B.nb[0]));
   if (covered && in_core(B, bx, by)) {
     for (int c = 0; c < 10; ++c) m[c] = F.m[c];
-  } else if (SKIP && covered) {
+  } else if ((SKIP >= 1 && covered) || (SKIP >= 2 && refluxed)) {
     return;
   } else {
     double g[NQ];
@@ -371,7 +383,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 template <int S, bool PE>
-__device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const double *__restrict__ rc, const double *__restrict__ rf, double inv_vol, float scale, float inv, int *__restrict__ err, long long t) {
+// With M, the wave also writes the donor moments of its cell's new state for the next coarse
+// step (its populations as stored, gathered to lane 0; every donor-block z index that maps
+// to its z).
+__device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const double *__restrict__ rc, const double *__restrict__ rf, double inv_vol, float scale, float inv, int *__restrict__ err, long long t, double *__restrict__ M = nullptr) {
   // rf holds fine face elements [f][2 nt][2 nz][4]
   const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
   const long long w = t / 64;
@@ -414,27 +429,25 @@ __device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, D
     if (!(g + wq(i) >= 0)) e |= isfinite(g) ? TE_NEGATIVE : TE_NONFINITE;
     stv<S>(A + slot, (typename St<S>::C)g, scale);
   }
+  if (M != nullptr) {
+    const double h = lane < NQ ? stored_value<S>(g, scale, inv) : 0.0;
+    double hv[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) hv[q] = __shfl(h, q, 64);
+    if (lane == 0) {
+      double md[10];
+      moments_of(hv, md, e);
+      const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
+      const int bx = x - B.lo[0], by = y - B.lo[1];
+      for (int k = ((zpos - B.lo[2] + 3) % L.nz + L.nz) % L.nz; k < dz; k += L.nz) {
+        double *o = M + ((((long long)k * dy) + (by + 3)) * dx + (bx + 3)) * 10;
+        for (int c = 0; c < 10; ++c) o[c] = md[c];
+      }
+    }
+  }
   if (e) atomicOr(err, e);
 }
 
-template <int S, bool PE>
-__global__ __launch_bounds__(256) void k_reflux(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
-                                                const double *__restrict__ rc, const double *__restrict__ rf,
-                                                double inv_vol, float scale, float inv, int *__restrict__ err,
-                                                int nb_reflux, const double *__restrict__ part, int nparts, int nelem,
-                                                double fscale, double vscale, StepCtl *ctl, double *__restrict__ hist,
-                                                int sample) {
-  // the blocks after the reflux range: one block ends the step (its loads on sampled steps)
-  if ((int)blockIdx.x < nb_reflux) {
-    reflux_body<S, PE>(A, L, B, rc, rf, inv_vol, scale, inv, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
-  } else if (sample) {
-    loads_final_body(part, nparts, nelem, fscale, fscale, vscale, ctl, hist, nullptr);
-  } else if (threadIdx.x == 0) {
-    ctl->step += 1;
-  }
-}
-
-// Moments of every owned fine cell outside the fine virtual core (first restriction stage).
 // Fine moments and restriction in one pass: a block takes a tile of RT_X x RT_Y covered
 // coarse cells of one coarse z layer; its 256 threads are their children, whose moments go
 // to LDS; then one thread per coarse cell averages its eight children (fixed order) as in
@@ -444,15 +457,6 @@ __global__ __launch_bounds__(256) void k_reflux(typename St<S>::T *__restrict__ 
 constexpr int RT_X = 16, RT_Y = 2;
 __host__ __device__ inline int restrict_tiles(const BoxGeom &B) {
   return ((B.nb[0] + RT_X - 1) / RT_X) * ((B.nb[1] + RT_Y - 1) / RT_Y) * B.nb[2];
-}
-
-// A population as the next coarse sweep (and the donor moments) read it back after
-// scatter_in stored it: the storage rounding, applied in registers.
-template <int S>
-__device__ __forceinline__ double stored_value(double v, float scale, float inv) {
-  typename St<S>::T t;
-  stv<S>(&t, (typename St<S>::C)v, scale);
-  return (double)ld<S>(&t, inv);
 }
 
 // With M, each restricted cell outside the core also writes its donor moments for the next
@@ -526,7 +530,33 @@ __global__ __launch_bounds__(256) void k_restrict_tiles(const typename St<S>::T 
   if ((int)blockIdx.x < ntiles)
     restrict_tile_body<S, PE, PEC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, (int)blockIdx.x, M, cinv);
   else
-    donor_moments_body<S, PEC, true>(Ac, Lc, B, F, cinv, M, err, (long long)((int)blockIdx.x - ntiles) * 256 + threadIdx.x);
+    donor_moments_body<S, PEC, 1>(Ac, Lc, B, F, cinv, M, err, (long long)((int)blockIdx.x - ntiles) * 256 + threadIdx.x);
+}
+
+// End of a coarse step in one launch over consecutive block ranges: the reflux waves
+// (coarse cells next to the box faces), one block for the loads (sampled step) or the step
+// tick, the restriction tiles (covered cells), and the next step's donor moments of the
+// remaining donor cells.  Reflux and restriction write their own cells' donor moments, so
+// every range reads and writes slots no other range touches.
+template <int S, bool PF, bool PC>
+__global__ __launch_bounds__(256) void k_step_end(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
+                                                  typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B, Levels V,
+                                                  FreeState F, float cscale, float cinv, double *__restrict__ M,
+                                                  const double *__restrict__ rc, const double *__restrict__ rf,
+                                                  double inv_vol, int *__restrict__ err, int nb_reflux, int ntiles,
+                                                  const double *__restrict__ part, int nparts, int nelem, double fscale,
+                                                  double vscale, StepCtl *ctl, double *__restrict__ hist, int sample) {
+  const int b = (int)blockIdx.x;
+  if (b < nb_reflux) {
+    reflux_body<S, PC>(Ac, Lc, B, rc, rf, inv_vol, cscale, cinv, err, (long long)b * 256 + threadIdx.x, M);
+  } else if (b == nb_reflux) {
+    if (sample) loads_final_body(part, nparts, nelem, fscale, fscale, vscale, ctl, hist, nullptr);
+    else if (threadIdx.x == 0) ctl->step += 1;
+  } else if (b < nb_reflux + 1 + ntiles) {
+    restrict_tile_body<S, PF, PC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, b - nb_reflux - 1, M, cinv);
+  } else {
+    donor_moments_body<S, PC, 2>(Ac, Lc, B, F, cinv, M, err, (long long)(b - nb_reflux - 1 - ntiles) * 256 + threadIdx.x);
+  }
 }
 
 // Initial pre-collision state of one level: rho = 1, seeded momentum, regularised

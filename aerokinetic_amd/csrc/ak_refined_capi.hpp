@@ -231,16 +231,19 @@ void phase_end(Refined *r, bool sample) {
   // wall partials, the reflux only the coarse populations.
   const double fscale = volf * (r->dxf / r->dtf) / r->dtc / (0.5 * r->span);
   const int nparts = f->nlinks ? 2 * r->wall_blocks : 0;
-  const dim3 grid(cwave_blocks + 1), block(256);
-  if (c->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_reflux<S, true>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf,
-                       inv_volc, c->scale, c->inv_scale, r->d_err, (int)cwave_blocks, (const double *)r->d_part, nparts,
-                       f->nelem, fscale, volf, c->d_ctl, c->d_hist, sample ? 1 : 0);
-  else
-    hipLaunchKernelGGL((k_reflux<S, false>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_regc, r->d_regf,
-                       inv_volc, c->scale, c->inv_scale, r->d_err, (int)cwave_blocks, (const double *)r->d_part, nparts,
-                       f->nelem, fscale, volf, c->d_ctl, c->d_hist, sample ? 1 : 0);
-  launch_restrict<S>(r);
+  // reflux, loads / tick, restriction and the next step's donor moments: one launch
+  const int ntiles = restrict_tiles(r->B);
+  const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
+  const dim3 grid(cwave_blocks + 1 + ntiles + blocks_for(ndon, 256)), block(256);
+  const bool pf = f->sweeps % 2 == 0, pc = c->sweeps % 2 == 0;
+#define AK_END(PF, PC)                                                                                             \
+  hipLaunchKernelGGL((k_step_end<S, PF, PC>), grid, block, 0, c->stream, (const T_ *)f->pop, f->L, f->inv_scale,  \
+                     (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, c->inv_scale, r->d_before, r->d_regc,        \
+                     r->d_regf, inv_volc, r->d_err, (int)cwave_blocks, ntiles, (const double *)r->d_part, nparts, \
+                     f->nelem, fscale, volf, c->d_ctl, c->d_hist, sample ? 1 : 0)
+  if (pf) { if (pc) AK_END(true, true); else AK_END(true, false); }
+  else { if (pc) AK_END(false, true); else AK_END(false, false); }
+#undef AK_END
 }
 
 // Donor moments of the current state, launched directly (not in a step graph) at the start
