@@ -121,15 +121,16 @@ template <int S>
 __global__ __launch_bounds__(256) void kb_restrict(const CaseDev *__restrict__ C, const int *__restrict__ f_rt,
                                                    const int *__restrict__ f_don, int nm, int lead, int nb_rt) {
   using T = typename St<S>::T;
+  __shared__ double sm[RESTRICT_LDS];  // one array for the four parity instances
   const int b = (int)blockIdx.x;
   if (b < nb_rt) {
     AK_BATCH_MEMBER_IN(f_rt, b)
     (void)t;
     const bool pf = AK_PE_FINE(2), pc = AK_PE_COARSE(1);
-    if (pf && pc) restrict_tile_body<S, true, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, c.before, c.c_inv);
-    else if (pf) restrict_tile_body<S, true, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, c.before, c.c_inv);
-    else if (pc) restrict_tile_body<S, false, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, c.before, c.c_inv);
-    else restrict_tile_body<S, false, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, c.before, c.c_inv);
+    if (pf && pc) restrict_tile_body<S, true, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, c.before, c.c_inv);
+    else if (pf) restrict_tile_body<S, true, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, c.before, c.c_inv);
+    else if (pc) restrict_tile_body<S, false, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, c.before, c.c_inv);
+    else restrict_tile_body<S, false, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, c.before, c.c_inv);
   } else {
     AK_BATCH_MEMBER_IN(f_don, b - nb_rt)
     if (AK_PE_COARSE(1)) donor_moments_body<S, true, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.before, c.err, t);
@@ -139,10 +140,11 @@ __global__ __launch_bounds__(256) void kb_restrict(const CaseDev *__restrict__ C
 
 // One 64-thread block per member: the loads of a sampled step, or the step tick.
 __global__ __launch_bounds__(64) void kb_loads(const CaseDev *__restrict__ C, int sample) {
+  __shared__ double acc[LOADS_LDS];
   const CaseDev &c = C[blockIdx.x];
   if (sample)
     loads_final_body(c.part, c.W.n ? 2 * c.wall_blocks : 0, c.W.nelem, c.loads_scale, c.loads_scale, c.volf, c.ctl,
-                     c.hist, nullptr);
+                     c.hist, nullptr, acc);
   else if (threadIdx.x == 0)
     c.ctl->step += 1;
 }

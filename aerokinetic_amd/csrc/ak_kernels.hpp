@@ -356,16 +356,19 @@ __global__ void k_tick(StepCtl *ctl) {
 // history row: cl = Fy fscale, cd = Fx fscale, cm = -Mz mscale, mass exchange = mass
 // vscale.  diag != nullptr:
 // also clear the diagnostic slots for the sweep that follows (k_diag_final ends
-// the step); diag == nullptr: this kernel ends the sampled step.
+// the step); diag == nullptr: this kernel ends the sampled step.  `acc` is LOADS_LDS
+// doubles of the calling kernel's shared memory (kernels that fuse several phases give
+// every phase the same array).
+constexpr int LOADS_LDS = MAX_ELEM * 5 * 64;
 __device__ __forceinline__ void loads_final_body(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
-                              double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
-  __shared__ double acc[MAX_ELEM * 5][64];
+                              double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag,
+                              double *__restrict__ acc) {
   const int lane = threadIdx.x;  // the first 64 threads of the block do the work
   if (lane < 64)
     for (int c = 0; c < nelem * 5; ++c) {
       double s = 0.0;
       for (int b = lane; b < nblocks; b += 64) s += partials[(long long)b * nelem * 5 + c];
-      acc[c][lane] = s;
+      acc[c * 64 + lane] = s;
     }
   __syncthreads();
   if (lane == 0) {
@@ -375,7 +378,7 @@ __device__ __forceinline__ void loads_final_body(const double *__restrict__ part
       double v[5];
       for (int c = 0; c < 5; ++c) {
         double s = 0.0;
-        for (int k = 0; k < 64; ++k) s += acc[e * 5 + c][k];
+        for (int k = 0; k < 64; ++k) s += acc[(e * 5 + c) * 64 + k];
         v[c] = s;
         tot[c] += s;
       }
@@ -402,7 +405,8 @@ __device__ __forceinline__ void loads_final_body(const double *__restrict__ part
 
 __global__ void k_loads_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
                               double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
-  loads_final_body(partials, nblocks, nelem, fscale, mscale, vscale, ctl, hist, diag);
+  __shared__ double acc[LOADS_LDS];
+  loads_final_body(partials, nblocks, nelem, fscale, mscale, vscale, ctl, hist, diag, acc);
 }
 
 __global__ void k_diag_final(const DiagAcc *diag, StepCtl *ctl, double *__restrict__ hist) {

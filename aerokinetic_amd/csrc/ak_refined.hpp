@@ -455,6 +455,9 @@ __device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, D
 // 2x2x2 mean of the children's moments, rescaled fine->coarse and regularised); the moments
 // never leave the chip.
 constexpr int RT_X = 16, RT_Y = 2;
+constexpr int RESTRICT_LDS = 10 * 256;  // doubles: the children's moments of one tile
+// shared memory of a kernel that fuses the restriction and the loads fold
+constexpr int END_LDS = RESTRICT_LDS > LOADS_LDS ? RESTRICT_LDS : LOADS_LDS;
 __host__ __device__ inline int restrict_tiles(const BoxGeom &B) {
   return ((B.nb[0] + RT_X - 1) / RT_X) * ((B.nb[1] + RT_Y - 1) / RT_Y) * B.nb[2];
 }
@@ -466,8 +469,8 @@ template <int S, bool PE, bool PEC>
 __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
                                                    typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B, Levels V,
                                                    FreeState F, float cscale, int *__restrict__ err, int tile,
-                                                   double *__restrict__ M = nullptr, float cinv = 1.0f) {
-  __shared__ double sm[10][256];
+                                                   double *__restrict__ sm, double *__restrict__ M = nullptr,
+                                                   float cinv = 1.0f) {
   const int ntx = (B.nb[0] + RT_X - 1) / RT_X, nty = (B.nb[1] + RT_Y - 1) / RT_Y;
   const int tbx = tile % ntx, tby = (tile / ntx) % nty, bz = tile / (ntx * nty);
   const int tid = threadIdx.x;
@@ -481,7 +484,7 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
     moments_of(g, m, e);
   }
 #pragma unroll
-  for (int k = 0; k < 10; ++k) sm[k][tid] = m[k];
+  for (int k = 0; k < 10; ++k) sm[k * 256 + tid] = m[k];
   __syncthreads();
   if (tid < RT_X * RT_Y) {
     const int ox = tid % RT_X, oy = tid / RT_X;
@@ -496,7 +499,7 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
           for (int b = 0; b < 2; ++b)
             for (int a = 0; a < 2; ++a) {
               const int ch = (2 * ox + a) + 2 * RT_X * ((2 * oy + b) + 2 * RT_Y * c);
-              for (int k = 0; k < 10; ++k) acc[k] += sm[k][ch];
+              for (int k = 0; k < 10; ++k) acc[k] += sm[k * 256 + ch];
             }
         for (int k = 0; k < 10; ++k) acc[k] *= 0.125;
         rescale(acc, V.shear_fc, V.bulk_fc);
@@ -527,8 +530,9 @@ __global__ __launch_bounds__(256) void k_restrict_tiles(const typename St<S>::T 
                                                         typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B,
                                                         Levels V, FreeState F, float cscale, float cinv,
                                                         double *__restrict__ M, int *__restrict__ err, int ntiles) {
+  __shared__ double sm[RESTRICT_LDS];
   if ((int)blockIdx.x < ntiles)
-    restrict_tile_body<S, PE, PEC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, (int)blockIdx.x, M, cinv);
+    restrict_tile_body<S, PE, PEC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, (int)blockIdx.x, sm, M, cinv);
   else
     donor_moments_body<S, PEC, 1>(Ac, Lc, B, F, cinv, M, err, (long long)((int)blockIdx.x - ntiles) * 256 + threadIdx.x);
 }
@@ -546,14 +550,15 @@ __global__ __launch_bounds__(256) void k_step_end(const typename St<S>::T *__res
                                                   double inv_vol, int *__restrict__ err, int nb_reflux, int ntiles,
                                                   const double *__restrict__ part, int nparts, int nelem, double fscale,
                                                   double vscale, StepCtl *ctl, double *__restrict__ hist, int sample) {
+  __shared__ double lds[END_LDS];
   const int b = (int)blockIdx.x;
   if (b < nb_reflux) {
     reflux_body<S, PC>(Ac, Lc, B, rc, rf, inv_vol, cscale, cinv, err, (long long)b * 256 + threadIdx.x, M);
   } else if (b == nb_reflux) {
-    if (sample) loads_final_body(part, nparts, nelem, fscale, fscale, vscale, ctl, hist, nullptr);
+    if (sample) loads_final_body(part, nparts, nelem, fscale, fscale, vscale, ctl, hist, nullptr, lds);
     else if (threadIdx.x == 0) ctl->step += 1;
   } else if (b < nb_reflux + 1 + ntiles) {
-    restrict_tile_body<S, PF, PC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, b - nb_reflux - 1, M, cinv);
+    restrict_tile_body<S, PF, PC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, b - nb_reflux - 1, lds, M, cinv);
   } else {
     donor_moments_body<S, PC, 2>(Ac, Lc, B, F, cinv, M, err, (long long)(b - nb_reflux - 1 - ntiles) * 256 + threadIdx.x);
   }
