@@ -6,8 +6,8 @@ step count and stream.  Per coarse step of the batch:
 
 * the members' coarse sweeps run as one launch over their concatenated tile lists, and
   likewise each fine substep's sweeps;
-* the members' transfer, farfield and wall kernels run on the members' own streams,
-  forked from and joined back into the batch stream, so they overlap on the device;
+* every transfer, farfield, wall and load kernel also runs once for all members, over the
+  members' concatenated work lists (a block finds its member in a table of first blocks);
 * the whole step is a captured HIP graph (re-captured when the membership changes).
 
 A member gives the same history alone and in any mix: the same kernels do the same

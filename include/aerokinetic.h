@@ -234,14 +234,15 @@ int ak_refined_get_populations(void *handle, int level, double *host_out);
  * refined_run.run, solver.py:353-537, refined_run.py:19-157; polar sweeps loop over such
  * runs).  A batch steps several refined cases of one storage type together: the members'
  * coarse sweeps run as ONE launch over their concatenated tile lists, likewise each fine
- * substep's sweeps, while the members' transfer kernels run on their own streams and
- * overlap.  Each batch step is a captured HIP graph (per launch variant).  A member keeps
+ * substep's sweeps and each transfer, farfield, wall and load kernel (over the members'
+ * concatenated work lists).  Each batch step is a captured HIP graph (per launch variant).  A member keeps
  * its own step count, history rows and buffers, and gives the same result alone or in any
  * mix (same kernels, same arithmetic, fixed-order reductions).  Errors: ak_last_error(NULL). */
 int ak_batch_create(void **handle, int device, int storage, void *stream);
 int ak_batch_destroy(void *handle);
-/* Add / remove a refined case (ak_refined_create handle; each member needs a stream of its
- * own).  Membership changes re-capture the batch graph on the next run. */
+/* Add / remove a refined case (ak_refined_create handle, at a coarse-step boundary; work
+ * queued on its stream is ordered before the batch's next step).  Membership changes
+ * re-capture the batch graph on the next run. */
 int ak_batch_add(void *handle, void *refined);
 int ak_batch_remove(void *handle, void *refined);
 int ak_batch_size(void *handle);
