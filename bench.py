@@ -59,13 +59,13 @@ BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
 # map the counters onto the 8 GiB read + 8 GiB written by the streaming copy), for the
 # kernels whose roofline is reported: (workload, storage, kernel) -> bytes.
 MEASURED_TRAFFIC = {  # profiles/hbm_traffic.md (mean over both AA parities)
-    ('two-level', 'fp64', 'fine k_sweep'): 117836397,
-    ('two-level', 'fp64', 'coarse k_sweep'): 173296587,
-    ('two-level', 'fp32', 'fine k_sweep'): 62828752,
-    ('two-level', 'fp32', 'coarse k_sweep'): 87692869,
-    ('uniform-100m', 'fp16', 'k_sweep'): 11126890556,
-    ('uniform-100m', 'fp32', 'k_sweep'): 22186918992,
-    ('uniform-100m', 'fp64', 'k_sweep'): 44630775772,
+    ('two-level', 'fp64', 'fine k_sweep'): 117861883,
+    ('two-level', 'fp64', 'coarse k_sweep'): 173247923,
+    ('two-level', 'fp32', 'fine k_sweep'): 62850558,
+    ('two-level', 'fp32', 'coarse k_sweep'): 87725610,
+    ('uniform-100m', 'fp16', 'k_sweep'): 11129113267,
+    ('uniform-100m', 'fp32', 'k_sweep'): 22186270460,
+    ('uniform-100m', 'fp64', 'k_sweep'): 44633250902,
 }
 METRIC = "MLUPS & %HBM roofline, FP64 vs FP16-store; cases/hour + p50/p99 latency"
 
