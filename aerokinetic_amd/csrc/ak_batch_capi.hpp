@@ -5,10 +5,12 @@
 // and history.  One batch step = one coarse step of every member, as the same kernel
 // sequence as a single case, where every kernel runs for ALL members in one launch over
 // their concatenated work lists (ak_batch.hpp):
-//   donor moments -> coarse sweeps -> coarse face flux -> legacy farfield -> donor moments
-//   -> 2 x [prolongation x, y, z -> fine sweeps -> fine face flux -> walls + load partials]
-//   -> reflux -> fine moments -> restriction -> loads.
-// The launch count per step is independent of the number of members (about 20), so small
+//   coarse sweeps -> [coarse face flux, legacy farfield, donor moments at t + dt]
+//   -> prolongation x pass of both substeps -> 2 x [prolongation y/z -> fine sweeps
+//   -> [fine face flux, walls + load partials]] -> reflux -> [fine moments + restriction,
+//   donor moments at t of the next step] -> loads
+// (brackets: one launch; a batch run call starts with the donor moments of every member).
+// The launch count per step is independent of the number of members (12), so small
 // cases that are latency-bound alone fill the GPU together.  The step is captured on the
 // batch stream as a HIP graph per launch variant (lead member's coarse parity x sampled)
 // and replayed while the membership stays the same.

@@ -2,10 +2,12 @@
 //
 // One refined case = a coarse lattice (legacy farfield, z periodic, covered cells kept
 // as donors) + a fine lattice over the box plus two donor layers in x and y (z
-// periodic, Bouzidi walls).  One coarse step:
-//   donor moments (before) -> coarse sweep -> coarse face flux -> coarse boundary
-//   -> donor moments (after) -> 2 x [prolong+inject, fine sweep, fine face flux,
-//   fine walls + loads] -> reflux -> restriction.
+// periodic, Bouzidi walls).  One coarse step (10 launches):
+//   coarse sweep -> [coarse face flux, coarse boundary, donor moments at t + dt]
+//   -> prolongation x pass of both substeps -> 2 x [prolongation y/z + inject, fine sweep,
+//   [fine face flux, fine walls + load partials]] -> [reflux, loads fold or tick,
+//   restriction, donor moments at t of the next step]
+// (brackets: one launch).  A run call starts with the donor moments of the current state.
 #include "ak_refined.hpp"
 
 namespace {
