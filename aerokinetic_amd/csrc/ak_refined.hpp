@@ -30,11 +30,12 @@ struct BoxGeom {
 
 enum { TE_DENSITY = 1, TE_NEGATIVE = 2, TE_MACH = 4, TE_NONFINITE = 8 };
 
-__device__ __forceinline__ void moments_of(const double (&g)[NQ], double (&m)[10], int &err) {
+// Moments of the shifted populations g = f - w as running sums in index order 0..26.  Every
+// moments computation goes through add(), so a form that streams the populations (no
+// 27-value array live in registers) is bit-identical to one that holds them.
+struct MomentSum {
   double d = 0, jx = 0, jy = 0, jz = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0, mn = 1e300;
-#pragma unroll
-  for (int i = 0; i < NQ; ++i) {
-    const double v = g[i];
+  __device__ __forceinline__ void add(int i, double v) {
     d += v;
     jx += cxq(i) * v; jy += cyq(i) * v; jz += czq(i) * v;
     sxx += cxq(i) * cxq(i) * v; syy += cyq(i) * cyq(i) * v; szz += czq(i) * czq(i) * v;
@@ -42,42 +43,59 @@ __device__ __forceinline__ void moments_of(const double (&g)[NQ], double (&m)[10
     const double f = v + wq(i);
     mn = f < mn ? f : mn;
   }
-  const double rho = d + 1.0;
-  if (!(rho > 0)) err |= TE_DENSITY;
-  if (mn < 0) err |= TE_NEGATIVE;
-  if (!isfinite(rho) || !isfinite(jx + jy + jz)) err |= TE_NONFINITE;
-  if (sqrt(jx * jx + jy * jy + jz * jz) / rho / 0.57735026918962576451 > 0.3) err |= TE_MACH;
-  m[0] = rho; m[1] = jx; m[2] = jy; m[3] = jz;
-  m[4] = sxx - d / 3.0 - jx * jx / rho;
-  m[5] = syy - d / 3.0 - jy * jy / rho;
-  m[6] = szz - d / 3.0 - jz * jz / rho;
-  m[7] = sxy - jx * jy / rho;
-  m[8] = sxz - jx * jz / rho;
-  m[9] = syz - jy * jz / rho;
+  __device__ __forceinline__ void finish(double (&m)[10], int &err) const {
+    const double rho = d + 1.0;
+    if (!(rho > 0)) err |= TE_DENSITY;
+    if (mn < 0) err |= TE_NEGATIVE;
+    if (!isfinite(rho) || !isfinite(jx + jy + jz)) err |= TE_NONFINITE;
+    if (sqrt(jx * jx + jy * jy + jz * jz) / rho / 0.57735026918962576451 > 0.3) err |= TE_MACH;
+    m[0] = rho; m[1] = jx; m[2] = jy; m[3] = jz;
+    m[4] = sxx - d / 3.0 - jx * jx / rho;
+    m[5] = syy - d / 3.0 - jy * jy / rho;
+    m[6] = szz - d / 3.0 - jz * jz / rho;
+    m[7] = sxy - jx * jy / rho;
+    m[8] = sxz - jx * jz / rho;
+    m[9] = syz - jy * jz / rho;
+  }
+};
+
+__device__ __forceinline__ void moments_of(const double (&g)[NQ], double (&m)[10], int &err) {
+  MomentSum s;
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) s.add(i, g[i]);
+  s.finish(m, err);
 }
 
-// tensor-product equilibrium + second-order Hermite stress, shifted by the weights
-__device__ __forceinline__ void populations_of(const double (&m)[10], double (&g)[NQ], int &err) {
-  const double rho = m[0];
-  if (!(rho > 0)) err |= TE_DENSITY;
-  const double u[3] = {m[1] / rho, m[2] / rho, m[3] / rho};
-  if (sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / 0.57735026918962576451 > 0.3) err |= TE_MACH;
-  double ph[3][3];
-  for (int a = 0; a < 3; ++a) {
-    ph[a][1] = 2.0 / 3.0 - u[a] * u[a];
-    ph[a][0] = (1.0 / 3.0 + u[a] * u[a] - u[a]) * 0.5;
-    ph[a][2] = (1.0 / 3.0 + u[a] * u[a] + u[a]) * 0.5;
+// tensor-product equilibrium + second-order Hermite stress, shifted by the weights; pop(i)
+// gives one population, so callers may store each as it is made
+struct PopBasis {
+  double rho, ph[3][3];
+  __device__ __forceinline__ PopBasis(const double (&m)[10], int &err) {
+    rho = m[0];
+    if (!(rho > 0)) err |= TE_DENSITY;
+    const double u[3] = {m[1] / rho, m[2] / rho, m[3] / rho};
+    if (sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / 0.57735026918962576451 > 0.3) err |= TE_MACH;
+    for (int a = 0; a < 3; ++a) {
+      ph[a][1] = 2.0 / 3.0 - u[a] * u[a];
+      ph[a][0] = (1.0 / 3.0 + u[a] * u[a] - u[a]) * 0.5;
+      ph[a][2] = (1.0 / 3.0 + u[a] * u[a] + u[a]) * 0.5;
+    }
   }
-#pragma unroll
-  for (int i = 0; i < NQ; ++i) {
+  __device__ __forceinline__ double pop(int i, const double (&m)[10], int &err) const {
     const int cx = cxq(i), cy = cyq(i), cz = czq(i);
     const double eq = rho * ph[0][cx + 1] * ph[1][cy + 1] * ph[2][cz + 1];
     const double h = (cx * cx - 1.0 / 3.0) * m[4] + (cy * cy - 1.0 / 3.0) * m[5] + (cz * cz - 1.0 / 3.0) * m[6]
                    + 2.0 * (cx * cy * m[7] + cx * cz * m[8] + cy * cz * m[9]);
     const double p = eq + wq(i) * 4.5 * h;
     if (!(p >= 0)) err |= (isfinite(p) ? TE_NEGATIVE : TE_NONFINITE);
-    g[i] = p - wq(i);
+    return p - wq(i);
   }
+};
+
+__device__ __forceinline__ void populations_of(const double (&m)[10], double (&g)[NQ], int &err) {
+  const PopBasis b(m, err);
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) g[i] = b.pop(i, m, err);
 }
 
 __device__ __forceinline__ void rescale(double (&m)[10], double shear, double bulk) {
@@ -140,9 +158,10 @@ __device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__re
   } else if ((SKIP >= 1 && covered) || (SKIP >= 2 && refluxed)) {
     return;
   } else {
-    double g[NQ];
-    gather_in<S, PE>(A, L, x, y, z, inv_scale, g);
-    moments_of(g, m, e);
+    MomentSum ms;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) ms.add(i, (double)ld<S>(A + loc_in<PE>(L, x, y, z, i), inv_scale));
+    ms.finish(m, e);
   }
   for (int c = 0; c < 10; ++c) M[t * 10 + c] = m[c];
   if (e) atomicOr(err, e);
@@ -258,10 +277,10 @@ __device__ __forceinline__ void prolong_yz_body(const double *__restrict__ X, ty
       for (int kk = 0; kk < 10; ++kk) m[kk] += w * q[kk];
     }
     rescale(m, V.shear_cf, V.bulk_cf);
-    double g[NQ];
     int e = 0;
-    populations_of(m, g, e);
-    scatter_in<S, PE>(A, L, x, y, z, scale, g);
+    const PopBasis pb(m, e);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) stv<S>(A + loc_in<PE>(L, x, y, z, i), (typename St<S>::C)pb.pop(i, m, e), scale);
     if (e) atomicOr(err, e);
   }
 }
@@ -431,12 +450,12 @@ __device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, D
   }
   if (M != nullptr) {
     const double h = lane < NQ ? stored_value<S>(g, scale, inv) : 0.0;
-    double hv[NQ];
+    MomentSum ms;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) hv[q] = __shfl(h, q, 64);
+    for (int q = 0; q < NQ; ++q) ms.add(q, __shfl(h, q, 64));
     if (lane == 0) {
       double md[10];
-      moments_of(hv, md, e);
+      ms.finish(md, e);
       const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
       const int bx = x - B.lo[0], by = y - B.lo[1];
       for (int k = ((zpos - B.lo[2] + 3) % L.nz + L.nz) % L.nz; k < dz; k += L.nz) {
@@ -479,9 +498,10 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
   int e = 0;
   double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (fx < 2 * B.nb[0] && fy < 2 * B.nb[1] && !in_core(B, fx / 2, fy / 2)) {
-    double g[NQ];
-    gather_in<S, PE>(Af, Lf, B.pad + fx, B.pad + fy, fz, finv, g);
-    moments_of(g, m, e);
+    MomentSum ms;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) ms.add(i, (double)ld<S>(Af + loc_in<PE>(Lf, B.pad + fx, B.pad + fy, fz, i), finv));
+    ms.finish(m, e);
   }
 #pragma unroll
   for (int k = 0; k < 10; ++k) sm[k * 256 + tid] = m[k];
@@ -490,9 +510,10 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
     const int ox = tid % RT_X, oy = tid / RT_X;
     const int bx = RT_X * tbx + ox, by = RT_Y * tby + oy;
     if (bx < B.nb[0] && by < B.nb[1]) {
-      double g[NQ];
+      const int X = B.lo[0] + bx, Y = B.lo[1] + by, Z = B.lo[2] + bz;
       if (in_core(B, bx, by)) {
-        for (int i = 0; i < NQ; ++i) g[i] = F.g[i];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) stv<S>(Ac + loc_in<PEC>(Lc, X, Y, Z, i), (typename St<S>::C)F.g[i], cscale);
       } else {
         double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int c = 0; c < 2; ++c)
@@ -503,11 +524,32 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
             }
         for (int k = 0; k < 10; ++k) acc[k] *= 0.125;
         rescale(acc, V.shear_fc, V.bulk_fc);
-        populations_of(acc, g, e);
+        // with M the stored values also feed the donor moments of the next step.  FP32/FP16
+        // store each population as it is made (holding all 27 beside their rounded values
+        // costs ~45 registers there); FP64 stores exactly what it computes and measured
+        // faster with the 27 held
+        const PopBasis pb(acc, e);
+        MomentSum ms;
+        if constexpr (S == 0) {
+          double g[NQ];
+#pragma unroll
+          for (int i = 0; i < NQ; ++i) g[i] = pb.pop(i, acc, e);
+          scatter_in<S, PEC>(Ac, Lc, X, Y, Z, cscale, g);
+          if (M != nullptr) {
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) ms.add(i, g[i]);
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < NQ; ++i) {
+            const double gi = pb.pop(i, acc, e);
+            stv<S>(Ac + loc_in<PEC>(Lc, X, Y, Z, i), (typename St<S>::C)gi, cscale);
+            if (M != nullptr) ms.add(i, stored_value<S>(gi, cscale, cinv));
+          }
+        }
         if (M != nullptr) {
-          double h[NQ], md[10];
-          for (int i = 0; i < NQ; ++i) h[i] = stored_value<S>(g[i], cscale, cinv);
-          moments_of(h, md, e);
+          double md[10];
+          ms.finish(md, e);
           const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
           for (int k = ((bz + 3) % Lc.nz + Lc.nz) % Lc.nz; k < dz; k += Lc.nz) {
             double *o = M + ((((long long)k * dy) + (by + 3)) * dx + (bx + 3)) * 10;
@@ -515,7 +557,6 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
           }
         }
       }
-      scatter_in<S, PEC>(Ac, Lc, B.lo[0] + bx, B.lo[1] + by, B.lo[2] + bz, cscale, g);
     }
   }
   if (e) atomicOr(err, e);
