@@ -105,12 +105,6 @@ __device__ __forceinline__ void rescale(double (&m)[10], double shear, double bu
 }
 
 template <int S, bool PE>
-__device__ __forceinline__ void gather_in(const typename St<S>::T *A, const Desc &L, int x, int y, int z, float inv,
-                                          double (&g)[NQ]) {
-#pragma unroll
-  for (int i = 0; i < NQ; ++i) g[i] = (double)ld<S>(A + loc_in<PE>(L, x, y, z, i), inv);
-}
-template <int S, bool PE>
 __device__ __forceinline__ void scatter_in(typename St<S>::T *A, const Desc &L, int x, int y, int z, float scale,
                                            const double (&g)[NQ]) {
 #pragma unroll
