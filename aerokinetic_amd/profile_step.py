@@ -9,7 +9,9 @@ farfield), `uniform-100m` (1536x1024x64) and
 `two-level` (coarse 256x192x8 + fine 160x96x16), all NACA0012 (see
 aerokinetic_amd.cases); `copy`, a device-to-device copy of a buffer of known size
 (the calibration kernel for memory counters); `serve`, the batched service over
-seeded two-level requests (--steps = the per-case step cap).  Each workload prints one
+seeded two-level requests (--steps = the per-case step cap); `slab-y` (1536x128x64) and
+`slab-x` (192x1024x64), one GPU's share of the 1536x1024x64 lattice at 8 GPUs as a plain
+fluid slab (sweep kernel only).  Each workload prints one
 line per phase with HIP-event timings of the lattice's stream, so a profiler's
 per-kernel averages can be set beside them.
 """
@@ -106,11 +108,34 @@ def run_serve(storage, cases, rate, chunk, window, max_steps):
          f"batch steps {rep.gpu_steps}, serving-thread seconds {s['host_s']}")
 
 
+def run_slab(shape, storage, steps, warmup):
+    """The sweep of one GPU's share of the 1536x1024x64 lattice at 8 GPUs: a plain fluid
+    slab (no body, seeded field, legacy farfield), the sweep kernel timed with its own events."""
+    import math
+    import numpy as np
+    from .lattice import Lattice
+    nx, ny, nz = shape
+    lat = Lattice(nx, ny, nz, storage=storage, omega_shear=1.99, omega_bulk=1.0)
+    lat.set_solid(np.zeros((nx, ny), dtype=np.uint8))
+    seed = {"U": 0.0577, "amp": 1e-4, "phase_y": 0.3, "phase_z": 0.1, "ky": 2 * math.pi, "kz": 2 * math.pi,
+            "lex": 1.0, "ley": 1.0}
+    lat.init_seeded(seed, (0.0, 0.0), 1.0 / 128)
+    lat.set_farfield("legacy", 0.0577)
+    lat.run_profiled(warmup, 0, 0)
+    sweep_ms, _ = lat.run_profiled(steps, warmup, 0)
+    ms = sweep_ms / steps
+    cells = nx * ny * nz
+    gbs = cells * 54 * {"fp64": 8, "fp32": 4, "fp16": 2}[storage] / (ms * 1e-3) / 1e9
+    _log(f"slab {nx}x{ny}x{nz} storage {storage}: sweep {ms:.4f} ms, {cells / (ms * 1e3):.0f} MLUPS, "
+         f"{gbs:.0f} GB/s ({gbs / 80:.1f} % of 8 TB/s)")
+    lat.close()
+
+
 def main(argv=None):
     from . import cases
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--workload", choices=("uniform-c32", "exterior-c32", "uniform-100m", "two-level", "copy", "serve"),
-                    required=True)
+    ap.add_argument("--workload", choices=("uniform-c32", "exterior-c32", "uniform-100m", "two-level", "copy", "serve",
+                                           "slab-y", "slab-x"), required=True)
     ap.add_argument("--storage", choices=("fp64", "fp32", "fp16"), default="fp64")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -125,6 +150,8 @@ def main(argv=None):
         run_copy(a.copy_gib, a.steps)
     elif a.workload == "serve":
         run_serve(a.storage, a.cases, a.rate, a.chunk, 400, a.steps)
+    elif a.workload in ("slab-y", "slab-x"):
+        run_slab((1536, 128, 64) if a.workload == "slab-y" else (192, 1024, 64), a.storage, a.steps, a.warmup)
     elif a.workload == "two-level":
         run_two_level(cases.NACA0012_REFINED_256x192x8, a.storage, a.steps, a.warmup, a.sample_every, not a.no_graphs)
     else:
