@@ -59,13 +59,13 @@ BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
 # map the counters onto the 8 GiB read + 8 GiB written by the streaming copy), for the
 # kernels whose roofline is reported: (workload, storage, kernel) -> bytes.
 MEASURED_TRAFFIC = {  # profiles/hbm_traffic.md (mean over both AA parities)
-    ('two-level', 'fp64', 'fine k_sweep'): 117861883,
-    ('two-level', 'fp64', 'coarse k_sweep'): 173247923,
-    ('two-level', 'fp32', 'fine k_sweep'): 62850558,
-    ('two-level', 'fp32', 'coarse k_sweep'): 87725610,
-    ('uniform-100m', 'fp16', 'k_sweep'): 11129113267,
-    ('uniform-100m', 'fp32', 'k_sweep'): 22186270460,
-    ('uniform-100m', 'fp64', 'k_sweep'): 44633250902,
+    ('two-level', 'fp64', 'fine k_sweep'): 117721287,
+    ('two-level', 'fp64', 'coarse k_sweep'): 173270528,
+    ('two-level', 'fp32', 'fine k_sweep'): 62851843,
+    ('two-level', 'fp32', 'coarse k_sweep'): 87715949,
+    ('uniform-100m', 'fp16', 'k_sweep'): 11123233305,
+    ('uniform-100m', 'fp32', 'k_sweep'): 22186486482,
+    ('uniform-100m', 'fp64', 'k_sweep'): 44633303245,
 }
 METRIC = "MLUPS & %HBM roofline, FP64 vs FP16-store; cases/hour + p50/p99 latency"
 
