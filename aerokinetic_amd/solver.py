@@ -18,8 +18,9 @@ loads, frames, residuals and the density mean cover the physical box only.
 All numerics run in the native HIP library: one AA-pattern lattice in HBM,
 boundary kernels before each sweep, loads and low-Mach guards reduced on the
 device into a history buffer read back in batches.  Extra keyword-only
-arguments select the storage precision ("fp64", "fp32", "fp16"), the device and
-an explicit plan (fixed lattice shapes).
+arguments select the storage precision ("fp64", "fp32", "fp16"), the device, an
+explicit plan (fixed lattice shapes) and the sweep tile (bx, by), or "auto" for the
+tile the autotuner (aerokinetic_amd.autotune) measures fastest on this lattice.
 """
 from __future__ import annotations
 
@@ -102,7 +103,7 @@ class Solver:
         self._progress("Building the D3Q27 cumulant lattice on the GPU")
         self.lattice = Lattice(X, Y, nz, storage=storage, omega_shear=self.plan.relaxation_rate,
                                omega_bulk=1.0, fp16_scale=fp16_scale, device=device)
-        if tile:
+        if tile and tile != "auto":
             self.lattice.set_tile(*tile)
         self.lattice.set_graphs(graphs)
         self.lattice.set_solid(storage_solid)
@@ -111,6 +112,13 @@ class Solver:
         self.lattice.set_walls(self.links, len(self.contours), self.storage_origin[0] / self.plan.dx,
                                self.storage_origin[1] / self.plan.dx, grid.cells_per_chord, flow.lattice_speed)
         self._initialize()
+        self.tile, self.tile_timings = tile, {}
+        if tile == "auto":
+            # time every candidate tile on this lattice (cached per shape and storage), then start
+            # again from the seeded field: the tuning steps advanced the state
+            from .autotune import tune
+            self.tile, self.tile_timings = tune(self.lattice)
+            self._initialize()
         self.last_loads = None
 
     # ------------------------------------------------------------------ state

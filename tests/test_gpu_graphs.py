@@ -119,3 +119,20 @@ def test_x_face_cache_single_steps_match_one_run(storage):
     for _ in range(7):
         many.step()
     np.testing.assert_array_equal(one.populations(), many.populations())
+
+
+def test_autotuned_tile_gives_identical_results():
+    """tile="auto" times every candidate sweep tile on the case's own lattice, keeps the
+    fastest and restarts from the seeded field: the run must equal one on the default tile
+    bit for bit (a tile changes only which threads sweep which cells)."""
+    from aerokinetic_amd import autotune
+    from aerokinetic_amd.solver import Solver
+    section, flow, grid, plan = small_case("characteristic_transport_v1")
+    print("case autotune", plan.shape, flush=True)
+    tuned = Solver(section, flow, grid, plan=plan, tile="auto")
+    plain = Solver(section, flow, grid, plan=plan)
+    assert tuned.tile in autotune.CANDIDATES
+    assert tuned.steps == 0
+    a, b = tuned.advance(12, 3), plain.advance(12, 3)
+    np.testing.assert_array_equal(_rows(a), _rows(b))
+    np.testing.assert_array_equal(tuned.populations(), plain.populations())
