@@ -84,6 +84,23 @@ def roofline(bytes_per_launch, seconds_per_launch, traffic_key=None, copy_gbs=No
     return out
 
 
+def refined_fields(s):
+    """rho and u of both levels (coarse cells, then the padded fine lattice without its solid
+    cells), flattened into one set for the field L2 against the FP64-storage run."""
+    import numpy as np
+    from aerokinetic_amd.d3q27 import C
+    rhos, us, masks = [], [], []
+    for level in (0, 1):
+        g = s.populations(level)
+        rho = 1.0 + g.sum(axis=-1)
+        u = (g @ C.astype(float)) / rho[..., None]
+        mask = np.ones(rho.shape, dtype=bool)
+        if level == 1:  # the owned part of the padded fine lattice holds the solid cells
+            mask[s.pad:-s.pad, s.pad:-s.pad] &= ~s._fine_solid_xy[:, :, None]
+        rhos.append(rho.ravel()); us.append(u.reshape(-1, 3)); masks.append(mask.ravel())
+    return np.concatenate(rhos), np.concatenate(us), np.concatenate(masks)
+
+
 def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
     import numpy as np
     import torch
@@ -99,6 +116,7 @@ def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
     recs = s.advance(steps, 1)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    fields = refined_fields(s)
     tot, cs, fs = s.run_profiled(profile_steps)
     fnx, fny = s.fine_shape[0] + 4, s.fine_shape[1] + 4
     fine_cells = (fnx * fny - int(s._fine_solid_xy.sum())) * s.fine_shape[2]
@@ -114,6 +132,7 @@ def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
            "coarse_sweep_roofline": roofline(coarse * 2 * 27 * b, coarse_sweep, ("two-level", storage, "coarse k_sweep"),
                                              copy_gbs),
            "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
+           "fields": fields,
            "diag": s.diagnostics()}
     s.close()
     torch.cuda.empty_cache()
