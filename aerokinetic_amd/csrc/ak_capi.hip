@@ -226,9 +226,11 @@ void launch_boundary(Lattice *s, bool loads) {
   }
 }
 
-void launch_loads_final(Lattice *s) {
+// end of a sampled step: diagnostics and loads into the history row, slots cleared, cursor
+// advanced (one block)
+void launch_step_final(Lattice *s) {
   const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
-  hipLaunchKernelGGL(k_loads_final, dim3(1), dim3(64), 0, s->stream, s->d_partials, s->nlinks ? s->wall_blocks : 0,
+  hipLaunchKernelGGL(k_step_final, dim3(1), dim3(64), 0, s->stream, s->d_partials, s->nlinks ? s->wall_blocks : 0,
                      s->nelem, 1.0 / qa, 1.0 / (qa * s->chord), 1.0, s->d_ctl, s->d_hist, s->d_diag);
 }
 
@@ -286,11 +288,10 @@ void one_step(Lattice *s, bool sample, hipEvent_t *ev = nullptr) {
   // the exterior farfield accumulates the wall mass exchange on every step
   if (pe) launch_boundary<S, true>(s, sample || ext);
   else launch_boundary<S, false>(s, sample || ext);
-  if (sample) launch_loads_final(s);
   if (ev) (void)hipEventRecord(ev[1], s->stream);
   launch_sweep<S>(s, pe, sample, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
   if (ext) launch_exterior<S>(s, pe, sample);
-  if (sample) hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, s->d_diag, s->d_ctl, s->d_hist);
+  if (sample) launch_step_final(s);
   else hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s->stream, s->d_ctl);
   s->sweeps++;
 }
@@ -583,6 +584,8 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
   }
   (void)hipMemset(s->d_solid, 0, (size_t)(nx + 2) * (ny + 2));
   (void)hipMemset(s->d_hist, 0, (size_t)s->hist_cap * s->hist_width * sizeof(double));
+  // the first sampled sweep accumulates into clean slots; every sampled step's end clears them
+  hipLaunchKernelGGL(k_diag_clear, dim3(1), dim3(256), 0, s->stream, s->d_diag);
   *handle = s;
   return 0;
 }

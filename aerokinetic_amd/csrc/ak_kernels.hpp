@@ -354,9 +354,8 @@ __global__ void k_tick(StepCtl *ctl) {
 
 // Fold the wall-load partials (per-block sums, added in a fixed order) into the current
 // history row: cl = Fy fscale, cd = Fx fscale, cm = -Mz mscale, mass exchange = mass
-// vscale.  diag != nullptr:
-// also clear the diagnostic slots for the sweep that follows (k_diag_final ends
-// the step); diag == nullptr: this kernel ends the sampled step.  `acc` is LOADS_LDS
+// vscale.  diag != nullptr: also clear the diagnostic slots (k_step_final has read them
+// and then ends the step); diag == nullptr: this body ends the sampled step.  `acc` is LOADS_LDS
 // doubles of the calling kernel's shared memory (kernels that fuse several phases give
 // every phase the same array).
 constexpr int LOADS_LDS = MAX_ELEM * 5 * 64;
@@ -403,27 +402,42 @@ __device__ __forceinline__ void loads_final_body(const double *__restrict__ part
   }
 }
 
-__global__ void k_loads_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
-                              double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
-  __shared__ double acc[LOADS_LDS];
-  loads_final_body(partials, nblocks, nelem, fscale, mscale, vscale, ctl, hist, diag, acc);
+// Diagnostics slots set for a sampled sweep to accumulate into (lattice creation; each
+// sampled step's end clears them again).
+__global__ void k_diag_clear(DiagAcc *diag) {
+  for (int k = threadIdx.x; k < DIAG_SLOTS; k += blockDim.x) {
+    diag->s[k].u2max = 0ull;
+    diag->s[k].drmax = 0ull;
+    diag->s[k].rhomin = (unsigned long long)__double_as_longlong(1e300);
+    diag->s[k].nonfinite = 0ull;
+  }
 }
 
-__global__ void k_diag_final(const DiagAcc *diag, StepCtl *ctl, double *__restrict__ hist) {
-  // 64 threads fold the slots; then a fixed-order wave reduction
+// End of a sampled uniform step, one block: the sweep's diagnostics slots reduced (64
+// threads, then a fixed-order wave reduction) into the current history row, the wall
+// loads folded into it, the slots cleared for the next sampled sweep, the step cursor
+// advanced.
+__global__ void k_step_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
+                             double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
+  __shared__ double acc[LOADS_LDS];
   unsigned long long u = 0ull, d = 0ull, r = (unsigned long long)__double_as_longlong(1e300), b = 0ull;
-  for (int k = threadIdx.x; k < DIAG_SLOTS; k += 64) {
-    const DiagSlot &sl = diag->s[k];
-    u = sl.u2max > u ? sl.u2max : u;
-    d = sl.drmax > d ? sl.drmax : d;
-    r = sl.rhomin < r ? sl.rhomin : r;
-    b = sl.nonfinite > b ? sl.nonfinite : b;
+  if (threadIdx.x < 64) {
+    for (int k = threadIdx.x; k < DIAG_SLOTS; k += 64) {
+      const DiagSlot &sl = diag->s[k];
+      u = sl.u2max > u ? sl.u2max : u;
+      d = sl.drmax > d ? sl.drmax : d;
+      r = sl.rhomin < r ? sl.rhomin : r;
+      b = sl.nonfinite > b ? sl.nonfinite : b;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+      const unsigned long long u2 = __shfl_xor(u, o, 64), d2 = __shfl_xor(d, o, 64);
+      const unsigned long long r2 = __shfl_xor(r, o, 64), b2 = __shfl_xor(b, o, 64);
+      u = u2 > u ? u2 : u; d = d2 > d ? d2 : d; r = r2 < r ? r2 : r; b = b2 > b ? b2 : b;
+    }
   }
-  for (int o = 32; o >= 1; o >>= 1) {
-    const unsigned long long u2 = __shfl_xor(u, o, 64), d2 = __shfl_xor(d, o, 64);
-    const unsigned long long r2 = __shfl_xor(r, o, 64), b2 = __shfl_xor(b, o, 64);
-    u = u2 > u ? u2 : u; d = d2 > d ? d2 : d; r = r2 < r ? r2 : r; b = b2 > b ? b2 : b;
-  }
+  // loads into the row; with `diag` the same lanes that read each slot above clear it after
+  // (and the cursor stays for the diagnostics columns below)
+  loads_final_body(partials, nblocks, nelem, fscale, mscale, vscale, ctl, hist, diag, acc);
   if (threadIdx.x == 0) {
     double *row = hist + (long long)ctl->row * ctl->width;
     row[4] = sqrt(3.0) * sqrt(__longlong_as_double((long long)u));
