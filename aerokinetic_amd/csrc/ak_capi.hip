@@ -57,6 +57,7 @@ struct Lattice {
   double rx_off = 0, ry_off = 0, chord = 1, U = 0;
   double *d_partials = nullptr;
   int wall_blocks = 0;
+  bool walls_clear = false;  // every wall link at least 3 cells from the x and y edges
   // farfield
   int ff_kind = AK_FARFIELD_PERIODIC;
   FFParams ffp{};
@@ -202,7 +203,29 @@ template <int S, bool PE>
 void launch_boundary(Lattice *s, bool loads) {
   using T = typename St<S>::T;
   T *A = (T *)s->pop;
-  if (s->ff_kind == AK_FARFIELD_CHARACTERISTIC && s->nff) {
+  const bool charff = s->ff_kind == AK_FARFIELD_CHARACTERISTIC && s->nff;
+  const bool leg = (s->ff_kind == AK_FARFIELD_LEGACY || s->ff_kind == AK_FARFIELD_EXTERIOR) && s->nleg;
+  if (s->nlinks && s->walls_clear && ((charff && s->d_fc) || leg)) {
+    // walls clear of the edges: one launch for the walls and the edge phase that does not
+    // depend on them (the x-face copy, or the legacy links), then the farfield
+    WallLinks W{s->d_wxy, s->d_wdir, s->d_wq, s->d_welem, s->nlinks, s->nelem, s->rx_off, s->ry_off, 1.0};
+    LegacyLinks K{s->d_legx, s->d_legz, s->d_legdir, s->d_legkind, s->d_pdf, s->d_pdfnd, s->nleg};
+    const long long nb_edge = charff ? blocks_for(2LL * s->ny * s->nz, 256) : blocks_for(s->nleg, 256);
+    const dim3 g((unsigned)(s->wall_blocks + nb_edge)), b(256);
+    T *fc = (T *)s->d_fc;
+#define AK_WE(LD, E) hipLaunchKernelGGL((k_walls_edge<S, PE, LD, E>), g, b, 0, s->stream, A, s->L, W, s->scale, \
+                                        s->inv_scale, s->d_partials, s->wall_blocks, K, s->U, fc)
+    if (charff) { if (loads) AK_WE(true, 1); else AK_WE(false, 1); }
+    else { if (loads) AK_WE(true, 0); else AK_WE(false, 0); }
+#undef AK_WE
+    if (charff) {
+      FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff, s->d_fc};
+      hipLaunchKernelGGL((k_farfield<S, PE>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, A, s->L, I,
+                         s->ffp, s->scale, s->inv_scale);
+    }
+    return;
+  }
+  if (charff) {
     if (s->d_fc)
       hipLaunchKernelGGL((k_face_fill<S, PE>), dim3(blocks_for(2LL * s->ny * s->nz, 256)), dim3(256), 0, s->stream,
                          (const T *)A, s->L, (T *)s->d_fc);
@@ -253,7 +276,7 @@ void launch_ext_predict(Lattice *s, const StepCtl *ctl, double t0) {
   hipLaunchKernelGGL(k_ext_stage<3>, g, b, 0, s->stream, E, ctl, t0);
 }
 
-// Exterior phases after the sweep of form `odd` (the ring was skipped by it).
+// Exterior phases after the sweep of form `odd` (the ring was skipped by it); they end the step.
 template <int S>
 void launch_exterior(Lattice *s, bool odd, bool sample) {
   using T = typename St<S>::T;
@@ -273,9 +296,11 @@ void launch_exterior(Lattice *s, bool odd, bool sample) {
   else { if (sample) AK_RING(false, true); else AK_RING(false, false); }
 #undef AK_RING
   launch_ext_predict(s, s->d_ctl, 0.0);
-  hipLaunchKernelGGL(k_ext_final, dim3(1), dim3(64), 0, s->stream, (const double *)s->d_partials,
-                     s->nlinks ? s->wall_blocks : 0, s->nelem, E, s->ext_ring_blocks, (const StepCtl *)s->d_ctl,
-                     s->d_hist, sample ? 1 : 0);
+  // accumulators and the end of the step
+  const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
+  hipLaunchKernelGGL(k_ext_end, dim3(1), dim3(64), 0, s->stream, (const double *)s->d_partials,
+                     s->nlinks ? s->wall_blocks : 0, s->nelem, E, s->ext_ring_blocks, s->d_ctl, s->d_hist,
+                     sample ? 1 : 0, 1.0 / qa, 1.0 / (qa * s->chord), 1.0, s->d_diag);
 }
 
 // One uniform step; the step number and history row come from s->d_ctl.  With `ev`, four
@@ -290,8 +315,8 @@ void one_step(Lattice *s, bool sample, hipEvent_t *ev = nullptr) {
   else launch_boundary<S, false>(s, sample || ext);
   if (ev) (void)hipEventRecord(ev[1], s->stream);
   launch_sweep<S>(s, pe, sample, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
-  if (ext) launch_exterior<S>(s, pe, sample);
-  if (sample) launch_step_final(s);
+  if (ext) launch_exterior<S>(s, pe, sample);  // ends the step
+  else if (sample) launch_step_final(s);
   else hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s->stream, s->d_ctl);
   s->sweeps++;
 }
@@ -660,6 +685,9 @@ int ak_set_walls(void *handle, int n, const int32_t *x, const int32_t *y, const 
   s->nelem = nelem;
   s->rx_off = rx_off; s->ry_off = ry_off; s->chord = chord_cells; s->U = lattice_speed;
   s->wall_blocks = (int)blocks_for((long long)n * s->nz, 256);
+  s->walls_clear = true;
+  for (int k = 0; k < n; ++k)
+    if (x[k] < 3 || x[k] > s->nx - 4 || y[k] < 3 || y[k] > s->ny - 4) s->walls_clear = false;
   if (s->d_partials) { dfree(s, s->d_partials); s->d_partials = nullptr; }
   if (dalloc(s, &s->d_partials, (size_t)std::max(s->wall_blocks, 1) * nelem * 5)) return -1;
   return 0;

@@ -14,8 +14,8 @@
 //                block partials, store the ring's macroscopic state for the next
 //                prediction, collide and scatter like the sweep;
 //   k_ext_stage  the three SSP-RK3 stages of the ring predictor (exterior.py:101-133);
-//   k_ext_final  fixed-order sums of the wall mass exchange and the ring source into the
-//                accumulators and, on sampled steps, the history row.
+//   k_ext_end    fixed-order sums of the wall mass exchange and the ring source into the
+//                accumulators, then the end of the step (on sampled steps the history row).
 // The state between steps is post-collision like the other farfields; the reference's
 // pre-collision state has the same rho and u.
 #pragma once
@@ -390,9 +390,10 @@ __global__ __launch_bounds__(128) void k_ext_ring(typename St<S>::T *__restrict_
 // One 64-thread block: fixed-order sums of this step's wall mass exchange (wall partials,
 // component 4) and ring source partials into the accumulators; on sampled steps the
 // exterior columns of the current history row.
-__global__ void k_ext_final(const double *__restrict__ wall_part, int wall_blocks, int nelem, ExtDev E,
-                            int ring_blocks, const StepCtl *__restrict__ ctl, double *__restrict__ hist, int sample) {
-  __shared__ double acc[6][64];
+// `acc` is at least 6 x 64 doubles of the calling kernel's shared memory.
+__device__ __forceinline__ void ext_final_body(const double *__restrict__ wall_part, int wall_blocks, int nelem,
+                                               const ExtDev &E, int ring_blocks, const StepCtl *__restrict__ ctl,
+                                               double *__restrict__ hist, int sample, double (*acc)[64]) {
   const int lane = threadIdx.x;
   double wm = 0.0, s[5] = {0, 0, 0, 0, 0};
   for (int b = lane; b < wall_blocks; b += 64)
@@ -425,6 +426,19 @@ __global__ void k_ext_final(const double *__restrict__ wall_part, int wall_block
       row[EXT_COL + 2] = a.src_total[0];
     }
   }
+}
+// End of an exterior step, one 64-thread block: the accumulators above, then on sampled
+// steps the end of a sampled step (diagnostics and loads into the history row, slots
+// cleared, cursor advanced), otherwise the cursor advanced.
+static_assert(LOADS_LDS >= 6 * 64, "shared accumulator size");
+__global__ void k_ext_end(const double *__restrict__ wall_part, int wall_blocks, int nelem, ExtDev E,
+                          int ring_blocks, StepCtl *ctl, double *__restrict__ hist, int sample, double fscale,
+                          double mscale, double vscale, DiagAcc *diag) {
+  __shared__ double acc[LOADS_LDS];
+  ext_final_body(wall_part, wall_blocks, nelem, E, ring_blocks, ctl, hist, sample, (double (*)[64])acc);
+  __syncthreads();  // the accumulator array is reused below
+  if (sample) step_final_body(wall_part, wall_blocks, nelem, fscale, mscale, vscale, ctl, hist, diag, acc);
+  else if (threadIdx.x == 0) ctl->step += 1;
 }
 
 // ------------------------------------------------------------------ set-up

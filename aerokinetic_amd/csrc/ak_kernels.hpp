@@ -417,9 +417,10 @@ __global__ void k_diag_clear(DiagAcc *diag) {
 // threads, then a fixed-order wave reduction) into the current history row, the wall
 // loads folded into it, the slots cleared for the next sampled sweep, the step cursor
 // advanced.
-__global__ void k_step_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
-                             double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
-  __shared__ double acc[LOADS_LDS];
+// `acc` is LOADS_LDS doubles of the calling kernel's shared memory.
+__device__ __forceinline__ void step_final_body(const double *__restrict__ partials, int nblocks, int nelem,
+                                                double fscale, double mscale, double vscale, StepCtl *ctl,
+                                                double *__restrict__ hist, DiagAcc *diag, double *__restrict__ acc) {
   unsigned long long u = 0ull, d = 0ull, r = (unsigned long long)__double_as_longlong(1e300), b = 0ull;
   if (threadIdx.x < 64) {
     for (int k = threadIdx.x; k < DIAG_SLOTS; k += 64) {
@@ -447,6 +448,11 @@ __global__ void k_step_final(const double *__restrict__ partials, int nblocks, i
     ctl->step += 1;
     ctl->row += 1;
   }
+}
+__global__ void k_step_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
+                             double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
+  __shared__ double acc[LOADS_LDS];
+  step_final_body(partials, nblocks, nelem, fscale, mscale, vscale, ctl, hist, diag, acc);
 }
 
 // ------------------------------------------------------------------ farfield
@@ -631,13 +637,17 @@ __global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict_
 
 // Gather pass of the x-face cache (state in form PE); thread = (y, side, z).
 template <int S, bool PE>
-__global__ void k_face_fill(const typename St<S>::T *__restrict__ A, Desc L, typename St<S>::T *__restrict__ fc) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void face_fill_body(const typename St<S>::T *__restrict__ A, Desc L,
+                                               typename St<S>::T *__restrict__ fc, long long t) {
   if (t >= 2LL * L.ny * L.nz) return;
   const int y = (int)(t % L.ny), side = (int)((t / L.ny) % 2), z = (int)(t / (2LL * L.ny));
   const int x = side ? L.nx - 1 : 0;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) fc[face_index(L, side, y, z, q)] = A[loc_out<PE>(L, x, y, z, q)];
+}
+template <int S, bool PE>
+__global__ void k_face_fill(const typename St<S>::T *__restrict__ A, Desc L, typename St<S>::T *__restrict__ fc) {
+  face_fill_body<S, PE>(A, L, fc, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Initial farfield state from the initial populations (post-even form): incoming =
@@ -746,6 +756,26 @@ template <int S, bool PE>
 __global__ __launch_bounds__(128) void k_legacy(typename St<S>::T *__restrict__ A, Desc L, LegacyLinks K,
                                                 double U, float scale, float inv_scale) {
   legacy_body<S, PE>(A, L, K, U, scale, inv_scale, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The walls and one edge phase as one launch over consecutive block ranges, so that their
+// latencies overlap.  The walls read fluid slots and write slots of solid cells; the edge
+// phase reads fluid slots only and writes ghost slots (legacy links, EDGE 0) or the x-face
+// copy (EDGE 1): no slot written by one is touched by the other.  Wall blocks come first,
+// so the wall partials keep their per-block layout.
+template <int S, bool PE, bool LOADS, int EDGE>
+__global__ __launch_bounds__(256) void k_walls_edge(typename St<S>::T *__restrict__ A, Desc L, WallLinks W,
+                                                    float scale, float inv_scale, double *__restrict__ partials,
+                                                    int nb_wall, LegacyLinks K, double U,
+                                                    typename St<S>::T *__restrict__ fc) {
+  const int b = blockIdx.x;
+  if (b < nb_wall) {
+    walls_body<S, PE, LOADS>(A, L, W, scale, inv_scale, partials, (long long)b * 256 + threadIdx.x, b);
+  } else if constexpr (EDGE == 0) {
+    legacy_body<S, PE>(A, L, K, U, scale, inv_scale, (long long)(b - nb_wall) * 256 + threadIdx.x);
+  } else {
+    face_fill_body<S, PE>(A, L, fc, (long long)(b - nb_wall) * 256 + threadIdx.x);
+  }
 }
 
 // ------------------------------------------------------------------ init / exchange
