@@ -253,7 +253,7 @@ void launch_boundary(Lattice *s, bool loads) {
 // advanced (one block)
 void launch_step_final(Lattice *s) {
   const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
-  hipLaunchKernelGGL(k_step_final, dim3(1), dim3(64), 0, s->stream, s->d_partials, s->nlinks ? s->wall_blocks : 0,
+  hipLaunchKernelGGL(k_step_final, dim3(1), dim3(128), 0, s->stream, s->d_partials, s->nlinks ? s->wall_blocks : 0,
                      s->nelem, 1.0 / qa, 1.0 / (qa * s->chord), 1.0, s->d_ctl, s->d_hist, s->d_diag);
 }
 
@@ -298,7 +298,7 @@ void launch_exterior(Lattice *s, bool odd, bool sample) {
   launch_ext_predict(s, s->d_ctl, 0.0);
   // accumulators and the end of the step
   const double qa = 0.5 * s->U * s->U * s->chord * s->nz;
-  hipLaunchKernelGGL(k_ext_end, dim3(1), dim3(64), 0, s->stream, (const double *)s->d_partials,
+  hipLaunchKernelGGL(k_ext_end, dim3(1), dim3(192), 0, s->stream, (const double *)s->d_partials,
                      s->nlinks ? s->wall_blocks : 0, s->nelem, E, s->ext_ring_blocks, s->d_ctl, s->d_hist,
                      sample ? 1 : 0, 1.0 / qa, 1.0 / (qa * s->chord), 1.0, s->d_diag);
 }

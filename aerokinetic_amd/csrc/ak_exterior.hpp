@@ -387,58 +387,80 @@ __global__ __launch_bounds__(128) void k_ext_ring(typename St<S>::T *__restrict_
 }
 
 // ------------------------------------------------------------------ accumulators
-// One 64-thread block: fixed-order sums of this step's wall mass exchange (wall partials,
-// component 4) and ring source partials into the accumulators; on sampled steps the
-// exterior columns of the current history row.
-// `acc` is at least 6 x 64 doubles of the calling kernel's shared memory.
-__device__ __forceinline__ void ext_final_body(const double *__restrict__ wall_part, int wall_blocks, int nelem,
-                                               const ExtDev &E, int ring_blocks, const StepCtl *__restrict__ ctl,
-                                               double *__restrict__ hist, int sample, double (*acc)[64]) {
-  const int lane = threadIdx.x;
+// Lane sums (lanes 0..63 of one wave) of this step's wall mass exchange (wall partials,
+// component 4) and of the ring source partials: acc[0][lane] and acc[1 + c][lane].
+__device__ __forceinline__ void ext_lane_sums(const double *__restrict__ wall_part, int wall_blocks, int nelem,
+                                              const ExtDev &E, int ring_blocks, double (*acc)[64], int lane) {
   double wm = 0.0, s[5] = {0, 0, 0, 0, 0};
-  for (int b = lane; b < wall_blocks; b += 64)
-    for (int e = 0; e < nelem; ++e) wm += wall_part[((long long)b * nelem + e) * 5 + 4];
+  for (int b = lane; b < wall_blocks; b += 64) {
+    const double *p = wall_part + (long long)b * nelem * 5 + 4;
+#pragma unroll
+    for (int e = 0; e < MAX_ELEM; ++e)
+      if (e < nelem) wm += p[e * 5];
+  }
   for (int b = lane; b < ring_blocks; b += 64)
+#pragma unroll
     for (int c = 0; c < 5; ++c) {
       const double x = E.part[(long long)b * 5 + c];
       s[c] = c == 4 ? fmax(s[c], x) : s[c] + x;
     }
   acc[0][lane] = wm;
   for (int c = 0; c < 5; ++c) acc[1 + c][lane] = s[c];
-  __syncthreads();
-  if (lane == 0) {
-    double tot[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = 0; k < 64; ++k)
-      for (int c = 0; c < 6; ++c) tot[c] = c == 5 ? fmax(tot[c], acc[c][k]) : tot[c] + acc[c][k];
-    ExtAcc &a = *E.acc;
-    a.wall_mass = tot[0];
-    a.wall_total += tot[0];
-    for (int c = 0; c < 4; ++c) {
-      a.src[c] = tot[1 + c];
-      a.src_total[c] += tot[1 + c];
-    }
-    a.moment_err = tot[5];
-    a.steps += 1;
-    if (sample) {
-      double *row = hist + (long long)ctl->row * ctl->width;
-      row[EXT_COL] = a.wall_total;
-      row[EXT_COL + 1] = a.src[0];
-      row[EXT_COL + 2] = a.src_total[0];
-    }
+}
+
+// One lane: fixed-order totals of the lane sums into the accumulators and, with a row, the
+// exterior columns of that history row.
+__device__ __forceinline__ void ext_acc_row(const double (*acc)[64], const ExtDev &E, double *__restrict__ row) {
+  double tot[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < 64; ++k)
+    for (int c = 0; c < 6; ++c) tot[c] = c == 5 ? fmax(tot[c], acc[c][k]) : tot[c] + acc[c][k];
+  ExtAcc &a = *E.acc;
+  a.wall_mass = tot[0];
+  a.wall_total += tot[0];
+  for (int c = 0; c < 4; ++c) {
+    a.src[c] = tot[1 + c];
+    a.src_total[c] += tot[1 + c];
+  }
+  a.moment_err = tot[5];
+  a.steps += 1;
+  if (row) {
+    row[EXT_COL] = a.wall_total;
+    row[EXT_COL + 1] = a.src[0];
+    row[EXT_COL + 2] = a.src_total[0];
   }
 }
-// End of an exterior step, one 64-thread block: the accumulators above, then on sampled
-// steps the end of a sampled step (diagnostics and loads into the history row, slots
-// cleared, cursor advanced), otherwise the cursor advanced.
-static_assert(LOADS_LDS >= 6 * 64, "shared accumulator size");
-__global__ void k_ext_end(const double *__restrict__ wall_part, int wall_blocks, int nelem, ExtDev E,
-                          int ring_blocks, StepCtl *ctl, double *__restrict__ hist, int sample, double fscale,
-                          double mscale, double vscale, DiagAcc *diag) {
+
+// End of an exterior step, one block of three waves whose load chains overlap: wave 2 sums
+// the wall mass exchange and the ring source into the accumulators; on sampled steps wave 0
+// reduces the diagnostics slots and wave 1 the wall loads.  After one barrier lane 0 of each
+// wave writes its columns of the current history row (sampled steps), wave 0 clears the
+// slots for the next sampled sweep and thread 0 advances the cursor (read by every thread
+// before the barrier).
+__global__ __launch_bounds__(192) void k_ext_end(const double *__restrict__ wall_part, int wall_blocks, int nelem,
+                                                 ExtDev E, int ring_blocks, StepCtl *ctl, double *__restrict__ hist,
+                                                 int sample, double fscale, double mscale, double vscale,
+                                                 DiagAcc *diag) {
   __shared__ double acc[LOADS_LDS];
-  ext_final_body(wall_part, wall_blocks, nelem, E, ring_blocks, ctl, hist, sample, (double (*)[64])acc);
-  __syncthreads();  // the accumulator array is reused below
-  if (sample) step_final_body(wall_part, wall_blocks, nelem, fscale, mscale, vscale, ctl, hist, diag, acc);
-  else if (threadIdx.x == 0) ctl->step += 1;
+  __shared__ double eacc[6][64];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const long long step = ctl->step;
+  const int rowi = ctl->row;
+  double *row = hist + (long long)rowi * ctl->width;
+  DiagTot t{};
+  if (w == 0) { if (sample) t = diag_reduce_wave(diag, lane); }
+  else if (w == 1) { if (sample) loads_lane_sums(wall_part, wall_blocks, nelem, acc, lane); }
+  else ext_lane_sums(wall_part, wall_blocks, nelem, E, ring_blocks, eacc, lane);
+  __syncthreads();
+  if (lane == 0) {
+    if (w == 2) ext_acc_row(eacc, E, sample ? row : nullptr);
+    else if (sample && w == 0) diag_row(t, row);
+    else if (sample) loads_row(acc, nelem, fscale, mscale, vscale, row, step);
+  }
+  if (sample && w == 0) diag_clear_lanes(diag, lane);
+  if (threadIdx.x == 0) {
+    ctl->step = step + 1;
+    if (sample) ctl->row = rowi + 1;
+  }
 }
 
 // ------------------------------------------------------------------ set-up

@@ -352,54 +352,81 @@ __global__ void k_tick(StepCtl *ctl) {
   if (threadIdx.x == 0) ctl->step += 1;
 }
 
-// Fold the wall-load partials (per-block sums, added in a fixed order) into the current
-// history row: cl = Fy fscale, cd = Fx fscale, cm = -Mz mscale, mass exchange = mass
-// vscale.  diag != nullptr: also clear the diagnostic slots (k_step_final has read them
-// and then ends the step); diag == nullptr: this body ends the sampled step.  `acc` is LOADS_LDS
-// doubles of the calling kernel's shared memory (kernels that fuse several phases give
-// every phase the same array).
-constexpr int LOADS_LDS = MAX_ELEM * 5 * 64;
-__device__ __forceinline__ void loads_final_body(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
-                              double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag,
-                              double *__restrict__ acc) {
-  const int lane = threadIdx.x;  // the first 64 threads of the block do the work
-  if (lane < 64)
-    for (int c = 0; c < nelem * 5; ++c) {
-      double s = 0.0;
-      for (int b = lane; b < nblocks; b += 64) s += partials[(long long)b * nelem * 5 + c];
-      acc[c * 64 + lane] = s;
-    }
-  __syncthreads();
-  if (lane == 0) {
-    double *row = hist + (long long)ctl->row * ctl->width;
-    double tot[5] = {0, 0, 0, 0, 0};
-    for (int e = 0; e < nelem; ++e) {
-      double v[5];
-      for (int c = 0; c < 5; ++c) {
-        double s = 0.0;
-        for (int k = 0; k < 64; ++k) s += acc[(e * 5 + c) * 64 + k];
-        v[c] = s;
-        tot[c] += s;
-      }
-      for (int c = 0; c < 4; ++c) row[AK_HIST_FIXED + e * 4 + c] = v[c];
-    }
-    row[0] = (double)ctl->step;
-    row[1] = tot[1] * fscale;
-    row[2] = tot[0] * fscale;
-    row[3] = -tot[3] * mscale;
-    row[8] = tot[4] * vscale;
-    if (!diag) {
-      ctl->step += 1;
-      ctl->row += 1;
-    }
+// Lane sums of the wall-load partials (lanes 0..63 of one wave): acc[c * 64 + lane] is the
+// sum of column c over the blocks lane, lane + 64, ... in that order.  The columns of one
+// block are loaded together (unrolled to the largest element count).
+__device__ __forceinline__ void loads_lane_sums(const double *__restrict__ partials, int nblocks, int nelem,
+                                                double *__restrict__ acc, int lane) {
+  constexpr int NC = MAX_ELEM * 5;
+  const int nc = nelem * 5;
+  double s[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) s[c] = 0.0;
+  for (int b = lane; b < nblocks; b += 64) {
+    const double *p = partials + (long long)b * nc;
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (c < nc) s[c] += p[c];
   }
-  if (!diag || lane >= 64) return;
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    if (c < nc) acc[c * 64 + lane] = s[c];
+}
+
+// One lane: the loads of a history row from the lane sums, added in a fixed order:
+// cl = Fy fscale, cd = Fx fscale, cm = -Mz mscale, mass exchange = mass vscale, and the
+// per-element forces and moment.
+__device__ __forceinline__ void loads_row(const double *__restrict__ acc, int nelem, double fscale, double mscale,
+                                          double vscale, double *__restrict__ row, long long step) {
+  double tot[5] = {0, 0, 0, 0, 0};
+  for (int e = 0; e < nelem; ++e) {
+    double v[5];
+    for (int c = 0; c < 5; ++c) {
+      double s = 0.0;
+      for (int k = 0; k < 64; ++k) s += acc[(e * 5 + c) * 64 + k];
+      v[c] = s;
+      tot[c] += s;
+    }
+    for (int c = 0; c < 4; ++c) row[AK_HIST_FIXED + e * 4 + c] = v[c];
+  }
+  row[0] = (double)step;
+  row[1] = tot[1] * fscale;
+  row[2] = tot[0] * fscale;
+  row[3] = -tot[3] * mscale;
+  row[8] = tot[4] * vscale;
+}
+
+// Diagnostics slots back to their start values (lanes 0..63 of one wave).
+__device__ __forceinline__ void diag_clear_lanes(DiagAcc *diag, int lane) {
   for (int k = lane; k < DIAG_SLOTS; k += 64) {
     diag->s[k].u2max = 0ull;
     diag->s[k].drmax = 0ull;
     diag->s[k].rhomin = (unsigned long long)__double_as_longlong(1e300);
     diag->s[k].nonfinite = 0ull;
   }
+}
+
+// Fold the wall-load partials into the current history row (the first wave of the block
+// sums, lane 0 writes).  diag != nullptr: the same lanes then clear the diagnostic slots
+// and the cursor stays; diag == nullptr: this body ends the sampled step.  `acc` is
+// LOADS_LDS doubles of the calling kernel's shared memory (kernels that fuse several phases
+// give every phase the same array).
+constexpr int LOADS_LDS = MAX_ELEM * 5 * 64;
+__device__ __forceinline__ void loads_final_body(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
+                              double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag,
+                              double *__restrict__ acc) {
+  const int lane = threadIdx.x;
+  if (lane < 64) loads_lane_sums(partials, nblocks, nelem, acc, lane);
+  __syncthreads();
+  if (lane == 0) {
+    loads_row(acc, nelem, fscale, mscale, vscale, hist + (long long)ctl->row * ctl->width, ctl->step);
+    if (!diag) {
+      ctl->step += 1;
+      ctl->row += 1;
+    }
+  }
+  if (!diag || lane >= 64) return;
+  diag_clear_lanes(diag, lane);
 }
 
 // Diagnostics slots set for a sampled sweep to accumulate into (lattice creation; each
@@ -413,46 +440,60 @@ __global__ void k_diag_clear(DiagAcc *diag) {
   }
 }
 
-// End of a sampled uniform step, one block: the sweep's diagnostics slots reduced (64
-// threads, then a fixed-order wave reduction) into the current history row, the wall
-// loads folded into it, the slots cleared for the next sampled sweep, the step cursor
-// advanced.
-// `acc` is LOADS_LDS doubles of the calling kernel's shared memory.
-__device__ __forceinline__ void step_final_body(const double *__restrict__ partials, int nblocks, int nelem,
-                                                double fscale, double mscale, double vscale, StepCtl *ctl,
-                                                double *__restrict__ hist, DiagAcc *diag, double *__restrict__ acc) {
+// Diagnostics slots reduced by one wave (lanes 0..63; every lane ends with the totals):
+// largest |u|^2 and |drho|, smallest rho, non-finite flag.
+struct DiagTot {
+  unsigned long long u, d, r, b;
+};
+__device__ __forceinline__ DiagTot diag_reduce_wave(const DiagAcc *diag, int lane) {
   unsigned long long u = 0ull, d = 0ull, r = (unsigned long long)__double_as_longlong(1e300), b = 0ull;
-  if (threadIdx.x < 64) {
-    for (int k = threadIdx.x; k < DIAG_SLOTS; k += 64) {
-      const DiagSlot &sl = diag->s[k];
-      u = sl.u2max > u ? sl.u2max : u;
-      d = sl.drmax > d ? sl.drmax : d;
-      r = sl.rhomin < r ? sl.rhomin : r;
-      b = sl.nonfinite > b ? sl.nonfinite : b;
-    }
-    for (int o = 32; o >= 1; o >>= 1) {
-      const unsigned long long u2 = __shfl_xor(u, o, 64), d2 = __shfl_xor(d, o, 64);
-      const unsigned long long r2 = __shfl_xor(r, o, 64), b2 = __shfl_xor(b, o, 64);
-      u = u2 > u ? u2 : u; d = d2 > d ? d2 : d; r = r2 < r ? r2 : r; b = b2 > b ? b2 : b;
-    }
+  for (int k = lane; k < DIAG_SLOTS; k += 64) {
+    const DiagSlot &sl = diag->s[k];
+    u = sl.u2max > u ? sl.u2max : u;
+    d = sl.drmax > d ? sl.drmax : d;
+    r = sl.rhomin < r ? sl.rhomin : r;
+    b = sl.nonfinite > b ? sl.nonfinite : b;
   }
-  // loads into the row; with `diag` the same lanes that read each slot above clear it after
-  // (and the cursor stays for the diagnostics columns below)
-  loads_final_body(partials, nblocks, nelem, fscale, mscale, vscale, ctl, hist, diag, acc);
-  if (threadIdx.x == 0) {
-    double *row = hist + (long long)ctl->row * ctl->width;
-    row[4] = sqrt(3.0) * sqrt(__longlong_as_double((long long)u));
-    row[5] = __longlong_as_double((long long)d);
-    row[6] = __longlong_as_double((long long)r);
-    row[7] = (double)b;
-    ctl->step += 1;
-    ctl->row += 1;
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned long long u2 = __shfl_xor(u, o, 64), d2 = __shfl_xor(d, o, 64);
+    const unsigned long long r2 = __shfl_xor(r, o, 64), b2 = __shfl_xor(b, o, 64);
+    u = u2 > u ? u2 : u; d = d2 > d ? d2 : d; r = r2 < r ? r2 : r; b = b2 > b ? b2 : b;
   }
+  return DiagTot{u, d, r, b};
 }
-__global__ void k_step_final(const double *__restrict__ partials, int nblocks, int nelem, double fscale,
-                             double mscale, double vscale, StepCtl *ctl, double *__restrict__ hist, DiagAcc *diag) {
+__device__ __forceinline__ void diag_row(const DiagTot &t, double *__restrict__ row) {
+  row[4] = sqrt(3.0) * sqrt(__longlong_as_double((long long)t.u));
+  row[5] = __longlong_as_double((long long)t.d);
+  row[6] = __longlong_as_double((long long)t.r);
+  row[7] = (double)t.b;
+}
+
+// End of a sampled uniform step, one block of two waves: wave 0 reduces the sweep's
+// diagnostics slots, wave 1 sums the wall-load partials (their load chains overlap); then
+// lane 0 of each writes its columns of the current history row, wave 0 clears the slots
+// for the next sampled sweep and thread 0 advances the cursor (read by every thread before
+// the barrier).
+__global__ __launch_bounds__(128) void k_step_final(const double *__restrict__ partials, int nblocks, int nelem,
+                                                    double fscale, double mscale, double vscale, StepCtl *ctl,
+                                                    double *__restrict__ hist, DiagAcc *diag) {
   __shared__ double acc[LOADS_LDS];
-  step_final_body(partials, nblocks, nelem, fscale, mscale, vscale, ctl, hist, diag, acc);
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const long long step = ctl->step;
+  const int rowi = ctl->row;
+  double *row = hist + (long long)rowi * ctl->width;
+  DiagTot t{};
+  if (w == 0) t = diag_reduce_wave(diag, lane);
+  else loads_lane_sums(partials, nblocks, nelem, acc, lane);
+  __syncthreads();
+  if (lane == 0) {
+    if (w == 0) diag_row(t, row);
+    else loads_row(acc, nelem, fscale, mscale, vscale, row, step);
+  }
+  if (w == 0) diag_clear_lanes(diag, lane);
+  if (threadIdx.x == 0) {
+    ctl->step = step + 1;
+    ctl->row = rowi + 1;
+  }
 }
 
 // ------------------------------------------------------------------ farfield
