@@ -73,6 +73,29 @@ def test_thousand_steps_laminar(farfield):
             assert a[k] == pytest.approx(b[k], abs=1e-9)
 
 
+@pytest.mark.parametrize("farfield", ["characteristic_transport_v1", "legacy"])
+def test_body_near_the_edge_takes_the_separate_boundary_launches(farfield):
+    # A wall link within 3 cells of the x = 0 edge: the walls no longer share a launch with
+    # the edge phase (x-face copy or legacy links); the separate launches keep oracle parity.
+    section = Section((naca0012(41),), 1.0, (0.25, 0.0))
+    flow = Flow(reynolds=10000, alpha_deg=4.0, lattice_speed=0.05, perturbation=1e-3)
+    grid = Grid(cells_per_chord=16, upstream_chords=0.125, downstream_chords=2.0, transverse_chords=1.0,
+                span_chords=0.5, memory_limit_gib=2, farfield=farfield)
+    plan = Plan.create(section, flow, grid)
+    print("case near edge", farfield, plan.shape, flush=True)
+    s = gpu_solver(section, flow, grid, plan)
+    assert int(s.links["x"].min()) < 3
+    o = O.from_case(section, flow, grid, plan)
+    fluid = np.broadcast_to(~s._solid_xy[:, :, None, None], (*plan.shape, 27))
+    recs = s.advance(20, 10)
+    refs = [o.step(sample=(k + 1) % 10 == 0) for k in range(20)]
+    refs = [r for r in refs if r is not None]
+    assert rel_err(s.populations(), o.pdf(), fluid) <= 1e-12
+    for a, b in zip(recs, refs):
+        for k in ("cl", "cd", "cm"):
+            assert a[k] == pytest.approx(b[k], abs=1e-11)
+
+
 def _golden(name, last):
     project = Project.load(GOLDEN / name / "project.json")
     plan = project.plan()
