@@ -1,11 +1,12 @@
 // ak_capi.hip -- C ABI of the gfx950 D3Q27 cumulant step (declared in include/aerokinetic.h).
 //
 // Host-side orchestration only: buffer sizing, boundary item tables, launch order.
-// Per step: boundary kernels (farfield, walls) write the incoming populations of
-// boundary links where the next AA sweep reads them; on sampled steps the wall
-// kernel also emits load partials, a one-block kernel folds them into a history
-// row, and the sweep folds the diagnostics (max |u|, max |rho-1|, min rho,
-// non-finite flag) into the same row.  No host synchronisation in the step path.
+// Per step: boundary kernels (farfield, walls; the walls share a launch with the
+// edge phase that does not depend on them) write the incoming populations of
+// boundary links where the next AA sweep reads them; on sampled steps the walls
+// also emit load partials and the sweep accumulates the diagnostics (max |u|,
+// max |rho-1|, min rho, non-finite flag) into slots, and a one-block kernel folds
+// both into a history row.  No host synchronisation in the step path.
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
