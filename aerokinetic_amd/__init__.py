@@ -12,11 +12,19 @@ from .convergence import StopCriteria
 
 __version__ = "0.5.3+mi355x.1"
 __all__ = ["Contour", "Section", "naca0012", "naca4", "naca4_profile", "Atmosphere", "Flow", "Grid", "Plan",
-           "Project", "RunSettings", "StopCriteria", "Solver", "NumericalFailure", "coefficients"]
+           "Project", "RunSettings", "StopCriteria", "Solver", "NumericalFailure", "coefficients", "RefinedSolver",
+           "RefinedPlan", "RefinementSettings"]
 
 
 def __getattr__(name):
+    # the solvers load the HIP library (and torch) on first use only
     if name in ("Solver", "NumericalFailure", "coefficients"):
         from . import solver
         return getattr(solver, name)
+    if name == "RefinedSolver":
+        from .refined import RefinedSolver
+        return RefinedSolver
+    if name in ("RefinedPlan", "RefinementSettings"):
+        from . import refined_plan
+        return getattr(refined_plan, name)
     raise AttributeError(name)

@@ -91,6 +91,7 @@ _SIGS = {
     "ak_refined_transfer_errors": (ctypes.c_int, [_vp]),
     "ak_refined_set_graph_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ak_refined_get_populations": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "ak_refined_budget": (ctypes.c_int, [_vp, _dp]),
 }
 
 _lib = None

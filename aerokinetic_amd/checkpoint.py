@@ -21,7 +21,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib
-from ._io import atomic_binary
+from ._io import PublishedFile
 
 FORMAT = "aerokinetic_mi355x_checkpoint"
 SCHEMA = 1
@@ -55,10 +55,12 @@ def save_checkpoint(solver, path, context=None):
             "library": _library(), "storage": solver.storage, "project": project, "case_fingerprint": fingerprint,
             "steps": int(solver.steps), "wall_fallback_links": int(solver.wall_fallback_links),
             "context": context or {}}
+    # the plan, so that a case built on an explicit lattice is rebuilt on the same one
+    meta["plan"] = json.loads(json.dumps(solver.plan.as_dict(), default=list))
     if not refined:
         meta["plan_shape"] = list(solver.plan.shape)
     arrays = {"state": state, "metadata": np.asarray(json.dumps(meta, ensure_ascii=False, allow_nan=False))}
-    with atomic_binary(path) as handle:
+    with PublishedFile(path) as handle:
         np.savez(handle, **arrays)
     return Path(path)
 
@@ -72,9 +74,9 @@ def read_metadata(path):
 
 
 def restore_checkpoint(solver_type, path, progress=None, device=0, plan=None):
-    """Rebuild the case of a checkpoint and load its device state (same storage).
-
-    `plan` is needed when the case was built on an explicit plan (a fixed lattice)."""
+    """Rebuild the case of a checkpoint on the plan it recorded (or `plan`) and load its
+    device state (same storage)."""
+    from .config import Plan
     from .project import Project
     from .refined import RefinedSolver
     from .refined_plan import RefinedPlan, RefinementSettings
@@ -93,8 +95,9 @@ def restore_checkpoint(solver_type, path, progress=None, device=0, plan=None):
             raise ValueError("Not a two-level checkpoint")
         project = Project.from_dict(meta["project"]["project"])
         settings = RefinementSettings(**meta["project"]["refinement"])
-        solver = solver_type(project, settings, progress, storage=meta["storage"], device=device,
-                             plan=plan or RefinedPlan.create(project, settings))
+        if plan is None:
+            plan = RefinedPlan.from_dict(meta["plan"]) if "plan" in meta else RefinedPlan.create(project, settings)
+        solver = solver_type(project, settings, progress, storage=meta["storage"], device=device, plan=plan)
         if solver.case_fingerprint != meta["case_fingerprint"]:
             raise ValueError("Checkpoint physical/numerical case is inconsistent")
         _lib.check(L.ak_refined_load_state(solver.h, _lib._vp(state.ctypes.data), state.size), None,
@@ -105,6 +108,8 @@ def restore_checkpoint(solver_type, path, progress=None, device=0, plan=None):
         project = Project.from_dict(meta["project"])
         if project.case_fingerprint != meta["case_fingerprint"]:
             raise ValueError("Checkpoint physical/numerical case is inconsistent")
+        if plan is None and "plan" in meta:
+            plan = Plan.from_dict(meta["plan"])
         solver = solver_type(project.section, project.flow, project.grid, progress, storage=meta["storage"],
                              device=device, plan=plan)
         if list(solver.plan.shape) != meta["plan_shape"]:

@@ -225,3 +225,10 @@ class Plan:
 
     def as_dict(self):
         return asdict(self)
+
+    @classmethod
+    def from_dict(cls, d):
+        """The plan that as_dict() described (JSON lists back to tuples)."""
+        d = dict(d)
+        d["shape"], d["origin"], d["notes"] = tuple(d["shape"]), tuple(d["origin"]), tuple(d["notes"])
+        return cls(**d)

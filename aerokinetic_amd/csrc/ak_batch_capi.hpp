@@ -342,7 +342,10 @@ int ak_batch_run(void *handle, int nsteps, int sample_every) {
   if (build_descriptors(b)) return -1;
   if (order_after_members(b)) return -1;
   const bool sample = sample_every == 1;
-  for (Refined *r : b->members) set_ctl_on(r->c, b->stream, r->steps + 1, r->c->hist_rows);
+  for (Refined *r : b->members) {
+    set_ctl_on(r->c, b->stream, r->steps + 1, r->c->hist_rows);
+    r->budget_complete = false;  // batched steps keep no conservation budget
+  }
   switch (b->storage) {
     case 0: batch_start<0>(b); break;
     case 1: batch_start<1>(b); break;

@@ -49,6 +49,7 @@ struct Lattice {
   void *pop = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  bool pre_collision = false;  // levels of a two-level case: the stored state is pre-collision
   uint8_t *d_solid = nullptr;
   // walls
   int nlinks = 0, nelem = 1;
@@ -996,10 +997,12 @@ namespace {
 template <int S, bool PE>
 void launch_diag_state(Lattice *s, double *part, unsigned nb) {
   using T = typename St<S>::T;
-  if (s->ff_kind == AK_FARFIELD_PERIODIC)
-    hipLaunchKernelGGL((k_diag_state<S, PE, true>), dim3(nb), dim3(256), 0, s->stream, (const T *)s->pop, s->d_solid, s->L, s->inv_scale, part);
+  if (s->pre_collision)
+    hipLaunchKernelGGL((k_diag_state<S, PE, false, true>), dim3(nb), dim3(256), 0, s->stream, (const T *)s->pop, s->d_solid, s->L, s->inv_scale, part);
+  else if (s->ff_kind == AK_FARFIELD_PERIODIC)
+    hipLaunchKernelGGL((k_diag_state<S, PE, true, false>), dim3(nb), dim3(256), 0, s->stream, (const T *)s->pop, s->d_solid, s->L, s->inv_scale, part);
   else
-    hipLaunchKernelGGL((k_diag_state<S, PE, false>), dim3(nb), dim3(256), 0, s->stream, (const T *)s->pop, s->d_solid, s->L, s->inv_scale, part);
+    hipLaunchKernelGGL((k_diag_state<S, PE, false, false>), dim3(nb), dim3(256), 0, s->stream, (const T *)s->pop, s->d_solid, s->L, s->inv_scale, part);
 }
 template <int S>
 void diag_any(Lattice *s, double *part, unsigned nb) {
@@ -1010,10 +1013,12 @@ template <int S, bool PE>
 void launch_plane(Lattice *s, int z, double *r, double *u) {
   using T = typename St<S>::T;
   const long long n = (long long)s->nx * s->ny;
-  if (s->ff_kind == AK_FARFIELD_PERIODIC)
-    hipLaunchKernelGGL((k_plane<S, PE, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, z, s->inv_scale, r, u);
+  if (s->pre_collision)
+    hipLaunchKernelGGL((k_plane<S, PE, false, true>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, z, s->inv_scale, r, u);
+  else if (s->ff_kind == AK_FARFIELD_PERIODIC)
+    hipLaunchKernelGGL((k_plane<S, PE, true, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, z, s->inv_scale, r, u);
   else
-    hipLaunchKernelGGL((k_plane<S, PE, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, z, s->inv_scale, r, u);
+    hipLaunchKernelGGL((k_plane<S, PE, false, false>), dim3(blocks_for(n, 256)), dim3(256), 0, s->stream, (const T *)s->pop, s->L, z, s->inv_scale, r, u);
 }
 template <int S>
 void plane_any(Lattice *s, int z, double *r, double *u) {

@@ -892,8 +892,9 @@ __global__ void k_import(typename St<S>::T *__restrict__ A, Desc L, const double
 
 // Read-only state diagnostics: per-block [sum rho, max |u|^2, max |rho-1|, min rho, bad]
 // over fluid cells (the guards also over the exterior ring, the density sum over the
-// physical fluid only); the host folds the blocks in a fixed order.
-template <int S, bool PE, bool PXY>
+// physical fluid only); the host folds the blocks in a fixed order.  IN reads the incoming
+// (pre-collision) populations, the stored state of a two-level case's levels.
+template <int S, bool PE, bool PXY, bool IN>
 __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__restrict__ A,
                                                     const uint8_t *__restrict__ solid, Desc L,
                                                     float inv_scale, double *__restrict__ part) {
@@ -911,7 +912,8 @@ __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__r
 #pragma unroll
       for (int j = 0; j < NQ; ++j) {
         long long src;
-        if constexpr (PE) src = c + (long long)opp(j) * L.QS;
+        if constexpr (IN) src = loc_in<PE>(L, x, y, z, j);
+        else if constexpr (PE) src = c + (long long)opp(j) * L.QS;
         else src = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
         const C v = ld<S>(A + src, inv_scale);
         r += v;
@@ -949,8 +951,9 @@ __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__r
   }
 }
 
-// rho and u of one z plane (interior x, y), x-major output: rho[x*ny + y], u[(x*ny + y)*3 + a]
-template <int S, bool PE, bool PXY>
+// rho and u of one z plane (interior x, y), x-major output: rho[x*ny + y], u[(x*ny + y)*3 + a];
+// IN as for k_diag_state
+template <int S, bool PE, bool PXY, bool IN>
 __global__ void k_plane(const typename St<S>::T *__restrict__ A, Desc L, int z, float inv_scale,
                         double *__restrict__ rho_out, double *__restrict__ u_out) {
   using C = typename St<S>::C;
@@ -961,7 +964,8 @@ __global__ void k_plane(const typename St<S>::T *__restrict__ A, Desc L, int z, 
   C r = 0, jx = 0, jy = 0, jz = 0;
   for (int j = 0; j < NQ; ++j) {
     long long src;
-    if constexpr (PE) src = c + (long long)opp(j) * L.QS;
+    if constexpr (IN) src = loc_in<PE>(L, x, y, z, j);
+    else if constexpr (PE) src = c + (long long)opp(j) * L.QS;
     else src = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
     const C v = ld<S>(A + src, inv_scale);
     r += v; jx += cxq(j) * v; jy += cyq(j) * v; jz += czq(j) * v;

@@ -379,6 +379,182 @@ __global__ __launch_bounds__(256) void k_flux_fine(const typename St<S>::T *__re
   flux_fine_body<S, PE>(A, L, B, T, vol, inv, accumulate, regf, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// ------------------------------------------------------------------ conservation budget
+// FP64 bookkeeping of one two-level case (refined_solver.py:291-345): per coarse step the
+// wall mass exchange and wall impulse of both fine substeps and the mass and momentum that
+// enter through the coarse domain's outer x/y faces, accumulated over the steps since the
+// initial state; the conserved integrals (mass, momentum) of both levels' owned cells are
+// taken at the initial state and around the last step of a run call.  Common units:
+// volumes dx^3 of each level, momenta times dx/dt (the same on both levels).
+struct BudgetAcc {
+  double wall[5];   // last step: wall impulse x/y/z, wall moment impulse z, wall mass exchange
+  double ext[4];    // last step: mass and momentum x/y/z entering through the outer faces
+  double cum_wall_mass, cum_wall_impulse[3], cum_ext[4];
+  double initial[4], before[4], after[4];  // conserved integrals: mass, momentum x/y/z
+  long long steps;        // coarse steps folded since the initial state
+  long long budget_step;  // the step that `before` and `after` bracket (0: none yet)
+};
+
+// Block sum of four values per thread (wave butterflies, then the waves in order by thread
+// 0); the whole block must call it.  Returns the totals in thread 0.
+__device__ __forceinline__ void block_sum4(double (&v)[4], double (&red)[16][4]) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1)
+    for (int c = 0; c < 4; ++c) v[c] += __shfl_xor(v[c], o, 64);
+  const int w = threadIdx.x / 64;
+  if (threadIdx.x % 64 == 0)
+    for (int c = 0; c < 4; ++c) red[w][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) / 64;
+    for (int c = 0; c < 4; ++c) {
+      double s = red[0][c];
+      for (int k = 1; k < nw; ++k) s += red[k][c];
+      v[c] = s;
+    }
+  }
+}
+
+// Mass and momentum entering the coarse domain through its four outer x/y faces during one
+// coarse step: the pull-streaming face flux (the reference's outer StreamingFaceFlux
+// samplers over the face between the ghost layer and the first interior row,
+// refined_solver.py:215-222, 276-279) of the post-collision state after the farfield has
+// written the incoming values.  A ghost cell's population pointing into the domain is the
+// incoming value of its link, stored where the next sweep gathers it (loc_in of the
+// interior cell it feeds); a ghost population whose target is another ghost cell is never
+// written by a boundary and holds the free-stream value of the initial state.  The z faces
+// are left out: on the periodic span they carry one and the same flux in and out.
+// Face elements (face, tangential cell, z) take FLUX_LANES lanes each; each block writes one
+// partial [4] (lattice units, sign -side applied per element).
+template <int S, bool PE>
+__device__ __forceinline__ void outer_flux_body(const typename St<S>::T *__restrict__ A, Desc L,
+                                                const FluxTable *__restrict__ T, const FreeState &F, float inv,
+                                                double *__restrict__ part, long long t, int blk) {
+  __shared__ double red[16][4];
+  const long long w = t / FLUX_LANES;
+  const int sub_lane = (int)(t % FLUX_LANES);
+  const long long nfx = (long long)L.ny * L.nz, nfy = (long long)L.nx * L.nz;
+  const bool active = w < 2 * (nfx + nfy);
+  int f = 0, ti = 0, zpos = 0;
+  if (active) {
+    long long r = w;
+    if (r < 2 * nfx) {
+      f = (int)(r / nfx);
+      r %= nfx;
+    } else {
+      r -= 2 * nfx;
+      f = 2 + (int)(r / nfy);
+      r %= nfy;
+    }
+    zpos = (int)(r % L.nz);
+    ti = (int)(r / L.nz);
+  }
+  const int axis = f / 2, side = f % 2;
+  const int face_cell = side ? (axis == 0 ? L.nx : L.ny) - 1 : -1;
+  double o[4] = {0.0, 0.0, 0.0, 0.0};
+  if (active) {
+    for (int k = sub_lane; k < T->n[axis]; k += FLUX_LANES) {
+      const FluxTerm &tm = T->term[axis][k];
+      const int d = tm.d;
+      const int nrm = axis == 0 ? cxq(d) : cyq(d);
+      const int a = face_cell + (nrm < 0 ? 1 : 0);
+      const int tt = ti - tm.s0;
+      const int zz = ((zpos - tm.s1) % L.nz + L.nz) % L.nz;
+      const int x = axis == 0 ? a : tt, y = axis == 0 ? tt : a;
+      double v;
+      if (x >= 0 && x < L.nx && y >= 0 && y < L.ny) {
+        v = (double)ld<S>(A + loc_out<PE>(L, x, y, zz, d), inv);
+      } else {
+        const int tx = x + cxq(d), ty = y + cyq(d);
+        if (tx >= 0 && tx < L.nx && ty >= 0 && ty < L.ny)
+          v = (double)ld<S>(A + loc_in<PE>(L, tx, ty, ((zz + czq(d)) % L.nz + L.nz) % L.nz, d), inv);
+        else
+          v = F.g[d];
+      }
+      for (int c = 0; c < 4; ++c) o[c] += v * tm.coef[c];
+    }
+  }
+#pragma unroll
+  for (int s = FLUX_LANES / 2; s >= 1; s >>= 1)
+    for (int c = 0; c < 4; ++c) o[c] += __shfl_xor(o[c], s, 64);
+  const double sgn = side ? -1.0 : 1.0;
+  double v[4];
+  for (int c = 0; c < 4; ++c) v[c] = (active && sub_lane == 0) ? sgn * o[c] : 0.0;
+  block_sum4(v, red);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; ++c) part[(long long)blk * 4 + c] = v[c];
+}
+
+// Conserved integrals of one level (refined_solver.py:291-299): sums of rho = 1 + sum g and of
+// j = sum c g over the owned cells (mask 0: coarse cells outside the box, fine fluid cells of
+// the box) from the pre-collision state, FP64; one partial [4] per block.
+template <int S, bool PE>
+__global__ __launch_bounds__(256) void k_integrals(const typename St<S>::T *__restrict__ A,
+                                                   const uint8_t *__restrict__ mask, Desc L, float inv,
+                                                   double *__restrict__ part) {
+  __shared__ double red[16][4];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (t < (long long)L.nx * L.ny * L.nz) {
+    const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
+    if (mask[(long long)(y + 1) * (L.nx + 2) + (x + 1)] == 0) {
+      double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        const double g = (double)ld<S>(A + loc_in<PE>(L, x, y, z, i), inv);
+        r += g;
+        if (cxq(i)) jx += cxq(i) * g;
+        if (cyq(i)) jy += cyq(i) * g;
+        if (czq(i)) jz += czq(i) * g;
+      }
+      v[0] = 1.0 + r; v[1] = jx; v[2] = jy; v[3] = jz;
+    }
+  }
+  block_sum4(v, red);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; ++c) part[(long long)blockIdx.x * 4 + c] = v[c];
+}
+
+// Both levels' integral partials folded in a fixed order into dst[4] (common units: mass
+// sum rho dx^3, momentum sum j dx^3 dx/dt; coarse level first).  One wave.  With acc, the
+// fold also records the step it brackets.
+__global__ void k_integrals_final(const double *__restrict__ part, int nb_c, int nb_f, double vol_c, double vol_f,
+                                  double uv, double *__restrict__ dst, BudgetAcc *acc, long long step) {
+  __shared__ double lane_sum[2][4][64];
+  const int lane = threadIdx.x;
+  for (int lev = 0; lev < 2; ++lev) {
+    const int first = lev ? nb_c : 0, n = lev ? nb_f : nb_c;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = lane; b < n; b += 64)
+      for (int c = 0; c < 4; ++c) s[c] += part[(long long)(first + b) * 4 + c];
+    for (int c = 0; c < 4; ++c) lane_sum[lev][c][lane] = s[c];
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double tot[2][4];
+    for (int lev = 0; lev < 2; ++lev)
+      for (int c = 0; c < 4; ++c) {
+        double s = 0.0;
+        for (int k = 0; k < 64; ++k) s += lane_sum[lev][c][k];
+        tot[lev][c] = s;
+      }
+    dst[0] = tot[0][0] * vol_c + tot[1][0] * vol_f;
+    for (int c = 1; c < 4; ++c) dst[c] = tot[0][c] * vol_c * uv + tot[1][c] * vol_f * uv;
+    if (acc) acc->budget_step = step;
+  }
+}
+
+// Budget accumulators back to the initial state (the integrals of the initial state are
+// written by k_integrals_final into acc->initial afterwards).
+__global__ void k_budget_reset(BudgetAcc *acc) {
+  if (threadIdx.x == 0) {
+    double *p = (double *)acc;
+    for (int k = 0; k < (int)(offsetof(BudgetAcc, steps) / sizeof(double)); ++k) p[k] = 0.0;
+    acc->steps = 0;
+    acc->budget_step = 0;
+  }
+}
+
 // Equilibrium reflux on the coarse cells just outside each face: one wavefront per
 // cell, lane i (< 27) owns population i; density and momentum by fixed-order butterflies.
 __device__ __forceinline__ double eq_dir(int i, double rho, double ux, double uy, double uz) {
@@ -469,8 +645,10 @@ __device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, D
 // never leave the chip.
 constexpr int RT_X = 16, RT_Y = 2;
 constexpr int RESTRICT_LDS = 10 * 256;  // doubles: the children's moments of one tile
-// shared memory of a kernel that fuses the restriction and the loads fold
-constexpr int END_LDS = RESTRICT_LDS > LOADS_LDS ? RESTRICT_LDS : LOADS_LDS;
+// shared memory of a kernel that fuses the restriction and the loads + budget fold (the
+// wall partials' lane sums, then four lane-sum columns of the outer-face flux partials)
+constexpr int FOLD_LDS = LOADS_LDS + 4 * 64;
+constexpr int END_LDS = RESTRICT_LDS > FOLD_LDS ? RESTRICT_LDS : FOLD_LDS;
 __host__ __device__ inline int restrict_tiles(const BoxGeom &B) {
   return ((B.nb[0] + RT_X - 1) / RT_X) * ((B.nb[1] + RT_Y - 1) / RT_Y) * B.nb[2];
 }
@@ -577,6 +755,63 @@ __global__ __launch_bounds__(256) void k_restrict_tiles(const typename St<S>::T 
 // tick, the restriction tiles (covered cells), and the next step's donor moments of the
 // remaining donor cells.  Reflux and restriction write their own cells' donor moments, so
 // every range reads and writes slots no other range touches.
+// Where the end of a step folds the budget: the outer-face flux partials of this step
+// (written during the first fine substep) and the accumulators; acc == nullptr skips it.
+struct BudgetArgs {
+  const double *opart;
+  int nb_outer;
+  double volc, uv;  // coarse cell volume, common velocity unit dx/dt
+  BudgetAcc *acc;
+};
+
+// The loads block of a step's end with the budget (one block, 256 threads): lanes 0..63 sum
+// the wall partials, lanes 64..127 the outer-face flux partials (lane-strided, fixed order);
+// thread 0 then writes the history row of a sampled step and adds this step's wall mass
+// exchange, wall impulse and outer in-flux to the accumulators (refined_solver.py:318-343),
+// and advances the step cursor.
+__device__ __forceinline__ void refined_fold_body(const double *__restrict__ part, int nparts, int nelem,
+                                                  double fscale, double volf, const BudgetArgs &G, StepCtl *ctl,
+                                                  double *__restrict__ hist, int sample, double *__restrict__ lds) {
+  const int lane = threadIdx.x;
+  double *oacc = lds + LOADS_LDS;
+  if (lane < 64) loads_lane_sums(part, nparts, nelem, lds, lane);
+  if (lane >= 64 && lane < 128) {
+    const int l = lane - 64;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = l; b < G.nb_outer; b += 64)
+      for (int c = 0; c < 4; ++c) s[c] += G.opart[(long long)b * 4 + c];
+    for (int c = 0; c < 4; ++c) oacc[c * 64 + l] = s[c];
+  }
+  __syncthreads();
+  if (lane != 0) return;
+  if (sample) loads_row(lds, nelem, fscale, fscale, volf, hist + (long long)ctl->row * ctl->width, ctl->step);
+  // the same lane sums in the same order as loads_row
+  double tot[5] = {0, 0, 0, 0, 0};
+  for (int e = 0; e < nelem; ++e)
+    for (int c = 0; c < 5; ++c) {
+      double s = 0.0;
+      for (int k = 0; k < 64; ++k) s += lds[(e * 5 + c) * 64 + k];
+      tot[c] += s;
+    }
+  double ext[4];
+  for (int c = 0; c < 4; ++c) {
+    double s = 0.0;
+    for (int k = 0; k < 64; ++k) s += oacc[c * 64 + k];
+    ext[c] = s;
+  }
+  BudgetAcc *a = G.acc;
+  for (int c = 0; c < 4; ++c) a->wall[c] = tot[c] * volf * G.uv;
+  a->wall[4] = tot[4] * volf;
+  a->ext[0] = ext[0] * G.volc;
+  for (int c = 1; c < 4; ++c) a->ext[c] = ext[c] * G.volc * G.uv;
+  a->cum_wall_mass += a->wall[4];
+  for (int c = 0; c < 3; ++c) a->cum_wall_impulse[c] += a->wall[c];
+  for (int c = 0; c < 4; ++c) a->cum_ext[c] += a->ext[c];
+  a->steps += 1;
+  ctl->step += 1;
+  if (sample) ctl->row += 1;
+}
+
 template <int S, bool PF, bool PC>
 __global__ __launch_bounds__(256) void k_step_end(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
                                                   typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B, Levels V,
@@ -584,13 +819,15 @@ __global__ __launch_bounds__(256) void k_step_end(const typename St<S>::T *__res
                                                   const double *__restrict__ rc, const double *__restrict__ rf,
                                                   double inv_vol, int *__restrict__ err, int nb_reflux, int ntiles,
                                                   const double *__restrict__ part, int nparts, int nelem, double fscale,
-                                                  double vscale, StepCtl *ctl, double *__restrict__ hist, int sample) {
+                                                  double vscale, StepCtl *ctl, double *__restrict__ hist, int sample,
+                                                  BudgetArgs G) {
   __shared__ double lds[END_LDS];
   const int b = (int)blockIdx.x;
   if (b < nb_reflux) {
     reflux_body<S, PC>(Ac, Lc, B, rc, rf, inv_vol, cscale, cinv, err, (long long)b * 256 + threadIdx.x, M);
   } else if (b == nb_reflux) {
-    if (sample) loads_final_body(part, nparts, nelem, fscale, fscale, vscale, ctl, hist, nullptr, lds);
+    if (G.acc) refined_fold_body(part, nparts, nelem, fscale, vscale, G, ctl, hist, sample, lds);
+    else if (sample) loads_final_body(part, nparts, nelem, fscale, fscale, vscale, ctl, hist, nullptr, lds);
     else if (threadIdx.x == 0) ctl->step += 1;
   } else if (b < nb_reflux + 1 + ntiles) {
     restrict_tile_body<S, PF, PC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, b - nb_reflux - 1, lds, M, cinv);
@@ -653,18 +890,26 @@ __global__ __launch_bounds__(256) void k_after_coarse(typename St<S>::T *__restr
 
 // After a fine sweep, the fine face flux (box faces) and the walls with their loads
 // (around the body, which keeps clear of the faces) likewise share one launch.  The wall
-// partials keep their per-block layout (block index within the wall range).
-template <int S, bool PE>
+// partials keep their per-block layout (block index within the wall range).  After the first
+// substep a third range adds the coarse domain's outer-face flux of this coarse step (the
+// coarse lattice is untouched between its farfield and the end of the step), with PC the
+// coarse lattice's form.
+template <int S, bool PE, bool PC>
 __global__ __launch_bounds__(256) void k_after_fine(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
                                                     const FluxTable *__restrict__ T, double volf, float scale, float inv,
                                                     int sub, double *__restrict__ regf, WallLinks W,
-                                                    double *__restrict__ part, int nb_flux) {
+                                                    double *__restrict__ part, int nb_flux, int nb_walls,
+                                                    const typename St<S>::T *__restrict__ Ac, Desc Lc, float cinv,
+                                                    FreeState F, double *__restrict__ opart) {
   const int b = blockIdx.x;
   if (b < nb_flux) {
     flux_fine_body<S, PE>(A, L, B, T, volf, inv, sub, regf, (long long)b * 256 + threadIdx.x);
-  } else {
+  } else if (b < nb_flux + nb_walls) {
     const int wb = b - nb_flux;
     walls_body<S, PE, true>(A, L, W, scale, inv, part, (long long)wb * 256 + threadIdx.x, wb);
+  } else {
+    const int ob = b - nb_flux - nb_walls;
+    outer_flux_body<S, PC>(Ac, Lc, T, F, cinv, opart, (long long)ob * 256 + threadIdx.x, ob);
   }
 }
 

@@ -30,9 +30,17 @@ struct Refined {
   hipStream_t own_stream = nullptr;  // created when the caller passes the null stream
   bool use_graphs = true;
   GraphCache graphs;  // slot = coarse parity * 4 + fine parity * 2 + sampled
+  // conservation budget: accumulators, outer-face flux partials of a step, integral partials
+  BudgetAcc *d_acc = nullptr;
+  double *d_opart = nullptr, *d_ipart = nullptr;
+  int nb_outer = 0, nb_int_c = 0, nb_int_f = 0;
+  bool budget_complete = true;  // false once a batch has stepped the case (no budget there)
 };
 
 Refined *R(void *h) { return (Refined *)h; }
+
+// Device address of a double field of the case's budget accumulators.
+double *acc_field(Refined *r, size_t offset) { return (double *)((char *)r->d_acc + offset); }
 
 // host copies of the transfer algebra (same formulas as the device)
 void host_populations(const double m[10], double g[27]) {
@@ -201,23 +209,28 @@ void phase_before_fine(Refined *r, int sub) {
     hipLaunchKernelGGL((k_prolong_yz<S, false>), gyz, dim3(64), lds, f->stream, X, (T_ *)f->pop, f->L, r->B, r->V, f->scale, r->d_err, r->d_cols, r->n_cols);
 }
 
-// fine substep `sub` after its sweep: fine face flux, walls + load partials
+// fine substep `sub` after its sweep: fine face flux, walls + load partials, and after the
+// first substep the coarse domain's outer-face flux of this coarse step (budget)
 template <int S>
 void phase_after_fine(Refined *r, int sub) {
-  Lattice *f = r->f;
+  Lattice *f = r->f, *c = r->c;
   using T_ = typename St<S>::T;
   const double volf = r->dxf * r->dxf * r->dxf;
   const unsigned fflux_blocks = blocks_for(4LL * (2 * r->nt) * f->nz * FLUX_LANES, 256);
   WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
   double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;
-  // one launch: fine face flux and the walls with their loads (disjoint slots)
-  const dim3 grid(fflux_blocks + (f->nlinks ? r->wall_blocks : 0)), block(256);
-  if (f->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_after_fine<S, true>), grid, block, 0, f->stream, (T_ *)f->pop, f->L, r->B, r->d_T, volf,
-                       f->scale, f->inv_scale, sub, r->d_regf, W, part, (int)fflux_blocks);
-  else
-    hipLaunchKernelGGL((k_after_fine<S, false>), grid, block, 0, f->stream, (T_ *)f->pop, f->L, r->B, r->d_T, volf,
-                       f->scale, f->inv_scale, sub, r->d_regf, W, part, (int)fflux_blocks);
+  const int nb_walls = f->nlinks ? r->wall_blocks : 0;
+  const int nb_outer = sub == 0 ? r->nb_outer : 0;
+  // one launch: fine face flux, the walls with their loads and the outer flux (disjoint slots)
+  const dim3 grid(fflux_blocks + nb_walls + nb_outer), block(256);
+  const bool pf = f->sweeps % 2 == 0, pc = c->sweeps % 2 == 0;
+#define AK_AF(PF, PC)                                                                                              \
+  hipLaunchKernelGGL((k_after_fine<S, PF, PC>), grid, block, 0, f->stream, (T_ *)f->pop, f->L, r->B, r->d_T, volf, \
+                     f->scale, f->inv_scale, sub, r->d_regf, W, part, (int)fflux_blocks, nb_walls,                 \
+                     (const T_ *)c->pop, c->L, c->inv_scale, r->F, r->d_opart)
+  if (pf) { if (pc) AK_AF(true, true); else AK_AF(true, false); }
+  else { if (pc) AK_AF(false, true); else AK_AF(false, false); }
+#undef AK_AF
 }
 
 // end of the coarse step: reflux, restriction, loads (sampled) or the step tick
@@ -242,7 +255,8 @@ void phase_end(Refined *r, bool sample) {
   hipLaunchKernelGGL((k_step_end<S, PF, PC>), grid, block, 0, c->stream, (const T_ *)f->pop, f->L, f->inv_scale,  \
                      (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, c->inv_scale, r->d_before, r->d_regc,        \
                      r->d_regf, inv_volc, r->d_err, (int)cwave_blocks, ntiles, (const double *)r->d_part, nparts, \
-                     f->nelem, fscale, volf, c->d_ctl, c->d_hist, sample ? 1 : 0)
+                     f->nelem, fscale, volf, c->d_ctl, c->d_hist, sample ? 1 : 0, G)
+  const BudgetArgs G{r->d_opart, r->nb_outer, r->dxc * r->dxc * r->dxc, r->dxc / r->dtc, r->d_acc};
   if (pf) { if (pc) AK_END(true, true); else AK_END(true, false); }
   else { if (pc) AK_END(false, true); else AK_END(false, false); }
 #undef AK_END
@@ -256,6 +270,33 @@ void run_start(Refined *r) {
     case 0: phase_before_coarse<0>(r); break;
     case 1: phase_before_coarse<1>(r); break;
     default: phase_before_coarse<2>(r); break;
+  }
+}
+
+// Conserved integrals of both levels' owned cells (pre-collision state) into dst[4] (a
+// field of the device accumulators); with step > 0 the fold also records that step.
+template <int S>
+void launch_integrals(Refined *r, double *dst, long long step) {
+  Lattice *c = r->c, *f = r->f;
+  using T_ = typename St<S>::T;
+  double *pc = r->d_ipart, *pf = r->d_ipart + (size_t)r->nb_int_c * 4;
+  if (c->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_integrals<S, true>), dim3(r->nb_int_c), dim3(256), 0, c->stream, (const T_ *)c->pop, c->d_solid, c->L, c->inv_scale, pc);
+  else
+    hipLaunchKernelGGL((k_integrals<S, false>), dim3(r->nb_int_c), dim3(256), 0, c->stream, (const T_ *)c->pop, c->d_solid, c->L, c->inv_scale, pc);
+  if (f->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_integrals<S, true>), dim3(r->nb_int_f), dim3(256), 0, c->stream, (const T_ *)f->pop, f->d_solid, f->L, f->inv_scale, pf);
+  else
+    hipLaunchKernelGGL((k_integrals<S, false>), dim3(r->nb_int_f), dim3(256), 0, c->stream, (const T_ *)f->pop, f->d_solid, f->L, f->inv_scale, pf);
+  hipLaunchKernelGGL(k_integrals_final, dim3(1), dim3(64), 0, c->stream, (const double *)r->d_ipart, r->nb_int_c,
+                     r->nb_int_f, r->dxc * r->dxc * r->dxc, r->dxf * r->dxf * r->dxf, r->dxc / r->dtc, dst,
+                     step > 0 ? r->d_acc : nullptr, step);
+}
+void integrals_any(Refined *r, double *dst, long long step) {
+  switch (r->c->storage) {
+    case 0: launch_integrals<0>(r, dst, step); break;
+    case 1: launch_integrals<1>(r, dst, step); break;
+    default: launch_integrals<2>(r, dst, step); break;
   }
 }
 
@@ -345,6 +386,19 @@ int ak_refined_create(void **handle, int device, int nx, int ny, int nz, const i
     delete r;
     return -1;
   }
+  // budget: outer-face flux partials (one per block of the outer range), integral partials
+  r->nb_outer = (int)blocks_for(2LL * ((long long)ny * nz + (long long)nx * nz) * FLUX_LANES, 256);
+  r->nb_int_c = (int)blocks_for((long long)r->c->nx * r->c->ny * r->c->nz, 256);
+  r->nb_int_f = (int)blocks_for((long long)r->f->nx * r->f->ny * r->f->nz, 256);
+  if (dalloc(r->c, &r->d_acc, 1) || dalloc(r->c, &r->d_opart, (size_t)r->nb_outer * 4) ||
+      dalloc(r->c, &r->d_ipart, (size_t)(r->nb_int_c + r->nb_int_f) * 4)) {
+    ak_lattice_destroy(hf); ak_lattice_destroy(hc);
+    if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
+    delete r;
+    return -1;
+  }
+  r->c->pre_collision = r->f->pre_collision = true;
+  hipLaunchKernelGGL(k_budget_reset, dim3(1), dim3(64), 0, r->c->stream, r->d_acc);
   (void)hipMemsetAsync(r->d_err, 0, sizeof(int), r->c->stream);
   if (build_prolong_lists(r)) {
     ak_lattice_destroy(hf); ak_lattice_destroy(hc);
@@ -435,6 +489,11 @@ int ak_refined_init_seeded(void *handle, const double *p) {
   HIPCHK(c, hipGetLastError());
   if (ak_set_farfield(c, AK_FARFIELD_LEGACY, r->U)) return -1;
   r->steps = 0;
+  // budget from the initial state: accumulators cleared, integrals of the initial state
+  hipLaunchKernelGGL(k_budget_reset, dim3(1), dim3(64), 0, c->stream, r->d_acc);
+  integrals_any(r, acc_field(r, offsetof(BudgetAcc, initial)), 0);
+  r->budget_complete = true;
+  HIPCHK(c, hipGetLastError());
   return 0;
 }
 
@@ -461,6 +520,8 @@ int ak_refined_run(void *handle, int nsteps, long long first_step, int sample_ev
         default: refined_coarse_step<2>(r, sample); break;
       }
     };
+    const bool last = k == nsteps - 1;
+    if (last) integrals_any(r, acc_field(r, offsetof(BudgetAcc, before)), 0);  // the budget brackets the call's last step
     if (r->use_graphs) {
       const int slot = (int)(c->sweeps % 2) * 4 + (int)(f->sweeps % 2) * 2 + (sample ? 1 : 0);
       const unsigned long long key = (c->gen << 32) ^ f->gen;
@@ -470,6 +531,7 @@ int ak_refined_run(void *handle, int nsteps, long long first_step, int sample_ev
     }
     if (sample) c->hist_rows++;
     r->steps++;
+    if (last) integrals_any(r, acc_field(r, offsetof(BudgetAcc, after)), number);
   }
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -502,6 +564,8 @@ int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int 
       if (flush_history(c)) return -1;
       set_ctl(c, number, 0);
     }
+    const bool last = k == nsteps - 1;
+    if (last) integrals_any(r, acc_field(r, offsetof(BudgetAcc, before)), 0);
     switch (c->storage) {
       case 0: refined_coarse_step<0>(r, sample, &ev[8 * k]); break;
       case 1: refined_coarse_step<1>(r, sample, &ev[8 * k]); break;
@@ -509,6 +573,7 @@ int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int 
     }
     if (sample) c->hist_rows++;
     r->steps++;
+    if (last) integrals_any(r, acc_field(r, offsetof(BudgetAcc, after)), number);
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double tot = 0, cs = 0, fs = 0;
@@ -530,7 +595,7 @@ int ak_refined_run_profiled(void *handle, int nsteps, long long first_step, int 
 // lattice states (ak_save_state layout).  The transfer buffers are rebuilt every step.
 size_t ak_refined_state_bytes(void *handle) {
   Refined *r = R(handle);
-  return r ? sizeof(long long) + ak_state_bytes(r->c) + ak_state_bytes(r->f) : 0;
+  return r ? sizeof(long long) + ak_state_bytes(r->c) + ak_state_bytes(r->f) + sizeof(BudgetAcc) + sizeof(int) : 0;
 }
 
 int ak_refined_save_state(void *handle, void *host, size_t bytes) {
@@ -541,7 +606,14 @@ int ak_refined_save_state(void *handle, void *host, size_t bytes) {
   std::memcpy(o, &r->steps, sizeof(long long));
   o += sizeof(long long);
   if (ak_save_state(r->c, o, ak_state_bytes(r->c))) return -1;
-  return ak_save_state(r->f, o + ak_state_bytes(r->c), ak_state_bytes(r->f));
+  o += ak_state_bytes(r->c);
+  if (ak_save_state(r->f, o, ak_state_bytes(r->f))) return -1;
+  o += ak_state_bytes(r->f);
+  // the budget accumulators (the stream was synchronised by the state copies)
+  HIPCHK(r->c, hipMemcpy(o, r->d_acc, sizeof(BudgetAcc), hipMemcpyDeviceToHost));
+  const int complete = r->budget_complete ? 1 : 0;
+  std::memcpy(o + sizeof(BudgetAcc), &complete, sizeof(int));
+  return 0;
 }
 
 int ak_refined_load_state(void *handle, const void *host, size_t bytes) {
@@ -554,9 +626,46 @@ int ak_refined_load_state(void *handle, const void *host, size_t bytes) {
   i += sizeof(long long);
   const size_t nc = state_blob_size(r->c, i, bytes - sizeof(long long));
   const size_t nf = nc ? state_blob_size(r->f, i + nc, bytes - sizeof(long long) - std::min(nc, bytes - sizeof(long long))) : 0;
-  if (!nc || !nf || bytes != sizeof(long long) + nc + nf) return fail(nullptr, "refined state of another size or layout");
+  const size_t tail = sizeof(BudgetAcc) + sizeof(int);
+  if (!nc || !nf || bytes != sizeof(long long) + nc + nf + tail) return fail(nullptr, "refined state of another size or layout");
   if (ak_load_state(r->c, i, nc) || ak_load_state(r->f, i + nc, nf)) return -1;
+  HIPCHK(r->c, hipMemcpy(r->d_acc, i + nc + nf, sizeof(BudgetAcc), hipMemcpyHostToDevice));
+  int complete = 0;
+  std::memcpy(&complete, i + nc + nf + sizeof(BudgetAcc), sizeof(int));
+  r->budget_complete = complete != 0;
   r->steps = steps;
+  return 0;
+}
+
+// Conservation budget of the case (refined_solver.py:291-361), FP64, common units:
+// out[0..2] wall impulse of the last step, [3] its wall moment impulse, [4] its wall mass
+// exchange, [5..8] mass and momentum in through the outer faces in the last step, [9]
+// cumulative wall mass exchange, [10..12] cumulative wall impulse, [13..16] cumulative outer
+// in-flux, [17..20] conserved integrals (mass, momentum) of the initial state, [21..24] and
+// [25..28] before and after the last step of the last run call, [29] coarse steps accumulated,
+// [30] the step [21..28] bracket (0: none), [31] 1 if every step since the initial state was
+// accumulated (a batch does not keep the budget).  Synchronises.
+int ak_refined_budget(void *handle, double *out) {
+  Refined *r = R(handle);
+  if (!r) return fail(nullptr, "null refined handle");
+  if (!out) return fail(nullptr, "null output");
+  BudgetAcc a{};
+  HIPCHK(r->c, hipSetDevice(r->c->dev));
+  HIPCHK(r->c, hipStreamSynchronize(r->c->stream));
+  HIPCHK(r->c, hipMemcpy(&a, r->d_acc, sizeof a, hipMemcpyDeviceToHost));
+  for (int k = 0; k < 5; ++k) out[k] = a.wall[k];
+  for (int k = 0; k < 4; ++k) out[5 + k] = a.ext[k];
+  out[9] = a.cum_wall_mass;
+  for (int k = 0; k < 3; ++k) out[10 + k] = a.cum_wall_impulse[k];
+  for (int k = 0; k < 4; ++k) out[13 + k] = a.cum_ext[k];
+  for (int k = 0; k < 4; ++k) {
+    out[17 + k] = a.initial[k];
+    out[21 + k] = a.before[k];
+    out[25 + k] = a.after[k];
+  }
+  out[29] = (double)a.steps;
+  out[30] = (double)a.budget_step;
+  out[31] = r->budget_complete ? 1.0 : 0.0;
   return 0;
 }
 

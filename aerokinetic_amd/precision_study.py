@@ -1,0 +1,159 @@
+"""Storage-precision study on the 1536x1024x64 lattice (~100.7M cells).
+
+    python -m aerokinetic_amd.precision_study --steps 20000 --average-last 10000 \
+        --out gpurun_out/precision_c3.json
+
+NACA0012, chord 128 cells at 8 deg, Re 1e5, Ma 0.1, z periodic, characteristic farfield,
+seeded perturbation (aerokinetic_amd.cases.NACA0012_C128_1536x1024x64).  The same case runs
+with populations stored in FP64, FP32 and FP16 shifted deviations g = f - w (FP32 collision
+arithmetic for FP32/FP16), loads recorded every step.  Reported per storage:
+
+* Cl/Cd/Cm averaged over the last `--average-last` steps, and their relative error against
+  the FP64-storage run (the FP64 GPU path, itself matched per population to the FP64 oracle
+  at smaller sizes);
+* the field L2 error of rho and u at the last step against the FP64 run, over the fluid
+  cells: ||d rho|| / ||rho||, ||d rho|| / ||rho - 1|| and ||d u|| / ||u||;
+* the FP64 mass check: the density sum over the fluid cells at the start and at the end;
+* the low-Mach guards (local Mach, max |rho - 1|) of the last step, and the step rate.
+
+One progress line is printed every `--progress` steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+
+def _log(msg):
+    print(f"[precision_study] {msg}", flush=True)
+
+
+def guard_failure(rec):
+    """The low-Mach guards of a history record (solver.py:281-297): None if they hold."""
+    import math
+    if rec["nonfinite"] or not all(math.isfinite(rec[k]) for k in ("cl", "cd", "cm")):
+        return "non-finite state"
+    if rec["local_mach_max"] > 0.3 or rec["density_error_max"] > 0.1 or rec["density_min"] <= 0:
+        return f"local Mach {rec['local_mach_max']:.3g}, max |rho-1| {rec['density_error_max']:.3g}"
+    return None
+
+
+def run_storage(case, storage, steps, average_last, progress):
+    import numpy as np
+    import torch
+    from .solver import Solver
+    section, flow, grid, plan = case.build()
+    t_build = time.perf_counter()
+    s = Solver(section, flow, grid, storage=storage, plan=plan)
+    t_build = time.perf_counter() - t_build
+    start = s.diagnostics()
+    nfluid = int(s._fluid.sum())
+    loads = []
+    failure = None
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps and failure is None:
+        n = min(progress, steps - done)
+        recs = s.advance(n, 1)
+        for r in recs:
+            why = guard_failure(r)
+            if why:
+                failure = {"step": r["step"], "reason": why}
+                break
+            loads.append((r["cl"], r["cd"], r["cm"]))
+        done += n
+        last = recs[-1]
+        _log(f"{storage}: step {s.steps}, cl {last['cl']:.6f} cd {last['cd']:.6f} cm {last['cm']:.6f}, "
+             f"local Mach {last['local_mach_max']:.4f}, {(time.perf_counter() - t0) / done * 1e3:.3f} ms/step"
+             + (f"; guards failed at step {failure['step']}: {failure['reason']}" if failure else ""))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    loads = np.array(loads)
+    tail = loads[-average_last:]
+    cells = int(np.prod(plan.shape))
+    out = {"storage": storage, "farfield": grid.farfield, "steps": s.steps, "cells": cells, "fluid_cells": nfluid,
+           "build_s": t_build, "wall_s": wall, "ms_per_step": wall / done * 1e3, "mlups": cells * done / wall / 1e6,
+           "guards_failed": failure, "loads": loads,
+           "mean_last": dict(zip(("cl", "cd", "cm"), map(float, tail.mean(axis=0)))),
+           "rms_last": dict(zip(("cl", "cd", "cm"), map(float, tail.std(axis=0)))),
+           "averaged_steps": [int(len(loads) - len(tail) + 1), int(len(loads))],
+           "mass_start": start["density_mean"] * nfluid}
+    fields = None
+    if failure is None:
+        end = s.diagnostics()
+        out.update(mass_end=end["density_mean"] * nfluid, local_mach_max=end["local_mach_max"],
+                   density_error_max=end["density_error_max"])
+        rho, u = s.lattice.macroscopic()
+        if s._exterior:  # the physical box of the padded lattice
+            rho, u = rho[s._core], u[s._core]
+        fields = (rho, u, s._fluid.copy())
+    s.lattice.close()
+    del s
+    torch.cuda.empty_cache()
+    return out, fields
+
+
+def compare(result, fields, ref, ref_fields, average_last):
+    """Field L2 at the last step (both runs complete) and the coefficients averaged over the
+    last `average_last` steps that both runs completed, against the FP64 run."""
+    import numpy as np
+    out = {}
+    if fields is not None and ref_fields is not None:
+        (ra, ua, fl), (rb, ub, _) = fields, ref_fields
+        dr, du = (ra - rb)[fl], (ua - ub)[fl]
+        out["field_l2_vs_fp64"] = {"rho_rel": float(np.linalg.norm(dr) / np.linalg.norm(rb[fl])),
+                                   "rho_minus_1_rel": float(np.linalg.norm(dr) / np.linalg.norm(rb[fl] - 1.0)),
+                                   "u_rel": float(np.linalg.norm(du) / np.linalg.norm(ub[fl])),
+                                   "rho_max_abs": float(np.abs(dr).max()), "u_max_abs": float(np.abs(du).max())}
+    n = min(len(result["loads"]), len(ref["loads"]))
+    a, b = result["loads"][:n][-average_last:].mean(axis=0), ref["loads"][:n][-average_last:].mean(axis=0)
+    out["common_window"] = [int(n - min(n, average_last) + 1), int(n)]
+    out["mean_coef_common_window"] = {k: [float(a[i]), float(b[i])] for i, k in enumerate(("cl", "cd", "cm"))}
+    out["mean_coef_rel_err_vs_fp64"] = {k: float(abs(a[i] - b[i]) / abs(b[i])) for i, k in enumerate(("cl", "cd", "cm"))}
+    return out
+
+
+def main(argv=None):
+    from . import cases
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--average-last", type=int, default=10000)
+    ap.add_argument("--storages", default="fp64,fp32,fp16", help="fp64 first: the others are compared with it")
+    ap.add_argument("--progress", type=int, default=1000)
+    ap.add_argument("--farfield", default="characteristic_transport_v1",
+                    choices=("characteristic_transport_v1", "exterior_regularized_fd_v1", "legacy"))
+    ap.add_argument("--out", default=None, help="JSON report path")
+    a = ap.parse_args(argv)
+    storages = [s for s in a.storages.split(",") if s]
+    if storages[0] != "fp64":
+        raise SystemExit("the first storage must be fp64 (the reference run)")
+    case = cases.NACA0012_C128_1536x1024x64
+    if a.farfield != case.farfield:
+        from dataclasses import replace
+        case = replace(case, name=case.name + " " + a.farfield, farfield=a.farfield)
+    report = {"workload": f"{case.name}, {a.farfield} farfield, loads every step", "steps": a.steps,
+              "average_last": a.average_last, "by_storage": {}}
+    ref = ref_fields = None
+    for st in storages:
+        _log(f"{case.name}: {st}, {a.steps} steps")
+        res, fields = run_storage(case, st, a.steps, a.average_last, a.progress)
+        if ref is None:
+            ref, ref_fields = res, fields
+        else:
+            res.update(compare(res, fields, ref, ref_fields, a.average_last))
+            del fields
+        report["by_storage"][st] = res
+        _log(json.dumps({k: v for k, v in res.items() if k != "loads"}))
+    for res in report["by_storage"].values():
+        res.pop("loads")
+    if a.out:
+        with open(a.out, "w", encoding="utf-8") as f:
+            json.dump(report, f, indent=1)
+    print(json.dumps(report))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

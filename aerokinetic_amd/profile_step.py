@@ -11,7 +11,8 @@ aerokinetic_amd.cases); `copy`, a device-to-device copy of a buffer of known siz
 (the calibration kernel for memory counters); `serve`, the batched service over
 seeded two-level requests (--steps = the per-case step cap); `slab-y` (1536x128x64) and
 `slab-x` (192x1024x64), one GPU's share of the 1536x1024x64 lattice at 8 GPUs as a plain
-fluid slab (sweep kernel only).  Each workload prints one
+fluid slab (sweep kernel only); `traffic`, the streaming copy of a known size and one
+sweep of each AA form of the 1536x1024x64 lattice per storage, for memory-counter runs.  Each workload prints one
 line per phase with HIP-event timings of the lattice's stream, so a profiler's
 per-kernel averages can be set beside them.
 """
@@ -97,6 +98,28 @@ def run_copy(gib, steps):
     _log(f"copy {int(gib * 2**30)} bytes: {ms:.4f} ms per copy, {gbs:.1f} GB/s (read + write)")
 
 
+def run_traffic(storages, copy_gib=4.0):
+    """The launches whose memory counters calibrate and measure HBM traffic: the streaming copy
+    of `copy_gib` GiB twice, then for each storage one sampled step of each AA form of the
+    1536x1024x64 lattice (direct launches, so every sweep is one dispatch)."""
+    import torch
+    from . import cases
+    from .solver import Solver
+    ms, gbs = copy_bandwidth(copy_gib, 2)
+    _log(f"copy {int(copy_gib * 2**30)} bytes x 2: {ms:.4f} ms per copy")
+    case = cases.NACA0012_C128_1536x1024x64
+    section, flow, grid, plan = case.build()
+    for storage in storages:
+        s = Solver(section, flow, grid, storage=storage, plan=plan, graphs=False)
+        sweep_ms, _ = s.lattice.run_profiled(2, 0, 1)
+        s.lattice.read_history()
+        _log(f"{case.name} {storage}: 2 sampled steps (one per AA form), sweep {sweep_ms / 2:.4f} ms per launch, "
+             f"{int(s._fluid.sum())} fluid cells")
+        s.lattice.close()
+        del s
+        torch.cuda.empty_cache()
+
+
 def run_serve(storage, cases, rate, chunk, window, max_steps):
     from . import serving as S
     reqs = S.polar_requests(cases, seed=2026, rate=rate, chord=(16, 24), span=(8, 8), reynolds=(1e4, 3e4))
@@ -135,7 +158,8 @@ def main(argv=None):
     from . import cases
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--workload", choices=("uniform-c32", "exterior-c32", "uniform-100m", "two-level", "copy", "serve",
-                                           "slab-y", "slab-x"), required=True)
+                                           "slab-y", "slab-x", "traffic"), required=True)
+    ap.add_argument("--storages", default="fp64,fp16", help="traffic: storages of the 1536x1024x64 sweeps")
     ap.add_argument("--storage", choices=("fp64", "fp32", "fp16"), default="fp64")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -148,6 +172,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.workload == "copy":
         run_copy(a.copy_gib, a.steps)
+    elif a.workload == "traffic":
+        run_traffic([x for x in a.storages.split(",") if x], a.copy_gib)
     elif a.workload == "serve":
         run_serve(a.storage, a.cases, a.rate, a.chunk, 400, a.steps)
     elif a.workload in ("slab-y", "slab-x"):

@@ -3,8 +3,9 @@
 Public surface of the reference (`src/aerokinetic/refined_solver.py:115-379`,
 `refined_run.py:19-157`): `RefinedSolver(project, refinement=RefinementSettings(),
 progress=None)` with `step() -> record` (step, t_star, cl, cd, cm[, time_s]),
-`diagnostics()` and `run(output, callback=None, stop_event=None)` writing
-`history.csv` (step,t_star,cl,cd,cm,time_s), `metadata.json`, `summary.json`.
+`diagnostics()` (low-Mach guards and the FP64 conservation budget kept on the device),
+`run(output, **kwargs)` writing `history.csv` (step,t_star,cl,cd,cm[,time_s]),
+`residuals.csv`, `metadata.json`, `summary.json`, and `from_checkpoint` / `resume_run`.
 
 The coarse lattice (legacy farfield, z periodic) and the fine lattice (box plus two
 donor layers in x and y, z periodic, Bouzidi walls) are native AA lattices; the
@@ -14,11 +15,9 @@ restriction and loads -- runs in the HIP library without host round trips.
 from __future__ import annotations
 
 from dataclasses import asdict
-import csv
 import ctypes
 import json
 import math
-import time
 from pathlib import Path
 
 import numpy as np
@@ -79,8 +78,10 @@ def host_setup(project: Project, plan, pad=2):
             "seed": seed_parameters(section, flow, q)}
 
 
+BUDGET_WIDTH = 32  # doubles returned by ak_refined_budget
+
+
 class RefinedSolver:
-    HISTORY_COLUMNS = ["step", "t_star", "cl", "cd", "cm", "time_s"]
     pad = 2
 
     def __init__(self, project: Project, refinement=RefinementSettings(), progress=None, *, storage="fp64",
@@ -137,6 +138,13 @@ class RefinedSolver:
         self._keep = (links, q, el)
         self.hist_width = int(self.L.ak_history_width(self._levels[0]))
         self._seed = hs["seed"]
+        # refined_run.py:31-33: time_s only when the physical time step is known
+        self.HISTORY_COLUMNS = ["step", "t_star", "cl", "cd", "cm"] + (
+            ["time_s"] if self.plan.dt_seconds is not None else [])
+        fine_fluid_cells = int((~self._fine_solid_xy).sum()) * self.fine_shape[2]
+        coarse_owned_cells = int(np.prod(p.shape)) - int(np.prod(box.shape))
+        self._reference_volume = (coarse_owned_cells * p.coarse_level.dx ** 3
+                                  + fine_fluid_cells * p.fine_level.dx ** 3)
         self.initialize_flow()
 
     def close(self):
@@ -192,9 +200,15 @@ class RefinedSolver:
         return rec
 
     def _read(self):
-        out = np.empty((4096, self.hist_width))
-        n = _lib.check(self.L.ak_read_history(self._levels[0], _lib.dptr(out), 4096), self._levels[0], "ak_read_history")
-        return out[:n]
+        """Every recorded history row (read in blocks until the device history is empty)."""
+        rows = []
+        while True:
+            out = np.empty((4096, self.hist_width))
+            n = _lib.check(self.L.ak_read_history(self._levels[0], _lib.dptr(out), 4096), self._levels[0],
+                           "ak_read_history")
+            rows.append(out[:n])
+            if n < 4096:
+                return np.concatenate(rows)
 
     def advance(self, nsteps, sample_every=1):
         if nsteps <= 0:
@@ -236,7 +250,44 @@ class RefinedSolver:
         _lib.check(self.L.ak_refined_get_populations(self.h, level, _lib.dptr(g)), None, "ak_refined_get_populations")
         return g
 
+    def budget(self):
+        """The device's FP64 budget accumulators (ak_refined_budget layout)."""
+        out = np.zeros(BUDGET_WIDTH)
+        _lib.check(self.L.ak_refined_budget(self.h, _lib.dptr(out)), None, "ak_refined_budget")
+        return out
+
+    @staticmethod
+    def _last_budget(b, volume):
+        """Budget of the last coarse step from the accumulators (refined_solver.py:330-340):
+        mass and momentum residuals per owned volume, wall mass exchange and impulse, and what
+        entered through the outer faces."""
+        if b[30] <= 0:
+            return {}
+        before, after = b[21:25], b[25:29]
+        wall_impulse, wall_mass, external = b[0:3], b[4], b[5:9]
+        mass_residual = after[0] - before[0] - external[0] - wall_mass
+        momentum_residual = after[1:] - before[1:] - external[1:] + wall_impulse
+        return {"mass_residual_per_volume": float(mass_residual / volume),
+                "momentum_residual_per_volume": (momentum_residual / volume).tolist(),
+                "wall_mass_exchange": float(wall_mass), "external_mass_in": float(external[0]),
+                "wall_impulse": wall_impulse.tolist(), "external_impulse_in": external[1:].tolist()}
+
+    @property
+    def last_budget(self):
+        """Mass/momentum budget of the last coarse step ({} before the first step)."""
+        return self._last_budget(self.budget(), self._reference_volume)
+
+    @property
+    def coarse_boundary_calls(self):
+        return int(self.budget()[29])
+
+    @property
+    def fine_boundary_calls(self):
+        return 2 * self.coarse_boundary_calls
+
     def diagnostics(self):
+        """Low-Mach guards over the leaf cells and the conservation budget
+        (refined_solver.py:347-361)."""
         mach, dens = 0.0, 0.0
         for lvl in self._levels:
             out = np.zeros(5)
@@ -246,8 +297,16 @@ class RefinedSolver:
             mach, dens = max(mach, out[0]), max(dens, out[1])
         if mach > 0.3 or dens > 0.1:
             raise RuntimeError(f"Refined low-Mach state invalid: max Ma={mach:.5g}, density deviation={dens:.5g}")
-        return {"local_mach_max": float(mach), "density_error_max": float(dens),
-                "wall_fallback_links_including_donors": self.wall_fallback_links}
+        b = self.budget()
+        out = {"local_mach_max": float(mach), "density_error_max": float(dens),
+               "wall_fallback_links_including_donors": self.wall_fallback_links,
+               "coarse_boundary_calls": int(b[29]), "fine_boundary_calls": 2 * int(b[29]),
+               "cumulative_wall_mass_exchange": float(b[9]),
+               "cumulative_wall_mass_exchange_over_initial_mass": float(b[9] / b[17]),
+               "last_budget": self._last_budget(b, self._reference_volume)}
+        if not b[31]:
+            out["budget_complete"] = False  # batched steps keep no budget
+        return out
 
     @property
     def residual_dt(self):
@@ -271,119 +330,146 @@ class RefinedSolver:
         from .checkpoint import save_checkpoint
         return save_checkpoint(self, path, context)
 
-    def run(self, output, callback=None, stop_event=None, resume=False, residual_callback=None):
-        """Run project.run.duration_convective coarse time (refined_run.py:19-157); with resume=True
-        continue an interrupted run in `output` from the checkpoint this solver was restored from."""
+    @classmethod
+    def from_checkpoint(cls, path, progress=None, **kwargs):
+        """Rebuild a case from its checkpoint and continue from it (refined_solver.py:367-370)."""
+        from .checkpoint import restore_checkpoint
+        return restore_checkpoint(cls, path, progress, **kwargs)
+
+    @classmethod
+    def resume_run(cls, output, **kwargs):
+        """Continue the interrupted run in `output` from its checkpoint (refined_run.py:160-163)."""
+        output = Path(output)
+        solver = cls.from_checkpoint(output / "checkpoint.npz", progress=kwargs.pop("progress", None))
+        return solver.run(output, resume=True, **kwargs)
+
+    def run(self, output, *, callback=None, stop_event=None, resume=False, residual_callback=None):
+        """Run project.run.duration_convective of coarse time into a new or empty directory
+        (refined_run.py:19-157); resume=True continues an interrupted run in `output` from the
+        checkpoint this solver was restored from.  callback(solver, record) may return False
+        to stop; history.csv holds every sample_every-th step's loads, residuals.csv every
+        check_every-th step's residuals and balance defects."""
+        from ._io import CsvLog, Recovery
         from .convergence import RunControl
         from .timing import duration_steps
-        from ._io import file_hash
         from . import __version__
-        import os
         settings = self.project.run
         output = Path(output)
-        history = output / "history.csv"
         if not resume and output.exists() and any(output.iterdir()):
-            raise FileExistsError("Use a new/empty output directory to avoid mixing histories")
+            raise FileExistsError("Use a new/empty refined result directory")
         output.mkdir(parents=True, exist_ok=True)
-        end_step = self.steps + duration_steps(settings.duration_convective, self.plan.coarse_level.dt)
-        rows = []
+        recovery = Recovery(output)
         if resume:
-            old = json.loads((output / "metadata.json").read_text(encoding="utf-8"))
+            metadata = json.loads((output / "metadata.json").read_text(encoding="utf-8"))
             context = getattr(self, "_checkpoint_context", {})
-            if (old.get("case_fingerprint") != self.case_fingerprint
+            saved_project, saved_refinement = load_refined_project(output / "project.json")
+            if (saved_project.case_fingerprint != self.project.case_fingerprint or saved_refinement != self.refinement
+                    or saved_project.run != settings or metadata.get("case_fingerprint") != self.case_fingerprint
                     or context.get("case_fingerprint") != self.case_fingerprint):
-                raise ValueError("Resume requires the original project, settings and checkpoint")
-            if old["status"] == "completed":
-                raise ValueError("This run has already completed")
-            end_step, prefix = context["end_step"], context["history_bytes"]
-            if file_hash(history, prefix) != context["history_sha256"]:
-                raise ValueError("History before the checkpoint has changed; refusing to mix results")
-            if history.stat().st_size > prefix:
-                import shutil
-                import uuid
-                recovery = output / "recovery" / uuid.uuid4().hex
-                recovery.mkdir(parents=True)
-                shutil.copy2(history, recovery / "history_before_resume.csv")
-                with history.open("r+b") as handle:
-                    handle.truncate(prefix)
-            with history.open(newline="", encoding="utf-8") as handle:
-                for row in csv.DictReader(handle):
-                    rows.append({"step": int(row["step"]), "t_star": float(row["t_star"]),
-                                 **{k: float(row[k]) for k in ("cl", "cd", "cm")}})
+                raise ValueError("Refined project/run settings changed; cannot mix histories")
+            if metadata["status"] == "completed":
+                raise ValueError("The requested refined run has already completed")
+            end_step = context["end_step"]
+            history = CsvLog(output / "history.csv", self.HISTORY_COLUMNS,
+                             resume=(context["history_bytes"], context["history_sha256"]), recovery=recovery)
+            metadata["resumed_with_version"] = __version__
         else:
+            end_step = self.steps + duration_steps(settings.duration_convective, self.plan.coarse_level.dt)
             write_json(output / "project.json", refined_project_dict(self.project, self.refinement))
-        control = RunControl(self, settings, output, resume=resume, callback=residual_callback)
-        metadata = {"version": __version__, "refined_plan": json.loads(json.dumps(self.plan.as_dict(), default=list)),
-                    "case_fingerprint": self.case_fingerprint, "status": "running", "end_step": end_step,
-                    "resume_supported": True, "device": {"backend": "hip-gfx950", "storage": self.storage}}
+            metadata = {"format": "aerokinetic_refined_run", "version": __version__,
+                        "case_fingerprint": self.case_fingerprint, "flow": asdict(self.flow), "grid": asdict(self.grid),
+                        "refinement": asdict(self.refinement),
+                        "plan": json.loads(json.dumps(self.plan.as_dict(), default=list)),
+                        "run": asdict(settings), "start_step": self.steps,
+                        "contours": [{"name": e.name, "xy": e.xy.tolist()} for e in self.section.elements],
+                        "coefficient_convention": "Cl up; Cd downstream; Cm nose-up about the chosen reference; "
+                                                  "area=c*periodic_span",
+                        "force_time": "average of both fine momentum-transfer intervals; timestamp at coarse "
+                                      "interval midpoint",
+                        "experimental_validation": "not_performed", "resume_supported": True,
+                        "reference_chord_input_units": self.section.chord,
+                        "moment_reference_input_units": self.section.moment_reference,
+                        "device": {"backend": "hip-gfx950", "storage": self.storage,
+                                   "library": self.L.ak_version().decode()}}
+            history = CsvLog(output / "history.csv", self.HISTORY_COLUMNS)
+        control = RunControl.open(self, settings, output, resume=resume, callback=residual_callback,
+                                  start_step=metadata.get("start_step", self.steps), recovery=recovery)
+        metadata.update(status="running", target_steps=end_step,
+                        residual_definition="Volume-weighted L2/Linf of d(rho,j_common)/dt*, owned fluid only; "
+                                            "integral balance defects of the last coarse step are a separate "
+                                            "diagnostic")
         write_json(output / "metadata.json", metadata)
-        state, reason, start = "completed", "duration_reached", time.monotonic()
+
+        def checkpoint():
+            size, digest = history.durable()
+            context = {"case_fingerprint": self.case_fingerprint, "end_step": end_step,
+                       "history_bytes": size, "history_sha256": digest}
+            context.update(control.checkpoint_context())
+            self.save_checkpoint(output / "checkpoint.npz", context)
+
+        status, reason, error = "completed", "duration_reached", None
+        se, ke, ce = settings.sample_every, settings.check_every, settings.checkpoint_every
         try:
-            with history.open("a" if resume else "w", newline="", encoding="utf-8") as handle:
-                writer = csv.DictWriter(handle, self.HISTORY_COLUMNS)
-                if not resume:
-                    writer.writeheader()
-
-                def checkpoint():
-                    handle.flush()
-                    os.fsync(handle.fileno())
-                    context = {"case_fingerprint": self.case_fingerprint, "end_step": end_step,
-                               "history_bytes": history.stat().st_size, "history_sha256": file_hash(history)}
-                    context.update(control.checkpoint_context())
-                    self.save_checkpoint(output / "checkpoint.npz", context)
-
-                se, ce, ke = settings.sample_every, settings.checkpoint_every, settings.check_every
-                while self.steps < end_step:
-                    if stop_event is not None and stop_event.is_set():
-                        state, reason = "stopped", "user_stopped"
+            while self.steps < end_step:
+                if stop_event is not None and stop_event.is_set():
+                    status, reason = "stopped", "user_stopped"
+                    break
+                # one device call up to the next event: a sample a callback must see, a check,
+                # a checkpoint, or the end
+                events = [end_step, self.steps + 64 * se] + [(self.steps // e + 1) * e for e in (ke, ce) if e]
+                if callback is not None:
+                    events.append((self.steps // se + 1) * se)
+                recs = self.advance(min(events) - self.steps, se)
+                for rec in recs:
+                    if not all(math.isfinite(rec[k]) for k in ("cl", "cd", "cm")):
+                        raise NumericalFailure("Non-finite loads; the run is invalid")
+                    history.append(rec)
+                    if callback is not None and callback(self, rec) is False:
+                        status, reason = "stopped", "user_stopped"
                         break
-                    nxt = min(end_step, (self.steps // se + 1) * se if callback else self.steps + 64 * se)
-                    for every in (ce, ke):
-                        if every:
-                            nxt = min(nxt, (self.steps // every + 1) * every)
-                    recs = self.advance(nxt - self.steps, se)
-                    stop = False
-                    for rec in recs:
-                        if not all(math.isfinite(rec[k]) for k in ("cl", "cd", "cm")):
-                            raise NumericalFailure("Non-finite loads; the run is invalid")
-                        writer.writerow({k: rec.get(k, "") for k in self.HISTORY_COLUMNS})
-                        rows.append(rec)
-                        if callback is not None and callback(self, rec) is False:
-                            state, reason, stop = "stopped", "user_stopped", True
-                            break
-                    handle.flush()
-                    if ke and self.steps % ke == 0:
-                        self.diagnostics()
-                        control.observe()
-                    if stop:
-                        break
-                    selected = control.stop_reason()
-                    if selected:
-                        reason = selected
-                        if reason == "wall_time_limit":
-                            state = "stopped"
-                        break
-                    if ce and self.steps % ce == 0 and self.steps < end_step:
-                        checkpoint()
-                self.diagnostics()
-                checkpoint()
-        except BaseException as error:
-            state, reason = "failed", "failed"
-            metadata["failure"] = f"{type(error).__name__}: {error}"
-            raise
-        finally:
-            sel = np.array([[r["cl"], r["cd"], r["cm"]] for r in rows if r["t_star"] >= settings.average_after])
-            ok = len(sel) and state != "failed"
-            summary = {"status": state, "steps": self.steps, "t_star": self.steps * self.plan.coarse_level.dt,
-                       "wall_clock_seconds": time.monotonic() - start, "samples": len(rows),
-                       "mean": dict(zip(("cl", "cd", "cm"), map(float, sel.mean(axis=0)))) if ok else None,
-                       "rms_fluctuation": dict(zip(("cl", "cd", "cm"), map(float, sel.std(axis=0)))) if ok else None}
-            summary.update(control.summary(reason))
-            control.close()
-            metadata["termination_reason"] = reason
-            metadata["status"] = state
-            write_json(output / "metadata.json", metadata)
-            write_json(output / "summary.json", summary)
+                history.flush()
+                if ke and self.steps % ke == 0:
+                    self.diagnostics()
+                    control.observe()
+                if status == "stopped":
+                    break
+                selected = control.stop_reason()
+                if selected:
+                    reason = selected
+                    status = "stopped" if selected == "wall_time_limit" else status
+                    break
+                if ce and self.steps % ce == 0 and self.steps < end_step:
+                    checkpoint()
+            self.diagnostics()
+            checkpoint()
+        except Exception as failure:
+            status, reason, error = "failed", "failed", f"{type(failure).__name__}: {failure}"
+        rows = history.read()
+        history.close()
+        selected_rows = [r for r in rows if float(r["t_star"]) >= settings.average_after]
+        coef = np.array([[float(r[k]) for k in ("cl", "cd", "cm")] for r in selected_rows]).reshape(-1, 3)
+        final = {"state": "invalid"}
+        if error is None:
+            try:
+                final = self.diagnostics()
+            except Exception as failure:
+                status, reason, error = "failed", "failed", f"{type(failure).__name__}: {failure}"
+        summary = {"status": status, "steps": self.steps, "fine_steps": 2 * self.steps,
+                   "t_star": self.steps * self.plan.coarse_level.dt, "average_after_t_star": settings.average_after,
+                   "samples": len(rows), "averaging_samples": len(selected_rows),
+                   "mean": dict(zip(("cl", "cd", "cm"), map(float, coef.mean(axis=0)))) if len(coef) else None,
+                   "rms": dict(zip(("cl", "cd", "cm"), map(float, coef.std(axis=0)))) if len(coef) else None,
+                   "experimental_validation": "not_performed", "statistical_convergence": "not_established",
+                   "diagnostics": final}
+        summary.update(control.summary(reason))
+        control.close()
+        if error:
+            summary["error"] = error
+        metadata.update(status=status, completed_steps=self.steps, termination_reason=reason)
+        write_json(output / "metadata.json", metadata)
+        write_json(output / "summary.json", summary)
+        if error:
+            raise RuntimeError(error)
         return summary
 
 

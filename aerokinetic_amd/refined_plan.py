@@ -171,6 +171,13 @@ class RefinedPlan:
                  fine_cells_per_chord=round(1 / self.fine_level.dx), fine_shape=self.fine_shape)
         return d
 
+    @classmethod
+    def from_dict(cls, d):
+        """The plan that as_dict() described (the derived keys are ignored)."""
+        return cls(tuple(d["shape"]), tuple(d["origin"]), FineBox(tuple(d["box"]["lower"]), tuple(d["box"]["upper"])),
+                   AcousticLevel(**d["coarse_level"]), AcousticLevel(**d["fine_level"]), d["span"], d["dt_seconds"],
+                   d["estimated_memory_gib"], d["minimum_wall_interface_distance_coarse_cells"], tuple(d["notes"]))
+
 
 def refined_project_dict(project, refinement):
     return {"format": "aerokinetic_refined_project", "schema_version": 1,

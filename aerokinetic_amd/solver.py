@@ -25,7 +25,6 @@ tile the autotuner (aerokinetic_amd.autotune) measures fastest on this lattice.
 from __future__ import annotations
 
 from dataclasses import asdict
-import csv
 import json
 import math
 import time
@@ -35,7 +34,7 @@ import numpy as np
 
 from .config import EXTERIOR_FARFIELD, Flow, Grid, Plan
 from .geometry import Section
-from ._io import atomic_binary, write_json
+from ._io import PublishedFile, write_json
 from .lattice import Lattice
 from .project import Project, RunSettings
 from .walls import labels, wall_links
@@ -256,6 +255,26 @@ class Solver:
         from .checkpoint import save_checkpoint
         return save_checkpoint(self, path, context)
 
+    @classmethod
+    def from_checkpoint(cls, path, progress=None, **kwargs):
+        """Rebuild a case from its checkpoint and continue from it (solver.py:340-343)."""
+        from .checkpoint import restore_checkpoint
+        return restore_checkpoint(cls, path, progress, **kwargs)
+
+    @classmethod
+    def resume_run(cls, output, callback=None, stop_event=None, progress=None, residual_callback=None):
+        """Continue the interrupted run in `output` with the run settings it recorded
+        (solver.py:345-351)."""
+        output = Path(output)
+        metadata = json.loads((output / "metadata.json").read_text(encoding="utf-8"))
+        solver = cls.from_checkpoint(output / "checkpoint.npz", progress)
+        run = dict(metadata["run"])
+        if isinstance(run.get("termination"), dict):
+            from .convergence import StopCriteria
+            run["termination"] = StopCriteria(**run["termination"])
+        return solver.run(output=output, callback=callback, stop_event=stop_event, residual_callback=residual_callback,
+                          resume=True, **run)
+
     def save_frame(self, path, max_pixels=800):
         """Mid-span speed, vorticity and Cp planes (solver.py:312-334)."""
         nx, ny, nz = self.plan.shape
@@ -270,7 +289,7 @@ class Solver:
         mask = self._solid_xy[::stride, ::stride]
         for a in (speed, omega_z, cp):
             a[mask] = np.nan
-        with atomic_binary(path) as handle:
+        with PublishedFile(path) as handle:
             np.savez_compressed(handle, speed=speed, omega_z=omega_z, cp=cp, mask=mask,
                                 origin=self.plan.origin, dx=self.plan.dx * stride,
                                 t_star=self.steps * self.plan.dt_convective)
@@ -283,13 +302,11 @@ class Solver:
         callback(solver, record) may return False to stop.  Mean/RMS coefficients
         are plain sample statistics over t_star >= average_after.
         """
+        from ._io import CsvLog, Recovery
         from .convergence import RunControl, StopCriteria
         from .timing import duration_steps
-        from ._io import file_hash
         from . import __version__
-        import os
         import shutil
-        import uuid
         termination = termination or StopCriteria()
         settings = RunSettings(duration_convective, sample_every, frame_every, check_every, average_after,
                                checkpoint_every, termination)
@@ -300,8 +317,7 @@ class Solver:
         output.mkdir(parents=True, exist_ok=True)
         (output / "frames").mkdir(exist_ok=resume)
         end_step = self.steps + duration_steps(duration_convective, self.plan.dt_convective)
-        history = output / "history.csv"
-        rows = []
+        recovery = Recovery(output)
         start_step = self.steps
         if resume:
             # checkpoint.py / solver.py:389-425: same project, settings and durable history prefix
@@ -316,25 +332,19 @@ class Solver:
                 raise ValueError("This run has already completed")
             start_step = old.get("start_step", 0)
             end_step = context["end_step"]
-            prefix = context["history_bytes"]
-            if file_hash(history, prefix) != context["history_sha256"]:
-                raise ValueError("History before the checkpoint has changed; refusing to mix results")
-            # samples and frames written after the last durable checkpoint are kept aside
-            future = [f for f in (output / "frames").glob("frame_*.npz") if int(f.stem.split("_")[-1]) > self.steps]
-            if history.stat().st_size > prefix or future:
-                recovery = output / "recovery" / uuid.uuid4().hex
-                recovery.mkdir(parents=True)
-                shutil.copy2(history, recovery / "history_before_resume.csv")
-                with history.open("r+b") as handle:
-                    handle.truncate(prefix)
-                for f in future:
-                    shutil.move(str(f), str(recovery / f.name))
-            with history.open(newline="", encoding="utf-8") as handle:
-                for row in csv.DictReader(handle):
-                    rows.append({"step": int(row["step"]), "t_star": float(row["t_star"]),
-                                 **{k: float(row[k]) for k in ("cl", "cd", "cm")}})
+            history = CsvLog(output / "history.csv", self.HISTORY_COLUMNS,
+                             resume=(context["history_bytes"], context["history_sha256"]), recovery=recovery)
+            # frames written after the last durable checkpoint are kept aside with the rows
+            for f in (output / "frames").glob("frame_*.npz"):
+                if int(f.stem.split("_")[-1]) > self.steps:
+                    shutil.move(str(f), str(recovery.directory() / f.name))
+            rows = [{"step": int(r["step"]), "t_star": float(r["t_star"]),
+                     **{k: float(r[k]) for k in ("cl", "cd", "cm")}} for r in history.read()]
             if rows and rows[-1]["step"] > self.steps:
                 raise ValueError("History extends beyond the checkpoint")
+        else:
+            history = CsvLog(output / "history.csv", self.HISTORY_COLUMNS)
+            rows = []
         if self.steps > end_step:
             raise ValueError("Checkpoint is past the requested end time")
         metadata = {"version": __version__, "flow": asdict(self.flow), "grid": asdict(self.grid),
@@ -351,7 +361,8 @@ class Solver:
                                "library": _lib.load().ak_version().decode()}}
         if not resume:
             project.save(output / "project.json")
-        control = RunControl(self, settings, output, resume=resume, callback=residual_callback)
+        control = RunControl.open(self, settings, output, resume=resume, callback=residual_callback,
+                                  recovery=recovery)
         metadata["residual_definition"] = ("Volume-weighted L2/Linf of d(rho,j_common)/dt*, owned fluid only; "
                                            "stationary-flow residual, not a turbulence-accuracy measure")
         if self._exterior:
@@ -359,67 +370,60 @@ class Solver:
         write_json(output / "metadata.json", metadata)
         state, reason = "completed", "duration_reached"
         start = time.monotonic()
+        def checkpoint():
+            size, digest = history.durable()
+            context = {"case_fingerprint": project.case_fingerprint, "end_step": end_step,
+                       "history_bytes": size, "history_sha256": digest}
+            context.update(control.checkpoint_context())
+            self.save_checkpoint(output / "checkpoint.npz", context)
+
         try:
             self.diagnostics()
             self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
-            with history.open("a" if resume else "w", newline="", encoding="utf-8") as handle:
-                writer = csv.DictWriter(handle, self.HISTORY_COLUMNS)
-                if not resume:
-                    writer.writeheader()
-
-                def checkpoint():
-                    handle.flush()
-                    os.fsync(handle.fileno())
-                    context = {"case_fingerprint": project.case_fingerprint, "end_step": end_step,
-                               "history_bytes": history.stat().st_size, "history_sha256": file_hash(history)}
-                    context.update(control.checkpoint_context())
-                    self.save_checkpoint(output / "checkpoint.npz", context)
-
-                stop = False
-                while self.steps < end_step and not stop:
-                    if stop_event is not None and stop_event.is_set():
-                        state, reason = "stopped", "user_stopped"
+            stop = False
+            while self.steps < end_step and not stop:
+                if stop_event is not None and stop_event.is_set():
+                    state, reason = "stopped", "user_stopped"
+                    break
+                # next event: a sample (when a callback must see it), a check, a frame, the end
+                nxt = end_step
+                for every in (check_every, frame_every, checkpoint_every) + ((sample_every,) if callback else ()):
+                    if every:
+                        nxt = min(nxt, (self.steps // every + 1) * every)
+                if not callback:
+                    nxt = min(nxt, self.steps + 256 * max(sample_every, 1))
+                recs = self.advance(nxt - self.steps, sample_every)
+                for rec in recs:
+                    if rec["nonfinite"] or not all(math.isfinite(rec[k]) for k in ("cl", "cd", "cm")):
+                        raise NumericalFailure("Non-finite loads; the run is invalid")
+                    if rec["local_mach_max"] > 0.3 or rec["density_error_max"] > 0.1 or rec["density_min"] <= 0:
+                        raise NumericalFailure(f"Low-Mach validity check failed: local Ma={rec['local_mach_max']:.3g}, "
+                                               f"|rho-1|max={rec['density_error_max']:.3g}")
+                    history.append(rec)
+                    rows.append(rec)
+                    self.last_loads = rec
+                    if callback is not None and callback(self, rec) is False:
+                        state, reason, stop = "stopped", "user_stopped", True
                         break
-                    # next event: a sample (when a callback must see it), a check, a frame, the end
-                    nxt = end_step
-                    for every in (check_every, frame_every, checkpoint_every) + ((sample_every,) if callback else ()):
-                        if every:
-                            nxt = min(nxt, (self.steps // every + 1) * every)
-                    if not callback:
-                        nxt = min(nxt, self.steps + 256 * max(sample_every, 1))
-                    recs = self.advance(nxt - self.steps, sample_every)
-                    for rec in recs:
-                        if rec["nonfinite"] or not all(math.isfinite(rec[k]) for k in ("cl", "cd", "cm")):
-                            raise NumericalFailure("Non-finite loads; the run is invalid")
-                        if rec["local_mach_max"] > 0.3 or rec["density_error_max"] > 0.1 or rec["density_min"] <= 0:
-                            raise NumericalFailure(f"Low-Mach validity check failed: local Ma={rec['local_mach_max']:.3g}, "
-                                                   f"|rho-1|max={rec['density_error_max']:.3g}")
-                        row = {k: rec.get(k, "") for k in self.HISTORY_COLUMNS}
-                        writer.writerow(row)
-                        rows.append(row)
-                        self.last_loads = rec
-                        if callback is not None and callback(self, rec) is False:
-                            state, reason, stop = "stopped", "user_stopped", True
-                            break
-                    handle.flush()
-                    if check_every and self.steps % check_every == 0:
-                        self.diagnostics()
-                        control.observe()
-                    if (frame_every and self.steps % frame_every == 0) or self.steps == end_step:
-                        self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
-                    if stop:
-                        break
-                    selected = control.stop_reason()
-                    if selected:
-                        reason = selected
-                        if reason == "wall_time_limit":
-                            state = "stopped"
-                        break
-                    if checkpoint_every and self.steps % checkpoint_every == 0 and self.steps < end_step:
-                        checkpoint()
-                self.diagnostics()
-                self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
-                checkpoint()
+                history.flush()
+                if check_every and self.steps % check_every == 0:
+                    self.diagnostics()
+                    control.observe()
+                if (frame_every and self.steps % frame_every == 0) or self.steps == end_step:
+                    self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
+                if stop:
+                    break
+                selected = control.stop_reason()
+                if selected:
+                    reason = selected
+                    if reason == "wall_time_limit":
+                        state = "stopped"
+                    break
+                if checkpoint_every and self.steps % checkpoint_every == 0 and self.steps < end_step:
+                    checkpoint()
+            self.diagnostics()
+            self.save_frame(output / "frames" / f"frame_{self.steps:010d}.npz")
+            checkpoint()
         except BaseException as error:
             state, reason = "failed", "failed"
             metadata["failure"] = f"{type(error).__name__}: {error}"
@@ -436,6 +440,7 @@ class Solver:
                        "experimental_validation": "not_performed"}
             summary.update(control.summary(reason))
             control.close()
+            history.close()
             if self._exterior:
                 boundary = self._failed_safe(self.boundary_diagnostics, state == "failed")
                 summary["boundary_diagnostics"] = metadata["boundary_model"] = boundary

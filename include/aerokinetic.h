@@ -229,6 +229,20 @@ int ak_refined_load_state(void *handle, const void *host, size_t bytes);
 /* Pre-collision populations of one level (0 coarse, 1 fine padded), exchange layout. */
 int ak_refined_get_populations(void *handle, int level, double *host_out);
 
+/* FP64 conservation budget of the case, common units (RefinedSolver.conserved_integrals,
+ * step and diagnostics bookkeeping: refined_solver.py:291-361; the outer face-flux samplers,
+ * refined_solver.py:215-222, 271-280).  Kept on the device: every coarse step adds its wall
+ * mass exchange, wall impulse and outer-face in-flux to the accumulators; the conserved
+ * integrals (mass, momentum) of both levels' owned cells are taken at the initial state and
+ * around the last step of every ak_refined_run call.  out[32] = {wall impulse x/y/z of the
+ * last step, its wall moment impulse, its wall mass exchange, mass and momentum x/y/z in
+ * through the outer faces in the last step, cumulative wall mass exchange, cumulative wall
+ * impulse [3], cumulative outer in-flux [4], integrals of the initial state [4], before [4]
+ * and after [4] the last step of the last run call, coarse steps accumulated, the step that
+ * before/after bracket (0: none), 1 if every step since the initial state was accumulated
+ * (0 once a batch has stepped the case: batches keep no budget)}.  Synchronises. */
+int ak_refined_budget(void *handle, double *out);
+
 /* ---- Batches of two-level cases (serving) --------------------------------------------------
  * No counterpart in the reference, which runs one case per process (Solver.run /
  * refined_run.run, solver.py:353-537, refined_run.py:19-157; polar sweeps loop over such

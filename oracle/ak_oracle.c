@@ -377,6 +377,23 @@ void ako_init_seeded(void *h, double U, double amp, double phy, double phz, doub
     s->steps = 0;
 }
 
+/* every cell of both arrays, ghosts included, set to one population vector: the reference
+ * fills both of its arrays with the free stream before writing the interior
+ * (refined_solver.py:228-230), so ghost entries no boundary writes keep that value */
+void ako_fill_all(void *h, const double *g) {
+    ako *s = (ako *)h;
+    for (long c = 0; c < s->ncell; ++c) {
+        memcpy(s->src + c * NQ, g, sizeof(double) * NQ);
+        memcpy(s->dst + c * NQ, g, sizeof(double) * NQ);
+    }
+}
+
+/* the current array with its ghost layer: (nx+2, ny+2, nz+2, 27) */
+void ako_get_padded(void *h, double *g) {
+    ako *s = (ako *)h;
+    memcpy(g, s->src, sizeof(double) * NQ * (size_t)s->ncell);
+}
+
 /* populations in the (nx, ny, nz, 27) interior layout */
 void ako_set_pdf(void *h, const double *g) {
     ako *s = (ako *)h;

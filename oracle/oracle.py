@@ -65,6 +65,8 @@ def lib():
         L.ako_get_cells.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
         L.ako_set_cells.argtypes = [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_double_p]
         L.ako_collided.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.ako_fill_all.argtypes = [ctypes.c_void_p, _c_double_p]
+        L.ako_get_padded.argtypes = [ctypes.c_void_p, _c_double_p]
         L.ako_wall_loads.argtypes = [ctypes.c_void_p, _c_double_p, _c_double_p]
         _lib = L
     return _lib
@@ -154,6 +156,18 @@ class Oracle:
     def pdf(self):
         g = np.zeros((*self.shape, 27))
         self.L.ako_get_pdf(self.h, _dp(g))
+        return g
+
+    def fill_all(self, g27):
+        """Every cell of both arrays, ghosts included, to one population vector."""
+        g27 = np.ascontiguousarray(g27, dtype=np.float64)
+        assert g27.shape == (27,)
+        self.L.ako_fill_all(self.h, _dp(g27))
+
+    def padded_pdf(self):
+        """The current array with its ghost layer, (nx+2, ny+2, nz+2, 27)."""
+        g = np.zeros((*(n + 2 for n in self.shape), 27))
+        self.L.ako_get_padded(self.h, _dp(g))
         return g
 
     def set_farfield(self, kind):

@@ -14,7 +14,13 @@ NumPy transfer algebra and the coarse/fine interface orchestration.  Followed:
   state (interface.py:297-352 with the overrides of refined_solver.py:183-203,
   271-284);
 * fine-wall loads averaged over the two substeps (refined_solver.py:311-345);
-* initial field (refined_solver.py:224-269).
+* initial field (refined_solver.py:224-269);
+* the conservation budget: conserved integrals of the owned cells of both levels
+  (refined_solver.py:291-299), the mass and momentum entering through the coarse
+  domain's outer faces (the outer face-flux samplers over the ghost-padded coarse array
+  after its boundaries, refined_solver.py:215-222, 271-280; both arrays start filled with
+  the free stream, ghosts included, refined_solver.py:228-230), the wall mass exchange and
+  impulse, `last_budget` and the cumulative sums (refined_solver.py:311-361).
 """
 from __future__ import annotations
 
@@ -141,6 +147,7 @@ class FaceFlux:
 
 class RefinedOracle:
     pad = 2
+    outer_flux_before_boundaries = False
 
     def __init__(self, project, settings, plan: RefinedPlan | None = None):
         self.project, self.settings = project, settings
@@ -170,6 +177,15 @@ class RefinedOracle:
         self.core = virtual_core(self.box.shape)
         self.fine_core = np.repeat(np.repeat(np.repeat(self.core, 2, 0), 2, 1), 2, 2)
         self.full_periodic_axes = [a for a in range(3) if self.box.lower[a] == 0 and self.box.upper[a] == self.shape[a]]
+        # outer face-flux samplers of the coarse domain, on the ghost-padded array
+        padded_coarse = tuple(n + 2 for n in self.shape)
+        self.outer = []
+        for axis in range(3):
+            ranges = [(1, self.shape[a] + 1) for a in range(3) if a != axis]
+            for side, index in ((-1, 0), (1, self.shape[axis])):
+                self.outer.append((side, FaceFlux(padded_coarse, axis, index, ranges, False)))
+        self.coarse_owned = np.ones(self.shape, bool)
+        self.coarse_owned[self.box.slices] = False
         self.faces, self.support = [], np.zeros(self.shape, bool)
         for axis in range(3):
             if axis in self.full_periodic_axes:
@@ -261,14 +277,35 @@ class RefinedOracle:
         origin = tuple(lo * dc - self.pad * df for lo in self.box.lower)
         fine = populations(self._initial_fields(self.padded, df, origin))
         coarse[self.box.slices] = self._restrict_fine(fine[self.fine_slices])
+        self.coarse.fill_all(self.free_pdf)
+        self.fine.fill_all(self.free_pdf)
         self.coarse.set_pdf(coarse)
         self.fine.set_pdf(fine)
         self.coarse.set_farfield("legacy")
         self.fine.set_farfield("periodic")
         self.steps = 0
+        self.fine_fluid = np.broadcast_to(~self.fine.solid()[self.fine_slices[:2]][:, :, None], self.fine_shape)
+        self.cumulative_wall_mass_exchange = 0.0
+        self.cumulative_wall_impulse = np.zeros(3)
+        self.cumulative_external_in = np.zeros(4)
+        self.last_budget = {}
+        self.initial_integrals = self.conserved_integrals()
+
+    def conserved_integrals(self):
+        """Mass and momentum (common units) of the owned cells of both levels."""
+        mass, impulse = 0.0, np.zeros(3)
+        fine = self.fine.pdf()[self.fine_slices]
+        for pdf, owned, level in ((self.coarse.pdf(), self.coarse_owned, self.coarse_level),
+                                  (fine, self.fine_fluid, self.fine_level)):
+            rho = pdf.sum(axis=-1) + 1
+            j = pdf @ STENCIL.astype(float)
+            mass += rho[owned].sum() * level.dx**3
+            impulse += j[owned].sum(axis=0) * level.dx**3 * level.lattice_velocity_unit
+        return {"mass": float(mass), "momentum": impulse.tolist()}
 
     # -------------------------------------------------------------- step
     def step(self):
+        start = self.conserved_integrals()
         L = self.fine.L
         dc, df = self.coarse_level.dx, self.fine_level.dx
         old = self.coarse.pdf()
@@ -276,7 +313,14 @@ class RefinedOracle:
         L.ako_collide(self.coarse.h)
         post = self.coarse.pdf()
         coarse_flux = [f[3](post) * dc**3 for f in self.faces]
+        if self.outer_flux_before_boundaries:  # a planted mistake for the tests only
+            padded = self.coarse.padded_pdf()
         L.ako_boundaries(self.coarse.h)
+        if not self.outer_flux_before_boundaries:
+            padded = self.coarse.padded_pdf()
+        external = np.zeros(4)
+        for side, sampler in self.outer:
+            external -= side * sampler(padded).sum(axis=(0, 1)) * dc**3
         L.ako_stream(self.coarse.h)
         prediction = self.coarse.pdf()
         before, after = self._donor_block(old), self._donor_block(prediction)
@@ -318,9 +362,38 @@ class RefinedOracle:
         vol, vel = df**3, self.fine_level.lattice_velocity_unit
         impulse = (walls[0][:3] + walls[1][:3]) * vol * vel
         moment = (walls[0][3] + walls[1][3]) * vol * vel
+        wall_mass = (walls[0][4] + walls[1][4]) * vol
         q_area = 0.5 * self.plan.span
         force = impulse / self.coarse_level.dt
         mz = moment / self.coarse_level.dt
+        # budget of this coarse step
+        end = self.conserved_integrals()
+        external[1:] *= vel
+        mass_residual = end["mass"] - start["mass"] - external[0] - wall_mass
+        momentum_residual = np.array(end["momentum"]) - start["momentum"] - external[1:] + impulse
+        volume = (np.count_nonzero(self.coarse_owned) * dc**3 + np.count_nonzero(self.fine_fluid) * df**3)
+        self.last_budget = {"mass_residual_per_volume": float(mass_residual / volume),
+                            "momentum_residual_per_volume": (momentum_residual / volume).tolist(),
+                            "wall_mass_exchange": float(wall_mass), "external_mass_in": float(external[0]),
+                            "wall_impulse": impulse.tolist(), "external_impulse_in": external[1:].tolist()}
+        self.cumulative_wall_mass_exchange += wall_mass
+        self.cumulative_wall_impulse += impulse
+        self.cumulative_external_in += external
         return {"step": self.steps, "t_star": (self.steps - 0.5) * self.coarse_level.dt,
                 "cl": float(force[1] / q_area), "cd": float(force[0] / q_area), "cm": float(-mz / q_area),
-                "wall_mass_exchange": float((walls[0][4] + walls[1][4]) * vol)}
+                "wall_mass_exchange": float(wall_mass)}
+
+    def diagnostics(self):
+        """Guards over the leaf cells and the budget (refined_solver.py:347-361)."""
+        mach, derr = 0.0, 0.0
+        for pdf, owned in ((self.coarse.pdf(), self.coarse_owned), (self.fine.pdf()[self.fine_slices], self.fine_fluid)):
+            rho = pdf.sum(axis=-1)[owned] + 1
+            u = (pdf @ STENCIL.astype(float))[owned] / rho[:, None]
+            mach = max(mach, float(np.linalg.norm(u, axis=-1).max() * math.sqrt(3)))
+            derr = max(derr, float(np.max(abs(rho - 1))))
+        return {"local_mach_max": mach, "density_error_max": derr,
+                "coarse_boundary_calls": self.steps, "fine_boundary_calls": 2 * self.steps,
+                "cumulative_wall_mass_exchange": self.cumulative_wall_mass_exchange,
+                "cumulative_wall_mass_exchange_over_initial_mass":
+                    self.cumulative_wall_mass_exchange / self.initial_integrals["mass"],
+                "last_budget": self.last_budget}

@@ -7,7 +7,8 @@ so the tests never need it.  Sources (reference paths):
   docs/verification_stage1_20261106/profile_threads1.json (last_sample after 272 steps)
   tests/fixtures/legacy051_project.json (fingerprint fixture)
   docs/verification_040/{startup_project.json,startup_history.csv} (two-level startup, every 4 coarse steps)
-  docs/verification_multielement_050/fowler/{project.json,history.csv} (two elements, every coarse step)
+  docs/verification_multielement_050/{fowler,double,slotted}/{project.json,history.csv,residuals.csv,summary.json}
+      (two- and three-element cases, every coarse step; summary diagnostics: budget and guards)
   docs/verification_user_tests_20261108/raw/final_smoke/study/cases/N16/{project.json,history.csv}
       (exterior farfield, 136x51x8, every step from step 1)
   docs/verification_exterior_library_20261108/raw/cold_offline_final_v2/runtime/parent/{project.json,history.csv}
@@ -48,6 +49,8 @@ for name, (rel, last_step) in CASES.items():
 REFINED = {
     "refined_startup_n32": ("docs/verification_040", "startup_project.json", "startup_history.csv", 40),
     "refined_fowler_n32": ("docs/verification_multielement_050/fowler", "project.json", "history.csv", 32),
+    "refined_double_n32": ("docs/verification_multielement_050/double", "project.json", "history.csv", 32),
+    "refined_slotted_n32": ("docs/verification_multielement_050/slotted", "project.json", "history.csv", 800),
 }
 for name, (rel, proj, hist, last_step) in REFINED.items():
     dst = OUT / name
@@ -61,6 +64,15 @@ for name, (rel, proj, hist, last_step) in REFINED.items():
             if int(row["step"]) > last_step:
                 break
             writer.writerow(row)
+
+# The two-level runs' conservation budget and residuals: summary.json "diagnostics" (guards,
+# cumulative wall mass exchange, last_budget) and residuals.csv (l2/linf/balance columns).
+for name in ("refined_fowler_n32", "refined_double_n32", "refined_slotted_n32"):
+    rel = REFINED[name][0]
+    summary = json.loads((REF / rel / "summary.json").read_text())
+    (OUT / name / "summary_diagnostics.json").write_text(json.dumps(
+        {"steps": summary["steps"], "diagnostics": summary["diagnostics"], "mean": summary["mean"]}, indent=1) + "\n")
+    shutil.copyfile(REF / rel / "residuals.csv", OUT / name / "residuals.csv")
 
 profile = json.loads((REF / "docs/verification_stage1_20261106/profile_threads1.json").read_text())
 (OUT / "profile_n32_naca0012_81.json").write_text(json.dumps({
