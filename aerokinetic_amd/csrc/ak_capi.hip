@@ -1004,6 +1004,11 @@ void launch_diag_state(Lattice *s, double *part, unsigned nb) {
   else
     hipLaunchKernelGGL((k_diag_state<S, PE, false, false>), dim3(nb), dim3(256), 0, s->stream, (const T *)s->pop, s->d_solid, s->L, s->inv_scale, part);
 }
+// blocks of the diagnostics reduction: one thread per cell up to DIAG_MAX_BLOCKS blocks, which
+// then stride over the lattice
+unsigned diag_blocks(Lattice *s) {
+  return (unsigned)std::min<long long>(blocks_for((long long)s->nx * s->ny * s->nz, 256), DIAG_MAX_BLOCKS);
+}
 template <int S>
 void diag_any(Lattice *s, double *part, unsigned nb) {
   if (s->sweeps % 2 == 0) launch_diag_state<S, true>(s, part, nb);
@@ -1032,25 +1037,15 @@ extern "C" {
 int ak_diagnostics(void *handle, double *out) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
-  const unsigned nb = blocks_for((long long)s->nx * s->ny * s->nz, 256);
+  const unsigned nb = diag_blocks(s);
   double *d = nullptr;
-  HIPCHK(s, hipMalloc((void **)&d, (size_t)nb * 5 * sizeof(double)));
+  HIPCHK(s, hipMalloc((void **)&d, ((size_t)nb * 5 + 8) * sizeof(double)));
   AK_DISPATCH(s, diag_any, s, d, nb);
-  std::vector<double> h((size_t)nb * 5);
-  hipError_t e = hipStreamSynchronize(s->stream);
-  if (e == hipSuccess) e = hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->stream, (const double *)d, (int)nb, d + (size_t)nb * 5);
+  hipError_t e = hipMemcpyAsync(out, d + (size_t)nb * 5, 5 * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(s, "diagnostics: %s", hipGetErrorString(e));
-  double sum = 0.0, u2 = 0.0, dr = 0.0, rmin = 1e300, bad = 0.0;
-  for (unsigned b = 0; b < nb; ++b) {
-    sum += h[5 * b]; u2 = std::max(u2, h[5 * b + 1]); dr = std::max(dr, h[5 * b + 2]);
-    rmin = std::min(rmin, h[5 * b + 3]); bad = std::max(bad, h[5 * b + 4]);
-  }
-  out[0] = std::sqrt(3.0) * std::sqrt(u2);
-  out[1] = dr;
-  out[2] = rmin;
-  out[3] = bad;
-  out[4] = sum;
   return 0;
 }
 

@@ -892,40 +892,49 @@ __global__ void k_import(typename St<S>::T *__restrict__ A, Desc L, const double
 
 // Read-only state diagnostics: per-block [sum rho, max |u|^2, max |rho-1|, min rho, bad]
 // over fluid cells (the guards also over the exterior ring, the density sum over the
-// physical fluid only); the host folds the blocks in a fixed order.  IN reads the incoming
-// (pre-collision) populations, the stored state of a two-level case's levels.
+// physical fluid only); k_diag_final folds the blocks in a fixed order.  The grid strides over
+// the lattice, so the partials need at most DIAG_MAX_BLOCKS x 5 doubles whatever its size.
+// IN reads the incoming (pre-collision) populations, the stored state of a two-level case's
+// levels.
+constexpr int DIAG_MAX_BLOCKS = 4096;
 template <int S, bool PE, bool PXY, bool IN>
 __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__restrict__ A,
                                                     const uint8_t *__restrict__ solid, Desc L,
                                                     float inv_scale, double *__restrict__ part) {
   using C = typename St<S>::C;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = (long long)L.nx * L.ny * L.nz;
+  const long long stride = (long long)gridDim.x * blockDim.x;
   double sr = 0.0, vu = 0.0, vd = 0.0, vr = 1e300;
   int bad = 0;
-  if (t < n) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
     const int x = (int)(t % L.nx), y = (int)((t / L.nx) % L.ny), z = (int)(t / ((long long)L.nx * L.ny));
     const uint8_t mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)];
-    if (mk == 0 || mk == 3) {
-      const long long c = L.base + cell_of(L, x, y, z);
-      C r = 0, jx = 0, jy = 0, jz = 0;
+    if (mk != 0 && mk != 3) continue;
+    const long long c = L.base + cell_of(L, x, y, z);
+    C r = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) {
-        long long src;
-        if constexpr (IN) src = loc_in<PE>(L, x, y, z, j);
-        else if constexpr (PE) src = c + (long long)opp(j) * L.QS;
-        else src = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
-        const C v = ld<S>(A + src, inv_scale);
-        r += v;
-        if (cxq(j)) jx += cxq(j) * v;
-        if (cyq(j)) jy += cyq(j) * v;
-        if (czq(j)) jz += czq(j) * v;
-      }
-      const double rho = 1.0 + (double)r;
-      const double ux = (double)jx / rho, uy = (double)jy / rho, uz = (double)jz / rho;
-      const double u2 = ux * ux + uy * uy + uz * uz;
-      if (!(isfinite(rho) && isfinite(u2))) bad = 1;
-      else { sr = mk == 0 ? rho : 0.0; vu = u2; vd = fabs((double)r); vr = rho; }
+    for (int j = 0; j < NQ; ++j) {
+      long long src;
+      if constexpr (IN) src = loc_in<PE>(L, x, y, z, j);
+      else if constexpr (PE) src = c + (long long)opp(j) * L.QS;
+      else src = c + (long long)j * L.QS + nb_off<PXY>(L, x, y, z, cxq(j), cyq(j), czq(j));
+      const C v = ld<S>(A + src, inv_scale);
+      r += v;
+      if (cxq(j)) jx += cxq(j) * v;
+      if (cyq(j)) jy += cyq(j) * v;
+      if (czq(j)) jz += czq(j) * v;
+    }
+    const double rho = 1.0 + (double)r;
+    const double ux = (double)jx / rho, uy = (double)jy / rho, uz = (double)jz / rho;
+    const double u2 = ux * ux + uy * uy + uz * uz;
+    if (!(isfinite(rho) && isfinite(u2))) {
+      bad = 1;
+    } else {
+      if (mk == 0) sr += rho;
+      vu = u2 > vu ? u2 : vu;
+      const double d = fabs((double)r);
+      vd = d > vd ? d : vd;
+      vr = rho < vr ? rho : vr;
     }
   }
 #pragma unroll
@@ -948,6 +957,38 @@ __global__ __launch_bounds__(256) void k_diag_state(const typename St<S>::T *__r
     }
     double *p = part + (long long)blockIdx.x * 5;
     p[0] = o0; p[1] = o1; p[2] = o2; p[3] = o3; p[4] = o4;
+  }
+}
+
+// Fold of the diagnostics partials (one wave, fixed order: lane-strided blocks, then lanes in
+// order): out = {local Mach max, max |rho-1|, min rho, non-finite flag, sum of rho}.
+__global__ void k_diag_final(const double *__restrict__ part, int nblocks, double *__restrict__ out) {
+  __shared__ double acc[5][64];
+  const int lane = threadIdx.x;
+  double s = 0.0, u = 0.0, d = 0.0, r = 1e300, b = 0.0;
+  for (int k = lane; k < nblocks; k += 64) {
+    const double *p = part + (long long)k * 5;
+    s += p[0];
+    u = p[1] > u ? p[1] : u;
+    d = p[2] > d ? p[2] : d;
+    r = p[3] < r ? p[3] : r;
+    b = p[4] > b ? p[4] : b;
+  }
+  acc[0][lane] = s; acc[1][lane] = u; acc[2][lane] = d; acc[3][lane] = r; acc[4][lane] = b;
+  __syncthreads();
+  if (lane == 0) {
+    for (int k = 1; k < 64; ++k) {
+      s += acc[0][k];
+      u = acc[1][k] > u ? acc[1][k] : u;
+      d = acc[2][k] > d ? acc[2][k] : d;
+      r = acc[3][k] < r ? acc[3][k] : r;
+      b = acc[4][k] > b ? acc[4][k] : b;
+    }
+    out[0] = sqrt(3.0) * sqrt(u);
+    out[1] = d;
+    out[2] = r;
+    out[3] = b;
+    out[4] = s;
   }
 }
 

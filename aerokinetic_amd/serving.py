@@ -81,6 +81,43 @@ def polar_requests(n, seed=2026, rate=None, chord=(32, 96), span=(8, 32), reynol
     return out
 
 
+def case_cells(req: CaseRequest):
+    """Lattice cells of a request's two-level case: coarse cells plus fine cells of the box."""
+    plan = project_of(req)[2]
+    return int(np.prod(plan.shape)) + 8 * int(np.prod(plan.box.shape))
+
+
+def serving_requests(n, seed=2026, rate=None, cells=(2e6, 2e8), span_ratio=0.25, reynolds=(1e4, 1e5),
+                     alpha=(-4.0, 12.0), camber=(0.0, 0.04), thickness=(0.09, 0.18), flap=(0.0, 30.0)):
+    """Seeded requests of the serving workload: a NACA 4-digit main element (camber and
+    thickness uniform) with a 25 %-chord flap (deflection uniform in `flap`, gap and overlap
+    1-3 % chord), angle uniform, Reynolds number log-uniform, and a lattice size log-uniform in
+    `cells` (coarse + fine cells): the coarse cells per chord are the integer whose case comes
+    closest to the drawn size, with span = max(8, span_ratio x chord) cells."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        if rate:
+            t += float(rng.exponential(1.0 / rate))
+        target = float(np.exp(rng.uniform(np.log(cells[0]), np.log(cells[1]))))
+        kw = dict(camber=float(rng.uniform(*camber)), position=0.4, thickness=float(rng.uniform(*thickness)),
+                  alpha_deg=float(rng.uniform(*alpha)),
+                  reynolds=float(np.exp(rng.uniform(np.log(reynolds[0]), np.log(reynolds[1])))),
+                  flap_deflection_deg=float(rng.uniform(*flap)), flap_gap=float(rng.uniform(0.01, 0.03)),
+                  flap_overlap=float(rng.uniform(0.01, 0.03)))
+        # the case size grows like the chord cubed (~28 c^3 for this domain and span ratio)
+        c0 = max(16, int(round((target / 28.0) ** (1.0 / 3.0))))
+        best = None
+        for c in range(max(16, c0 - 3), c0 + 4):
+            req = CaseRequest(i, t, chord_cells=c, span_cells=max(8, int(round(span_ratio * c))), **kw)
+            err = abs(math.log(case_cells(req) / target))
+            if best is None or err < best[0]:
+                best = (err, req)
+        out.append(best[1])
+    return out
+
+
 def section_of(req: CaseRequest):
     """Main element (chord 1) and, if requested, a flap below and behind its trailing edge."""
     from .geometry import Contour, Section, naca4_profile

@@ -1,42 +1,40 @@
 """Throughput of the D3Q27 cumulant lattice-Boltzmann step on one MI355X.
 
-Headline workload (the two-level configuration): NACA0012, chord 64 fine cells
-(32 coarse) at 6 deg, Re 3e4, Ma 0.1; coarse lattice 256x192x8 (legacy farfield,
-z periodic) with a fine patch of 160x96x16 fine cells (80x48x8 coarse) sub-cycled
-twice per coarse step; Bouzidi walls on the fine level; Cl/Cd/Cm recorded every
-coarse step.  One step = one coarse step: donor moments, coarse sweep, coarse face
-flux and farfield, two fine substeps (prolongation in time, fine sweep, fine face
-flux, walls + loads), reflux and restriction.  Populations are resident in HBM
-before timing.  MLUPS counts coarse cells + 2 x fine cells per coarse step (the
-reference's own two-level MLUPS convention).  `value` is the FP64-storage run; the
-FP32-storage run of the same case is reported beside it, with its loads compared
-to FP64.
+Headline workload (the precision-study configuration): a uniform 1536x1024x64 lattice
+(100,663,296 cells) around a NACA0012 of chord 128 cells at 8 deg, Re 1e5, Ma 0.1, z
+periodic, characteristic farfield on the x/y faces, seeded perturbation, Cl/Cd/Cm recorded
+every step.  One step = boundary phase (Bouzidi walls + loads, farfield) + the fused AA
+stream-collide sweep.  `value` is the FP64-storage MLUPS (cells x steps / s over exactly
+--steps timed steps, populations resident in HBM before timing); the same case in FP32 and
+FP16 shifted storage (g = f - w, FP32 collision arithmetic) is reported beside it
+(`by_storage`), with its loads and rho/u fields compared to FP64.
 
-roofline: the dominant kernel (largest total time inside the step) is the fine
-sweep; its algorithmic bytes per launch = updated fine cells x 2 x 27 x bytes per
-population (one read + one write per population, AA pattern), divided by its mean
-duration from HIP events recorded on the stream it is launched on.
+roofline: the dominant kernel is the sweep (~97 % of a step).  Algorithmic bytes per launch
+= fluid cells x 2 x 27 x bytes per population (one read + one write per population, AA
+pattern in place), divided by its mean duration from HIP events that the sweep launch
+records itself on the stream it runs on (both AA forms, direct launches).  `traffic` is
+measured in this run: two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over a
+child process that runs the streaming copy of 4 GiB and one sweep of each AA form per
+storage; each counter is scaled by what it reads for the copy of known size.
 
-exterior_farfield: the plain 384x256x8 NACA0012 case (chord 32, 4 deg, Re 1e4, Ma 0.1)
-with the exterior farfield (a 386x258x8 padded lattice whose ring is rebuilt every step):
-step rate and the split of a step into sweep, boundary and ring phases.
+two_level: coarse 256x192x8 + fine 160x96x16 around a NACA0012 of chord 64 fine cells at
+6 deg, Re 3e4 (one step = one coarse step with both fine substeps and all transfers; MLUPS
+counts coarse cells + 2 x fine cells per coarse step), FP64 and FP32 storage.
 
-uniform_sweep: the same step on a uniform 1536x1024x64 lattice (100.7M cells,
-NACA0012 chord 128 at 8 deg, Re 1e5, characteristic farfield) in FP64, FP32 and
-FP16 shifted storage -- the HBM-roofline view of the fused AA sweep at full size.
+exterior_farfield: the plain 384x256x8 NACA0012 case (chord 32, 4 deg, Re 1e4) with the
+exterior farfield (a 386x258x8 padded lattice whose ring is rebuilt every step).
 
-copy: the library's streaming device copy of 4 GiB (attainable HBM read+write
-bandwidth on this GPU), reported beside the vendor peak.
+serving: continuous batching of two-level requests in FP16 storage against one-at-a-time
+FP64 solves of the same requests: seeded Poisson arrivals of a NACA 4-digit main element +
+25 %-chord flap (0-30 deg), lattice sizes log-uniform in 2M-200M cells, Re 1e4-1e5,
+retirement when the 2000-step Cl window varies < 0.5 % or at the step cap; cases/hour and
+p50/p99 latency.
 
-traffic: HBM bytes per launch of the dominant kernel from the memory counters
-(FETCH_SIZE, WRITE_SIZE per dispatch), each scaled by what the counter reads for the
-streaming copy of a known size; measured separately (the counters serialise the
-GPU) and recorded below with the command that produced them.
+copy: the library's streaming device copy of 4 GiB (attainable HBM read+write bandwidth).
 
-cpu_baseline: the FP64 oracle (C/OpenMP lattices + NumPy transfer, the
-reference's data flow) on the host, same two-level case, a bounded number of
-coarse steps, all host cores; beside it the uniform 384x256x8 oracle step on all
-cores and on one core.
+cpu_baseline: the FP64 oracle (C/OpenMP, the reference's two-array pull data flow) on the
+host cores: a 1536x1024x8 slice of the headline lattice (same airfoil, same farfield) on all
+cores, with the plain 384x256x8 case on all cores and on one core beside it.
 """
 from __future__ import annotations
 
@@ -53,20 +51,8 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
 BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
+STORE_ID = {"fp64": 0, "fp32": 1, "fp16": 2}
 
-# HBM bytes per launch from the memory counters (rocprofv3 --pmc FETCH_SIZE, then
-# WRITE_SIZE, KiB per dispatch; FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors that
-# map the counters onto the 8 GiB read + 8 GiB written by the streaming copy), for the
-# kernels whose roofline is reported: (workload, storage, kernel) -> bytes.
-MEASURED_TRAFFIC = {  # profiles/hbm_traffic.md (mean over both AA parities)
-    ('two-level', 'fp64', 'fine k_sweep'): 117721287,
-    ('two-level', 'fp64', 'coarse k_sweep'): 173270528,
-    ('two-level', 'fp32', 'fine k_sweep'): 62851843,
-    ('two-level', 'fp32', 'coarse k_sweep'): 87715949,
-    ('uniform-100m', 'fp16', 'k_sweep'): 11123233305,
-    ('uniform-100m', 'fp32', 'k_sweep'): 22186486482,
-    ('uniform-100m', 'fp64', 'k_sweep'): 44633303245,
-}
 METRIC = "MLUPS & %HBM roofline, FP64 vs FP16-store; cases/hour + p50/p99 latency"
 
 
@@ -74,14 +60,78 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def roofline(bytes_per_launch, seconds_per_launch, traffic_key=None, copy_gbs=None):
+def roofline(bytes_per_launch, seconds_per_launch, copy_gbs=None, traffic=None):
     achieved = bytes_per_launch / seconds_per_launch / 1e9
     out = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-           "frac": achieved / PEAK_HBM_GBS, "traffic": MEASURED_TRAFFIC.get(traffic_key),
+           "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
            "algorithmic_bytes_per_launch": bytes_per_launch, "launch_ms": seconds_per_launch * 1e3}
     if copy_gbs:
         out["frac_of_copy"] = achieved / copy_gbs
     return out
+
+
+def _counter_rows(directory):
+    """(kernel name, counter value) of every dispatch in a rocprofv3 counter-collection CSV."""
+    import csv
+    rows = []
+    for path in Path(directory).rglob("*counter_collection.csv"):
+        with open(path, newline="") as fh:
+            for r in csv.DictReader(fh):
+                name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+                value = r.get("Counter_Value") or r.get("Counter-Value")
+                if value is not None:
+                    rows.append((name, float(value)))
+    return rows
+
+
+def measure_traffic(storages, timeout=300):
+    """HBM bytes per launch of the headline sweep, per storage, from two rocprofv3 counter
+    passes (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch) over `profile_step --workload
+    traffic` in a child process: the streaming copy of 4 GiB (which reads and writes 4 GiB:
+    the calibration) and one sweep of each AA form of the 1536x1024x64 lattice.  Returns
+    {storage: bytes per launch (mean of the two forms), ...} plus the details, or a reason."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if Path("/opt/rocm/bin/rocprofv3").exists() else None)
+    if exe is None:
+        return None, "rocprofv3 not found"
+    copy_bytes = 4 * 2**30
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out_dir = tempfile.mkdtemp(prefix=f"ak_pmc_{counter.lower()}_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "-d", out_dir, "-o", "pmc", "--output-format", "csv", "--", sys.executable,
+               "-m", "aerokinetic_amd.profile_step", "--workload", "traffic", "--storages", ",".join(storages)]
+        env = dict(os.environ, TMPDIR="/tmp", PYTHONPATH=str(ROOT))
+        log(f"memory counters: {counter}")
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, timeout=timeout, capture_output=True, text=True)
+        except subprocess.TimeoutExpired:
+            return None, f"rocprofv3 --pmc {counter} timed out"
+        rows = _counter_rows(out_dir)
+        shutil.rmtree(out_dir, ignore_errors=True)
+        if not rows:
+            return None, f"rocprofv3 --pmc {counter} gave no counters (rc {r.returncode}): {r.stderr[-300:]}"
+        copy = [v for n, v in rows if "k_stream_copy" in n]
+        if not copy:
+            return None, f"no streaming-copy dispatch in the {counter} pass"
+        scale = copy_bytes / (float(np.mean(copy)) * 1024.0)  # bytes per counter KiB
+        per[counter] = {"scale": scale, "copy_kib": float(np.mean(copy)),
+                        "sweeps": {st: [v * 1024.0 * scale for n, v in rows if n.startswith(f"k_sweep<{STORE_ID[st]},")]
+                                   for st in storages}}
+    out = {}
+    for st in storages:
+        fetch, write = per["FETCH_SIZE"]["sweeps"][st], per["WRITE_SIZE"]["sweeps"][st]
+        if len(fetch) != 2 or len(write) != 2:
+            return None, f"expected two {st} sweep dispatches per pass, got {len(fetch)} and {len(write)}"
+        out[st] = {"read_bytes": float(np.mean(fetch)), "written_bytes": float(np.mean(write)),
+                   "bytes_per_launch": float(np.mean(fetch) + np.mean(write)),
+                   "by_form": [float(f + w) for f, w in zip(fetch, write)]}
+    out["calibration"] = {"copy_bytes_each_way": copy_bytes,
+                          "fetch_size_kib_of_copy": per["FETCH_SIZE"]["copy_kib"],
+                          "write_size_kib_of_copy": per["WRITE_SIZE"]["copy_kib"],
+                          "fetch_factor": per["FETCH_SIZE"]["scale"], "write_factor": per["WRITE_SIZE"]["scale"]}
+    return out, None
 
 
 def refined_fields(s):
@@ -128,9 +178,8 @@ def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
            "event_ms_per_step": tot / profile_steps, "coarse_sweep_ms": coarse_sweep * 1e3,
            "fine_sweep_ms": fine_sweep * 1e3,
            "fine_sweep_cells": fine_cells,
-           "roofline": roofline(fine_cells * 2 * 27 * b, fine_sweep, ("two-level", storage, "fine k_sweep"), copy_gbs),
-           "coarse_sweep_roofline": roofline(coarse * 2 * 27 * b, coarse_sweep, ("two-level", storage, "coarse k_sweep"),
-                                             copy_gbs),
+           "roofline": roofline(fine_cells * 2 * 27 * b, fine_sweep, copy_gbs),
+           "coarse_sweep_roofline": roofline(coarse * 2 * 27 * b, coarse_sweep, copy_gbs),
            "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
            "fields": fields,
            "diag": s.diagnostics()}
@@ -160,7 +209,7 @@ def uniform_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
     sweep = sweep_ms / profile_steps / 1e3
     out = {"storage": storage, "mlups": cells * steps / dt / 1e6, "ms_per_step": dt / steps * 1e3,
            "sweep_ms": sweep * 1e3, "boundary_ms": bnd_ms / profile_steps,
-           "roofline": roofline(fluid * 2 * 27 * BYTES[storage], sweep, ("uniform-100m", storage, "k_sweep"), copy_gbs),
+           "roofline": roofline(fluid * 2 * 27 * BYTES[storage], sweep, copy_gbs), "fluid_cells": fluid,
            "cells": cells, "steps_compared": warmup + steps,
            "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
            "fields": (rho, u, s._fluid)}
@@ -201,7 +250,7 @@ def exterior_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
            "sweep_ms": sweep * 1e3, "boundary_ms": bnd_ms / profile_steps,
            "ring_and_bookkeeping_ms": (tot_ms - sweep_ms - bnd_ms) / profile_steps,
            "step_ms_direct": tot_ms / profile_steps,
-           "sweep_roofline": roofline(swept * 2 * 27 * BYTES[storage], sweep, None, copy_gbs),
+           "sweep_roofline": roofline(swept * 2 * 27 * BYTES[storage], sweep, copy_gbs),
            "last_sample": {k: recs[-1][k] for k in ("step", "cl", "cd", "cm", "exterior_source_mass_lattice",
                                                      "wall_mass_exchange_lattice")}}
     s.lattice.close()
@@ -233,23 +282,40 @@ def compare_loads(results, ref="fp64"):
     return out
 
 
-def cpu_uniform_oracle(seconds, threads):
-    """The uniform 384x256x8 oracle step (loads every step) on `threads` host cores."""
+def host_cpu():
+    """CPU model, usable cores and the OpenMP placement of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo", encoding="utf-8"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"cpu_model": model, "usable_cores": usable, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
+def cpu_uniform_oracle(case, seconds, threads, warmup=2, min_steps=4):
+    """The FP64 oracle's uniform step (loads every step) of `case` on `threads` host cores:
+    `warmup` untimed steps, then steps until `seconds` have passed (at least `min_steps`)."""
     from oracle import oracle as O
-    from aerokinetic_amd import cases
-    section, flow, grid, plan = cases.NACA0012_C32_384x256x8.build()
+    section, flow, grid, plan = case.build()
     O.Oracle.set_threads(threads)
     o = O.from_case(section, flow, grid, plan)
     cells = plan.shape[0] * plan.shape[1] * plan.shape[2]
-    for _ in range(2):
+    for _ in range(warmup):
         o.step(sample=True)
     n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
+    while n < min_steps or time.perf_counter() - t0 < seconds:
         o.step(sample=True)
         n += 1
     dt = time.perf_counter() - t0
+    del o
     return {"value": cells * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"uniform 384x256x8 FP64 oracle, {n} steps with loads ({dt:.1f} s)"}
+            "sample": f"{case.name}: FP64 oracle, {warmup} warm-up + {n} timed steps with loads every step "
+                      f"({dt:.1f} s)"}
 
 
 def cpu_baseline_refined(case, seconds, threads):
@@ -273,24 +339,28 @@ def cpu_baseline_refined(case, seconds, threads):
 
 
 def serving_gpu(args):
-    """Continuous batching (FP16 storage) vs one-at-a-time FP64 solves of the same requests."""
+    """Continuous batching (FP16 storage) vs one-at-a-time FP64 solves of the same requests:
+    the serving workload (main element + 25 %-chord flap at 0-30 deg, lattices of 2M-200M
+    cells log-uniform, Re 1e4-1e5, Poisson arrivals), a reduced number of requests."""
     from aerokinetic_amd import serving as S
-    reqs = S.polar_requests(args.serve_cases, seed=2026, rate=args.serve_rate, chord=(16, 24), span=(8, 8),
-                            reynolds=(1e4, 3e4))
+    reqs = S.serving_requests(args.serve_cases, seed=2026, rate=args.serve_rate, cells=(2e6, 2e8), flap=(0.0, 30.0))
     rule = dict(chunk=args.serve_chunk, window=args.serve_window, tol=0.005, max_steps=args.serve_max_steps)
-    out = {"workload": (f"{len(reqs)} two-level NACA 4-digit requests (seed 2026: camber 0-4 %, thickness 9-15 %, "
-                        f"alpha -4..12 deg, Re 1e4-3e4 log-uniform, coarse chord 16-24 cells, 12c x 8c x 8 cells, "
-                        f"fine patch 2:1), Poisson arrivals {args.serve_rate}/s; retire when the "
-                        f"{args.serve_window}-step Cl window varies < 0.5 % or at {args.serve_max_steps} steps"),
-           "rule": rule}
+    cells = [S.case_cells(r) for r in reqs]
+    out = {"workload": (f"{len(reqs)} two-level requests (seed 2026): NACA 4-digit main element (camber 0-4 %, "
+                        f"thickness 9-18 %) + 25 %-chord flap at 0-30 deg, gap/overlap 1-3 % chord, alpha -4..12 deg, "
+                        f"Re 1e4-1e5 log-uniform, 12c x 8c domains, lattice sizes log-uniform in 2M-200M cells "
+                        f"(drawn: {min(cells) / 1e6:.1f}M-{max(cells) / 1e6:.1f}M, total {sum(cells) / 1e6:.0f}M), "
+                        f"Poisson arrivals {args.serve_rate}/s; retire when the {args.serve_window}-step Cl window "
+                        f"varies < 0.5 % or at {args.serve_max_steps} coarse steps"),
+           "rule": rule, "cells": cells}
     for st in [s for s in args.serve_storages.split(",") if s]:
         log(f"serving batched {st}")
-        rep = S.Server(storage=st, **rule).serve(reqs)
+        rep = S.Server(storage=st, **rule).serve(reqs, log=log)
         out[f"batched_{st}"] = rep.summary()
         log(f"  {out[f'batched_{st}']['cases_per_hour']:.0f} cases/h, p50 {out[f'batched_{st}']['latency_p50_s']:.2f} s, "
             f"p99 {out[f'batched_{st}']['latency_p99_s']:.2f} s, peak members {rep.peak_members}")
     log("serving sequential fp64")
-    seq = S.serve_sequential(reqs, storage="fp64", **rule)
+    seq = S.serve_sequential(reqs, storage="fp64", log=log, **rule)
     out["sequential_fp64"] = seq.summary()
     log(f"  {out['sequential_fp64']['cases_per_hour']:.0f} cases/h, p50 {out['sequential_fp64']['latency_p50_s']:.2f} s")
     for k in list(out):
@@ -301,23 +371,27 @@ def serving_gpu(args):
 
 def main():
     faulthandler.enable()
+    # the oracle's OpenMP threads stay on their cores (read when the runtime starts)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--storages", default="fp64,fp32")
-    ap.add_argument("--profile-steps", type=int, default=40)
-    ap.add_argument("--uniform-storages", default="fp64,fp32,fp16", help="empty to skip the 100M-cell sweep")
-    ap.add_argument("--uniform-steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200, help="timed steps of the headline lattice (each storage)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--storages", default="fp64,fp32,fp16", help="headline storages; the first is `value`")
+    ap.add_argument("--profile-steps", type=int, default=20, help="event-timed steps for the roofline")
+    ap.add_argument("--traffic", type=int, default=1, help="0 skips the memory-counter passes")
+    ap.add_argument("--two-level-storages", default="fp64,fp32", help="empty to skip the two-level case")
+    ap.add_argument("--two-level-steps", type=int, default=2000)
     ap.add_argument("--exterior-storages", default="fp64,fp16", help="empty to skip the exterior-farfield case")
     ap.add_argument("--exterior-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--serve-cases", type=int, default=48, help="0 skips the serving measurement")
-    ap.add_argument("--serve-rate", type=float, default=64.0)
-    ap.add_argument("--serve-storages", default="fp16,fp32")
-    ap.add_argument("--serve-chunk", type=int, default=200)
-    ap.add_argument("--serve-window", type=int, default=400)
-    ap.add_argument("--serve-max-steps", type=int, default=2000)
+    ap.add_argument("--serve-cases", type=int, default=6, help="0 skips the serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-storages", default="fp16")
+    ap.add_argument("--serve-chunk", type=int, default=250)
+    ap.add_argument("--serve-window", type=int, default=2000)
+    ap.add_argument("--serve-max-steps", type=int, default=3000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: a case runs on a single MI355X here; "
@@ -327,23 +401,25 @@ def main():
     from aerokinetic_amd.profile_step import copy_bandwidth
     copy_ms, copy_gbs = copy_bandwidth(4.0, 10)
     log(f"streaming copy 4 GiB: {copy_ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
-    case = cases.NACA0012_REFINED_256x192x8
+    # headline: the 1536x1024x64 lattice in every storage
+    ucase = cases.NACA0012_C128_1536x1024x64
     storages = [s for s in args.storages.split(",") if s]
-    results = {}
+    uni = {}
     for st in storages:
+        log(f"{ucase.name} {st}")
+        uni[st] = uniform_gpu(ucase, st, args.steps, args.warmup, args.profile_steps, copy_gbs)
+        r = uni[st]
+        log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, sweep {r['sweep_ms']:.3f} ms "
+            f"({100 * r['roofline']['frac']:.1f}% HBM), boundary {r['boundary_ms']:.3f} ms")
+    uni_cmp = compare_loads(uni)
+    two = {}
+    case2 = cases.NACA0012_REFINED_256x192x8
+    for st in [s for s in args.two_level_storages.split(",") if s]:
         log(f"two-level {st}")
-        results[st] = refined_gpu(case, st, args.steps, args.warmup, args.profile_steps, copy_gbs)
-        r = results[st]
+        two[st] = refined_gpu(case2, st, args.two_level_steps, 500, 40, copy_gbs)
+        r = two[st]
         log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, fine sweep {r['fine_sweep_ms']:.4f} ms "
             f"({100 * r['roofline']['frac']:.1f}% HBM), coarse sweep {r['coarse_sweep_ms']:.4f} ms")
-    uni = {}
-    ucase = cases.NACA0012_C128_1536x1024x64
-    for st in [s for s in args.uniform_storages.split(",") if s]:
-        log(f"uniform 100M {st}")
-        uni[st] = uniform_gpu(ucase, st, args.uniform_steps, 10, 20, copy_gbs)
-        r = uni[st]
-        log(f"  {r['mlups']:.0f} MLUPS, sweep {r['sweep_ms']:.3f} ms ({100 * r['roofline']['frac']:.1f}% HBM)")
-    uni_cmp = compare_loads(uni)
     ext = {}
     for st in [s for s in args.exterior_storages.split(",") if s]:
         log(f"exterior farfield 384x256x8 {st}")
@@ -352,29 +428,50 @@ def main():
         log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.4f} ms/step; sweep {r['sweep_ms']:.4f} ms, boundary "
             f"{r['boundary_ms']:.4f} ms, ring + bookkeeping {r['ring_and_bookkeeping_ms']:.4f} ms")
     serving = serving_gpu(args) if args.serve_cases > 0 else None
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    import torch
+    torch.cuda.empty_cache()
+    traffic, why = (measure_traffic(storages) if args.traffic else (None, "skipped (--traffic 0)"))
+    if traffic:
+        for st in storages:
+            uni[st]["roofline"]["traffic"] = traffic[st]["bytes_per_launch"]
+            uni[st]["roofline"]["traffic_detail"] = {k: v for k, v in traffic[st].items() if k != "bytes_per_launch"}
+            uni[st]["roofline"]["traffic_calibration"] = traffic["calibration"]
+    else:
+        log(f"memory counters not measured: {why}")
+        for st in storages:
+            uni[st]["roofline"]["traffic_note"] = why
     cpu = None
     if args.cpu_seconds > 0:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        cores = min(omp or 10**9, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
         log(f"cpu baseline on {cores} cores")
-        cpu = cpu_baseline_refined(case, args.cpu_seconds, cores)
-        cpu["uniform_384x256x8"] = {"all_cores": cpu_uniform_oracle(args.cpu_seconds / 3, cores),
-                                    "one_core": cpu_uniform_oracle(args.cpu_seconds / 3, 1)}
-    head = results[storages[0]]
+        slice_case = ucase.with_shape((1536, 1024, 8))
+        cpu = cpu_uniform_oracle(slice_case, args.cpu_seconds, cores)
+        cpu["sample"] += " -- a 1536x1024x8 slice of the headline 1536x1024x64 lattice (per-cell work is the same)"
+        cpu["host"] = host_cpu()
+        c1 = cases.NACA0012_C32_384x256x8
+        cpu["uniform_384x256x8"] = {"all_cores": cpu_uniform_oracle(c1, args.cpu_seconds / 3, cores),
+                                    "one_core": cpu_uniform_oracle(c1, args.cpu_seconds / 3, 1)}
+        if two:
+            cpu["two_level"] = cpu_baseline_refined(case2, args.cpu_seconds / 2, cores)
+    head = uni[storages[0]]
     strip = lambda r: {k: v for k, v in r.items() if k not in ("loads", "diag", "fields")}
     line = {"metric": METRIC, "value": head["mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64" if storages[0] == "fp64" else "f32", "data": "synthetic",
-            "config": {"workload": case.name + ", legacy coarse farfield, fine Bouzidi walls, loads every coarse step",
-                       "storage": storages[0], "updates_per_step": head["updates_per_step"],
-                       "parallelism": "single GPU", "step_graphs": True},
+            "dtype": {"fp64": "f64", "fp32": "f32", "fp16": "f16-store/f32"}[storages[0]], "data": "synthetic",
+            "config": {"workload": ucase.name + " (1536x1024x64 = 100,663,296 cells), characteristic farfield, "
+                                               "loads every step",
+                       "storage": storages[0], "cells": head["cells"], "parallelism": "single GPU",
+                       "step_graphs": True, "sweep_tile": "64x4 (row pitch aligned, XCD-remapped blocks)"},
             "roofline": head["roofline"], "cpu_baseline": cpu,
             "copy": {"bytes": 4 * 2**30, "ms": copy_ms, "gbs_read_write": copy_gbs},
-            "by_storage": {st: strip(r) for st, r in results.items()},
-            "loads_vs_fp64": compare_loads(results),
-            "uniform_sweep": {"workload": ucase.name + ", characteristic farfield, loads every step",
-                              "by_storage": {st: strip(r) for st, r in uni.items()},
-                              "vs_fp64": uni_cmp},
+            "by_storage": {st: strip(r) for st, r in uni.items()},
+            "vs_fp64": uni_cmp,
+            "two_level": {"workload": case2.name + ", legacy coarse farfield, fine Bouzidi walls, loads every coarse "
+                                                   "step; MLUPS = (coarse + 2 x fine cells) per coarse step",
+                          "by_storage": {st: strip(r) for st, r in two.items()},
+                          "loads_vs_fp64": compare_loads(two)},
             "exterior_farfield": {"workload": cases.NACA0012_C32_384x256x8_EXTERIOR.name + " (exterior_regularized_fd_v1; "
                                               "MLUPS over the physical cells)",
                                   "by_storage": {st: strip(r) for st, r in ext.items()},

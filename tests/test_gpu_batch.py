@@ -113,3 +113,52 @@ def test_service_admits_steps_and_retires_a_request_stream():
     assert s["peak_members"] >= 2
     for r in rep.results:
         assert r.latency > 0 and 400 <= r.steps <= 800 and np.isfinite(r.cl)
+
+
+def test_polar_sweep_sized_members_identical_alone_and_in_a_mix():
+    """Batched polar-sweep sizes (FP32 storage): three members drawn from the sweep's
+    distribution -- NACA 4-digit camber 0-4 %, thickness 9-15 %, chord 32-96 coarse cells,
+    span 8-32 cells, Re 1e4-1e5 -- one of them at the largest chord; each member's loads and
+    populations after 6 coarse steps in the mix are bit-identical to the member stepped alone."""
+    from aerokinetic_amd.batch import RefinedBatch
+    reqs = S.polar_requests(3, seed=2026, chord=(32, 96), span=(8, 32))
+    reqs[2] = S.CaseRequest(2, 0.0, 0.03, 0.4, 0.12, 6.0, 5e4, 96, 32)  # the largest case of the sweep
+    print("requests", [(r.chord_cells, r.span_cells) for r in reqs],
+          "cells", [S.case_cells(r) for r in reqs], flush=True)
+    assert all(32 <= r.chord_cells <= 96 and 8 <= r.span_cells <= 32 for r in reqs)
+    mixed = [_solver(r, "fp32") for r in reqs]
+    mix = RefinedBatch("fp32")
+    for m in mixed:
+        mix.add(m)
+    got, failed = mix.advance(6)
+    assert not failed
+    mix.close()
+    for req, m in zip(reqs, mixed):
+        alone_m = _solver(req, "fp32")
+        alone = RefinedBatch("fp32")
+        alone.add(alone_m)
+        ref, _ = alone.advance(6)
+        np.testing.assert_array_equal(_loads(ref[alone_m]), _loads(got[m]))
+        for level in (0, 1):
+            np.testing.assert_array_equal(alone_m.populations(level), m.populations(level))
+        alone.close()
+        alone_m.close()
+        m.close()
+
+
+def test_service_serves_a_flapped_request_of_millions_of_cells():
+    """A serving-workload request (main element + 25 %-chord flap, FP16 storage, a lattice of
+    more than 2M cells) through the service alongside a small one: both retire by the rule."""
+    reqs = S.serving_requests(1, seed=11, rate=None, cells=(2.2e6, 2.6e6), flap=(20.0, 30.0))
+    reqs += S.polar_requests(1, seed=3, rate=None, chord=(16, 16), span=(8, 8))
+    reqs[1] = S.CaseRequest(1, 0.0, reqs[1].camber, 0.4, reqs[1].thickness, reqs[1].alpha_deg, reqs[1].reynolds, 16, 8)
+    big = reqs[0]
+    print("flap request", big, "cells", S.case_cells(big), flush=True)
+    assert S.case_cells(big) >= 2_000_000 and big.flap_deflection_deg >= 20.0
+    srv = S.Server(storage="fp16", chunk=100, window=400, tol=0.005, max_steps=600, builders=2)
+    rep = srv.serve(reqs)
+    s = rep.summary()
+    assert s["cases"] == 2 and s["reasons"]["failed"] == 0
+    by_id = {r.id: r for r in rep.results}
+    assert by_id[0].cells >= 2_000_000 and 400 <= by_id[0].steps <= 600
+    assert np.isfinite([by_id[0].cl, by_id[0].cd, by_id[0].cm]).all()
