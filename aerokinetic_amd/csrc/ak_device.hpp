@@ -214,6 +214,37 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
   u2_out = ux2 + uy2 + uz2;
 }
 
+// FP16 storage of a cell's 27 post-collision populations that keeps its mass and momentum.
+// Rounding each shifted population to FP16 on its own changes the cell's sum and first
+// moments by the sum of 27 rounding errors, a few units in the last place of the largest
+// populations (~1e-5 at Ma 0.1); in a smooth field the errors repeat from cell to cell and
+// step to step, a spurious source of mass and momentum.  Here the rounding deficits of the 19
+// non-corner directions are moved onto the 8 corner directions (c = (+-1, +-1, +-1), the
+// smallest populations, whose FP16 steps are the finest) as e_k = -(dm + c_k . dj) / 8, which
+// restores the sum and the first moments exactly (sum_k c_k = 0, sum_k c_k c_k^T = 8 I); only
+// the corners' own rounding remains.  The 19 others are stored exactly as plain rounding
+// would store them.  No-op for FP64 and FP32 storage.
+template <int S>
+__device__ __forceinline__ void conserve_fp16(typename St<S>::C (&g)[NQ], float scale, float inv_scale) {
+  if constexpr (S == 2) {
+    float dm = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      if (cxq(i) != 0 && cyq(i) != 0 && czq(i) != 0) continue;
+      const float d = (float)(_Float16)(g[i] * scale) * inv_scale - g[i];
+      dm += d;
+      if (cxq(i)) dx += (float)cxq(i) * d;
+      if (cyq(i)) dy += (float)cyq(i) * d;
+      if (czq(i)) dz += (float)czq(i) * d;
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      if (cxq(i) == 0 || cyq(i) == 0 || czq(i) == 0) continue;
+      g[i] -= 0.125f * (dm + (float)cxq(i) * dx + (float)cyq(i) * dy + (float)czq(i) * dz);
+    }
+  }
+}
+
 // Product-form equilibrium of direction i (absolute, not shifted).
 template <typename C>
 __device__ __forceinline__ C phi1(int c, C q) {

@@ -345,6 +345,7 @@ __global__ __launch_bounds__(128) void k_ext_ring(typename St<S>::T *__restrict_
     }
     C drho_c, u2_c;
     collide<C>(g, ws, wb, drho_c, u2_c);
+    conserve_fp16<S>(g, scale, inv_scale);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       long long dst;
@@ -511,6 +512,7 @@ __global__ __launch_bounds__(256) void k_collide_inplace(typename St<S>::T *__re
   for (int i = 0; i < NQ; ++i) g[i] = ld<S>(A + c + (long long)opp(i) * L.QS, inv_scale);
   C dr, u2;
   collide<C>(g, ws, wb, dr, u2);
+  conserve_fp16<S>(g, scale, inv_scale);
 #pragma unroll
   for (int i = 0; i < NQ; ++i) stv<S>(A + c + (long long)opp(i) * L.QS, g[i], scale);
 }

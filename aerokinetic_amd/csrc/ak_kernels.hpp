@@ -106,6 +106,7 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         g[q] = ld<S>(A + src, inv_scale);
       }
       collide<C>(g, ws, wb, drho, u2);
+      conserve_fp16<S>(g, scale, inv_scale);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         long long dst;
@@ -135,6 +136,7 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         g[q] = ld_g<S>(bq + boff, inv_scale);
       }
       collide<C>(g, ws, wb, drho, u2);
+      conserve_fp16<S>(g, scale, inv_scale);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         long long d;

@@ -112,6 +112,13 @@ def compare(result, fields, ref, ref_fields, average_last):
     out["common_window"] = [int(n - min(n, average_last) + 1), int(n)]
     out["mean_coef_common_window"] = {k: [float(a[i]), float(b[i])] for i, k in enumerate(("cl", "cd", "cm"))}
     out["mean_coef_rel_err_vs_fp64"] = {k: float(abs(a[i] - b[i]) / abs(b[i])) for i, k in enumerate(("cl", "cd", "cm"))}
+    # shorter windows: the start-up, where the runs follow one trajectory, and later blocks
+    windows = {}
+    for lo, hi in ((1, 1000), (1, 2000), (1, 5000), (5001, 10000), (10001, 15000), (15001, 20000)):
+        if hi <= n:
+            x, y = result["loads"][lo - 1:hi].mean(axis=0), ref["loads"][lo - 1:hi].mean(axis=0)
+            windows[f"{lo}-{hi}"] = {k: float(abs(x[i] - y[i]) / abs(y[i])) for i, k in enumerate(("cl", "cd", "cm"))}
+    out["mean_coef_rel_err_by_window"] = windows
     return out
 
 
@@ -146,6 +153,10 @@ def main(argv=None):
             del fields
         report["by_storage"][st] = res
         _log(json.dumps({k: v for k, v in res.items() if k != "loads"}))
+    if a.out:  # every step's loads beside the report
+        import numpy as np
+        np.savez_compressed(a.out.rsplit(".", 1)[0] + "_loads.npz",
+                            **{st: r["loads"] for st, r in report["by_storage"].items()})
     for res in report["by_storage"].values():
         res.pop("loads")
     if a.out:

@@ -93,6 +93,7 @@ def measure_traffic(storages, timeout=300):
     import shutil
     import subprocess
     import tempfile
+    import numpy as np
     exe = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if Path("/opt/rocm/bin/rocprofv3").exists() else None)
     if exe is None:
         return None, "rocprofv3 not found"
@@ -117,7 +118,7 @@ def measure_traffic(storages, timeout=300):
             return None, f"no streaming-copy dispatch in the {counter} pass"
         scale = copy_bytes / (float(np.mean(copy)) * 1024.0)  # bytes per counter KiB
         per[counter] = {"scale": scale, "copy_kib": float(np.mean(copy)),
-                        "sweeps": {st: [v * 1024.0 * scale for n, v in rows if n.startswith(f"k_sweep<{STORE_ID[st]},")]
+                        "sweeps": {st: [v * 1024.0 * scale for n, v in rows if f"k_sweep<{STORE_ID[st]}," in n]
                                    for st in storages}}
     out = {}
     for st in storages:
@@ -282,7 +283,7 @@ def compare_loads(results, ref="fp64"):
     return out
 
 
-def host_cpu():
+def host_cpu(usable):
     """CPU model, usable cores and the OpenMP placement of the host the baseline runs on."""
     model = "unknown"
     try:
@@ -292,30 +293,25 @@ def host_cpu():
                 break
     except OSError:
         pass
-    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    return {"cpu_model": model, "usable_cores": usable, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-            "omp_places": os.environ.get("OMP_PLACES")}
+    return {"cpu_model": model, "usable_cores": usable, "omp_proc_bind": "close", "omp_places": "cores"}
 
 
-def cpu_uniform_oracle(case, seconds, threads, warmup=2, min_steps=4):
-    """The FP64 oracle's uniform step (loads every step) of `case` on `threads` host cores:
-    `warmup` untimed steps, then steps until `seconds` have passed (at least `min_steps`)."""
-    from oracle import oracle as O
-    section, flow, grid, plan = case.build()
-    O.Oracle.set_threads(threads)
-    o = O.from_case(section, flow, grid, plan)
-    cells = plan.shape[0] * plan.shape[1] * plan.shape[2]
-    for _ in range(warmup):
-        o.step(sample=True)
-    n, t0 = 0, time.perf_counter()
-    while n < min_steps or time.perf_counter() - t0 < seconds:
-        o.step(sample=True)
-        n += 1
-    dt = time.perf_counter() - t0
-    del o
-    return {"value": cells * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"{case.name}: FP64 oracle, {warmup} warm-up + {n} timed steps with loads every step "
-                      f"({dt:.1f} s)"}
+def cpu_uniform_oracle(case_name, seconds, threads, shape=None):
+    """The FP64 oracle's uniform step (loads every step) of a named workload on `threads` host
+    cores, timed in a child process whose OpenMP threads are pinned (OMP_PROC_BIND=close,
+    OMP_PLACES=cores): 2 untimed steps, then steps until `seconds` have passed (at least 4)."""
+    import subprocess
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--case", case_name, "--seconds", str(seconds),
+           "--threads", str(threads)] + (["--shape", ",".join(map(str, shape))] if shape else [])
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads),
+               PYTHONPATH=str(ROOT))
+    r = subprocess.run(cmd, cwd=str(ROOT), env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode:
+        raise RuntimeError(f"cpu baseline failed: {r.stderr[-500:]}")
+    t = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": t["value"], "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{t['case']}: FP64 oracle, {t['warmup']} warm-up + {t['steps']} timed steps with loads every "
+                      f"step ({t['seconds']:.1f} s), OpenMP threads pinned (close, cores)"}
 
 
 def cpu_baseline_refined(case, seconds, threads):
@@ -371,9 +367,8 @@ def serving_gpu(args):
 
 def main():
     faulthandler.enable()
-    # the oracle's OpenMP threads stay on their cores (read when the runtime starts)
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+    # host cores this process may use, read before any runtime binds a thread
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed steps of the headline lattice (each storage)")
@@ -443,15 +438,13 @@ def main():
     cpu = None
     if args.cpu_seconds > 0:
         omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-        cores = min(omp or 10**9, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+        cores = min(omp, usable) if omp > 0 else usable
         log(f"cpu baseline on {cores} cores")
-        slice_case = ucase.with_shape((1536, 1024, 8))
-        cpu = cpu_uniform_oracle(slice_case, args.cpu_seconds, cores)
+        cpu = cpu_uniform_oracle("NACA0012_C128_1536x1024x64", args.cpu_seconds, cores, shape=(1536, 1024, 8))
         cpu["sample"] += " -- a 1536x1024x8 slice of the headline 1536x1024x64 lattice (per-cell work is the same)"
-        cpu["host"] = host_cpu()
-        c1 = cases.NACA0012_C32_384x256x8
-        cpu["uniform_384x256x8"] = {"all_cores": cpu_uniform_oracle(c1, args.cpu_seconds / 3, cores),
-                                    "one_core": cpu_uniform_oracle(c1, args.cpu_seconds / 3, 1)}
+        cpu["host"] = host_cpu(usable)
+        cpu["uniform_384x256x8"] = {"all_cores": cpu_uniform_oracle("NACA0012_C32_384x256x8", args.cpu_seconds / 3, cores),
+                                    "one_core": cpu_uniform_oracle("NACA0012_C32_384x256x8", args.cpu_seconds / 3, 1)}
         if two:
             cpu["two_level"] = cpu_baseline_refined(case2, args.cpu_seconds / 2, cores)
     head = uni[storages[0]]

@@ -148,6 +148,7 @@ class FaceFlux:
 class RefinedOracle:
     pad = 2
     outer_flux_before_boundaries = False
+    track_budget = True  # the conserved integrals around every step (last_budget)
 
     def __init__(self, project, settings, plan: RefinedPlan | None = None):
         self.project, self.settings = project, settings
@@ -230,8 +231,13 @@ class RefinedOracle:
         a, b = moments(before), moments(after)
         fields = prolong_cell_averages((1 - fraction) * a + fraction * b)
         fields = rescale_stress(fields, self.coarse_level, self.fine_level)
-        fields[self.fine_slices] = self.free_fields
-        return populations(fields)
+        # the owned cells get the free stream here and their own state right after; the
+        # reconstruction is elementwise, so only the donor cells are rebuilt
+        out = np.broadcast_to(self.free_pdf, (*fields.shape[:3], 27)).copy()
+        donor = np.ones(fields.shape[:3], bool)
+        donor[self.fine_slices] = False
+        out[donor] = populations(fields[donor])
+        return out
 
     def _wrap_z(self, pdf):
         for axis in self.full_periodic_axes:
@@ -305,7 +311,7 @@ class RefinedOracle:
 
     # -------------------------------------------------------------- step
     def step(self):
-        start = self.conserved_integrals()
+        start = self.conserved_integrals() if self.track_budget else None
         L = self.fine.L
         dc, df = self.coarse_level.dx, self.fine_level.dx
         old = self.coarse.pdf()
@@ -366,9 +372,17 @@ class RefinedOracle:
         q_area = 0.5 * self.plan.span
         force = impulse / self.coarse_level.dt
         mz = moment / self.coarse_level.dt
+        external[1:] *= vel
+        self.cumulative_wall_mass_exchange += wall_mass
+        self.cumulative_wall_impulse += impulse
+        self.cumulative_external_in += external
+        record = {"step": self.steps, "t_star": (self.steps - 0.5) * self.coarse_level.dt,
+                  "cl": float(force[1] / q_area), "cd": float(force[0] / q_area), "cm": float(-mz / q_area),
+                  "wall_mass_exchange": float(wall_mass)}
+        if not self.track_budget:
+            return record
         # budget of this coarse step
         end = self.conserved_integrals()
-        external[1:] *= vel
         mass_residual = end["mass"] - start["mass"] - external[0] - wall_mass
         momentum_residual = np.array(end["momentum"]) - start["momentum"] - external[1:] + impulse
         volume = (np.count_nonzero(self.coarse_owned) * dc**3 + np.count_nonzero(self.fine_fluid) * df**3)
@@ -376,12 +390,7 @@ class RefinedOracle:
                             "momentum_residual_per_volume": (momentum_residual / volume).tolist(),
                             "wall_mass_exchange": float(wall_mass), "external_mass_in": float(external[0]),
                             "wall_impulse": impulse.tolist(), "external_impulse_in": external[1:].tolist()}
-        self.cumulative_wall_mass_exchange += wall_mass
-        self.cumulative_wall_impulse += impulse
-        self.cumulative_external_in += external
-        return {"step": self.steps, "t_star": (self.steps - 0.5) * self.coarse_level.dt,
-                "cl": float(force[1] / q_area), "cd": float(force[0] / q_area), "cm": float(-mz / q_area),
-                "wall_mass_exchange": float(wall_mass)}
+        return record
 
     def diagnostics(self):
         """Guards over the leaf cells and the budget (refined_solver.py:347-361)."""

@@ -79,8 +79,10 @@ def test_precision_study_lattice_three_storages():
     characteristic farfield) at full size in FP64, FP32 and FP16 shifted storage for 40 steps
     with loads every step: the state stays finite and inside the low-Mach guards, the FP64
     density sum over the fluid cells (the global mass check) changes by less than 2e-6 of the
-    mass (measured 4.7e-7 in FP64: the characteristic farfield is open, and the impulsive start
-    pushes a little mass through it), and the mean Cl and Cd of steps 21-40 of
+    mass in FP64 and FP32 storage (measured 4.7e-7: the characteristic farfield is open, and the
+    impulsive start pushes a little mass through it) and by less than 3e-5 in FP16 storage
+    (measured 1.0e-5: each cell's stored populations carry the FP16 rounding, a few units in
+    the last place of the largest shifted population, ~7e-6 at Ma 0.1), and the mean Cl and Cd of steps 21-40 of
     FP32 and FP16 storage are within 1 % of FP64 storage."""
     import torch
     from aerokinetic_amd import cases
@@ -102,7 +104,7 @@ def test_precision_study_lattice_three_storages():
         m1 = d["density_mean"] * nfluid
         print(storage, "mass", m0, m1, "relative change", (m1 - m0) / m0, "guards", d["local_mach_max"],
               d["density_error_max"], flush=True)
-        assert abs(m1 - m0) / m0 < 2e-6
+        assert abs(m1 - m0) / m0 < (3e-5 if storage == "fp16" else 2e-6)
         means[storage] = np.mean([[r["cl"], r["cd"]] for r in recs[20:]], axis=0)
         s.lattice.close()
         del s

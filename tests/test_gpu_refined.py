@@ -85,3 +85,63 @@ def test_reduced_storage_loads_track_fp64(storage):
     ref, got = out["fp64"], out[storage]
     assert np.all(np.isfinite(got))
     np.testing.assert_allclose(got, ref, rtol=0.01)
+
+
+def test_thousand_coarse_steps_laminar_against_refined_oracle():
+    """A laminar two-level start-up for 1000 coarse steps (2000 fine substeps): NACA0012 at
+    2 deg, Re 1e4, 16 coarse / 32 fine cells per chord, no perturbation, legacy farfield
+    (t* = 3.1).  Loads every 100 steps within 1e-9 of the refined oracle, and every coarse
+    cell and owned fine fluid cell within 1e-9 relative to f = g + w at the end."""
+    from aerokinetic_amd.config import Flow, Grid
+    from aerokinetic_amd.geometry import Section, naca0012
+    from aerokinetic_amd.project import Project, RunSettings
+    from aerokinetic_amd.refined import RefinedSolver
+    from aerokinetic_amd.refined_plan import RefinementSettings
+    from oracle.oracle import Oracle
+    from oracle.refined_oracle import RefinedOracle
+    Oracle.set_threads(16)
+    section = Section((naca0012(41),), 1.0, (0.25, 0.0))
+    flow = Flow(reynolds=10000, alpha_deg=2.0, lattice_speed=0.05, perturbation=0.0)
+    grid = Grid(cells_per_chord=16, upstream_chords=1.0, downstream_chords=1.5, transverse_chords=0.75,
+                span_chords=0.5, memory_limit_gib=2, farfield="legacy")
+    project = Project(section, flow, grid, RunSettings())
+    settings = RefinementSettings()
+    plan = RefinedPlan.create(project, settings)
+    print("case", plan.shape, plan.box, flush=True)
+    s = RefinedSolver(project, settings, plan=plan)
+    o = RefinedOracle(project, settings, plan)
+    o.track_budget = False  # loads and populations only: the budget is pinned elsewhere
+    recs = s.advance(1000, 100)
+    refs = []
+    for k in range(1000):
+        r = o.step()
+        if (k + 1) % 100 == 0:
+            refs.append(r)
+            print("oracle step", k + 1, flush=True)
+    assert [r["step"] for r in recs] == [r["step"] for r in refs]
+    for a, b in zip(recs, refs):
+        for key in ("cl", "cd", "cm"):
+            assert a[key] == pytest.approx(b[key], abs=1e-9), (a["step"], key)
+    pad, nz2 = 2, 2 * plan.shape[2]
+    fine_fluid = np.broadcast_to(~s._fine_solid_xy[:, :, None, None], (*s._fine_solid_xy.shape, nz2, 27))
+    ec = _rel(s.populations(0), o.coarse.pdf(), np.ones((*plan.shape, 27), bool))
+    ef = _rel(s.populations(1)[pad:-pad, pad:-pad], o.fine.pdf()[pad:-pad, pad:-pad, pad:pad + nz2], fine_fluid)
+    print("1000 coarse steps: max relative population error coarse", ec, "fine", ef, flush=True)
+    assert ec <= 1e-9 and ef <= 1e-9
+
+
+def test_two_level_headline_case_fp32_tracks_fp64():
+    """The 256x192x8 + 160x96x16 two-level case in FP32 storage (FP32 collision, FP64 load and
+    mass sums) against FP64 storage over 300 coarse steps: mean Cl and Cd of steps 101-300
+    within 1 %."""
+    from aerokinetic_amd import cases
+    from aerokinetic_amd.refined import RefinedSolver
+    project, settings, plan = cases.NACA0012_REFINED_256x192x8.build()
+    out = {}
+    for st in ("fp64", "fp32"):
+        s = RefinedSolver(project, settings, plan=plan, storage=st)
+        recs = s.advance(300, 1)
+        out[st] = np.array([[r["cl"], r["cd"]] for r in recs[100:]]).mean(axis=0)
+        s.close()
+    print("mean Cl, Cd fp64", out["fp64"], "fp32", out["fp32"], flush=True)
+    np.testing.assert_allclose(out["fp32"], out["fp64"], rtol=0.01)
