@@ -57,6 +57,7 @@ _SIGS = {
     "ak_set_populations": (ctypes.c_int, [_vp, _dp]),
     "ak_get_macroscopic": (ctypes.c_int, [_vp, _dp, _dp]),
     "ak_diagnostics": (ctypes.c_int, [_vp, _dp]),
+    "ak_diagnostics_noalloc": (ctypes.c_int, [_vp, _dp]),
     "ak_get_plane": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "ak_synchronize": (ctypes.c_int, [_vp]),
     "ak_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),

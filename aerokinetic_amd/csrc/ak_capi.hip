@@ -1301,3 +1301,5 @@ int ak_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
 #include "ak_refined_capi.hpp"
 #include "ak_batch.hpp"
 #include "ak_batch_capi.hpp"
+
+#include "ak_noalloc.hpp"

@@ -142,6 +142,15 @@ int ak_get_macroscopic(void *handle, double *rho, double *u);
  * out = {local Mach max, max |rho-1|, min rho, non-finite flag, sum of rho}. */
 int ak_diagnostics(void *handle, double *out);
 
+/* ak_diagnostics without any allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it).  Same kernels, same reduction order and the same result as
+ * ak_diagnostics; the block partials live in a file-scope device array of 160 KiB per GPU
+ * (4096 blocks x 5 doubles + the result), held until the process ends.  The reduction strides
+ * over the lattice with at most 4096 blocks, so every lattice fits.  Calls that overlap on one
+ * GPU share the array and race. */
+int ak_diagnostics_noalloc(void *handle, double *out);
+
 /* rho[x][y] and u[x][y][3] of one z plane (host, FP64; synchronises). */
 int ak_get_plane(void *handle, int z, double *rho, double *u);
 
