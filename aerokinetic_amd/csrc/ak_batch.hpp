@@ -22,7 +22,8 @@ struct CaseDev {
   Levels V;
   FreeState F;
   const FluxTable *T;
-  double *before, *after, *regc, *regf, *part, *px;
+  double *before, *after, *regc, *regf, *px;  // transfer buffers, read as TR<S> through tb<S>()
+  double *part;                               // wall load partials (FP64)
   const int32_t *lpx, *cols;  // prolongation x-pass items; fine interface-band columns
   int npx, ncols;
   int *err;
@@ -42,6 +43,10 @@ enum BatchPhase {
   BP_DONOR, BP_FLUXC, BP_LEGACY, BP_PX, BP_PYZ, BP_FLUXF, BP_WALLS, BP_REFLUX, BP_RESTRICT,
   BP_COUNT
 };
+
+// A transfer buffer of a member as the kernels of storage S read it.
+template <int S>
+__device__ __forceinline__ TR<S> *tb(double *p) { return (TR<S> *)p; }
 
 // Member owning block b: the last m with first[m] <= b (members with no blocks are skipped).
 __device__ __forceinline__ int batch_owner(const int *__restrict__ first, int nm, int b) {
@@ -76,16 +81,17 @@ __global__ __launch_bounds__(256) void kb_donor(const CaseDev *__restrict__ C, c
                                                 int lead, int delta, int after) {
   using T = typename St<S>::T;
   AK_BATCH_MEMBER
-  double *out = after ? c.after : c.before;
+  TR<S> *out = tb<S>(after ? c.after : c.before);
   if (AK_PE_COARSE(delta)) donor_moments_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, out, c.err, t);
   else donor_moments_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, out, c.err, t);
 }
 
 // x pass of both substeps (after the coarse step's donor moments at t + dt)
+template <int S>
 __global__ __launch_bounds__(256) void kb_prolong_x(const CaseDev *__restrict__ C, const int *__restrict__ first,
                                                     int nm) {
   AK_BATCH_MEMBER
-  if (t < c.npx) prolong_x_body(c.before, c.after, c.B, c.px, c.lpx[t]);
+  if (t < c.npx) prolong_x_body<TR<S>>(tb<S>(c.before), tb<S>(c.after), c.B, tb<S>(c.px), c.lpx[t]);
 }
 
 // y and z passes + injection: one 64-thread block per group of PZ_COLS band columns
@@ -96,7 +102,7 @@ __global__ __launch_bounds__(64) void kb_prolong_yz(const CaseDev *__restrict__ 
   using T = typename St<S>::T;
   AK_BATCH_MEMBER
   (void)t;
-  const double *X = c.px + sub * prolong_x_half(c.B);
+  const TR<S> *X = tb<S>(c.px) + sub * prolong_x_half(c.B);
   if (AK_PE_FINE(sub)) prolong_yz_body<S, true>(X, (T *)c.Af, c.Lf, c.B, c.V, c.f_scale, c.err, c.cols, c.ncols, lb_);
   else prolong_yz_body<S, false>(X, (T *)c.Af, c.Lf, c.B, c.V, c.f_scale, c.err, c.cols, c.ncols, lb_);
 }
@@ -107,9 +113,11 @@ __global__ __launch_bounds__(256) void kb_reflux(const CaseDev *__restrict__ C, 
   using T = typename St<S>::T;
   AK_BATCH_MEMBER
   if (AK_PE_COARSE(delta))
-    reflux_body<S, true>((T *)c.Ac, c.Lc, c.B, c.regc, c.regf, c.inv_volc, c.c_scale, c.c_inv, c.err, t);
+    reflux_body<S, true>((T *)c.Ac, c.Lc, c.B, tb<S>(c.regc), tb<S>(c.regf), TR<S>(c.inv_volc), c.c_scale, c.c_inv,
+                         c.err, t);
   else
-    reflux_body<S, false>((T *)c.Ac, c.Lc, c.B, c.regc, c.regf, c.inv_volc, c.c_scale, c.c_inv, c.err, t);
+    reflux_body<S, false>((T *)c.Ac, c.Lc, c.B, tb<S>(c.regc), tb<S>(c.regf), TR<S>(c.inv_volc), c.c_scale, c.c_inv,
+                          c.err, t);
 }
 
 
@@ -121,20 +129,21 @@ template <int S>
 __global__ __launch_bounds__(256) void kb_restrict(const CaseDev *__restrict__ C, const int *__restrict__ f_rt,
                                                    const int *__restrict__ f_don, int nm, int lead, int nb_rt) {
   using T = typename St<S>::T;
-  __shared__ double sm[RESTRICT_LDS];  // one array for the four parity instances
+  __shared__ double sm_[RESTRICT_LDS];  // one array for the four parity instances
+  TR<S> *sm = (TR<S> *)sm_;
   const int b = (int)blockIdx.x;
   if (b < nb_rt) {
     AK_BATCH_MEMBER_IN(f_rt, b)
     (void)t;
     const bool pf = AK_PE_FINE(2), pc = AK_PE_COARSE(1);
-    if (pf && pc) restrict_tile_body<S, true, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, c.before, c.c_inv);
-    else if (pf) restrict_tile_body<S, true, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, c.before, c.c_inv);
-    else if (pc) restrict_tile_body<S, false, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, c.before, c.c_inv);
-    else restrict_tile_body<S, false, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, c.before, c.c_inv);
+    if (pf && pc) restrict_tile_body<S, true, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, tb<S>(c.before), c.c_inv);
+    else if (pf) restrict_tile_body<S, true, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, tb<S>(c.before), c.c_inv);
+    else if (pc) restrict_tile_body<S, false, true>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, tb<S>(c.before), c.c_inv);
+    else restrict_tile_body<S, false, false>((const T *)c.Af, c.Lf, c.f_inv, (T *)c.Ac, c.Lc, c.B, c.V, c.F, c.c_scale, c.err, lb_, sm, tb<S>(c.before), c.c_inv);
   } else {
     AK_BATCH_MEMBER_IN(f_don, b - nb_rt)
-    if (AK_PE_COARSE(1)) donor_moments_body<S, true, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.before, c.err, t);
-    else donor_moments_body<S, false, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.before, c.err, t);
+    if (AK_PE_COARSE(1)) donor_moments_body<S, true, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, tb<S>(c.before), c.err, t);
+    else donor_moments_body<S, false, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, tb<S>(c.before), c.err, t);
   }
 }
 
@@ -162,16 +171,16 @@ __global__ __launch_bounds__(256) void kb_after_coarse(const CaseDev *__restrict
   const int b = (int)blockIdx.x;
   if (b < nb_flux) {
     AK_BATCH_MEMBER_IN(f_flux, b)
-    if (AK_PE_COARSE(delta)) flux_coarse_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.T, c.volc, c.c_inv, c.regc, t);
-    else flux_coarse_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.T, c.volc, c.c_inv, c.regc, t);
+    if (AK_PE_COARSE(delta)) flux_coarse_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.T, TR<S>(c.volc), c.c_inv, tb<S>(c.regc), t);
+    else flux_coarse_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.T, TR<S>(c.volc), c.c_inv, tb<S>(c.regc), t);
   } else if (b < nb_flux + nb_leg) {
     AK_BATCH_MEMBER_IN(f_leg, b - nb_flux)
     if (AK_PE_COARSE(delta)) legacy_body<S, true>((T *)c.Ac, c.Lc, c.K, c.U, c.c_scale, c.c_inv, t);
     else legacy_body<S, false>((T *)c.Ac, c.Lc, c.K, c.U, c.c_scale, c.c_inv, t);
   } else {
     AK_BATCH_MEMBER_IN(f_don, b - nb_flux - nb_leg)
-    if (AK_PE_COARSE(delta)) donor_moments_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.after, c.err, t);
-    else donor_moments_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, c.after, c.err, t);
+    if (AK_PE_COARSE(delta)) donor_moments_body<S, true>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, tb<S>(c.after), c.err, t);
+    else donor_moments_body<S, false>((const T *)c.Ac, c.Lc, c.B, c.F, c.c_inv, tb<S>(c.after), c.err, t);
   }
 }
 
@@ -183,8 +192,8 @@ __global__ __launch_bounds__(256) void kb_after_fine(const CaseDev *__restrict__
   const int b = (int)blockIdx.x;
   if (b < nb_flux) {
     AK_BATCH_MEMBER_IN(f_flux, b)
-    if (AK_PE_FINE(delta)) flux_fine_body<S, true>((const T *)c.Af, c.Lf, c.B, c.T, c.volf, c.f_inv, sub, c.regf, t);
-    else flux_fine_body<S, false>((const T *)c.Af, c.Lf, c.B, c.T, c.volf, c.f_inv, sub, c.regf, t);
+    if (AK_PE_FINE(delta)) flux_fine_body<S, true>((const T *)c.Af, c.Lf, c.B, c.T, TR<S>(c.volf), c.f_inv, sub, tb<S>(c.regf), t);
+    else flux_fine_body<S, false>((const T *)c.Af, c.Lf, c.B, c.T, TR<S>(c.volf), c.f_inv, sub, tb<S>(c.regf), t);
   } else {
     AK_BATCH_MEMBER_IN(f_walls, b - nb_flux)
     double *part = c.part + (size_t)sub * c.wall_blocks * c.W.nelem * 5;

@@ -164,7 +164,7 @@ int build_descriptors(Batch *b) {
   }
   for (int k = 0; k < BP_COUNT; ++k) b->nblk[k] = first[(size_t)k * (b->cap + 1) + n];
   b->yz_lds = 0;
-  for (const Refined *r : b->members) b->yz_lds = std::max(b->yz_lds, prolong_yz_lds(r->B));
+  for (const Refined *r : b->members) b->yz_lds = std::max(b->yz_lds, prolong_yz_lds(r->B, r->c->storage == 0 ? sizeof(double) : sizeof(float)));
   BCHK(hipMemcpy(b->d_cases, cd.data(), sizeof(CaseDev) * cd.size(), hipMemcpyHostToDevice));
   BCHK(hipMemcpy(b->d_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice));
   b->desc_key = key;
@@ -206,7 +206,7 @@ void batch_step(Batch *b, bool sample) {
                          F(BP_DONOR), nm, lead, nfl, nlg);
   }
   // x pass of the prolongation for both substeps
-  if (go(BP_PX)) hipLaunchKernelGGL(kb_prolong_x, dim3(b->nblk[BP_PX]), dim3(64), 0, st, C, F(BP_PX), nm);
+  if (go(BP_PX)) hipLaunchKernelGGL((kb_prolong_x<S>), dim3(b->nblk[BP_PX]), dim3(64), 0, st, C, F(BP_PX), nm);
   for (int sub = 0; sub < 2; ++sub) {
     if (go(BP_PYZ)) hipLaunchKernelGGL((kb_prolong_yz<S>), dim3(b->nblk[BP_PYZ]), dim3(64), b->yz_lds, st, C, F(BP_PYZ), nm, sub);
     batched_sweep<S>(b, b->d_desc + (size_t)(2 + sub) * nc, b->nb_fine);

@@ -11,7 +11,8 @@
 //   reflux          -- equilibrium correction of the coarse cells outside the faces
 //   restrict        -- 2x2x2 mean of fine moments, rescaled and regularised, written
 //                      into the covered coarse cells' slots
-// Transfer arithmetic is FP64 for every storage precision.
+// Transfer arithmetic runs in the storage's compute type (FP64 for FP64 storage, FP32 for
+// FP32 and FP16 storage: TR<S>); the load sums and the conservation budget stay FP64.
 #pragma once
 #include "ak_kernels.hpp"
 
@@ -30,37 +31,43 @@ struct BoxGeom {
 
 enum { TE_DENSITY = 1, TE_NEGATIVE = 2, TE_MACH = 4, TE_NONFINITE = 8 };
 
+// compute type of the transfer arithmetic of storage S
+template <int S>
+using TR = typename St<S>::C;
+
 // Moments of the shifted populations g = f - w as running sums in index order 0..26.  Every
 // moments computation goes through add(), so a form that streams the populations (no
 // 27-value array live in registers) is bit-identical to one that holds them.
+template <typename R>
 struct MomentSum {
-  double d = 0, jx = 0, jy = 0, jz = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0, mn = 1e300;
-  __device__ __forceinline__ void add(int i, double v) {
+  R d = 0, jx = 0, jy = 0, jz = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0, mn = R(1e30);
+  __device__ __forceinline__ void add(int i, R v) {
     d += v;
     jx += cxq(i) * v; jy += cyq(i) * v; jz += czq(i) * v;
     sxx += cxq(i) * cxq(i) * v; syy += cyq(i) * cyq(i) * v; szz += czq(i) * czq(i) * v;
     sxy += cxq(i) * cyq(i) * v; sxz += cxq(i) * czq(i) * v; syz += cyq(i) * czq(i) * v;
-    const double f = v + wq(i);
+    const R f = v + R(wq(i));
     mn = f < mn ? f : mn;
   }
-  __device__ __forceinline__ void finish(double (&m)[10], int &err) const {
-    const double rho = d + 1.0;
+  __device__ __forceinline__ void finish(R (&m)[10], int &err) const {
+    const R rho = d + R(1);
     if (!(rho > 0)) err |= TE_DENSITY;
     if (mn < 0) err |= TE_NEGATIVE;
     if (!isfinite(rho) || !isfinite(jx + jy + jz)) err |= TE_NONFINITE;
-    if (sqrt(jx * jx + jy * jy + jz * jz) / rho / 0.57735026918962576451 > 0.3) err |= TE_MACH;
+    if (sqrt(jx * jx + jy * jy + jz * jz) / rho / R(0.57735026918962576451) > R(0.3)) err |= TE_MACH;
     m[0] = rho; m[1] = jx; m[2] = jy; m[3] = jz;
-    m[4] = sxx - d / 3.0 - jx * jx / rho;
-    m[5] = syy - d / 3.0 - jy * jy / rho;
-    m[6] = szz - d / 3.0 - jz * jz / rho;
+    m[4] = sxx - d / R(3) - jx * jx / rho;
+    m[5] = syy - d / R(3) - jy * jy / rho;
+    m[6] = szz - d / R(3) - jz * jz / rho;
     m[7] = sxy - jx * jy / rho;
     m[8] = sxz - jx * jz / rho;
     m[9] = syz - jy * jz / rho;
   }
 };
 
-__device__ __forceinline__ void moments_of(const double (&g)[NQ], double (&m)[10], int &err) {
-  MomentSum s;
+template <typename R>
+__device__ __forceinline__ void moments_of(const R (&g)[NQ], R (&m)[10], int &err) {
+  MomentSum<R> s;
 #pragma unroll
   for (int i = 0; i < NQ; ++i) s.add(i, g[i]);
   s.finish(m, err);
@@ -68,45 +75,49 @@ __device__ __forceinline__ void moments_of(const double (&g)[NQ], double (&m)[10
 
 // tensor-product equilibrium + second-order Hermite stress, shifted by the weights; pop(i)
 // gives one population, so callers may store each as it is made
+template <typename R>
 struct PopBasis {
-  double rho, ph[3][3];
-  __device__ __forceinline__ PopBasis(const double (&m)[10], int &err) {
+  R rho, ph[3][3];
+  __device__ __forceinline__ PopBasis(const R (&m)[10], int &err) {
     rho = m[0];
     if (!(rho > 0)) err |= TE_DENSITY;
-    const double u[3] = {m[1] / rho, m[2] / rho, m[3] / rho};
-    if (sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / 0.57735026918962576451 > 0.3) err |= TE_MACH;
+    const R u[3] = {m[1] / rho, m[2] / rho, m[3] / rho};
+    if (sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / R(0.57735026918962576451) > R(0.3)) err |= TE_MACH;
     for (int a = 0; a < 3; ++a) {
-      ph[a][1] = 2.0 / 3.0 - u[a] * u[a];
-      ph[a][0] = (1.0 / 3.0 + u[a] * u[a] - u[a]) * 0.5;
-      ph[a][2] = (1.0 / 3.0 + u[a] * u[a] + u[a]) * 0.5;
+      ph[a][1] = R(2.0 / 3.0) - u[a] * u[a];
+      ph[a][0] = (R(1.0 / 3.0) + u[a] * u[a] - u[a]) * R(0.5);
+      ph[a][2] = (R(1.0 / 3.0) + u[a] * u[a] + u[a]) * R(0.5);
     }
   }
-  __device__ __forceinline__ double pop(int i, const double (&m)[10], int &err) const {
+  __device__ __forceinline__ R pop(int i, const R (&m)[10], int &err) const {
     const int cx = cxq(i), cy = cyq(i), cz = czq(i);
-    const double eq = rho * ph[0][cx + 1] * ph[1][cy + 1] * ph[2][cz + 1];
-    const double h = (cx * cx - 1.0 / 3.0) * m[4] + (cy * cy - 1.0 / 3.0) * m[5] + (cz * cz - 1.0 / 3.0) * m[6]
-                   + 2.0 * (cx * cy * m[7] + cx * cz * m[8] + cy * cz * m[9]);
-    const double p = eq + wq(i) * 4.5 * h;
+    const R eq = rho * ph[0][cx + 1] * ph[1][cy + 1] * ph[2][cz + 1];
+    const R t3 = R(1.0 / 3.0);
+    const R h = (R(cx * cx) - t3) * m[4] + (R(cy * cy) - t3) * m[5] + (R(cz * cz) - t3) * m[6]
+              + R(2) * (R(cx * cy) * m[7] + R(cx * cz) * m[8] + R(cy * cz) * m[9]);
+    const R p = eq + R(wq(i) * 4.5) * h;
     if (!(p >= 0)) err |= (isfinite(p) ? TE_NEGATIVE : TE_NONFINITE);
-    return p - wq(i);
+    return p - R(wq(i));
   }
 };
 
-__device__ __forceinline__ void populations_of(const double (&m)[10], double (&g)[NQ], int &err) {
-  const PopBasis b(m, err);
+template <typename R>
+__device__ __forceinline__ void populations_of(const R (&m)[10], R (&g)[NQ], int &err) {
+  const PopBasis<R> b(m, err);
 #pragma unroll
   for (int i = 0; i < NQ; ++i) g[i] = b.pop(i, m, err);
 }
 
-__device__ __forceinline__ void rescale(double (&m)[10], double shear, double bulk) {
-  const double iso = (m[4] + m[5] + m[6]) / 3.0;
+template <typename R>
+__device__ __forceinline__ void rescale(R (&m)[10], R shear, R bulk) {
+  const R iso = (m[4] + m[5] + m[6]) / R(3);
   for (int k = 4; k < 10; ++k) m[k] *= shear;
   for (int k = 4; k < 7; ++k) m[k] += (bulk - shear) * iso;
 }
 
-template <int S, bool PE>
+template <int S, bool PE, typename R>
 __device__ __forceinline__ void scatter_in(typename St<S>::T *A, const Desc &L, int x, int y, int z, float scale,
-                                           const double (&g)[NQ]) {
+                                           const R (&g)[NQ]) {
 #pragma unroll
   for (int i = 0; i < NQ; ++i) stv<S>(A + loc_in<PE>(L, x, y, z, i), (typename St<S>::C)g[i], scale);
 }
@@ -114,10 +125,10 @@ __device__ __forceinline__ void scatter_in(typename St<S>::T *A, const Desc &L, 
 // A population as the next coarse sweep (and the donor moments) read it back after
 // scatter_in stored it: the storage rounding, applied in registers.
 template <int S>
-__device__ __forceinline__ double stored_value(double v, float scale, float inv) {
+__device__ __forceinline__ TR<S> stored_value(TR<S> v, float scale, float inv) {
   typename St<S>::T t;
-  stv<S>(&t, (typename St<S>::C)v, scale);
-  return (double)ld<S>(&t, inv);
+  stv<S>(&t, v, scale);
+  return ld<S>(&t, inv);
 }
 
 __device__ __forceinline__ bool in_core(const BoxGeom &B, int bx, int by) {
@@ -135,26 +146,27 @@ struct FreeState {
 // populations writes their moments too); SKIP 2: also the reflux cells (the coarse cells
 // next to the box faces, whose reflux writes theirs).
 template <int S, bool PE, int SKIP = 0>
-__device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F, float inv_scale, double *__restrict__ M, int *__restrict__ err, long long t) {
+__device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F, float inv_scale, TR<S> *__restrict__ M, int *__restrict__ err, long long t) {
+  using R = TR<S>;
   const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
   if (t >= (long long)dx * dy * dz) return;
   const int i = (int)(t % dx), j = (int)((t / dx) % dy), k = (int)(t / ((long long)dx * dy));
   const int x = B.lo[0] - 3 + i, y = B.lo[1] - 3 + j;
   const int z = ((B.lo[2] - 3 + k) % L.nz + L.nz) % L.nz;
-  double m[10];
+  R m[10];
   int e = 0;
   const int bx = x - B.lo[0], by = y - B.lo[1];
   const bool inx = bx >= 0 && bx < B.nb[0], iny = by >= 0 && by < B.nb[1];
   const bool covered = inx && iny;
   const bool refluxed = (inx && (by == -1 || by == B.nb[1])) || (iny && (bx == -1 || bx == B.nb[0]));
   if (covered && in_core(B, bx, by)) {
-    for (int c = 0; c < 10; ++c) m[c] = F.m[c];
+    for (int c = 0; c < 10; ++c) m[c] = R(F.m[c]);
   } else if ((SKIP >= 1 && covered) || (SKIP >= 2 && refluxed)) {
     return;
   } else {
-    MomentSum ms;
+    MomentSum<R> ms;
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) ms.add(i, (double)ld<S>(A + loc_in<PE>(L, x, y, z, i), inv_scale));
+    for (int i = 0; i < NQ; ++i) ms.add(i, ld<S>(A + loc_in<PE>(L, x, y, z, i), inv_scale));
     ms.finish(m, e);
   }
   for (int c = 0; c < 10; ++c) M[t * 10 + c] = m[c];
@@ -163,7 +175,7 @@ __device__ __forceinline__ void donor_moments_body(const typename St<S>::T *__re
 
 template <int S, bool PE>
 __global__ void k_donor_moments(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, FreeState F,
-                                float inv_scale, double *__restrict__ M, int *__restrict__ err) {
+                                float inv_scale, TR<S> *__restrict__ M, int *__restrict__ err) {
   donor_moments_body<S, PE>(A, L, B, F, inv_scale, M, err, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -176,21 +188,23 @@ __global__ void k_donor_moments(const typename St<S>::T *__restrict__ A, Desc L,
 //   the gather slots of the fine lattice.
 // Child 2P of parent P takes a0 - off, child 2P+1 takes a0 + off, with
 // off = 11/64 (a1 - a_-1) - 3/128 (a2 - a_-2).
-__device__ __forceinline__ double prolong_weight(int side, int k) {
+template <typename R>
+__device__ __forceinline__ R prolong_weight(int side, int k) {
   // k = -2..2
-  const double lo[5] = {-3.0 / 128.0, 11.0 / 64.0, 1.0, -11.0 / 64.0, 3.0 / 128.0};
-  return side ? -lo[k + 2] + (k == 0 ? 2.0 : 0.0) : lo[k + 2];
+  const R lo[5] = {R(-3.0 / 128.0), R(11.0 / 64.0), R(1), R(-11.0 / 64.0), R(3.0 / 128.0)};
+  return side ? -lo[k + 2] + (k == 0 ? R(2) : R(0)) : lo[k + 2];
 }
 
-// Doubles of one substep's x-pass output (the x pass of both substeps is one launch: it
+// Values of one substep's x-pass output (the x pass of both substeps is one launch: it
 // needs only the coarse donor moments at t and t + dt, which the fine substeps leave alone).
 __host__ __device__ inline size_t prolong_x_half(const BoxGeom &B) {
   return (size_t)(2 * B.nb[0] + 4) * (B.nb[1] + 6) * (B.nb[2] + 6) * 10;
 }
 
 // x pass of both substeps: substep s (time fraction s / 2) goes to X + s * prolong_x_half.
-__device__ __forceinline__ void prolong_x_body(const double *__restrict__ M0, const double *__restrict__ M1, BoxGeom B,
-                                               double *__restrict__ X, long long t) {
+template <typename R>
+__device__ __forceinline__ void prolong_x_body(const R *__restrict__ M0, const R *__restrict__ M1, BoxGeom B,
+                                               R *__restrict__ X, long long t) {
   const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4;
   if (t >= (long long)cxn * dy * dz) return;
   const int cx = (int)(t % cxn), qy = (int)((t / cxn) % dy), qz = (int)(t / ((long long)cxn * dy));
@@ -200,18 +214,18 @@ __device__ __forceinline__ void prolong_x_body(const double *__restrict__ M0, co
   const int P = cx / 2 + 2, sd = cx & 1;
   // each donor value is loaded once for both time fractions (0 and 1/2); every term keeps
   // the form w * ((1 - frac) * M0 + frac * M1)
-  double m0[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, m1[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  R m0[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, m1[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int a = -2; a <= 2; ++a) {
-    const double w = prolong_weight(sd, a);
+    const R w = prolong_weight<R>(sd, a);
     const long long idx = (((long long)qz * dy + qy) * dx + (P + a)) * 10;
 #pragma unroll
     for (int k = 0; k < 10; ++k) {
-      const double v0 = M0[idx + k], v1 = M1[idx + k];
-      m0[k] += w * ((1.0 - 0.0) * v0 + 0.0 * v1);
-      m1[k] += w * ((1.0 - 0.5) * v0 + 0.5 * v1);
+      const R v0 = M0[idx + k], v1 = M1[idx + k];
+      m0[k] += w * ((R(1) - R(0)) * v0 + R(0) * v1);
+      m1[k] += w * ((R(1) - R(0.5)) * v0 + R(0.5) * v1);
     }
   }
-  double *o0 = X + t * 10, *o1 = X + prolong_x_half(B) + t * 10;
+  R *o0 = X + t * 10, *o1 = X + prolong_x_half(B) + t * 10;
 #pragma unroll
   for (int k = 0; k < 10; ++k) { o0[k] = m0[k]; o1[k] = m1[k]; }
 }
@@ -219,28 +233,31 @@ __device__ __forceinline__ void prolong_x_body(const double *__restrict__ M0, co
 // The prolongation kernels run over compact lists of the items that do work (the host
 // builds them with the bodies' own conditions): the fine interface band is a few percent of
 // the box volume.
-__global__ void k_prolong_x(const double *__restrict__ M0, const double *__restrict__ M1, BoxGeom B,
-                            double *__restrict__ X, const int32_t *__restrict__ list, int n) {
+template <typename R>
+__global__ void k_prolong_x(const R *__restrict__ M0, const R *__restrict__ M1, BoxGeom B,
+                            R *__restrict__ X, const int32_t *__restrict__ list, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) prolong_x_body(M0, M1, B, X, list[i]);
 }
 
 // The y pass and the z pass + injection of the prolongation for a group of PZ_COLS fine
 // columns of the interface band (fine columns outside the owned box): the y pass
-// of a column at every donor z goes to LDS (dz * 10 doubles per column, dynamic shared
+// of a column at every donor z goes to LDS (dz * 10 values per column, dynamic shared
 // memory), then its fine cells (the fine cells outside the owned box, every z) interpolate
 // in z from there, are rescaled fine <- coarse, regularised and written into the slots the
 // next fine sweep reads.
 constexpr int PZ_COLS = 4;
-__host__ __device__ inline size_t prolong_yz_lds(const BoxGeom &B) {
-  return (size_t)PZ_COLS * (B.nb[2] + 6) * 10 * sizeof(double);
+__host__ __device__ inline size_t prolong_yz_lds(const BoxGeom &B, size_t value_bytes) {
+  return (size_t)PZ_COLS * (B.nb[2] + 6) * 10 * value_bytes;
 }
 
 template <int S, bool PE>
-__device__ __forceinline__ void prolong_yz_body(const double *__restrict__ X, typename St<S>::T *__restrict__ A, Desc L,
+__device__ __forceinline__ void prolong_yz_body(const TR<S> *__restrict__ X, typename St<S>::T *__restrict__ A, Desc L,
                                                 BoxGeom B, Levels V, float scale, int *__restrict__ err,
                                                 const int32_t *__restrict__ cols, int ncol, int group) {
-  extern __shared__ double yz[];  // [PZ_COLS][dz][10]
+  using R = TR<S>;
+  extern __shared__ __align__(16) unsigned char yz_bytes[];
+  R *yz = (R *)yz_bytes;  // [PZ_COLS][dz][10]
   const int dy = B.nb[1] + 6, dz = B.nb[2] + 6, cxn = 2 * B.nb[0] + 4;
   const int c0 = group * PZ_COLS;
   const int nc = ncol - c0 < PZ_COLS ? ncol - c0 : PZ_COLS;
@@ -248,13 +265,13 @@ __device__ __forceinline__ void prolong_yz_body(const double *__restrict__ X, ty
     const int k = i / dz, qz = i % dz;
     const int col = cols[c0 + k], cx = col % cxn, cy = col / cxn;
     const int P = cy / 2 + 2, sd = cy & 1;
-    double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    R m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = -2; b <= 2; ++b) {
-      const double w = prolong_weight(sd, b);
-      const double *q = X + (((long long)qz * dy + (P + b)) * cxn + cx) * 10;
+      const R w = prolong_weight<R>(sd, b);
+      const R *q = X + (((long long)qz * dy + (P + b)) * cxn + cx) * 10;
       for (int kk = 0; kk < 10; ++kk) m[kk] += w * q[kk];
     }
-    double *o = yz + ((size_t)k * dz + qz) * 10;
+    R *o = yz + ((size_t)k * dz + qz) * 10;
     for (int kk = 0; kk < 10; ++kk) o[kk] = m[kk];
   }
   __syncthreads();
@@ -264,15 +281,15 @@ __device__ __forceinline__ void prolong_yz_body(const double *__restrict__ X, ty
     const int col = cols[c0 + k], x = col % cxn, y = col / cxn;
     const int pz = z + p;
     const int P = pz / 2 + 2, sd = pz & 1;
-    double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    R m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = -2; c <= 2; ++c) {
-      const double w = prolong_weight(sd, c);
-      const double *q = yz + ((size_t)k * dz + (P + c)) * 10;
+      const R w = prolong_weight<R>(sd, c);
+      const R *q = yz + ((size_t)k * dz + (P + c)) * 10;
       for (int kk = 0; kk < 10; ++kk) m[kk] += w * q[kk];
     }
-    rescale(m, V.shear_cf, V.bulk_cf);
+    rescale(m, R(V.shear_cf), R(V.bulk_cf));
     int e = 0;
-    const PopBasis pb(m, e);
+    const PopBasis<R> pb(m, e);
 #pragma unroll
     for (int i = 0; i < NQ; ++i) stv<S>(A + loc_in<PE>(L, x, y, z, i), (typename St<S>::C)pb.pop(i, m, e), scale);
     if (e) atomicOr(err, e);
@@ -280,7 +297,7 @@ __device__ __forceinline__ void prolong_yz_body(const double *__restrict__ X, ty
 }
 
 template <int S, bool PE>
-__global__ __launch_bounds__(64) void k_prolong_yz(const double *__restrict__ X, typename St<S>::T *__restrict__ A,
+__global__ __launch_bounds__(64) void k_prolong_yz(const TR<S> *__restrict__ X, typename St<S>::T *__restrict__ A,
                                                    Desc L, BoxGeom B, Levels V, float scale, int *__restrict__ err,
                                                    const int32_t *__restrict__ cols, int ncol) {
   prolong_yz_body<S, PE>(X, A, L, B, V, scale, err, cols, ncol, (int)blockIdx.x);
@@ -304,8 +321,9 @@ constexpr int FLUX_LANES = 8;
 template <int S, bool PE>
 __device__ __forceinline__ void flux_terms(const typename St<S>::T *A, const Desc &L, const FluxTable *__restrict__ T,
                                            int axis, int face_cell, int tpos, int zpos, float inv, int sub_lane,
-                                           bool active, double (&out)[4]) {
-  out[0] = out[1] = out[2] = out[3] = 0.0;
+                                           bool active, TR<S> (&out)[4]) {
+  using R = TR<S>;
+  out[0] = out[1] = out[2] = out[3] = R(0);
   if (active) {
     for (int k = sub_lane; k < T->n[axis]; k += FLUX_LANES) {
       const FluxTerm &tm = T->term[axis][k];
@@ -314,8 +332,8 @@ __device__ __forceinline__ void flux_terms(const typename St<S>::T *A, const Des
       const int tt = tpos - tm.s0;
       const int zz = ((zpos - tm.s1) % L.nz + L.nz) % L.nz;
       const int x = axis == 0 ? a : tt, y = axis == 0 ? tt : a;
-      const double v = (double)ld<S>(A + loc_out<PE>(L, x, y, zz, tm.d), inv);
-      for (int c = 0; c < 4; ++c) out[c] += v * tm.coef[c];
+      const R v = ld<S>(A + loc_out<PE>(L, x, y, zz, tm.d), inv);
+      for (int c = 0; c < 4; ++c) out[c] += v * R(tm.coef[c]);
     }
   }
 #pragma unroll
@@ -325,7 +343,7 @@ __device__ __forceinline__ void flux_terms(const typename St<S>::T *A, const Des
 
 // Coarse face registers reg[f][t][z][4] (integrated over one coarse step: x dx_c^3).
 template <int S, bool PE>
-__device__ __forceinline__ void flux_coarse_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T, double vol, float inv, double *__restrict__ reg, long long t) {
+__device__ __forceinline__ void flux_coarse_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T, TR<S> vol, float inv, TR<S> *__restrict__ reg, long long t) {
   const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
   const long long w = t / FLUX_LANES;  // face element; its lanes are consecutive within a wave
   const int sub_lane = (int)(t % FLUX_LANES);
@@ -335,25 +353,25 @@ __device__ __forceinline__ void flux_coarse_body(const typename St<S>::T *__rest
   const int other = 1 - axis;
   const bool active = w < ne && ti < B.nb[other];
   const int face_cell = side ? B.hi[axis] - 1 : B.lo[axis] - 1;
-  double o[4];
+  TR<S> o[4];
   flux_terms<S, PE>(A, L, T, axis, face_cell, B.lo[other] + ti, zpos, inv, sub_lane, active, o);
   if (active && sub_lane == 0) {
-    double *r = reg + (((long long)f * nt + ti) * L.nz + zpos) * 4;
+    TR<S> *r = reg + (((long long)f * nt + ti) * L.nz + zpos) * 4;
     for (int c = 0; c < 4; ++c) r[c] = o[c] * vol;
   }
 }
 
 template <int S, bool PE>
 __global__ __launch_bounds__(256) void k_flux_coarse(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
-                                                     const FluxTable *__restrict__ T, double vol, float inv,
-                                                     double *__restrict__ reg) {
+                                                     const FluxTable *__restrict__ T, TR<S> vol, float inv,
+                                                     TR<S> *__restrict__ reg) {
   flux_coarse_body<S, PE>(A, L, B, T, vol, inv, reg, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Fine face registers per fine face element regf[f][t_f][z_f][4] (x dx_f^3), summed over
 // the two substeps (accumulate = 1 on the second); the reflux folds the 2x2 children.
 template <int S, bool PE>
-__device__ __forceinline__ void flux_fine_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T, double vol, float inv, int accumulate, double *__restrict__ regf, long long t) {
+__device__ __forceinline__ void flux_fine_body(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const FluxTable *__restrict__ T, TR<S> vol, float inv, int accumulate, TR<S> *__restrict__ regf, long long t) {
   const int nt = 2 * (B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1]);
   const long long w = t / FLUX_LANES;  // face element; its lanes are consecutive within a wave
   const int sub_lane = (int)(t % FLUX_LANES);
@@ -364,18 +382,18 @@ __device__ __forceinline__ void flux_fine_body(const typename St<S>::T *__restri
   const bool active = w < ne && tf < 2 * B.nb[other];
   const int n_axis = 2 * B.nb[axis];
   const int face_cell = side ? B.pad + n_axis - 1 : B.pad - 1;
-  double o[4];
+  TR<S> o[4];
   flux_terms<S, PE>(A, L, T, axis, face_cell, B.pad + tf, zf, inv, sub_lane, active, o);
   if (active && sub_lane == 0) {
-    double *r = regf + (((long long)f * nt + tf) * L.nz + zf) * 4;
-    for (int c = 0; c < 4; ++c) r[c] = (accumulate ? r[c] : 0.0) + o[c] * vol;
+    TR<S> *r = regf + (((long long)f * nt + tf) * L.nz + zf) * 4;
+    for (int c = 0; c < 4; ++c) r[c] = (accumulate ? r[c] : TR<S>(0)) + o[c] * vol;
   }
 }
 
 template <int S, bool PE>
 __global__ __launch_bounds__(256) void k_flux_fine(const typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
-                                                   const FluxTable *__restrict__ T, double vol, float inv,
-                                                   int accumulate, double *__restrict__ regf) {
+                                                   const FluxTable *__restrict__ T, TR<S> vol, float inv,
+                                                   int accumulate, TR<S> *__restrict__ regf) {
   flux_fine_body<S, PE>(A, L, B, T, vol, inv, accumulate, regf, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -557,15 +575,17 @@ __global__ void k_budget_reset(BudgetAcc *acc) {
 
 // Equilibrium reflux on the coarse cells just outside each face: one wavefront per
 // cell, lane i (< 27) owns population i; density and momentum by fixed-order butterflies.
-__device__ __forceinline__ double eq_dir(int i, double rho, double ux, double uy, double uz) {
-  const double u[3] = {ux, uy, uz};
+template <typename R>
+__device__ __forceinline__ R eq_dir(int i, R rho, R ux, R uy, R uz) {
+  const R u[3] = {ux, uy, uz};
   const int c[3] = {cxq(i), cyq(i), czq(i)};
-  double e = rho;
+  R e = rho;
   for (int a = 0; a < 3; ++a)
-    e *= c[a] == 0 ? 2.0 / 3.0 - u[a] * u[a] : (1.0 / 3.0 + u[a] * u[a] + c[a] * u[a]) * 0.5;
+    e *= c[a] == 0 ? R(2.0 / 3.0) - u[a] * u[a] : (R(1.0 / 3.0) + u[a] * u[a] + R(c[a]) * u[a]) * R(0.5);
   return e;
 }
-__device__ __forceinline__ double wave_sum(double v) {
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -575,7 +595,8 @@ template <int S, bool PE>
 // With M, the wave also writes the donor moments of its cell's new state for the next coarse
 // step (its populations as stored, gathered to lane 0; every donor-block z index that maps
 // to its z).
-__device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const double *__restrict__ rc, const double *__restrict__ rf, double inv_vol, float scale, float inv, int *__restrict__ err, long long t, double *__restrict__ M = nullptr) {
+__device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B, const TR<S> *__restrict__ rc, const TR<S> *__restrict__ rf, TR<S> inv_vol, float scale, float inv, int *__restrict__ err, long long t, TR<S> *__restrict__ M = nullptr) {
+  using R = TR<S>;
   // rf holds fine face elements [f][2 nt][2 nz][4]
   const int nt = B.nb[0] > B.nb[1] ? B.nb[0] : B.nb[1];
   const long long w = t / 64;
@@ -588,30 +609,30 @@ __device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, D
   const int a = side ? B.hi[axis] : B.lo[axis] - 1;
   const int x = axis == 0 ? a : B.lo[other] + ti, y = axis == 0 ? B.lo[other] + ti : a;
   const long long ri = (((long long)f * nt + ti) * L.nz + zpos) * 4;
-  const double sgn = side ? 1.0 : -1.0;
-  double corr[4];
+  const R sgn = side ? R(1) : R(-1);
+  R corr[4];
   for (int c = 0; c < 4; ++c) {
-    double ff = 0.0;
+    R ff = R(0);
     for (int p = 0; p < 2; ++p)
       for (int q = 0; q < 2; ++q) ff += rf[(((long long)f * 2 * nt + 2 * ti + p) * 2 * L.nz + 2 * zpos + q) * 4 + c];
     corr[c] = sgn * (ff - rc[ri + c]) * inv_vol;
   }
   const int i = lane < NQ ? lane : 0;
   const long long slot = loc_in<PE>(L, x, y, zpos, i);
-  double g = lane < NQ ? (double)ld<S>(A + slot, inv) : 0.0;
-  const double d = wave_sum(g);
-  const double jx = wave_sum(cxq(i) * g * (lane < NQ));
-  const double jy = wave_sum(cyq(i) * g * (lane < NQ));
-  const double jz = wave_sum(czq(i) * g * (lane < NQ));
+  R g = lane < NQ ? R(ld<S>(A + slot, inv)) : R(0);
+  const R on = lane < NQ ? R(1) : R(0);
+  const R d = wave_sum(g);
+  const R jx = wave_sum(R(cxq(i)) * g * on);
+  const R jy = wave_sum(R(cyq(i)) * g * on);
+  const R jz = wave_sum(R(czq(i)) * g * on);
   int e = 0;
-  const double rho = d + 1.0, rho2 = rho + corr[0];
+  const R rho = d + R(1), rho2 = rho + corr[0];
   if (!(rho > 0) || !(rho2 > 0)) e |= TE_DENSITY;
-  const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
-  const double vx = (jx + corr[1]) / rho2, vy = (jy + corr[2]) / rho2, vz = (jz + corr[3]) / rho2;
-  if (sqrt(ux * ux + uy * uy + uz * uz) > 0.3 * 0.57735026918962576451 ||
-      sqrt(vx * vx + vy * vy + vz * vz) > 0.3 * 0.57735026918962576451)
-    e |= TE_MACH;
-  const double eo = eq_dir(i, rho, ux, uy, uz), en = eq_dir(i, rho2, vx, vy, vz);
+  const R ux = jx / rho, uy = jy / rho, uz = jz / rho;
+  const R vx = (jx + corr[1]) / rho2, vy = (jy + corr[2]) / rho2, vz = (jz + corr[3]) / rho2;
+  const R mach_cap = R(0.3 * 0.57735026918962576451);
+  if (sqrt(ux * ux + uy * uy + uz * uz) > mach_cap || sqrt(vx * vx + vy * vy + vz * vz) > mach_cap) e |= TE_MACH;
+  const R eo = eq_dir(i, rho, ux, uy, uz), en = eq_dir(i, rho2, vx, vy, vz);
   if (lane < NQ) {
     if (!(eo >= 0) || !(en >= 0)) e |= TE_NEGATIVE;
     g += en - eo;
@@ -619,17 +640,17 @@ __device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, D
     stv<S>(A + slot, (typename St<S>::C)g, scale);
   }
   if (M != nullptr) {
-    const double h = lane < NQ ? stored_value<S>(g, scale, inv) : 0.0;
-    MomentSum ms;
+    const R h = lane < NQ ? stored_value<S>(g, scale, inv) : R(0);
+    MomentSum<R> ms;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) ms.add(q, __shfl(h, q, 64));
     if (lane == 0) {
-      double md[10];
+      R md[10];
       ms.finish(md, e);
       const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
       const int bx = x - B.lo[0], by = y - B.lo[1];
       for (int k = ((zpos - B.lo[2] + 3) % L.nz + L.nz) % L.nz; k < dz; k += L.nz) {
-        double *o = M + ((((long long)k * dy) + (by + 3)) * dx + (bx + 3)) * 10;
+        R *o = M + ((((long long)k * dy) + (by + 3)) * dx + (bx + 3)) * 10;
         for (int c = 0; c < 10; ++c) o[c] = md[c];
       }
     }
@@ -644,7 +665,7 @@ __device__ __forceinline__ void reflux_body(typename St<S>::T *__restrict__ A, D
 // 2x2x2 mean of the children's moments, rescaled fine->coarse and regularised); the moments
 // never leave the chip.
 constexpr int RT_X = 16, RT_Y = 2;
-constexpr int RESTRICT_LDS = 10 * 256;  // doubles: the children's moments of one tile
+constexpr int RESTRICT_LDS = 10 * 256;  // doubles: room for the children's moments of one tile
 // shared memory of a kernel that fuses the restriction and the loads + budget fold (the
 // wall partials' lane sums, then four lane-sum columns of the outer-face flux partials)
 constexpr int FOLD_LDS = LOADS_LDS + 4 * 64;
@@ -660,19 +681,20 @@ template <int S, bool PE, bool PEC>
 __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
                                                    typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B, Levels V,
                                                    FreeState F, float cscale, int *__restrict__ err, int tile,
-                                                   double *__restrict__ sm, double *__restrict__ M = nullptr,
+                                                   TR<S> *__restrict__ sm, TR<S> *__restrict__ M = nullptr,
                                                    float cinv = 1.0f) {
+  using R = TR<S>;
   const int ntx = (B.nb[0] + RT_X - 1) / RT_X, nty = (B.nb[1] + RT_Y - 1) / RT_Y;
   const int tbx = tile % ntx, tby = (tile / ntx) % nty, bz = tile / (ntx * nty);
   const int tid = threadIdx.x;
   const int cx = tid % (2 * RT_X), cy = (tid / (2 * RT_X)) % (2 * RT_Y), cz = tid / (4 * RT_X * RT_Y);
   const int fx = 2 * RT_X * tbx + cx, fy = 2 * RT_Y * tby + cy, fz = 2 * bz + cz;
   int e = 0;
-  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  R m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (fx < 2 * B.nb[0] && fy < 2 * B.nb[1] && !in_core(B, fx / 2, fy / 2)) {
-    MomentSum ms;
+    MomentSum<R> ms;
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) ms.add(i, (double)ld<S>(Af + loc_in<PE>(Lf, B.pad + fx, B.pad + fy, fz, i), finv));
+    for (int i = 0; i < NQ; ++i) ms.add(i, R(ld<S>(Af + loc_in<PE>(Lf, B.pad + fx, B.pad + fy, fz, i), finv)));
     ms.finish(m, e);
   }
 #pragma unroll
@@ -687,23 +709,23 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
 #pragma unroll
         for (int i = 0; i < NQ; ++i) stv<S>(Ac + loc_in<PEC>(Lc, X, Y, Z, i), (typename St<S>::C)F.g[i], cscale);
       } else {
-        double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        R acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int c = 0; c < 2; ++c)
           for (int b = 0; b < 2; ++b)
             for (int a = 0; a < 2; ++a) {
               const int ch = (2 * ox + a) + 2 * RT_X * ((2 * oy + b) + 2 * RT_Y * c);
               for (int k = 0; k < 10; ++k) acc[k] += sm[k * 256 + ch];
             }
-        for (int k = 0; k < 10; ++k) acc[k] *= 0.125;
-        rescale(acc, V.shear_fc, V.bulk_fc);
+        for (int k = 0; k < 10; ++k) acc[k] *= R(0.125);
+        rescale(acc, R(V.shear_fc), R(V.bulk_fc));
         // with M the stored values also feed the donor moments of the next step.  FP32/FP16
         // store each population as it is made (holding all 27 beside their rounded values
         // costs ~45 registers there); FP64 stores exactly what it computes and measured
         // faster with the 27 held
-        const PopBasis pb(acc, e);
-        MomentSum ms;
+        const PopBasis<R> pb(acc, e);
+        MomentSum<R> ms;
         if constexpr (S == 0) {
-          double g[NQ];
+          R g[NQ];
 #pragma unroll
           for (int i = 0; i < NQ; ++i) g[i] = pb.pop(i, acc, e);
           scatter_in<S, PEC>(Ac, Lc, X, Y, Z, cscale, g);
@@ -714,17 +736,17 @@ __device__ __forceinline__ void restrict_tile_body(const typename St<S>::T *__re
         } else {
 #pragma unroll
           for (int i = 0; i < NQ; ++i) {
-            const double gi = pb.pop(i, acc, e);
+            const R gi = pb.pop(i, acc, e);
             stv<S>(Ac + loc_in<PEC>(Lc, X, Y, Z, i), (typename St<S>::C)gi, cscale);
             if (M != nullptr) ms.add(i, stored_value<S>(gi, cscale, cinv));
           }
         }
         if (M != nullptr) {
-          double md[10];
+          R md[10];
           ms.finish(md, e);
           const int dx = B.nb[0] + 6, dy = B.nb[1] + 6, dz = B.nb[2] + 6;
           for (int k = ((bz + 3) % Lc.nz + Lc.nz) % Lc.nz; k < dz; k += Lc.nz) {
-            double *o = M + ((((long long)k * dy) + (by + 3)) * dx + (bx + 3)) * 10;
+            R *o = M + ((((long long)k * dy) + (by + 3)) * dx + (bx + 3)) * 10;
             for (int c = 0; c < 10; ++c) o[c] = md[c];
           }
         }
@@ -742,10 +764,10 @@ template <int S, bool PE, bool PEC>
 __global__ __launch_bounds__(256) void k_restrict_tiles(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
                                                         typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B,
                                                         Levels V, FreeState F, float cscale, float cinv,
-                                                        double *__restrict__ M, int *__restrict__ err, int ntiles) {
+                                                        TR<S> *__restrict__ M, int *__restrict__ err, int ntiles) {
   __shared__ double sm[RESTRICT_LDS];
   if ((int)blockIdx.x < ntiles)
-    restrict_tile_body<S, PE, PEC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, (int)blockIdx.x, sm, M, cinv);
+    restrict_tile_body<S, PE, PEC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, (int)blockIdx.x, (TR<S> *)sm, M, cinv);
   else
     donor_moments_body<S, PEC, 1>(Ac, Lc, B, F, cinv, M, err, (long long)((int)blockIdx.x - ntiles) * 256 + threadIdx.x);
 }
@@ -815,9 +837,9 @@ __device__ __forceinline__ void refined_fold_body(const double *__restrict__ par
 template <int S, bool PF, bool PC>
 __global__ __launch_bounds__(256) void k_step_end(const typename St<S>::T *__restrict__ Af, Desc Lf, float finv,
                                                   typename St<S>::T *__restrict__ Ac, Desc Lc, BoxGeom B, Levels V,
-                                                  FreeState F, float cscale, float cinv, double *__restrict__ M,
-                                                  const double *__restrict__ rc, const double *__restrict__ rf,
-                                                  double inv_vol, int *__restrict__ err, int nb_reflux, int ntiles,
+                                                  FreeState F, float cscale, float cinv, TR<S> *__restrict__ M,
+                                                  const TR<S> *__restrict__ rc, const TR<S> *__restrict__ rf,
+                                                  TR<S> inv_vol, int *__restrict__ err, int nb_reflux, int ntiles,
                                                   const double *__restrict__ part, int nparts, int nelem, double fscale,
                                                   double vscale, StepCtl *ctl, double *__restrict__ hist, int sample,
                                                   BudgetArgs G) {
@@ -830,7 +852,7 @@ __global__ __launch_bounds__(256) void k_step_end(const typename St<S>::T *__res
     else if (sample) loads_final_body(part, nparts, nelem, fscale, fscale, vscale, ctl, hist, nullptr, lds);
     else if (threadIdx.x == 0) ctl->step += 1;
   } else if (b < nb_reflux + 1 + ntiles) {
-    restrict_tile_body<S, PF, PC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, b - nb_reflux - 1, lds, M, cinv);
+    restrict_tile_body<S, PF, PC>(Af, Lf, finv, Ac, Lc, B, V, F, cscale, err, b - nb_reflux - 1, (TR<S> *)lds, M, cinv);
   } else {
     donor_moments_body<S, PC, 2>(Ac, Lc, B, F, cinv, M, err, (long long)(b - nb_reflux - 1 - ntiles) * 256 + threadIdx.x);
   }
@@ -874,9 +896,9 @@ __global__ void k_init_level(typename St<S>::T *__restrict__ A, const uint8_t *_
 // they run as one launch over consecutive block ranges: their latencies overlap.
 template <int S, bool PE>
 __global__ __launch_bounds__(256) void k_after_coarse(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
-                                                      const FluxTable *__restrict__ T, double volc, float scale,
-                                                      float inv, double *__restrict__ regc, LegacyLinks K, double U,
-                                                      FreeState F, double *__restrict__ after, int *__restrict__ err,
+                                                      const FluxTable *__restrict__ T, TR<S> volc, float scale,
+                                                      float inv, TR<S> *__restrict__ regc, LegacyLinks K, double U,
+                                                      FreeState F, TR<S> *__restrict__ after, int *__restrict__ err,
                                                       int nb_flux, int nb_leg) {
   const int b = blockIdx.x;
   if (b < nb_flux) {
@@ -896,8 +918,8 @@ __global__ __launch_bounds__(256) void k_after_coarse(typename St<S>::T *__restr
 // coarse lattice's form.
 template <int S, bool PE, bool PC>
 __global__ __launch_bounds__(256) void k_after_fine(typename St<S>::T *__restrict__ A, Desc L, BoxGeom B,
-                                                    const FluxTable *__restrict__ T, double volf, float scale, float inv,
-                                                    int sub, double *__restrict__ regf, WallLinks W,
+                                                    const FluxTable *__restrict__ T, TR<S> volf, float scale, float inv,
+                                                    int sub, TR<S> *__restrict__ regf, WallLinks W,
                                                     double *__restrict__ part, int nb_flux, int nb_walls,
                                                     const typename St<S>::T *__restrict__ Ac, Desc Lc, float cinv,
                                                     FreeState F, double *__restrict__ opart) {

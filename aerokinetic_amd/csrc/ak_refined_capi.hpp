@@ -18,8 +18,11 @@ struct Refined {
   Levels V{};
   FreeState F{};
   double dxc = 0, dtc = 0, dxf = 0, dtf = 0, span = 0, U = 0;
-  double *d_before = nullptr, *d_after = nullptr, *d_regc = nullptr, *d_regf = nullptr, *d_part = nullptr;
+  // transfer buffers hold values of the transfer arithmetic TR<S> (FP64 for FP64 storage,
+  // FP32 otherwise), sized for doubles and read through tr<S>()
+  double *d_before = nullptr, *d_after = nullptr, *d_regc = nullptr, *d_regf = nullptr;
   double *d_px = nullptr;                   // prolongation x pass of both substeps
+  double *d_part = nullptr;                 // wall load partials (FP64)
   int32_t *d_lpx = nullptr, *d_cols = nullptr;  // x-pass work items; fine interface-band columns
   int n_lpx = 0, n_cols = 0;
   int *d_err = nullptr;
@@ -38,6 +41,10 @@ struct Refined {
 };
 
 Refined *R(void *h) { return (Refined *)h; }
+
+// A transfer buffer as the kernels of storage S read it.
+template <int S>
+TR<S> *tr(double *p) { return (TR<S> *)p; }
 
 // Device address of a double field of the case's budget accumulators.
 double *acc_field(Refined *r, size_t offset) { return (double *)((char *)r->d_acc + offset); }
@@ -119,7 +126,8 @@ int build_prolong_lists(Refined *r) {
   const Desc &L = r->f->L;
   if ((long long)cxn * dy * dz >= INT32_MAX || (long long)L.nx * L.ny >= INT32_MAX)
     return fail(nullptr, "refined box too large for 32-bit prolongation work items");
-  if (prolong_yz_lds(B) > 65536) return fail(nullptr, "periodic span too long for the prolongation tiles");
+  if (prolong_yz_lds(B, r->c->storage == 0 ? sizeof(double) : sizeof(float)) > 65536)
+    return fail(nullptr, "periodic span too long for the prolongation tiles");
   std::vector<int32_t> lx, cols;
   for (int qz = 0; qz < dz; ++qz)
     for (int qy = 0; qy < dy; ++qy)
@@ -145,8 +153,8 @@ void launch_restrict(Refined *r) {
   const bool pf = f->sweeps % 2 == 0, pc = c->sweeps % 2 == 0;
 #define AK_RT(PF, PC)                                                                                            \
   hipLaunchKernelGGL((k_restrict_tiles<S, PF, PC>), grid, block, 0, c->stream, (const T_ *)f->pop, f->L,         \
-                     f->inv_scale, (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, c->inv_scale, r->d_before,   \
-                     r->d_err, ntiles)
+                     f->inv_scale, (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, c->inv_scale,                \
+                     tr<S>(r->d_before), r->d_err, ntiles)
   if (pf) { if (pc) AK_RT(true, true); else AK_RT(true, false); }
   else { if (pc) AK_RT(false, true); else AK_RT(false, false); }
 #undef AK_RT
@@ -165,9 +173,9 @@ void phase_before_coarse(Refined *r) {
   using T_ = typename St<S>::T;
   const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
   if (c->sweeps % 2 == 0)
-    hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_before, r->d_err);
+    hipLaunchKernelGGL((k_donor_moments<S, true>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, tr<S>(r->d_before), r->d_err);
   else
-    hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, r->d_before, r->d_err);
+    hipLaunchKernelGGL((k_donor_moments<S, false>), dim3(blocks_for(ndon, 128)), dim3(128), 0, c->stream, (const T_ *)c->pop, c->L, r->B, r->F, c->inv_scale, tr<S>(r->d_before), r->d_err);
 }
 
 // after the coarse sweep: coarse face flux, coarse farfield, donor moments at t + dt
@@ -177,21 +185,24 @@ void phase_after_coarse(Refined *r) {
   using T_ = typename St<S>::T;
   const long long ndon = (long long)(r->B.nb[0] + 6) * (r->B.nb[1] + 6) * (r->B.nb[2] + 6);
   const unsigned cflux_blocks = blocks_for(4LL * r->nt * c->nz * FLUX_LANES, 256);
-  const double volc = r->dxc * r->dxc * r->dxc;
+  const TR<S> volc = TR<S>(r->dxc * r->dxc * r->dxc);
   // one launch: face flux, legacy farfield and donor moments (disjoint slots)
   const LegacyLinks K{c->d_legx, c->d_legz, c->d_legdir, c->d_legkind, c->d_pdf, c->d_pdfnd, c->nleg};
   const int nb_leg = (int)blocks_for(c->nleg, 256);
   const dim3 grid(cflux_blocks + nb_leg + blocks_for(ndon, 256)), block(256);
   if (c->sweeps % 2 == 0)
     hipLaunchKernelGGL((k_after_coarse<S, true>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_T, volc,
-                       c->scale, c->inv_scale, r->d_regc, K, c->U, r->F, r->d_after, r->d_err, (int)cflux_blocks, nb_leg);
+                       c->scale, c->inv_scale, tr<S>(r->d_regc), K, c->U, r->F, tr<S>(r->d_after), r->d_err,
+                       (int)cflux_blocks, nb_leg);
   else
     hipLaunchKernelGGL((k_after_coarse<S, false>), grid, block, 0, c->stream, (T_ *)c->pop, c->L, r->B, r->d_T, volc,
-                       c->scale, c->inv_scale, r->d_regc, K, c->U, r->F, r->d_after, r->d_err, (int)cflux_blocks, nb_leg);
+                       c->scale, c->inv_scale, tr<S>(r->d_regc), K, c->U, r->F, tr<S>(r->d_after), r->d_err,
+                       (int)cflux_blocks, nb_leg);
   // x pass of the prolongation for both fine substeps (time fractions 0 and 1/2); 64-thread
   // blocks spread the few thousand items over every CU
-  hipLaunchKernelGGL(k_prolong_x, dim3(blocks_for(r->n_lpx, 64)), dim3(64), 0, c->stream, r->d_before, r->d_after, r->B,
-                     r->d_px, r->d_lpx, r->n_lpx);
+  hipLaunchKernelGGL(k_prolong_x<TR<S>>, dim3(blocks_for(r->n_lpx, 64)), dim3(64), 0, c->stream,
+                     (const TR<S> *)tr<S>(r->d_before), (const TR<S> *)tr<S>(r->d_after), r->B, tr<S>(r->d_px),
+                     r->d_lpx, r->n_lpx);
 }
 
 // fine substep `sub` before its sweep: y and z passes of the time-interpolated prolongation
@@ -200,9 +211,9 @@ template <int S>
 void phase_before_fine(Refined *r, int sub) {
   Lattice *f = r->f;
   using T_ = typename St<S>::T;
-  const double *X = r->d_px + sub * prolong_x_half(r->B);
+  const TR<S> *X = tr<S>(r->d_px) + sub * prolong_x_half(r->B);
   const dim3 gyz(blocks_for(r->n_cols, PZ_COLS));
-  const size_t lds = prolong_yz_lds(r->B);
+  const size_t lds = prolong_yz_lds(r->B, sizeof(TR<S>));
   if (f->sweeps % 2 == 0)
     hipLaunchKernelGGL((k_prolong_yz<S, true>), gyz, dim3(64), lds, f->stream, X, (T_ *)f->pop, f->L, r->B, r->V, f->scale, r->d_err, r->d_cols, r->n_cols);
   else
@@ -215,7 +226,7 @@ template <int S>
 void phase_after_fine(Refined *r, int sub) {
   Lattice *f = r->f, *c = r->c;
   using T_ = typename St<S>::T;
-  const double volf = r->dxf * r->dxf * r->dxf;
+  const TR<S> volf = TR<S>(r->dxf * r->dxf * r->dxf);
   const unsigned fflux_blocks = blocks_for(4LL * (2 * r->nt) * f->nz * FLUX_LANES, 256);
   WallLinks W{f->d_wxy, f->d_wdir, f->d_wq, f->d_welem, f->nlinks, f->nelem, f->rx_off, f->ry_off, r->dxf};
   double *part = r->d_part + (size_t)sub * r->wall_blocks * f->nelem * 5;
@@ -226,7 +237,7 @@ void phase_after_fine(Refined *r, int sub) {
   const bool pf = f->sweeps % 2 == 0, pc = c->sweeps % 2 == 0;
 #define AK_AF(PF, PC)                                                                                              \
   hipLaunchKernelGGL((k_after_fine<S, PF, PC>), grid, block, 0, f->stream, (T_ *)f->pop, f->L, r->B, r->d_T, volf, \
-                     f->scale, f->inv_scale, sub, r->d_regf, W, part, (int)fflux_blocks, nb_walls,                 \
+                     f->scale, f->inv_scale, sub, tr<S>(r->d_regf), W, part, (int)fflux_blocks, nb_walls,          \
                      (const T_ *)c->pop, c->L, c->inv_scale, r->F, r->d_opart)
   if (pf) { if (pc) AK_AF(true, true); else AK_AF(true, false); }
   else { if (pc) AK_AF(false, true); else AK_AF(false, false); }
@@ -253,8 +264,9 @@ void phase_end(Refined *r, bool sample) {
   const bool pf = f->sweeps % 2 == 0, pc = c->sweeps % 2 == 0;
 #define AK_END(PF, PC)                                                                                             \
   hipLaunchKernelGGL((k_step_end<S, PF, PC>), grid, block, 0, c->stream, (const T_ *)f->pop, f->L, f->inv_scale,  \
-                     (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, c->inv_scale, r->d_before, r->d_regc,        \
-                     r->d_regf, inv_volc, r->d_err, (int)cwave_blocks, ntiles, (const double *)r->d_part, nparts, \
+                     (T_ *)c->pop, c->L, r->B, r->V, r->F, c->scale, c->inv_scale, tr<S>(r->d_before),            \
+                     tr<S>(r->d_regc), tr<S>(r->d_regf), TR<S>(inv_volc), r->d_err, (int)cwave_blocks, ntiles,    \
+                     (const double *)r->d_part, nparts,                                                           \
                      f->nelem, fscale, volf, c->d_ctl, c->d_hist, sample ? 1 : 0, G)
   const BudgetArgs G{r->d_opart, r->nb_outer, r->dxc * r->dxc * r->dxc, r->dxc / r->dtc, r->d_acc};
   if (pf) { if (pc) AK_END(true, true); else AK_END(true, false); }
