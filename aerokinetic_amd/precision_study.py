@@ -1,22 +1,30 @@
 """Storage-precision study on the 1536x1024x64 lattice (~100.7M cells).
 
-    python -m aerokinetic_amd.precision_study --steps 20000 --average-last 10000 \
-        --out gpurun_out/precision_c3.json
+    python -m aerokinetic_amd.precision_study --farfield exterior_regularized_fd_v1 \
+        --steps 20000 --average-last 10000 --out gpurun_out/precision_c3.json
 
-NACA0012, chord 128 cells at 8 deg, Re 1e5, Ma 0.1, z periodic, characteristic farfield,
-seeded perturbation (aerokinetic_amd.cases.NACA0012_C128_1536x1024x64).  The same case runs
-with populations stored in FP64, FP32 and FP16 shifted deviations g = f - w (FP32 collision
-arithmetic for FP32/FP16), loads recorded every step.  Reported per storage:
+NACA0012, chord 128 cells at 8 deg, Re 1e5, Ma 0.1, z periodic, seeded perturbation
+(aerokinetic_amd.cases.NACA0012_C128_1536x1024x64; farfield chosen by --farfield, the
+characteristic one by default).  The same case runs with populations stored in FP64, FP32 and
+FP16 shifted deviations g = f - w (FP32 collision arithmetic for FP32/FP16), loads recorded
+every step.  Reported per storage:
 
 * Cl/Cd/Cm averaged over the last `--average-last` steps, and their relative error against
   the FP64-storage run (the FP64 GPU path, itself matched per population to the FP64 oracle
   at smaller sizes);
+* the statistical uncertainty of those averages: the standard error of the mean from batch
+  means (`--batches` equal batches of the averaging window), and the difference of each mean
+  from the FP64 one in units of the two runs' combined standard error;
+* the relative error of the means over fixed windows (1-1000, ..., 15001-20000) and the step
+  where a run leaves the FP64 trajectory (|d Cl| > 0.01, about 1 % of the mean Cl);
 * the field L2 error of rho and u at the last step against the FP64 run, over the fluid
   cells: ||d rho|| / ||rho||, ||d rho|| / ||rho - 1|| and ||d u|| / ||u||;
 * the FP64 mass check: the density sum over the fluid cells at the start and at the end;
 * the low-Mach guards (local Mach, max |rho - 1|) of the last step, and the step rate.
 
-One progress line is printed every `--progress` steps.
+One progress line is printed every `--progress` steps.  With `--out` the loads of every step
+are saved beside the report (`*_loads.npz`); `--reanalyse REPORT.json` recomputes the load
+statistics from that file on the host (no GPU) and rewrites the report.
 """
 from __future__ import annotations
 
@@ -40,7 +48,7 @@ def guard_failure(rec):
     return None
 
 
-def run_storage(case, storage, steps, average_last, progress):
+def run_storage(case, storage, steps, average_last, progress, batches=10):
     import numpy as np
     import torch
     from .solver import Solver
@@ -76,8 +84,9 @@ def run_storage(case, storage, steps, average_last, progress):
     out = {"storage": storage, "farfield": grid.farfield, "steps": s.steps, "cells": cells, "fluid_cells": nfluid,
            "build_s": t_build, "wall_s": wall, "ms_per_step": wall / done * 1e3, "mlups": cells * done / wall / 1e6,
            "guards_failed": failure, "loads": loads,
-           "mean_last": dict(zip(("cl", "cd", "cm"), map(float, tail.mean(axis=0)))),
-           "rms_last": dict(zip(("cl", "cd", "cm"), map(float, tail.std(axis=0)))),
+           "mean_last": dict(zip(COEFS, map(float, tail.mean(axis=0)))),
+           "rms_last": dict(zip(COEFS, map(float, tail.std(axis=0)))),
+           "stderr_last": dict(zip(COEFS, map(float, mean_stderr(tail, batches)))),
            "averaged_steps": [int(len(loads) - len(tail) + 1), int(len(loads))],
            "mass_start": start["density_mean"] * nfluid}
     fields = None
@@ -95,7 +104,52 @@ def run_storage(case, storage, steps, average_last, progress):
     return out, fields
 
 
-def compare(result, fields, ref, ref_fields, average_last):
+COEFS = ("cl", "cd", "cm")
+
+
+def mean_stderr(loads, batches):
+    """Standard error of the mean of each column of `loads` from `batches` equal batch means
+    (trailing rows that do not fill a batch are left out)."""
+    import numpy as np
+    n = len(loads) // batches * batches
+    means = loads[len(loads) - n:].reshape(batches, -1, loads.shape[1]).mean(axis=1)
+    return means.std(axis=0, ddof=1) / np.sqrt(batches)
+
+
+def departure_step(a, b, tol=1e-2):
+    """First step (1-based) where the Cl of two runs differ by more than `tol`, or None."""
+    import numpy as np
+    n = min(len(a), len(b))
+    off = np.nonzero(np.abs(a[:n, 0] - b[:n, 0]) > tol)[0]
+    return int(off[0]) + 1 if len(off) else None
+
+
+def load_statistics(loads, ref_loads, average_last, batches):
+    """The load-based comparison of a run with the FP64 run (both as [steps, 3] arrays)."""
+    import numpy as np
+    out = {}
+    n = min(len(loads), len(ref_loads))
+    a, b = loads[:n][-average_last:], ref_loads[:n][-average_last:]
+    ma, mb = a.mean(axis=0), b.mean(axis=0)
+    sa, sb = mean_stderr(a, batches), mean_stderr(b, batches)
+    out["common_window"] = [int(n - min(n, average_last) + 1), int(n)]
+    out["mean_coef_common_window"] = {k: [float(ma[i]), float(mb[i])] for i, k in enumerate(COEFS)}
+    out["mean_coef_rel_err_vs_fp64"] = {k: float(abs(ma[i] - mb[i]) / abs(mb[i])) for i, k in enumerate(COEFS)}
+    out["stderr_of_mean"] = {k: [float(sa[i]), float(sb[i])] for i, k in enumerate(COEFS)}
+    out["difference_in_combined_stderr"] = {k: float(abs(ma[i] - mb[i]) / np.hypot(sa[i], sb[i]))
+                                            for i, k in enumerate(COEFS)}
+    out["leaves_fp64_trajectory_at_step"] = departure_step(loads, ref_loads)
+    # shorter windows: the start-up, where the runs follow one trajectory, and later blocks
+    windows = {}
+    for lo, hi in ((1, 1000), (1, 2000), (1, 5000), (5001, 10000), (10001, 15000), (15001, 20000)):
+        if hi <= n:
+            x, y = loads[lo - 1:hi].mean(axis=0), ref_loads[lo - 1:hi].mean(axis=0)
+            windows[f"{lo}-{hi}"] = {k: float(abs(x[i] - y[i]) / abs(y[i])) for i, k in enumerate(COEFS)}
+    out["mean_coef_rel_err_by_window"] = windows
+    return out
+
+
+def compare(result, fields, ref, ref_fields, average_last, batches=10):
     """Field L2 at the last step (both runs complete) and the coefficients averaged over the
     last `average_last` steps that both runs completed, against the FP64 run."""
     import numpy as np
@@ -107,19 +161,27 @@ def compare(result, fields, ref, ref_fields, average_last):
                                    "rho_minus_1_rel": float(np.linalg.norm(dr) / np.linalg.norm(rb[fl] - 1.0)),
                                    "u_rel": float(np.linalg.norm(du) / np.linalg.norm(ub[fl])),
                                    "rho_max_abs": float(np.abs(dr).max()), "u_max_abs": float(np.abs(du).max())}
-    n = min(len(result["loads"]), len(ref["loads"]))
-    a, b = result["loads"][:n][-average_last:].mean(axis=0), ref["loads"][:n][-average_last:].mean(axis=0)
-    out["common_window"] = [int(n - min(n, average_last) + 1), int(n)]
-    out["mean_coef_common_window"] = {k: [float(a[i]), float(b[i])] for i, k in enumerate(("cl", "cd", "cm"))}
-    out["mean_coef_rel_err_vs_fp64"] = {k: float(abs(a[i] - b[i]) / abs(b[i])) for i, k in enumerate(("cl", "cd", "cm"))}
-    # shorter windows: the start-up, where the runs follow one trajectory, and later blocks
-    windows = {}
-    for lo, hi in ((1, 1000), (1, 2000), (1, 5000), (5001, 10000), (10001, 15000), (15001, 20000)):
-        if hi <= n:
-            x, y = result["loads"][lo - 1:hi].mean(axis=0), ref["loads"][lo - 1:hi].mean(axis=0)
-            windows[f"{lo}-{hi}"] = {k: float(abs(x[i] - y[i]) / abs(y[i])) for i, k in enumerate(("cl", "cd", "cm"))}
-    out["mean_coef_rel_err_by_window"] = windows
+    out.update(load_statistics(result["loads"], ref["loads"], average_last, batches))
     return out
+
+
+def reanalyse(report_path, batches):
+    """Recompute the load statistics of a saved report from its `*_loads.npz` (host only)."""
+    import numpy as np
+    with open(report_path, encoding="utf-8") as f:
+        report = json.load(f)
+    loads = np.load(report_path.rsplit(".", 1)[0] + "_loads.npz")
+    avg = report["average_last"]
+    ref = loads["fp64"]
+    for st, res in report["by_storage"].items():
+        tail = loads[st][-avg:]
+        res["stderr_last"] = dict(zip(COEFS, map(float, mean_stderr(tail, batches))))
+        if st != "fp64":
+            res.update(load_statistics(loads[st], ref, avg, batches))
+    report["batches"] = batches
+    with open(report_path, "w", encoding="utf-8") as f:
+        json.dump(report, f, indent=1)
+    return report
 
 
 def main(argv=None):
@@ -131,8 +193,14 @@ def main(argv=None):
     ap.add_argument("--progress", type=int, default=1000)
     ap.add_argument("--farfield", default="characteristic_transport_v1",
                     choices=("characteristic_transport_v1", "exterior_regularized_fd_v1", "legacy"))
+    ap.add_argument("--batches", type=int, default=10, help="batch means for the standard error of an average")
     ap.add_argument("--out", default=None, help="JSON report path")
+    ap.add_argument("--reanalyse", default=None, metavar="REPORT.json",
+                    help="recompute the load statistics of a saved report from its loads file (no GPU)")
     a = ap.parse_args(argv)
+    if a.reanalyse:
+        print(json.dumps(reanalyse(a.reanalyse, a.batches)))
+        return 0
     storages = [s for s in a.storages.split(",") if s]
     if storages[0] != "fp64":
         raise SystemExit("the first storage must be fp64 (the reference run)")
@@ -141,15 +209,15 @@ def main(argv=None):
         from dataclasses import replace
         case = replace(case, name=case.name + " " + a.farfield, farfield=a.farfield)
     report = {"workload": f"{case.name}, {a.farfield} farfield, loads every step", "steps": a.steps,
-              "average_last": a.average_last, "by_storage": {}}
+              "average_last": a.average_last, "batches": a.batches, "by_storage": {}}
     ref = ref_fields = None
     for st in storages:
         _log(f"{case.name}: {st}, {a.steps} steps")
-        res, fields = run_storage(case, st, a.steps, a.average_last, a.progress)
+        res, fields = run_storage(case, st, a.steps, a.average_last, a.progress, a.batches)
         if ref is None:
             ref, ref_fields = res, fields
         else:
-            res.update(compare(res, fields, ref, ref_fields, a.average_last))
+            res.update(compare(res, fields, ref, ref_fields, a.average_last, a.batches))
             del fields
         report["by_storage"][st] = res
         _log(json.dumps({k: v for k, v in res.items() if k != "loads"}))
