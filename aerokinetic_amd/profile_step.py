@@ -9,7 +9,8 @@ farfield), `uniform-100m` (1536x1024x64) and
 `two-level` (coarse 256x192x8 + fine 160x96x16), all NACA0012 (see
 aerokinetic_amd.cases); `copy`, a device-to-device copy of a buffer of known size
 (the calibration kernel for memory counters); `serve`, the batched service over
-seeded two-level requests (--steps = the per-case step cap); `slab-y` (1536x128x64) and
+seeded two-level requests (--steps = the per-case step cap); `batch`, a fixed batch of
+--cases seeded two-level serving requests (coarse chord 16-24) stepped together; `slab-y` (1536x128x64) and
 `slab-x` (192x1024x64), one GPU's share of the 1536x1024x64 lattice at 8 GPUs as a plain
 fluid slab (sweep kernel only); `traffic`, the streaming copy of a known size and one
 sweep of each AA form of the 1536x1024x64 lattice per storage, for memory-counter runs.  Each workload prints one
@@ -131,6 +132,39 @@ def run_serve(storage, cases, rate, chunk, window, max_steps):
          f"batch steps {rep.gpu_steps}, serving-thread seconds {s['host_s']}")
 
 
+def run_batch(storage, ncases, steps, warmup, chunk=50):
+    """A fixed batch of `ncases` two-level requests (coarse chord 16-24, span 8, Re 1e4-3e4;
+    seed 2026) stepped together in chunks of `chunk` coarse steps; wall time per batch step."""
+    import torch
+    from . import serving as S
+    from .batch import RefinedBatch
+    from .refined import RefinedSolver
+    reqs = S.polar_requests(ncases, seed=2026, chord=(16, 24), span=(8, 8), reynolds=(1e4, 3e4))
+    members = []
+    for r in reqs:
+        project, settings, plan = S.project_of(r)
+        members.append(RefinedSolver(project, settings, plan=plan, storage=storage))
+    b = RefinedBatch(storage)
+    for m in members:
+        b.add(m)
+    done = 0
+    while done < warmup:
+        b.advance(min(chunk, warmup - done))
+        done += min(chunk, warmup - done)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        b.advance(min(chunk, steps - done))
+        done += min(chunk, steps - done)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    _log(f"batch of {ncases} {storage}: {dt / steps * 1e3:.4f} ms per batch step over {steps} steps")
+    b.close()
+    for m in members:
+        m.close()
+
+
 def run_slab(shape, storage, steps, warmup):
     """The sweep of one GPU's share of the 1536x1024x64 lattice at 8 GPUs: a plain fluid
     slab (no body, seeded field, legacy farfield), the sweep kernel timed with its own events."""
@@ -158,7 +192,7 @@ def main(argv=None):
     from . import cases
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--workload", choices=("uniform-c32", "exterior-c32", "uniform-100m", "two-level", "copy", "serve",
-                                           "slab-y", "slab-x", "traffic"), required=True)
+                                           "batch", "slab-y", "slab-x", "traffic"), required=True)
     ap.add_argument("--storages", default="fp64,fp16", help="traffic: storages of the 1536x1024x64 sweeps")
     ap.add_argument("--storage", choices=("fp64", "fp32", "fp16"), default="fp64")
     ap.add_argument("--steps", type=int, default=10)
@@ -166,7 +200,7 @@ def main(argv=None):
     ap.add_argument("--sample-every", type=int, default=1)
     ap.add_argument("--copy-gib", type=float, default=4.0)
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly")
-    ap.add_argument("--cases", type=int, default=24, help="serve: number of requests")
+    ap.add_argument("--cases", type=int, default=24, help="serve / batch: number of requests")
     ap.add_argument("--rate", type=float, default=16.0, help="serve: Poisson arrival rate (1/s)")
     ap.add_argument("--chunk", type=int, default=100, help="serve: coarse steps per batch call")
     a = ap.parse_args(argv)
@@ -174,6 +208,8 @@ def main(argv=None):
         run_copy(a.copy_gib, a.steps)
     elif a.workload == "traffic":
         run_traffic([x for x in a.storages.split(",") if x], a.copy_gib)
+    elif a.workload == "batch":
+        run_batch(a.storage, a.cases, a.steps, a.warmup)
     elif a.workload == "serve":
         run_serve(a.storage, a.cases, a.rate, a.chunk, 400, a.steps)
     elif a.workload in ("slab-y", "slab-x"):
