@@ -67,7 +67,11 @@ struct Lattice {
   int32_t *d_ffg = nullptr, *d_ffz = nullptr, *d_ffmask = nullptr;
   int8_t *d_ffobj = nullptr;
   double *d_inc = nullptr, *d_der = nullptr, *d_rows = nullptr;
-  uint8_t *d_fc = nullptr;  // x-face cache of the characteristic farfield (storage type)
+  // perimeter moments of the characteristic farfield (compute type): written by every sweep
+  // and part of the restartable state; computed from the populations when the state was set
+  // from outside (pm_valid false).  Used while no perimeter cell is solid.
+  uint8_t *d_pm = nullptr;
+  bool perim_fluid = true, pm_valid = false;
   long long *d_clock = nullptr;
   int nleg = 0;
   int32_t *d_legx = nullptr, *d_legz = nullptr;
@@ -170,6 +174,29 @@ void set_geom(Lattice *s, int bx, int by) { s->geom = make_geom(s->nx, s->ny, s-
 
 // ---------------------------------------------------------------- launches
 
+// The perimeter moments the sweep keeps and the farfield reads, or null (no characteristic
+// farfield, or a solid cell on the perimeter: the farfield then reads the lattice).
+template <int S>
+typename St<S>::C *perimeter_moments(Lattice *s) {
+  const bool on = s->ff_kind == AK_FARFIELD_CHARACTERISTIC && s->nff && s->d_pm && s->perim_fluid;
+  return on ? (typename St<S>::C *)s->d_pm : nullptr;
+}
+
+// Perimeter moments of the current populations, when the state was set from outside since
+// the last sweep (a sweep writes them for the state it leaves).
+template <int S>
+void fill_perimeter(Lattice *s) {
+  using T = typename St<S>::T;
+  typename St<S>::C *pm = perimeter_moments<S>(s);
+  if (!pm || s->pm_valid) return;
+  s->pm_valid = true;
+  const dim3 g(blocks_for(2LL * (s->ny + s->nx) * s->nz, 256)), b(256);
+  if (s->sweeps % 2 == 0)
+    hipLaunchKernelGGL((k_perimeter_fill<S, true>), g, b, 0, s->stream, (const T *)s->pop, s->L, pm, s->inv_scale);
+  else
+    hipLaunchKernelGGL((k_perimeter_fill<S, false>), g, b, 0, s->stream, (const T *)s->pop, s->L, pm, s->inv_scale);
+}
+
 // With e0/e1 the launch records the kernel's own start and end on those events
 // (hipExtLaunchKernel), so the measured time is the sweep alone.
 template <int S>
@@ -182,14 +209,15 @@ void launch_sweep(Lattice *s, bool odd, bool diag, hipEvent_t e0 = nullptr, hipE
   const SweepGeom &G = pxy && s->geom.bx * s->geom.by > 256 ? s->geom_small : s->geom;
   const dim3 grid(G.nblocks), block(G.bx * G.by);
   const C ws = (C)s->ws, wb = (C)s->wb;
+  C *pm = perimeter_moments<S>(s);
 #define AK_SWEEP(O, P, D)                                                                                           \
   do {                                                                                                              \
     if (e0)                                                                                                         \
       hipExtLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, e0, e1, 0, A, s->d_solid, s->L, G, ws, \
-                            wb, s->scale, s->inv_scale, s->d_diag);                                                 \
+                            wb, s->scale, s->inv_scale, s->d_diag, pm);                                             \
     else                                                                                                            \
       hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, A, s->d_solid, s->L, G, ws, wb, s->scale, \
-                         s->inv_scale, s->d_diag);                                                                  \
+                         s->inv_scale, s->d_diag, pm);                                                              \
   } while (0)
   if (odd) {
     if (pxy) { if (diag) AK_SWEEP(true, true, true); else AK_SWEEP(true, true, false); }
@@ -207,31 +235,23 @@ void launch_boundary(Lattice *s, bool loads) {
   T *A = (T *)s->pop;
   const bool charff = s->ff_kind == AK_FARFIELD_CHARACTERISTIC && s->nff;
   const bool leg = (s->ff_kind == AK_FARFIELD_LEGACY || s->ff_kind == AK_FARFIELD_EXTERIOR) && s->nleg;
-  if (s->nlinks && s->walls_clear && ((charff && s->d_fc) || leg)) {
-    // walls clear of the edges: one launch for the walls and the edge phase that does not
-    // depend on them (the x-face copy, or the legacy links), then the farfield
+  const FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff,
+                  perimeter_moments<S>(s)};
+  if (s->nlinks && s->walls_clear && (charff || leg)) {
+    // walls clear of the edges: the walls and the edge phase (farfield items or legacy
+    // links) in one launch
     WallLinks W{s->d_wxy, s->d_wdir, s->d_wq, s->d_welem, s->nlinks, s->nelem, s->rx_off, s->ry_off, 1.0};
     LegacyLinks K{s->d_legx, s->d_legz, s->d_legdir, s->d_legkind, s->d_pdf, s->d_pdfnd, s->nleg};
-    const long long nb_edge = charff ? blocks_for(2LL * s->ny * s->nz, 256) : blocks_for(s->nleg, 256);
+    const long long nb_edge = charff ? blocks_for(s->nff, 256) : blocks_for(s->nleg, 256);
     const dim3 g((unsigned)(s->wall_blocks + nb_edge)), b(256);
-    T *fc = (T *)s->d_fc;
 #define AK_WE(LD, E) hipLaunchKernelGGL((k_walls_edge<S, PE, LD, E>), g, b, 0, s->stream, A, s->L, W, s->scale, \
-                                        s->inv_scale, s->d_partials, s->wall_blocks, K, s->U, fc)
+                                        s->inv_scale, s->d_partials, s->wall_blocks, K, s->U, I, s->ffp)
     if (charff) { if (loads) AK_WE(true, 1); else AK_WE(false, 1); }
     else { if (loads) AK_WE(true, 0); else AK_WE(false, 0); }
 #undef AK_WE
-    if (charff) {
-      FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff, s->d_fc};
-      hipLaunchKernelGGL((k_farfield<S, PE>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, A, s->L, I,
-                         s->ffp, s->scale, s->inv_scale);
-    }
     return;
   }
   if (charff) {
-    if (s->d_fc)
-      hipLaunchKernelGGL((k_face_fill<S, PE>), dim3(blocks_for(2LL * s->ny * s->nz, 256)), dim3(256), 0, s->stream,
-                         (const T *)A, s->L, (T *)s->d_fc);
-    FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff, s->d_fc};
     hipLaunchKernelGGL((k_farfield<S, PE>), dim3(blocks_for(s->nff, 128)), dim3(128), 0, s->stream, A, s->L, I,
                        s->ffp, s->scale, s->inv_scale);
   } else if ((s->ff_kind == AK_FARFIELD_LEGACY || s->ff_kind == AK_FARFIELD_EXTERIOR) && s->nleg) {
@@ -639,6 +659,7 @@ int ak_bind_populations(void *handle, void *device_ptr, size_t bytes) {
   if (bytes < s->pop_bytes) return fail(s, "population buffer of %zu bytes < required %zu", bytes, s->pop_bytes);
   if (((uintptr_t)device_ptr) % 128) return fail(s, "population buffer must be 128-byte aligned");
   s->pop = device_ptr;
+  s->pm_valid = false;
   HIPCHK(s, hipSetDevice(s->dev));
   // ghosts and padding start at zero
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s->stream, (uint8_t *)s->pop, (long long)s->pop_bytes, (uint8_t)0);
@@ -653,6 +674,12 @@ int ak_set_solid(void *handle, const uint8_t *mask_xy) {
   std::vector<uint8_t> m((size_t)(s->nx + 2) * (s->ny + 2), 0);
   for (int x = 0; x < s->nx; ++x)
     for (int y = 0; y < s->ny; ++y) m[(size_t)(y + 1) * (s->nx + 2) + (x + 1)] = mask_xy[(size_t)x * s->ny + y] > 2 ? 1 : mask_xy[(size_t)x * s->ny + y];
+  s->perim_fluid = true;
+  s->pm_valid = false;
+  for (int x = 0; x < s->nx; ++x)
+    for (int y = 0; y < s->ny; ++y)
+      if ((x == 0 || y == 0 || x == s->nx - 1 || y == s->ny - 1) && m[(size_t)(y + 1) * (s->nx + 2) + (x + 1)] == 1)
+        s->perim_fluid = false;
   HIPCHK(s, hipSetDevice(s->dev));
   HIPCHK(s, hipMemcpy(s->d_solid, m.data(), m.size(), hipMemcpyHostToDevice));
   return 0;
@@ -700,6 +727,7 @@ int ak_init_seeded(void *handle, const double *p) {
   if (check_ready(s)) return -1;
   s->gen++;  // invalidates captured step graphs
   SeedParams sp{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8], p[9], p[10]};
+  s->pm_valid = false;
   const long long n = (long long)s->nx * s->ny * s->nz;
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s->stream, (uint8_t *)s->pop, (long long)s->pop_bytes, (uint8_t)0);
   switch (s->storage) {
@@ -726,8 +754,9 @@ int ak_set_farfield(void *handle, int kind, double U) {
     return fail(s, "the exterior farfield needs a padded lattice of at least 6 x 6 x 4");
   s->ff_kind = kind;
   s->U = U;
-  dfree(s, s->d_fc);
-  s->d_fc = nullptr;
+  dfree(s, s->d_pm);
+  s->d_pm = nullptr;
+  s->pm_valid = false;
   const int nx = s->nx, ny = s->ny, nz = s->nz;
   if (kind == AK_FARFIELD_CHARACTERISTIC) {
     // objects in the order normals (-1,-1),(-1,0),(-1,1),(0,-1),(0,1),(1,-1),(1,0),(1,1)
@@ -795,7 +824,7 @@ int ak_set_farfield(void *handle, int kind, double U) {
     if (upload(s, &s->d_inc, zeros) || upload(s, &s->d_der, zeros) || upload(s, &s->d_clock, zc)) return -1;
     std::vector<double> rows((size_t)2 * nz * nx, 0.0);
     if (upload(s, &s->d_rows, rows)) return -1;
-    if (nx >= 2 && dalloc(s, &s->d_fc, (size_t)2 * NQ * ny * nz * s->elem_size)) return -1;
+    if (dalloc(s, &s->d_pm, pm_count(nx, ny, nz) * (s->storage == 0 ? sizeof(double) : sizeof(float)))) return -1;
     FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff, nullptr};
     const long long nr = 2LL * nz * nx;
     switch (s->storage) {
@@ -867,7 +896,10 @@ int ak_set_sweep_tile(void *handle, int bx, int by) {
 int ak_run(void *handle, int nsteps, long long first_step, int sample_every) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
-  if (nsteps > 0) set_ctl(s, first_step + 1, s->hist_rows);
+  if (nsteps > 0) {
+    set_ctl(s, first_step + 1, s->hist_rows);
+    AK_DISPATCH(s, fill_perimeter, s);
+  }
   for (int k = 0; k < nsteps; ++k) {
     const long long number = first_step + k + 1;
     const bool sample = sample_every > 0 && number % sample_every == 0;
@@ -909,7 +941,10 @@ int ak_run_profiled(void *handle, int nsteps, long long first_step, int sample_e
   if (check_ready(s)) return -1;
   std::vector<hipEvent_t> ev(4 * (size_t)nsteps);
   for (auto &e : ev) HIPCHK(s, hipEventCreate(&e));
-  if (nsteps > 0) set_ctl(s, first_step + 1, s->hist_rows);
+  if (nsteps > 0) {
+    set_ctl(s, first_step + 1, s->hist_rows);
+    AK_DISPATCH(s, fill_perimeter, s);
+  }
   for (int k = 0; k < nsteps; ++k) {
     const long long number = first_step + k + 1;
     const bool sample = sample_every > 0 && number % sample_every == 0;
@@ -978,6 +1013,7 @@ int ak_set_populations(void *handle, const double *host_in) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
   s->gen++;  // invalidates captured step graphs
+  s->pm_valid = false;
   const size_t n = (size_t)s->nx * s->ny * s->nz * 27;
   double *d = nullptr;
   HIPCHK(s, hipMalloc((void **)&d, n * sizeof(double)));
@@ -1112,7 +1148,7 @@ struct StateHeader {
   long long sweeps;
   unsigned long long pop_bytes;
 };
-constexpr int STATE_VERSION = 3;
+constexpr int STATE_VERSION = 4;
 
 struct StatePart {
   void *dev;
@@ -1126,6 +1162,8 @@ std::vector<StatePart> state_parts(Lattice *s, bool with_prev) {
     p.push_back({s->d_der, (size_t)s->nff * sizeof(double)});
     p.push_back({s->d_clock, (size_t)s->nff * sizeof(long long)});
     p.push_back({s->d_rows, (size_t)2 * s->nz * s->nx * sizeof(double)});
+    if (s->d_pm)
+      p.push_back({s->d_pm, pm_count(s->nx, s->ny, s->nz) * (s->storage == 0 ? sizeof(double) : sizeof(float))});
   }
   if (s->ff_kind == AK_FARFIELD_LEGACY && s->nleg) {
     p.push_back({s->d_pdf, (size_t)s->nleg * sizeof(double)});
@@ -1184,6 +1222,7 @@ int ak_save_state(void *handle, void *host, size_t bytes) {
   Lattice *s = H(handle);
   if (check_ready(s)) return -1;
   if (!host || bytes < ak_state_bytes(handle)) return fail(s, "state buffer of %zu bytes < %zu", bytes, ak_state_bytes(handle));
+  AK_DISPATCH(s, fill_perimeter, s);  // the saved perimeter moments are those of the saved populations
   HIPCHK(s, hipStreamSynchronize(s->stream));
   const StateHeader h = state_header(s);
   char *o = (char *)host;
@@ -1221,6 +1260,7 @@ int ak_load_state(void *handle, const void *host, size_t bytes) {
     i += p.bytes;
   }
   s->sweeps = h.sweeps;
+  s->pm_valid = s->d_pm != nullptr;
   return 0;
 }
 

@@ -104,9 +104,9 @@ __device__ __forceinline__ long long loc_in(const Desc &L, int x, int y, int z, 
 // Everything is carried as deviations from the rest state (central moments
 // of f minus those of w about the same velocity), so no O(1) quantity is
 // subtracted from another: the arithmetic keeps the precision of g itself.
-// Returns drho and |u|^2 for the fused diagnostics.
+// Returns drho and |u|^2 for the fused diagnostics, and u (conserved by the collision).
 template <typename C>
-__device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out) {
+__device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out, C (&u_out)[3]) {
   // forward: raw moments up to second order by three one-dimensional passes
   C Z0[9], Z1[9], Z2[9];
 #pragma unroll
@@ -212,6 +212,12 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
   }
   drho_out = drho;
   u2_out = ux2 + uy2 + uz2;
+  u_out[0] = ux; u_out[1] = uy; u_out[2] = uz;
+}
+template <typename C>
+__device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out) {
+  C u[3];
+  collide<C>(g, ws, wb, drho_out, u2_out, u);
 }
 
 // FP16 storage of a cell's 27 post-collision populations that keeps its mass and momentum.
@@ -242,6 +248,29 @@ __device__ __forceinline__ void conserve_fp16(typename St<S>::C (&g)[NQ], float 
       if (cxq(i) == 0 || cyq(i) == 0 || czq(i) == 0) continue;
       g[i] -= 0.125f * (dm + (float)cxq(i) * dx + (float)cyq(i) * dy + (float)czq(i) * dz);
     }
+  }
+}
+
+// The FP16 values conserve_fp16 followed by the storage rounding stores, computed in units of
+// the scaled value s = g * scale (scale is a power of two, so each step gives the same
+// result in either unit and the rounding of each direction is done once).
+__device__ __forceinline__ void fp16_conserving(const float (&g)[NQ], float scale, _Float16 (&h)[NQ]) {
+  float dm = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    if (cxq(i) != 0 && cyq(i) != 0 && czq(i) != 0) continue;
+    const float si = g[i] * scale;
+    h[i] = (_Float16)si;
+    const float d = (float)h[i] - si;
+    dm += d;
+    if (cxq(i)) dx += (float)cxq(i) * d;
+    if (cyq(i)) dy += (float)cyq(i) * d;
+    if (czq(i)) dz += (float)czq(i) * d;
+  }
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    if (cxq(i) == 0 || cyq(i) == 0 || czq(i) == 0) continue;
+    h[i] = (_Float16)(g[i] * scale - 0.125f * (dm + (float)cxq(i) * dx + (float)cyq(i) * dy + (float)czq(i) * dz));
   }
 }
 

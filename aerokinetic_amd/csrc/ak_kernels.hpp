@@ -69,13 +69,68 @@ __device__ __forceinline__ void st_g(gchar *p, typename St<S>::C v, float scale)
   else *(gT *)p = v;
 }
 
+// Density deviation and velocity of one cell from its f_out(x, j), j = 0..26 in order
+// (`load(j)`), so that every reader of a cell's moments gets bit-identical values.
+template <typename C, class Load>
+__device__ __forceinline__ void macro_of(Load &&load, C &drho, C (&u)[3]) {
+  C r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) {
+    const C v = load(j);
+    r += v;
+    if (cxq(j)) jx += cxq(j) * v;
+    if (cyq(j)) jy += cyq(j) * v;
+    if (czq(j)) jz += czq(j) * v;
+  }
+  drho = r;
+  const C rho = C(1) + r;
+  u[0] = jx / rho; u[1] = jy / rho; u[2] = jz / rho;
+}
+
+template <int S, bool PE>
+__device__ __forceinline__ void macro_cell(const typename St<S>::T *A, const Desc &L, int x, int y, int z,
+                                           float inv_scale, typename St<S>::C &drho, typename St<S>::C (&u)[3]) {
+  macro_of([&](int j) { return ld<S>(A + loc_out<PE>(L, x, y, z, j), inv_scale); }, drho, u);
+}
+
+// Perimeter moments of the characteristic farfield: the density deviation and velocity
+// (macro_of of the cell's f_out, in the collision's compute type) of every cell on the x
+// faces (x = 0 and nx - 1: [z][side][c][y]) and on the y faces (y = 0 and ny - 1:
+// [z][side][c][x] after the x faces), c = drho, ux, uy, uz; side = 0 for the low face.  The
+// sweep writes them for every cell it updates, and the farfield reads four coalesced values
+// per cell instead of 27 populations, which on an x face lie in 27 different 128-byte lines
+// that no other lane of a wave uses.
+__device__ __forceinline__ long long pm_x(const Desc &L, int side, int y, int z) {
+  return (long long)(z * 2 + side) * 4 * L.ny + y;  // component stride L.ny
+}
+__device__ __forceinline__ long long pm_y(const Desc &L, int side, int x, int z) {
+  return 8LL * L.ny * L.nz + (long long)(z * 2 + side) * 4 * L.nx + x;  // component stride L.nx
+}
+__host__ __device__ inline size_t pm_count(int nx, int ny, int nz) { return 8 * ((size_t)ny * nz + (size_t)nx * nz); }
+
+// Writes the perimeter moments of cell (x, y, z).
+template <typename C>
+__device__ __forceinline__ void perimeter_put(C *__restrict__ pm, const Desc &L, int x, int y, int z, C dr,
+                                              const C (&u)[3]) {
+  if (x == 0 || x == L.nx - 1) {
+    C *p = pm + pm_x(L, x != 0, y, z);
+    p[0] = dr; p[L.ny] = u[0]; p[2 * L.ny] = u[1]; p[3 * L.ny] = u[2];
+  }
+  if (y == 0 || y == L.ny - 1) {
+    C *p = pm + pm_y(L, y != 0, x, z);
+    p[0] = dr; p[L.nx] = u[0]; p[2 * L.nx] = u[1]; p[3 * L.nx] = u[2];
+  }
+}
+
 // One tile (block `lb` of the lattice's sweep grid) of the fused AA stream + collide.
-// `slot` picks the diagnostics accumulator slot of this block.
+// `slot` picks the diagnostics accumulator slot of this block; with `pm` the cells on the
+// x and y faces also write their perimeter moments.
 template <int S, bool ODD, bool PXY, bool DIAG>
 __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, const uint8_t *__restrict__ solid,
                                             const Desc &L, const SweepGeom &G, typename St<S>::C ws,
                                             typename St<S>::C wb, float scale, float inv_scale,
-                                            DiagAcc *__restrict__ diag, int lb, int slot) {
+                                            DiagAcc *__restrict__ diag, int lb, int slot,
+                                            typename St<S>::C *__restrict__ pm = nullptr) {
   using C = typename St<S>::C;
   int x, y, z;
   if (G.lin) {
@@ -135,8 +190,11 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         const gchar *bq = uniform_ptr(A + (plane + d));
         g[q] = ld_g<S>(bq + boff, inv_scale);
       }
-      collide<C>(g, ws, wb, drho, u2);
-      conserve_fp16<S>(g, scale, inv_scale);
+      C uc[3];
+      collide<C>(g, ws, wb, drho, u2, uc);
+      // FP16: the mass- and momentum-conserving rounding, each value rounded once
+      _Float16 h[S == 2 ? NQ : 1];
+      if constexpr (S == 2) fp16_conserving(g, scale, h);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         long long d;
@@ -147,8 +205,12 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
           d = (long long)opp(q) * L.QS;
         }
         gchar *bq = uniform_ptr(A + (plane + d));
-        st_g<S>(bq + boff, g[q], scale);
+        if constexpr (S == 2) *(__attribute__((address_space(1))) _Float16 *)(bq + boff) = h[q];
+        else st_g<S>(bq + boff, g[q], scale);
       }
+      // cells on the x and y faces keep their density deviation and velocity (conserved by
+      // the collision) for the next farfield pass
+      if (pm != nullptr && (x == 0 || y == 0 || x == L.nx - 1 || y == L.ny - 1)) perimeter_put<C>(pm, L, x, y, z, drho, uc);
     }
   }
   if constexpr (DIAG) {
@@ -189,9 +251,10 @@ template <int S, bool ODD, bool PXY, bool DIAG>
 __global__ __launch_bounds__(PXY ? 256 : 1024) void k_sweep(typename St<S>::T *__restrict__ A,
                                                             const uint8_t *__restrict__ solid, Desc L, SweepGeom G,
                                                             typename St<S>::C ws, typename St<S>::C wb, float scale,
-                                                            float inv_scale, DiagAcc *__restrict__ diag) {
+                                                            float inv_scale, DiagAcc *__restrict__ diag,
+                                                            typename St<S>::C *__restrict__ pm) {
   sweep_block<S, ODD, PXY, DIAG>(A, solid, L, G, ws, wb, scale, inv_scale, diag, xcd_remap(blockIdx.x, G.nblocks),
-                                 (int)(blockIdx.x % DIAG_SLOTS));
+                                 (int)(blockIdx.x % DIAG_SLOTS), PXY ? nullptr : pm);
 }
 
 // One lattice of a batched sweep: its arguments and the first block of its tiles in the
@@ -512,19 +575,8 @@ struct FFItems {
   long long *clock;
   const double *initial;   // glancing trace rows: [side][z][x], side 0 = y low, 1 = y high
   int n;
-  const void *fc;          // x-face cache of the last sweep (storage type), or null
+  const void *pm;          // perimeter moments of the current state (compute type), or null
 };
-
-// x-face cache: the post-collision populations f_out(x, q) of the cells x = 0 (side 0) and
-// x = nx - 1 (side 1) of every row, stored [z][side][q][y] in the storage type.  In the
-// lattice each such cell's 27 values lie in 27 different 128-byte lines that no other
-// lane of a wave uses; the characteristic farfield of an x face reads every one of them
-// five times (its cell and four transverse neighbours), so one gather pass (thread =
-// (y, side, z), consecutive rows per wave) copies them out first and the farfield reads
-// the copy with coalesced loads.
-__device__ __forceinline__ long long face_index(const Desc &L, int side, int y, int z, int q) {
-  return ((long long)(z * 2 + side) * NQ + q) * L.ny + y;
-}
 
 struct FFObj {
   int n[3];
@@ -536,30 +588,6 @@ struct FFParams {
   FFObj o[8];
   double U, rho0;
 };
-
-// Density deviation and velocity of one cell from its f_out(x, j), j = 0..26 in order
-// (`load(j)`), so that the lattice and the x-face cache give bit-identical moments.
-template <typename C, class Load>
-__device__ __forceinline__ void macro_of(Load &&load, C &drho, C (&u)[3]) {
-  C r = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int j = 0; j < NQ; ++j) {
-    const C v = load(j);
-    r += v;
-    if (cxq(j)) jx += cxq(j) * v;
-    if (cyq(j)) jy += cyq(j) * v;
-    if (czq(j)) jz += czq(j) * v;
-  }
-  drho = r;
-  const C rho = C(1) + r;
-  u[0] = jx / rho; u[1] = jy / rho; u[2] = jz / rho;
-}
-
-template <int S, bool PE>
-__device__ __forceinline__ void macro_cell(const typename St<S>::T *A, const Desc &L, int x, int y, int z,
-                                           float inv_scale, typename St<S>::C &drho, typename St<S>::C (&u)[3]) {
-  macro_of([&](int j) { return ld<S>(A + loc_out<PE>(L, x, y, z, j), inv_scale); }, drho, u);
-}
 
 template <typename C>
 __device__ __forceinline__ C log_rho(C drho, C rho0) {
@@ -604,11 +632,11 @@ __device__ __forceinline__ void ff_unpack(int32_t p, int &gx, int &gy) {
 }
 
 template <int S, bool PE>
-__global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict__ A, Desc L, FFItems I,
-                                                  FFParams Pm, float scale, float inv_scale) {
+__device__ __forceinline__ void farfield_body(typename St<S>::T *__restrict__ A, const Desc &L, const FFItems &I,
+                                              const FFParams &Pm, float scale, float inv_scale, long long t) {
   using C = typename St<S>::C;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= I.n) return;
+  if (t >= I.n) return;
+  const int k = (int)t;
   int gx, gy;
   ff_unpack(I.g[k], gx, gy);
   const int gz = I.gz[k];
@@ -616,14 +644,18 @@ __global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict_
   int foot[3] = {gx - o.n[0], gy - o.n[1], gz};
   const int edge[2] = {foot[0] == 0 ? -1 : (foot[0] == L.nx - 1 ? 1 : 0),
                        foot[1] == 0 ? -1 : (foot[1] == L.ny - 1 ? 1 : 0)};
-  // x faces (normal (+-1, 0, 0)) read their cells, and the transverse neighbours of the
-  // transport (same x), from the x-face cache
-  using T = typename St<S>::T;
-  const T *fc = (o.n[1] == 0 && I.fc != nullptr) ? (const T *)I.fc : nullptr;
-  const int side = o.n[0] > 0 ? 1 : 0;
+  // the foot and the transverse neighbours of the transport are perimeter cells: their
+  // moments come from the perimeter moments when the sweep keeps them
+  const C *pm = (const C *)I.pm;
   auto macro = [&](int x, int y, int z, C &dr, C (&uu)[3]) {
-    if (fc) macro_of([&](int j) { return ld<S>(fc + face_index(L, side, y, z, j), inv_scale); }, dr, uu);
-    else macro_cell<S, PE>(A, L, x, y, z, inv_scale, dr, uu);
+    if (pm) {
+      const bool xf = x == 0 || x == L.nx - 1;
+      const C *p = pm + (xf ? pm_x(L, x != 0, y, z) : pm_y(L, y != 0, x, z));
+      const int st = xf ? L.ny : L.nx;
+      dr = p[0]; uu[0] = p[st]; uu[1] = p[2 * st]; uu[2] = p[3 * st];
+    } else {
+      macro_cell<S, PE>(A, L, x, y, z, inv_scale, dr, uu);
+    }
   };
   C drho, u[3];
   macro(foot[0], foot[1], foot[2], drho, u);
@@ -669,8 +701,7 @@ __global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict_
   for (int d = 0; d < NQ; ++d) {
     if (!((mask >> d) & 1)) continue;
     const int id = opp(d);
-    const C fo = fc ? ld<S>(fc + face_index(L, side, foot[1], foot[2], id), inv_scale)
-                    : ld<S>(A + loc_out<PE>(L, foot[0], foot[1], foot[2], id), inv_scale);
+    const C fo = ld<S>(A + loc_out<PE>(L, foot[0], foot[1], foot[2], id), inv_scale);
     const C val = fo + (equilibrium<C>(id, rhob, ub[0], ub[1], ub[2]) - equilibrium<C>(id, rho, u[0], u[1], u[2]));
     // fluid cell x = g - c_d reads direction id
     const int xz = ((gz - czq(d)) % L.nz + L.nz) % L.nz;
@@ -678,19 +709,41 @@ __global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict_
   }
 }
 
-// Gather pass of the x-face cache (state in form PE); thread = (y, side, z).
 template <int S, bool PE>
-__device__ __forceinline__ void face_fill_body(const typename St<S>::T *__restrict__ A, Desc L,
-                                               typename St<S>::T *__restrict__ fc, long long t) {
-  if (t >= 2LL * L.ny * L.nz) return;
-  const int y = (int)(t % L.ny), side = (int)((t / L.ny) % 2), z = (int)(t / (2LL * L.ny));
-  const int x = side ? L.nx - 1 : 0;
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) fc[face_index(L, side, y, z, q)] = A[loc_out<PE>(L, x, y, z, q)];
+__global__ __launch_bounds__(128) void k_farfield(typename St<S>::T *__restrict__ A, Desc L, FFItems I,
+                                                  FFParams Pm, float scale, float inv_scale) {
+  farfield_body<S, PE>(A, L, I, Pm, scale, inv_scale, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
+
+// Perimeter moments of a state in form PE from the lattice (start of a run call; every
+// sweep then keeps them).  Thread = (y, side, z) of the x faces, then (x, side, z) of the
+// y faces.
 template <int S, bool PE>
-__global__ void k_face_fill(const typename St<S>::T *__restrict__ A, Desc L, typename St<S>::T *__restrict__ fc) {
-  face_fill_body<S, PE>(A, L, fc, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void k_perimeter_fill(const typename St<S>::T *__restrict__ A, Desc L, typename St<S>::C *__restrict__ pm,
+                                 float inv_scale) {
+  using C = typename St<S>::C;
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nxf = 2LL * L.ny * L.nz;
+  int x, y, z;
+  bool xface;
+  if (t < nxf) {
+    y = (int)(t % L.ny);
+    x = (int)((t / L.ny) % 2) ? L.nx - 1 : 0;
+    z = (int)(t / (2LL * L.ny));
+    xface = true;
+  } else {
+    t -= nxf;
+    if (t >= 2LL * L.nx * L.nz) return;
+    x = (int)(t % L.nx);
+    y = (int)((t / L.nx) % 2) ? L.ny - 1 : 0;
+    z = (int)(t / (2LL * L.nx));
+    xface = false;
+  }
+  C dr, u[3];
+  macro_cell<S, PE>(A, L, x, y, z, inv_scale, dr, u);
+  C *p = pm + (xface ? pm_x(L, x != 0, y, z) : pm_y(L, y != 0, x, z));
+  const int st = xface ? L.ny : L.nx;
+  p[0] = dr; p[st] = u[0]; p[2 * st] = u[1]; p[3 * st] = u[2];
 }
 
 // Initial farfield state from the initial populations (post-even form): incoming =
@@ -801,23 +854,23 @@ __global__ __launch_bounds__(128) void k_legacy(typename St<S>::T *__restrict__ 
   legacy_body<S, PE>(A, L, K, U, scale, inv_scale, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// The walls and one edge phase as one launch over consecutive block ranges, so that their
-// latencies overlap.  The walls read fluid slots and write slots of solid cells; the edge
-// phase reads fluid slots only and writes ghost slots (legacy links, EDGE 0) or the x-face
-// copy (EDGE 1): no slot written by one is touched by the other.  Wall blocks come first,
-// so the wall partials keep their per-block layout.
+// The walls and the edge phase as one launch over consecutive block ranges, so that their
+// latencies overlap: the walls (blocks below nb_wall, so the wall partials keep their
+// per-block layout), then the legacy links (EDGE 0) or the characteristic farfield items
+// (EDGE 1).  With every wall link at least three cells from the x and y edges the walls
+// read and write slots of cells next to the body only, and the edge phase those of the
+// cells next to the edges: no slot written by one is touched by the other.
 template <int S, bool PE, bool LOADS, int EDGE>
 __global__ __launch_bounds__(256) void k_walls_edge(typename St<S>::T *__restrict__ A, Desc L, WallLinks W,
                                                     float scale, float inv_scale, double *__restrict__ partials,
-                                                    int nb_wall, LegacyLinks K, double U,
-                                                    typename St<S>::T *__restrict__ fc) {
+                                                    int nb_wall, LegacyLinks K, double U, FFItems I, FFParams Pm) {
   const int b = blockIdx.x;
   if (b < nb_wall) {
     walls_body<S, PE, LOADS>(A, L, W, scale, inv_scale, partials, (long long)b * 256 + threadIdx.x, b);
   } else if constexpr (EDGE == 0) {
     legacy_body<S, PE>(A, L, K, U, scale, inv_scale, (long long)(b - nb_wall) * 256 + threadIdx.x);
   } else {
-    face_fill_body<S, PE>(A, L, fc, (long long)(b - nb_wall) * 256 + threadIdx.x);
+    farfield_body<S, PE>(A, L, I, Pm, scale, inv_scale, (long long)(b - nb_wall) * 256 + threadIdx.x);
   }
 }
 

@@ -1,0 +1,200 @@
+// ak_probe.hip -- memory-pattern probe of the in-place AA sweep (measurement tool, not part
+// of the step).
+//
+// One launch moves the sweep's exact population traffic with no collision: every cell loads
+// its 27 values and stores them back, in the even form (read (x, q), write (x, opp q)) or the
+// odd form (read (x - c_q, opp q), write (x + c_q, q)), over a fully periodic box (every
+// neighbour index wraps, so every access stays inside the buffer).  Each thread reads exactly
+// the slots it writes, as in the sweep, so a launch has no races.  Layouts:
+//   0 rows       interleaved direction rows [z][y][q][x] (the sweep's layout), row pitch a
+//                multiple of 128 bytes
+//   1 planes     one plane per direction [q][z][y][x], same pitch
+//   2 xblocks64  blocks of 64 cells along x, directions inside: [z][y][x/64][q][64]
+//   3 blocks1024 blocks of 1024 consecutive cells in (z, y, x) order: [n/1024][q][1024]
+// Threads: 64 x 4 tiles of one z plane with the sweep's XCD-contiguous block order, or (cells
+// per lane k = 2 or 4, even form, layouts 0-2) 64 lanes x k consecutive cells as one k-wide
+// access per direction.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+
+namespace {
+
+__host__ __device__ constexpr int pcx(int i) { return i / 9 - 1; }
+__host__ __device__ constexpr int pcy(int i) { return (i / 3) % 3 - 1; }
+__host__ __device__ constexpr int pcz(int i) { return i % 3 - 1; }
+__host__ __device__ constexpr int popp(int i) { return 26 - i; }
+
+struct Box {
+  int nx, ny, nz;
+  long long R;  // row pitch (elements), layouts 0 and 1
+  int gx, gy, nblocks;
+};
+
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+template <int LAYOUT>
+__device__ __forceinline__ long long addr(const Box &B, int x, int y, int z, int q) {
+  if constexpr (LAYOUT == 0) return (((long long)z * B.ny + y) * 27 + q) * B.R + x;
+  else if constexpr (LAYOUT == 1) return (((long long)q * B.nz + z) * B.ny + y) * B.R + x;
+  else if constexpr (LAYOUT == 2) {
+    const int nxb = B.nx / 64;
+    return ((((long long)z * B.ny + y) * nxb + x / 64) * 27 + q) * 64 + (x % 64);
+  } else {
+    const long long n = ((long long)z * B.ny + y) * B.nx + x;
+    return ((n / 1024) * 27 + q) * 1024 + (n % 1024);
+  }
+}
+
+__device__ __forceinline__ int xcd_order(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, xcd = b % 8, k = b / 8;
+  return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+template <typename T, int LAYOUT, bool ODD>
+__global__ __launch_bounds__(256) void kp_pattern(T *__restrict__ A, Box B) {
+  const int b = xcd_order((int)blockIdx.x, B.nblocks);
+  const int tx = b % B.gx, ty = (b / B.gx) % B.gy, z = b / (B.gx * B.gy);
+  const int x = tx * 64 + (int)(threadIdx.x % 64), y = ty * 4 + (int)(threadIdx.x / 64);
+  if (x >= B.nx || y >= B.ny) return;
+  T v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if constexpr (ODD)
+      v[q] = A[addr<LAYOUT>(B, wrapi(x - pcx(q), B.nx), wrapi(y - pcy(q), B.ny), wrapi(z - pcz(q), B.nz), popp(q))];
+    else
+      v[q] = A[addr<LAYOUT>(B, x, y, z, q)];
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if constexpr (ODD)
+      A[addr<LAYOUT>(B, wrapi(x + pcx(q), B.nx), wrapi(y + pcy(q), B.ny), wrapi(z + pcz(q), B.nz), q)] = v[q];
+    else
+      A[addr<LAYOUT>(B, x, y, z, popp(q))] = v[q];
+  }
+}
+
+// Even form with K consecutive cells per lane: one K-element access per direction (W is an
+// integer type of K elements).  The cell groups never straddle a row or a 64-cell block.
+template <typename W, int LAYOUT, int K>
+__global__ __launch_bounds__(256) void kp_pattern_wide(W *__restrict__ A, Box B) {
+  const int b = xcd_order((int)blockIdx.x, B.nblocks);
+  const int tx = b % B.gx, ty = (b / B.gx) % B.gy, z = b / (B.gx * B.gy);
+  const int x = (tx * 64 + (int)(threadIdx.x % 64)) * K, y = ty * 4 + (int)(threadIdx.x / 64);
+  if (x >= B.nx || y >= B.ny) return;
+  W v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) v[q] = A[addr<LAYOUT>(B, x, y, z, q) / K];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) A[addr<LAYOUT>(B, x, y, z, popp(q)) / K] = v[q];
+}
+
+template <typename T, int LAYOUT>
+hipError_t launch_one(T *A, const Box &B, bool odd, hipStream_t st) {
+  if (odd) hipLaunchKernelGGL((kp_pattern<T, LAYOUT, true>), dim3(B.nblocks), dim3(256), 0, st, A, B);
+  else hipLaunchKernelGGL((kp_pattern<T, LAYOUT, false>), dim3(B.nblocks), dim3(256), 0, st, A, B);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_layout(void *A, const Box &B, int layout, bool odd, hipStream_t st) {
+  switch (layout) {
+    case 0: return launch_one<T, 0>((T *)A, B, odd, st);
+    case 1: return launch_one<T, 1>((T *)A, B, odd, st);
+    case 2: return launch_one<T, 2>((T *)A, B, odd, st);
+    default: return launch_one<T, 3>((T *)A, B, odd, st);
+  }
+}
+
+template <typename W, int K>
+hipError_t launch_wide(void *A, const Box &B, int layout, hipStream_t st) {
+  switch (layout) {
+    case 0: hipLaunchKernelGGL((kp_pattern_wide<W, 0, K>), dim3(B.nblocks), dim3(256), 0, st, (W *)A, B); break;
+    case 1: hipLaunchKernelGGL((kp_pattern_wide<W, 1, K>), dim3(B.nblocks), dim3(256), 0, st, (W *)A, B); break;
+    default: hipLaunchKernelGGL((kp_pattern_wide<W, 2, K>), dim3(B.nblocks), dim3(256), 0, st, (W *)A, B); break;
+  }
+  return hipGetLastError();
+}
+
+char g_err[256] = "";
+
+}  // namespace
+
+extern "C" {
+
+const char *akp_last_error() { return g_err; }
+
+// Mean milliseconds per launch of one pattern over an nx x ny x nz box of elem_bytes-byte
+// elements (2, 4 or 8), `reps` timed launches after two untimed ones, on a buffer of its own.
+// cells_per_lane 1 (both forms, every layout) or 2 / 4 (even form, layouts 0-2, nx a multiple
+// of 64 x cells_per_lane).  out = {ms per launch, bytes per launch}.  Returns 0, or -1 with
+// akp_last_error() set.
+int akp_pattern(int layout, int elem_bytes, int odd, int cells_per_lane, int nx, int ny, int nz, int reps,
+                double *out) {
+  if (layout < 0 || layout > 3 || (elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8) || nx < 2 || ny < 2 ||
+      nz < 2 || reps < 1 || !out) {
+    snprintf(g_err, sizeof g_err, "bad probe arguments");
+    return -1;
+  }
+  const int k = cells_per_lane;
+  if (!(k == 1 || ((k == 2 || k == 4) && !odd && layout <= 2 && nx % (64 * k) == 0 && elem_bytes * k <= 8))) {
+    snprintf(g_err, sizeof g_err, "cells_per_lane %d needs the even form, layouts 0-2, nx a multiple of %d", k, 64 * k);
+    return -1;
+  }
+  if (layout == 2 && nx % 64) {
+    snprintf(g_err, sizeof g_err, "layout 2 needs nx a multiple of 64");
+    return -1;
+  }
+  Box B{};
+  B.nx = nx; B.ny = ny; B.nz = nz;
+  const int per_line = 128 / elem_bytes;
+  B.R = ((long long)nx + per_line - 1) / per_line * per_line;
+  B.gx = (nx / k + 63) / 64;
+  B.gy = (ny + 3) / 4;
+  B.nblocks = B.gx * B.gy * nz;
+  long long elems;
+  if (layout <= 1) elems = B.R * ny * nz * 27;
+  else if (layout == 2) elems = (long long)nx * ny * nz * 27;
+  else elems = ((long long)nx * ny * nz + 1023) / 1024 * 1024 * 27;
+  const size_t bytes = (size_t)elems * elem_bytes;
+  void *A = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&A, bytes);
+  if (e == hipSuccess) e = hipMemset(A, 0, bytes);
+  if (e == hipSuccess) e = hipStreamCreate(&st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  auto launch = [&]() -> hipError_t {
+    if (k == 1) {
+      switch (elem_bytes) {
+        case 2: return launch_layout<uint16_t>(A, B, layout, odd != 0, st);
+        case 4: return launch_layout<uint32_t>(A, B, layout, odd != 0, st);
+        default: return launch_layout<uint64_t>(A, B, layout, odd != 0, st);
+      }
+    }
+    if (elem_bytes * k == 4) return launch_wide<uint32_t, 2>(A, B, layout, st);  // 2 x FP16
+    if (k == 2) return launch_wide<uint64_t, 2>(A, B, layout, st);              // 2 x FP32
+    return launch_wide<uint64_t, 4>(A, B, layout, st);                          // 4 x FP16
+  };
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (A) (void)hipFree(A);
+  if (e != hipSuccess) {
+    snprintf(g_err, sizeof g_err, "%s", hipGetErrorString(e));
+    return -1;
+  }
+  out[0] = ms / reps;
+  out[1] = 2.0 * 27.0 * elem_bytes * (double)nx * ny * nz;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+"""Memory-pattern probe of the in-place AA sweep: what rate the sweep's population traffic
+reaches without the collision, per storage precision, layout and AA form.
+
+    python -m aerokinetic_amd.pattern_probe --shape 1536,1024,64 --reps 10
+
+Each line is one launch pattern of `aerokinetic_amd/lib/libaerokinetic_probe.so`
+(csrc/ak_probe.hip): 27 loads and 27 stores per cell, even form (read (x, q), write
+(x, opp q)) or odd form (read (x - c_q, opp q), write (x + c_q, q)), fully periodic box,
+64 x 4 tiles in the sweep's XCD-contiguous block order.  Layouts: `rows` (interleaved
+direction rows, the sweep's layout), `planes` (one plane per direction), `xblocks64` (64
+cells along x with their 27 directions together), `blocks1024` (1024 consecutive cells with
+their 27 directions together); `rows x2` / `x4` are the even form with 2 or 4 cells per lane
+(one 4- or 8-byte access per direction).  Rates are algorithmic bytes (2 x 27 x element bytes
+per cell) per second; `mean` is the rate of one even plus one odd launch, which is what a
+run of the sweep alternates.  Prints a markdown table (and one JSON line with --json).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libaerokinetic_probe.so"
+LAYOUTS = {"rows": 0, "planes": 1, "xblocks64": 2, "blocks1024": 3}
+BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
+PEAK_GBS = 8000.0
+
+
+def _lib():
+    if not LIB_PATH.exists():
+        raise FileNotFoundError(f"{LIB_PATH} is missing: build it with make -C aerokinetic_amd/csrc")
+    L = ctypes.CDLL(str(LIB_PATH))
+    L.akp_pattern.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
+    L.akp_pattern.restype = ctypes.c_int
+    L.akp_last_error.restype = ctypes.c_char_p
+    return L
+
+
+def measure(L, layout, storage, odd, shape, reps, cells_per_lane=1):
+    """(ms per launch, GB/s) of one pattern."""
+    out = (ctypes.c_double * 2)()
+    rc = L.akp_pattern(LAYOUTS[layout], BYTES[storage], 1 if odd else 0, cells_per_lane, *shape, reps, out)
+    if rc:
+        raise RuntimeError(f"probe {layout} {storage} odd={odd}: {L.akp_last_error().decode()}")
+    return out[0], out[1] / (out[0] * 1e-3) / 1e9
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--shape", default="1536,1024,64")
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--storages", default="fp64,fp32,fp16")
+    ap.add_argument("--json", action="store_true")
+    a = ap.parse_args(argv)
+    shape = tuple(int(v) for v in a.shape.split(","))
+    L = _lib()
+    rows = []
+    print("| storage | layout | even TB/s | odd TB/s | mean TB/s | mean % of 8 TB/s |", flush=True)
+    print("|---|---|---:|---:|---:|---:|", flush=True)
+    for st in [s for s in a.storages.split(",") if s]:
+        for lay in LAYOUTS:
+            te, ge = measure(L, lay, st, False, shape, a.reps)
+            to, go = measure(L, lay, st, True, shape, a.reps)
+            mean = 2 * ge * te / (te + to)  # two launches' bytes over their summed time
+            rows.append({"storage": st, "layout": lay, "even_gbs": ge, "odd_gbs": go, "mean_gbs": mean})
+            print(f"| {st} | {lay} | {ge / 1e3:.2f} | {go / 1e3:.2f} | {mean / 1e3:.2f} | {mean / PEAK_GBS * 100:.1f} |",
+                  flush=True)
+        for k in (2, 4):
+            if BYTES[st] * k > 8:
+                continue
+            te, ge = measure(L, "rows", st, False, shape, a.reps, cells_per_lane=k)
+            rows.append({"storage": st, "layout": f"rows x{k}", "even_gbs": ge})
+            print(f"| {st} | rows x{k} (even only) | {ge / 1e3:.2f} | — | — | — |", flush=True)
+    if a.json:
+        print(json.dumps({"shape": shape, "reps": a.reps, "rows": rows}))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
