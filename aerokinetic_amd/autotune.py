@@ -8,6 +8,11 @@ parities (odd sweeps gather from neighbours, even sweeps read in place) by timin
 even number of steps.  The lattice state advances while tuning, so tune on a
 scratch copy (or before the initial state is set).  Results are cached per
 (shape, storage) for the process.
+
+`tune_refined` does the same for both levels of a two-level case at once (the coarse and
+fine sweeps are timed separately inside each coarse step); the case's state is saved before
+and restored after, so tuning leaves it as it was.  `tune_batch` picks one 256-thread tile
+for every member sweep of a batch (the batched sweep launches 256-thread blocks).
 """
 from __future__ import annotations
 
@@ -34,3 +39,45 @@ def tune(lattice, steps=8, candidates=CANDIDATES, first_step=0):
     lattice.set_tile(*best)
     _cache[key] = best
     return best, timings
+
+
+def default_tile(nx):
+    """The library's tile when none is set: 64 x 4 on whole-wave rows, else row-linear 256."""
+    return (64, 4) if nx % 64 == 0 else (0, 256)
+
+
+def _fits(tile, nx):
+    return not (tile[0] > 64 and nx < tile[0])
+
+
+def tune_refined(solver, steps=4, candidates=CANDIDATES):
+    """Fastest sweep tile of each level of a RefinedSolver: ((bx, by) coarse, (bx, by) fine),
+    timings {level: {tile: ms per sweep}}."""
+    import numpy as np
+    from . import _lib
+    nxs = (solver.plan.shape[0], solver.fine_shape[0] + 2 * solver.pad)
+    key = ("refined", tuple(solver.plan.shape), tuple(solver.fine_shape), solver.storage)
+    if key in _cache:
+        solver.set_tiles(*_cache[key])
+        return _cache[key], {}
+    L, h = solver.L, solver.h
+    n = int(L.ak_refined_state_bytes(h))
+    state = np.empty(n, dtype=np.uint8)
+    _lib.check(L.ak_refined_save_state(h, _lib._vp(state.ctypes.data), n), None, "ak_refined_save_state")
+    steps0 = solver.steps
+    timings = ({}, {})
+    for tile in candidates:
+        use = [tile if _fits(tile, nx) else default_tile(nx) for nx in nxs]
+        solver.set_tiles(*use)
+        solver.run_profiled(2, 0)
+        _, cs, fs = solver.run_profiled(steps, 0)
+        if use[0] == tile:
+            timings[0][tile] = cs / steps
+        if use[1] == tile:
+            timings[1][tile] = fs / (2 * steps)
+    best = tuple(min(t, key=t.get) for t in timings)
+    solver.set_tiles(*best)
+    _lib.check(L.ak_refined_load_state(h, _lib._vp(state.ctypes.data), n), None, "ak_refined_load_state")
+    solver.steps = steps0
+    _cache[key] = best
+    return best, {"coarse": timings[0], "fine": timings[1]}

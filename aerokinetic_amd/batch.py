@@ -11,7 +11,10 @@ step count and stream.  Per coarse step of the batch:
 * the whole step is a captured HIP graph (re-captured when the membership changes).
 
 A member gives the same history alone and in any mix: the same kernels do the same
-arithmetic per member, and the load reductions have a fixed order.
+arithmetic per member, and the load reductions have a fixed order.  The sweep tile changes
+which thread updates a cell, never what it computes, so `tune` (one 256-thread tile for
+every member sweep, the fastest of BATCH_TILES over real steps of the batch) leaves the
+histories as they are.
 """
 from __future__ import annotations
 
@@ -20,6 +23,9 @@ import ctypes
 from . import _lib
 
 _vp = _lib._vp
+
+# 256-thread tiles of the batched sweep: (bx, by), bx = 0 for row-linear blocks
+BATCH_TILES = ((64, 4), (128, 2), (256, 1), (0, 256))
 
 
 class RefinedBatch:
@@ -38,11 +44,14 @@ class RefinedBatch:
         self.h = h
         _lib.check(self.L.ak_batch_set_graph_mode(h, 1 if graphs else 0), None, "ak_batch_set_graph_mode")
         self.members = []
+        self.tile, self.tile_timings = None, {}
 
     def add(self, solver):
         """Admit a RefinedSolver (same storage; at a coarse-step boundary)."""
         if solver.storage != self.storage:
             raise ValueError(f"a {solver.storage} case cannot join a {self.storage} batch")
+        if self.tile is not None:
+            self._apply_tile(solver, self.tile)
         _lib.check(self.L.ak_batch_add(self.h, solver.h), None, "ak_batch_add")
         self.members.append(solver)
 
@@ -85,6 +94,45 @@ class RefinedBatch:
             return {m: [] for m in self.members}, {}
         self.launch(nsteps, sample)
         return self.collect()
+
+    @staticmethod
+    def _apply_tile(solver, tile):
+        from .autotune import default_tile
+        nxs = (solver.plan.shape[0], solver.fine_shape[0] + 2 * solver.pad)
+        solver.set_tiles(*[tile if not (tile[0] > 64 and nx < tile[0]) else default_tile(nx) for nx in nxs])
+
+    def tune(self, steps=20, tiles=BATCH_TILES):
+        """Time `steps` real coarse steps of the batch with each tile applied to every member
+        and keep the fastest (members joining later get it too).  The steps are the members'
+        own (sampled every step).  Returns (tile, {tile: ms per batch step}, {member: history
+        rows of those steps, as collect(raw=True)})."""
+        import time
+        import numpy as np
+        import torch
+        if not self.members:
+            return self.tile, {}, {}
+        timings, rows = {}, {m: [] for m in self.members}
+
+        def run(n):
+            self.launch(n, sample=True)
+            got, _ = self.collect(raw=True)
+            for m, r in got.items():
+                rows[m].append(r)
+
+        for tile in tiles:
+            for m in self.members:
+                self._apply_tile(m, tile)
+            run(2)  # descriptors and graphs of this tile
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run(steps)
+            torch.cuda.synchronize()
+            timings[tile] = (time.perf_counter() - t0) / steps * 1e3
+        self.tile = min(timings, key=timings.get)
+        for m in self.members:
+            self._apply_tile(m, self.tile)
+        self.tile_timings = timings
+        return self.tile, timings, {m: np.concatenate(r) for m, r in rows.items()}
 
     def close(self):
         if getattr(self, "h", None):

@@ -61,8 +61,12 @@ unsigned long long batch_key(const Batch *b) {
   return k;
 }
 
-// batched sweeps run 256-thread blocks: the default geometry of each member
-SweepGeom tile_geom(const Lattice *s) { return default_geom(s->nx, s->ny, s->nz); }
+// batched sweeps run 256-thread blocks: a member lattice's own tile when it has 256 threads
+// (the descriptors are rebuilt when a member's tile changes: it bumps the lattice's
+// generation, which is part of batch_key), else its default geometry
+SweepGeom tile_geom(const Lattice *s) {
+  return s->geom.bx * s->geom.by == 256 ? s->geom : default_geom(s->nx, s->ny, s->nz);
+}
 
 BatchMember member_of(const Lattice *s, bool odd, int first) {
   BatchMember m{};

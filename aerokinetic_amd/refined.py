@@ -85,7 +85,7 @@ class RefinedSolver:
     pad = 2
 
     def __init__(self, project: Project, refinement=RefinementSettings(), progress=None, *, storage="fp64",
-                 device=0, base_plan=None, plan=None, fp16_scale=0.0, graphs=True, host=None):
+                 device=0, base_plan=None, plan=None, fp16_scale=0.0, graphs=True, host=None, tile=None):
         import torch
         if not torch.cuda.is_available():
             raise _lib.NativeLibraryError("no HIP device visible to PyTorch-ROCm")
@@ -146,6 +146,23 @@ class RefinedSolver:
         self._reference_volume = (coarse_owned_cells * p.coarse_level.dx ** 3
                                   + fine_fluid_cells * p.fine_level.dx ** 3)
         self.initialize_flow()
+        # sweep tiles of the two levels: library defaults, explicit ((bx, by), (bx, by)), or
+        # "auto" (aerokinetic_amd.autotune.tune_refined: fastest measured, state kept)
+        self.tile_timings = {}
+        if tile == "auto":
+            from .autotune import tune_refined
+            self.tile, self.tile_timings = tune_refined(self)
+        elif tile:
+            self.set_tiles(*tile)
+        else:
+            from .autotune import default_tile
+            self.tile = (default_tile(nx), default_tile(self.fine_shape[0] + 2 * self.pad))
+
+    def set_tiles(self, coarse, fine):
+        """Sweep tiles (bx, by) of the coarse and fine lattices (bx = 0: row-linear blocks)."""
+        for lvl, (bx, by) in zip(self._levels, (coarse, fine)):
+            _lib.check(self.L.ak_set_sweep_tile(lvl, int(bx), int(by)), lvl, "ak_set_sweep_tile")
+        self.tile = (tuple(coarse), tuple(fine))
 
     def close(self):
         if getattr(self, "h", None):

@@ -189,6 +189,8 @@ class ServeReport:
     peak_members: int = 0
     gpu_steps: int = 0
     host_s: dict = field(default_factory=dict)  # serving-thread time by activity
+    tile: tuple | None = None  # sweep tile of the batch (None: library defaults)
+    tile_timings: dict = field(default_factory=dict)  # tile -> ms per batch step, when tuned
 
     def summary(self):
         lat = np.array([r.latency for r in self.results]) if self.results else np.zeros(1)
@@ -199,7 +201,9 @@ class ServeReport:
                 "latency_p99_s": float(np.percentile(lat, 99)), "latency_mean_s": float(lat.mean()),
                 "peak_members": self.peak_members, "coarse_steps": int(sum(r.steps for r in self.results)),
                 "host_s": {k: round(v, 3) for k, v in self.host_s.items()},
-                "reasons": {k: sum(r.reason == k for r in self.results) for k in ("converged", "max_steps", "failed")}}
+                "reasons": {k: sum(r.reason == k for r in self.results) for k in ("converged", "max_steps", "failed")},
+                "tile": list(self.tile) if self.tile else None,
+                "tile_ms_per_batch_step": {f"{t[0]}x{t[1]}": round(v, 4) for t, v in self.tile_timings.items()}}
 
 
 def _window_variation(cl, window, cl_floor):
@@ -211,7 +215,8 @@ class Server:
     """Continuous-batching service (see the module docstring)."""
 
     def __init__(self, storage="fp16", device=0, memory_bytes=None, max_members=64, chunk=200, window=2000,
-                 tol=0.005, cl_floor=0.1, max_steps=20000, builders=4, graphs=True):
+                 tol=0.005, cl_floor=0.1, max_steps=20000, builders=4, graphs=True, tile=None, tune_members=4,
+                 tune_steps=10):
         if chunk < 4:
             raise ValueError("chunk must be at least 4 coarse steps")
         self.storage, self.device = storage, device
@@ -219,6 +224,9 @@ class Server:
         self.max_members, self.chunk = max_members, chunk
         self.window, self.tol, self.cl_floor, self.max_steps = window, tol, cl_floor, max_steps
         self.builders, self.graphs = builders, graphs
+        # sweep tile of the batch: library default (None), fixed (bx, by), or "auto": the fastest
+        # of batch.BATCH_TILES, measured on real steps once tune_members cases have joined
+        self.tile, self.tune_members, self.tune_steps = tile, tune_members, tune_steps
 
     def _budget(self):
         if self.memory_bytes:
@@ -263,11 +271,13 @@ class Server:
         budget = self._budget()
         queue = deque(sorted(requests, key=lambda r: r.arrival))
         batch = RefinedBatch(self.storage, self.device, graphs=self.graphs)
+        if self.tile not in (None, "auto"):
+            batch.tile = tuple(self.tile)
         building, ready, active = {}, deque(), {}  # prepare futures; prepared; in the batch
         reserved = 0
         t0 = time.perf_counter()
         staged = []  # cases whose device part was built while the last chunk ran
-        clock = {"build": 0.0, "launch": 0.0, "collect": 0.0, "retire": 0.0, "admit": 0.0, "idle": 0.0}
+        clock = {"build": 0.0, "launch": 0.0, "collect": 0.0, "retire": 0.0, "admit": 0.0, "idle": 0.0, "tune": 0.0}
 
         def timed(key, fn, *a, **k):
             t = time.perf_counter()
@@ -295,6 +305,12 @@ class Server:
                     log(f"admit {st['req'].id} ({st['cells']} cells) t={st['admitted']:.2f}s members={len(batch)}")
                 staged = []
                 report.peak_members = max(report.peak_members, len(batch))
+                if self.tile == "auto" and batch.tile is None and len(active) >= self.tune_members:
+                    _, report.tile_timings, rows = timed("tune", batch.tune, self.tune_steps)
+                    report.gpu_steps += (self.tune_steps + 2) * len(report.tile_timings)
+                    for solver, r in rows.items():
+                        active[solver]["cl"].extend(r[:, 1].tolist())
+                        active[solver]["loads"].append(r[:, 1:4].copy())
                 if active:
                     timed("launch", batch.launch, self.chunk, sample=True)
                 # admission: a free slot and the footprint within the budget; the device part is
@@ -331,6 +347,7 @@ class Server:
                         log(f"retire {st['req'].id} {reason} after {len(st['cl'])} steps, t={now:.2f}s")
         report.makespan_s = time.perf_counter() - t0
         report.host_s = clock
+        report.tile = batch.tile
         batch.close()
         return report
 

@@ -19,7 +19,9 @@ storage; each counter is scaled by what it reads for the copy of known size.
 
 two_level: coarse 256x192x8 + fine 160x96x16 around a NACA0012 of chord 64 fine cells at
 6 deg, Re 3e4 (one step = one coarse step with both fine substeps and all transfers; MLUPS
-counts coarse cells + 2 x fine cells per coarse step), FP64 and FP32 storage.
+counts coarse cells + 2 x fine cells per coarse step), FP64 and FP32 storage.  With
+--tile auto (the default) the sweep tile of each level is the autotuner's pick
+(aerokinetic_amd.autotune.tune_refined, reported as `tiles`).
 
 exterior_farfield: the plain 384x256x8 NACA0012 case (chord 32, 4 deg, Re 1e4) with the
 exterior farfield (a 386x258x8 padded lattice whose ring is rebuilt every step).
@@ -28,7 +30,8 @@ serving: continuous batching of two-level requests in FP16 storage against one-a
 FP64 solves of the same requests: seeded Poisson arrivals of a NACA 4-digit main element +
 25 %-chord flap (0-30 deg), lattice sizes log-uniform in 2M-200M cells, Re 1e4-1e5,
 retirement when the 2000-step Cl window varies < 0.5 % or at the step cap; cases/hour and
-p50/p99 latency.
+p50/p99 latency.  With --tile auto the batch picks its sweep tile on its first real steps
+(RefinedBatch.tune; reported as `tile`).
 
 copy: the library's streaming device copy of 4 GiB (attainable HBM read+write bandwidth).
 
@@ -152,12 +155,12 @@ def refined_fields(s):
     return np.concatenate(rhos), np.concatenate(us), np.concatenate(masks)
 
 
-def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
+def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None, tile=None):
     import numpy as np
     import torch
     from aerokinetic_amd.refined import RefinedSolver
     project, settings, plan = case.build()
-    s = RefinedSolver(project, settings, plan=plan, storage=storage)
+    s = RefinedSolver(project, settings, plan=plan, storage=storage, tile=tile)
     fine_owned = 8 * plan.box.shape[0] * plan.box.shape[1] * plan.box.shape[2]
     coarse = int(np.prod(plan.shape))
     updates = coarse + 2 * fine_owned
@@ -181,6 +184,8 @@ def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
            "fine_sweep_cells": fine_cells,
            "roofline": roofline(fine_cells * 2 * 27 * b, fine_sweep, copy_gbs),
            "coarse_sweep_roofline": roofline(coarse * 2 * 27 * b, coarse_sweep, copy_gbs),
+           "tiles": {"coarse": list(s.tile[0]), "fine": list(s.tile[1])},
+           "tile_ms": {lvl: {f"{t[0]}x{t[1]}": round(v, 5) for t, v in d.items()} for lvl, d in s.tile_timings.items()},
            "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
            "fields": fields,
            "diag": s.diagnostics()}
@@ -351,7 +356,7 @@ def serving_gpu(args):
            "rule": rule, "cells": cells}
     for st in [s for s in args.serve_storages.split(",") if s]:
         log(f"serving batched {st}")
-        rep = S.Server(storage=st, **rule).serve(reqs, log=log)
+        rep = S.Server(storage=st, tile=args.tile, **rule).serve(reqs, log=log)
         out[f"batched_{st}"] = rep.summary()
         log(f"  {out[f'batched_{st}']['cases_per_hour']:.0f} cases/h, p50 {out[f'batched_{st}']['latency_p50_s']:.2f} s, "
             f"p99 {out[f'batched_{st}']['latency_p99_s']:.2f} s, peak members {rep.peak_members}")
@@ -378,6 +383,8 @@ def main():
     ap.add_argument("--traffic", type=int, default=1, help="0 skips the memory-counter passes")
     ap.add_argument("--two-level-storages", default="fp64,fp32", help="empty to skip the two-level case")
     ap.add_argument("--two-level-steps", type=int, default=2000)
+    ap.add_argument("--tile", default="auto", help="sweep tiles of the two-level case and the serving batch: "
+                    "'auto' (autotuned) or 'default'")
     ap.add_argument("--exterior-storages", default="fp64,fp16", help="empty to skip the exterior-farfield case")
     ap.add_argument("--exterior-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -388,6 +395,7 @@ def main():
     ap.add_argument("--serve-window", type=int, default=2000)
     ap.add_argument("--serve-max-steps", type=int, default=3000)
     args = ap.parse_args()
+    args.tile = None if args.tile == "default" else args.tile
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: a case runs on a single MI355X here; "
               "multi-GPU runs are not supported (use --gpus 1)", file=sys.stderr)
@@ -411,7 +419,7 @@ def main():
     case2 = cases.NACA0012_REFINED_256x192x8
     for st in [s for s in args.two_level_storages.split(",") if s]:
         log(f"two-level {st}")
-        two[st] = refined_gpu(case2, st, args.two_level_steps, 500, 40, copy_gbs)
+        two[st] = refined_gpu(case2, st, args.two_level_steps, 500, 40, copy_gbs, tile=args.tile)
         r = two[st]
         log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, fine sweep {r['fine_sweep_ms']:.4f} ms "
             f"({100 * r['roofline']['frac']:.1f}% HBM), coarse sweep {r['coarse_sweep_ms']:.4f} ms")

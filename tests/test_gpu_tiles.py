@@ -1,0 +1,70 @@
+"""Sweep tiles change which thread updates a cell, never what it computes: the autotuned
+two-level case and a tuned batch give the same histories and populations, bit for bit, as
+the library's default tiles, and tuning leaves a case's state as it was."""
+import numpy as np
+import pytest
+
+from aerokinetic_amd import serving as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(req, storage, tile=None):
+    from aerokinetic_amd.refined import RefinedSolver
+    project, settings, plan = S.project_of(req)
+    return RefinedSolver(project, settings, plan=plan, storage=storage, tile=tile)
+
+
+def _reqs(n):
+    return S.polar_requests(n, seed=11, chord=(20, 24), span=(8, 8), reynolds=(1e4, 3e4), alpha=(2.0, 8.0))
+
+
+def _loads(recs):
+    return np.array([[r["step"], r["cl"], r["cd"], r["cm"]] for r in recs])
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp16"])
+def test_autotuned_two_level_case_matches_default_tiles(storage):
+    req = _reqs(1)[0]
+    print("request", storage, req, flush=True)
+    a = _solver(req, storage)
+    b = _solver(req, storage, tile="auto")
+    print("tiles", a.tile, b.tile, flush=True)
+    assert b.steps == 0
+    for (bx, by) in b.tile:
+        assert (bx == 0 and by % 64 == 0) or (bx % 64 == 0 and bx * by <= 1024)
+    for level in (0, 1):  # tuning restored the initial state
+        np.testing.assert_array_equal(a.populations(level), b.populations(level))
+    ra, rb = a.advance(12, 1), b.advance(12, 1)
+    np.testing.assert_array_equal(_loads(ra), _loads(rb))
+    for level in (0, 1):
+        np.testing.assert_array_equal(a.populations(level), b.populations(level))
+    np.testing.assert_array_equal(np.asarray(a.budget()), np.asarray(b.budget()))
+    a.close()
+    b.close()
+
+
+def test_tuned_batch_matches_single_cases():
+    """Three members tuned over the batched tiles (real, sampled steps), then stepped on: every
+    member's history and final populations equal the single-case solver's."""
+    from aerokinetic_amd.batch import BATCH_TILES, RefinedBatch
+    storage = "fp32"
+    reqs = _reqs(3)
+    singles = [_solver(r, storage) for r in reqs]
+    members = [_solver(r, storage) for r in reqs]
+    batch = RefinedBatch(storage)
+    for m in members:
+        batch.add(m)
+    tile, timings, rows = batch.tune(steps=2)
+    print("batch tile", tile, timings, flush=True)
+    assert tile in BATCH_TILES and set(timings) == set(BATCH_TILES)
+    recs, failed = batch.advance(4)
+    assert not failed
+    tuned_steps = 4 * len(BATCH_TILES)
+    for s, m in zip(singles, members):
+        want = _loads(s.advance(tuned_steps + 4, 1))
+        got = np.concatenate([rows[m][:, :4], _loads(recs[m])])
+        np.testing.assert_array_equal(got, want)
+        for level in (0, 1):
+            np.testing.assert_array_equal(s.populations(level), m.populations(level))
+    batch.close()
