@@ -34,11 +34,20 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// Wave maximum of finite non-negative values (callers replace non-finite inputs first): for
+// FP32 an unsigned maximum of the bit patterns (the same order), one instruction per step.
 template <typename C>
 __device__ __forceinline__ C wave_max(C v) {
+  if constexpr (sizeof(C) == 4) {
+    unsigned b = __float_as_uint(v);
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) { C t = __shfl_xor(v, o, 64); v = t > v ? t : v; }
-  return v;
+    for (int o = 32; o >= 1; o >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, o, 64));
+    return __uint_as_float(b);
+  } else {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+  }
 }
 template <typename C>
 __device__ __forceinline__ C wave_min(C v) {
@@ -55,6 +64,16 @@ __device__ __forceinline__ gchar *uniform_ptr(const void *p) {
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
   const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
   return (gchar *)(((unsigned long long)hi << 32) | lo);
+}
+// A wave-uniform 64-bit value in scalar registers, opaque to the optimiser (an empty asm
+// that "modifies" it), so that values derived from it are recomputed where they are used
+// instead of being kept alive from an earlier identical computation.
+__device__ __forceinline__ long long fresh_sgpr64(long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+  unsigned long long r = ((unsigned long long)hi << 32) | lo;
+  asm volatile("" : "+s"(r));
+  return (long long)r;
 }
 template <int S>
 __device__ __forceinline__ typename St<S>::C ld_g(const gchar *p, float inv_scale) {
@@ -132,17 +151,21 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
                                             DiagAcc *__restrict__ diag, int lb, int slot,
                                             typename St<S>::C *__restrict__ pm = nullptr) {
   using C = typename St<S>::C;
+  // the block's tile coordinates are wave-uniform: kept in scalar registers, so that the
+  // per-direction plane bases below are scalar arithmetic, not 64-bit vector adds
   int x, y, z;
   if (G.lin) {
-    const int j = (lb % G.gx) * G.bx + (int)threadIdx.x;
-    z = lb / G.gx;
+    const int j0 = __builtin_amdgcn_readfirstlane((lb % G.gx) * G.bx);
+    z = __builtin_amdgcn_readfirstlane(lb / G.gx);
+    const int j = j0 + (int)threadIdx.x;
     y = j / L.nx;
     x = j - y * L.nx;
   } else {
-    const int tx = lb % G.gx, ty = (lb / G.gx) % G.gy;
+    const int tx = __builtin_amdgcn_readfirstlane(lb % G.gx);
+    const int ty = __builtin_amdgcn_readfirstlane((lb / G.gx) % G.gy);
     x = tx * G.bx + (int)(threadIdx.x % G.bx);
     y = ty * G.by + (int)(threadIdx.x / G.bx);
-    z = lb / (G.gx * G.gy);
+    z = __builtin_amdgcn_readfirstlane(lb / (G.gx * G.gy));
   }
   bool active = x < L.nx && y < L.ny;
   uint8_t mk = 1;
@@ -175,7 +198,9 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
       // byte offset of the cell inside a direction plane (32 bits): global loads then use a
       // scalar base plus this vector offset, with no 64-bit vector address arithmetic
       const unsigned boff = (unsigned)((y + 1) * L.P + x) * (unsigned)sizeof(typename St<S>::T);
-      const long long plane = L.base + (long long)z * L.PS;
+      // the even form keeps the plane base in scalar registers explicitly (left to itself the
+      // compiler keeps it, and the 27 direction bases derived from it, in vector registers)
+      const long long plane = ODD ? L.base + (long long)z * L.PS : fresh_sgpr64(L.base + (long long)z * L.PS);
       const long long zp = (z == L.nz - 1) ? -(long long)(L.nz - 1) * L.PS : L.PS;
       const long long zm = (z == 0) ? (long long)(L.nz - 1) * L.PS : -L.PS;
 #pragma unroll
@@ -195,6 +220,7 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
       // FP16: the mass- and momentum-conserving rounding, each value rounded once
       _Float16 h[S == 2 ? NQ : 1];
       if constexpr (S == 2) fp16_conserving(g, scale, h);
+      const long long splane = plane;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         long long d;
@@ -204,7 +230,7 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         } else {
           d = (long long)opp(q) * L.QS;
         }
-        gchar *bq = uniform_ptr(A + (plane + d));
+        gchar *bq = uniform_ptr(A + (splane + d));
         if constexpr (S == 2) *(__attribute__((address_space(1))) _Float16 *)(bq + boff) = h[q];
         else st_g<S>(bq + boff, g[q], scale);
       }
