@@ -77,6 +77,7 @@ _SIGS = {
     "ak_batch_remove": (ctypes.c_int, [_vp, _vp]),
     "ak_batch_size": (ctypes.c_int, [_vp]),
     "ak_batch_set_graph_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ak_batch_set_sweep_grid": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ak_batch_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ak_batch_synchronize": (ctypes.c_int, [_vp]),
     "ak_refined_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

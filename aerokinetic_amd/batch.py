@@ -29,7 +29,7 @@ BATCH_TILES = ((64, 4), (128, 2), (256, 1), (0, 256))
 
 
 class RefinedBatch:
-    def __init__(self, storage="fp32", device=0, graphs=True, stream=None):
+    def __init__(self, storage="fp32", device=0, graphs=True, stream=None, sweep_grid=0):
         import torch
         if not torch.cuda.is_available():
             raise _lib.NativeLibraryError("no HIP device visible to PyTorch-ROCm")
@@ -43,6 +43,8 @@ class RefinedBatch:
                    "ak_batch_create")
         self.h = h
         _lib.check(self.L.ak_batch_set_graph_mode(h, 1 if graphs else 0), None, "ak_batch_set_graph_mode")
+        # batched sweep launch: 0 one block per tile, n > 0 a persistent grid of n blocks
+        _lib.check(self.L.ak_batch_set_sweep_grid(h, int(sweep_grid)), None, "ak_batch_set_sweep_grid")
         self.members = []
         self.tile, self.tile_timings = None, {}
 

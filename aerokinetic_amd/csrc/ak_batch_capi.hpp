@@ -34,6 +34,7 @@ struct Batch {
   std::vector<hipEvent_t> join;
   bool use_graphs = true;
   GraphCache graphs;  // slot = lead member's coarse parity * 2 + sampled
+  int sweep_grid = 0;  // batched sweep: 0 one block per tile, else a persistent grid of this many blocks
 };
 
 Batch *BT(void *h) { return (Batch *)h; }
@@ -177,7 +178,8 @@ int build_descriptors(Batch *b) {
 
 template <int S>
 void batched_sweep(Batch *b, const BatchMember *d, int nblocks) {
-  hipLaunchKernelGGL((k_sweep_batch<S, false>), dim3(nblocks), dim3(256), 0, b->stream, d, (int)b->members.size(),
+  const int grid = b->sweep_grid > 0 ? std::min(nblocks, b->sweep_grid) : nblocks;
+  hipLaunchKernelGGL((k_sweep_batch<S, false>), dim3(grid), dim3(256), 0, b->stream, d, (int)b->members.size(),
                      nblocks);
 }
 
@@ -330,6 +332,15 @@ int ak_batch_set_graph_mode(void *handle, int enable) {
   Batch *b = BT(handle);
   if (!b) return bfail("null batch handle");
   b->use_graphs = enable != 0;
+  return 0;
+}
+
+int ak_batch_set_sweep_grid(void *handle, int blocks) {
+  Batch *b = BT(handle);
+  if (!b) return bfail("null batch handle");
+  if (blocks < 0) return bfail("sweep grid of %d blocks", blocks);
+  b->sweep_grid = blocks == 0 ? 0 : std::max(8, blocks / 8 * 8);
+  b->gen++;  // the captured graphs hold the old launch shape
   return 0;
 }
 

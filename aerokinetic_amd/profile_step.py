@@ -132,7 +132,7 @@ def run_serve(storage, cases, rate, chunk, window, max_steps):
          f"batch steps {rep.gpu_steps}, serving-thread seconds {s['host_s']}")
 
 
-def run_batch(storage, ncases, steps, warmup, chunk=50):
+def run_batch(storage, ncases, steps, warmup, chunk=50, sweep_grid=0):
     """A fixed batch of `ncases` two-level requests (coarse chord 16-24, span 8, Re 1e4-3e4;
     seed 2026) stepped together in chunks of `chunk` coarse steps; wall time per batch step."""
     import torch
@@ -144,7 +144,7 @@ def run_batch(storage, ncases, steps, warmup, chunk=50):
     for r in reqs:
         project, settings, plan = S.project_of(r)
         members.append(RefinedSolver(project, settings, plan=plan, storage=storage))
-    b = RefinedBatch(storage)
+    b = RefinedBatch(storage, sweep_grid=sweep_grid)
     for m in members:
         b.add(m)
     done = 0
@@ -159,7 +159,8 @@ def run_batch(storage, ncases, steps, warmup, chunk=50):
         done += min(chunk, steps - done)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    _log(f"batch of {ncases} {storage}: {dt / steps * 1e3:.4f} ms per batch step over {steps} steps")
+    _log(f"batch of {ncases} {storage} (sweep grid {sweep_grid or 'one block per tile'}): "
+         f"{dt / steps * 1e3:.4f} ms per batch step over {steps} steps")
     b.close()
     for m in members:
         m.close()
@@ -203,13 +204,14 @@ def main(argv=None):
     ap.add_argument("--cases", type=int, default=24, help="serve / batch: number of requests")
     ap.add_argument("--rate", type=float, default=16.0, help="serve: Poisson arrival rate (1/s)")
     ap.add_argument("--chunk", type=int, default=100, help="serve: coarse steps per batch call")
+    ap.add_argument("--sweep-grid", type=int, default=0, help="batch: persistent sweep grid (0: one block per tile)")
     a = ap.parse_args(argv)
     if a.workload == "copy":
         run_copy(a.copy_gib, a.steps)
     elif a.workload == "traffic":
         run_traffic([x for x in a.storages.split(",") if x], a.copy_gib)
     elif a.workload == "batch":
-        run_batch(a.storage, a.cases, a.steps, a.warmup)
+        run_batch(a.storage, a.cases, a.steps, a.warmup, sweep_grid=a.sweep_grid)
     elif a.workload == "serve":
         run_serve(a.storage, a.cases, a.rate, a.chunk, 400, a.steps)
     elif a.workload in ("slab-y", "slab-x"):

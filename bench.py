@@ -33,6 +33,9 @@ retirement when the 2000-step Cl window varies < 0.5 % or at the step cap; cases
 p50/p99 latency.  With --tile auto the batch picks its sweep tile on its first real steps
 (RefinedBatch.tune; reported as `tile`).
 
+polar_sweep: 64 two-level FP32 cases of uneven size (chord 32-96, span 8-32, 540M cells in
+all) stepped as one batch, against the same cases one at a time; MLUPS of both.
+
 copy: the library's streaming device copy of 4 GiB (attainable HBM read+write bandwidth).
 
 cpu_baseline: the FP64 oracle (C/OpenMP, the reference's two-array pull data flow) on the
@@ -370,6 +373,65 @@ def serving_gpu(args):
     return out
 
 
+def polar_gpu(args):
+    """The batched polar sweep: `--polar-cases` two-level FP32 cases of uneven size (NACA 4-digit,
+    camber 0-4 %, thickness 9-15 %, alpha -4..12 deg, Re 1e4-1e5 log-uniform, chord 32-96 coarse
+    cells, span 8-32 cells, 12c x 8c domains; seed 2026), each kernel of a batch step one launch
+    over all members' concatenated work lists, against the same cases stepped one at a time.
+    Both modes run `--polar-steps` coarse steps per case (loads every step) after 10 untimed
+    warm-up steps, on the same solver objects (the flow is re-initialised in between)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    import torch
+    from aerokinetic_amd import serving as S
+    from aerokinetic_amd.batch import RefinedBatch
+    from aerokinetic_amd.refined import RefinedSolver, host_setup
+    reqs = S.polar_requests(args.polar_cases, seed=2026, chord=(32, 96), span=(8, 32))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(8) as pool:
+        bundles = list(pool.map(lambda r: (lambda b: (b, host_setup(b[0], b[2])))(S.project_of(r)), reqs))
+    solvers = [RefinedSolver(b[0], b[1], plan=b[2], storage="fp32", host=h) for b, h in bundles]
+    build_s = time.perf_counter() - t0
+    updates = [int(np.prod(m.plan.shape)) + 2 * 8 * int(np.prod(m.plan.box.shape)) for m in solvers]
+    steps = args.polar_steps
+    log(f"polar sweep: {len(solvers)} cases, {sum(updates) / 1e6:.0f}M updates per coarse step, built in {build_s:.1f} s")
+    # one at a time
+    seq = 0.0
+    for m in solvers:
+        m.advance(10, 1)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        m.advance(steps, 1)
+        torch.cuda.synchronize()
+        seq += time.perf_counter() - t
+        m.initialize_flow()
+    # all together
+    batch = RefinedBatch("fp32")
+    for m in solvers:
+        batch.add(m)
+    batch.advance(10)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    _, failed = batch.advance(steps)
+    torch.cuda.synchronize()
+    bat = time.perf_counter() - t
+    batch.close()
+    for m in solvers:
+        m.close()
+    torch.cuda.empty_cache()
+    total = sum(updates) * steps
+    out = {"workload": (f"{len(reqs)} two-level FP32 cases (seed 2026): NACA 4-digit camber 0-4 %, thickness 9-15 %, "
+                        f"alpha -4..12 deg, Re 1e4-1e5, chord 32-96 coarse cells, span 8-32 cells, 12c x 8c domains; "
+                        f"{steps} coarse steps per case, loads every step"),
+           "cases": len(reqs), "updates_per_coarse_step": int(sum(updates)),
+           "smallest_largest_updates": [int(min(updates)), int(max(updates))], "host_build_s": build_s,
+           "batched_s": bat, "batched_mlups": total / bat / 1e6, "transfer_failures": len(failed),
+           "sequential_s": seq, "sequential_mlups": total / seq / 1e6, "batched_vs_sequential": seq / bat}
+    log(f"  batched {out['batched_mlups']:.0f} MLUPS ({bat:.2f} s), one at a time {out['sequential_mlups']:.0f} MLUPS "
+        f"({seq:.2f} s): {seq / bat:.2f}x")
+    return out
+
+
 def main():
     faulthandler.enable()
     # host cores this process may use, read before any runtime binds a thread
@@ -389,6 +451,8 @@ def main():
     ap.add_argument("--exterior-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve-cases", type=int, default=6, help="0 skips the serving measurement")
+    ap.add_argument("--polar-cases", type=int, default=64, help="0 skips the batched polar sweep")
+    ap.add_argument("--polar-steps", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=1.0)
     ap.add_argument("--serve-storages", default="fp16")
     ap.add_argument("--serve-chunk", type=int, default=250)
@@ -431,6 +495,7 @@ def main():
         log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.4f} ms/step; sweep {r['sweep_ms']:.4f} ms, boundary "
             f"{r['boundary_ms']:.4f} ms, ring + bookkeeping {r['ring_and_bookkeeping_ms']:.4f} ms")
     serving = serving_gpu(args) if args.serve_cases > 0 else None
+    polar = polar_gpu(args) if args.polar_cases > 0 else None
     import torch
     torch.cuda.empty_cache()
     traffic, why = (measure_traffic(storages) if args.traffic else (None, "skipped (--traffic 0)"))
@@ -477,7 +542,8 @@ def main():
                                               "MLUPS over the physical cells)",
                                   "by_storage": {st: strip(r) for st, r in ext.items()},
                                   "vs_fp64": compare_loads(ext)},
-            "serving": serving}
+            "serving": serving,
+            "polar_sweep": polar}
     print(json.dumps(line))
 
 

@@ -270,6 +270,11 @@ int ak_batch_add(void *handle, void *refined);
 int ak_batch_remove(void *handle, void *refined);
 int ak_batch_size(void *handle);
 int ak_batch_set_graph_mode(void *handle, int enable);
+/* Launch shape of the batched sweep (no counterpart in the reference, which runs one case per
+ * process): 0 (default) one block per 256-cell tile of the concatenated tile list; blocks > 0
+ * a persistent grid of that many blocks (rounded down to a multiple of 8, at least 8) that
+ * loops over the tiles.  Results are the same either way. */
+int ak_batch_set_sweep_grid(void *handle, int blocks);
 /* Enqueue nsteps coarse steps of every member on the batch stream (returns before they run);
  * sample_every 1 (loads of every step into each member's history) or 0.  Call
  * ak_batch_synchronize before reading a member's history (ak_read_history on

@@ -68,3 +68,27 @@ def test_tuned_batch_matches_single_cases():
         for level in (0, 1):
             np.testing.assert_array_equal(s.populations(level), m.populations(level))
     batch.close()
+
+
+def test_persistent_batched_sweep_matches_one_block_per_tile():
+    """The batched sweep as a persistent grid of 64 blocks looping over the members' tiles gives
+    the same histories and populations as one block per tile."""
+    from aerokinetic_amd.batch import RefinedBatch
+    storage = "fp16"
+    reqs = _reqs(3)
+    plain = [_solver(r, storage) for r in reqs]
+    looped = [_solver(r, storage) for r in reqs]
+    a, b = RefinedBatch(storage), RefinedBatch(storage, sweep_grid=64)
+    for m in plain:
+        a.add(m)
+    for m in looped:
+        b.add(m)
+    ra, fa = a.advance(6)
+    rb, fb = b.advance(6)
+    assert not fa and not fb
+    for p, q in zip(plain, looped):
+        np.testing.assert_array_equal(_loads(ra[p]), _loads(rb[q]))
+        for level in (0, 1):
+            np.testing.assert_array_equal(p.populations(level), q.populations(level))
+    a.close()
+    b.close()
