@@ -104,9 +104,11 @@ __device__ __forceinline__ long long loc_in(const Desc &L, int x, int y, int z, 
 // Everything is carried as deviations from the rest state (central moments
 // of f minus those of w about the same velocity), so no O(1) quantity is
 // subtracted from another: the arithmetic keeps the precision of g itself.
-// Returns drho and |u|^2 for the fused diagnostics, and u (conserved by the collision).
+// Returns drho and |u|^2 for the fused diagnostics, and u (conserved by the collision).  The
+// output populations are multiplied by out_scale (a power of two: the FP16 storage scale, or 1).
 template <typename C>
-__device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out, C (&u_out)[3]) {
+__device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out, C (&u_out)[3],
+                                        C out_scale = C(1)) {
   // forward: raw moments up to second order by three one-dimensional passes
   C Z0[9], Z1[9], Z2[9];
 #pragma unroll
@@ -181,15 +183,19 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
           + C(2) * (ax * b2yz + ay * b2xz + az * b2xy) * ir2 + C(8) * bxy * bxz * byz * ir2
           - ((ux2 + uy2 + uz2) * (C(1) / C(9)) + (ux2 * uy2 + ux2 * uz2 + uy2 * uz2) * third + ux2 * uy2 * uz2);
   }
-  // back-transform: per axis, central moments (k0,k1,k2) about u -> populations (-1,0,+1)
-  const C half = C(0.5);
+  // back-transform: per axis, central moments (k0,k1,k2) about u -> populations (-1,0,+1).  The
+  // halves of the two outer populations of each line are not applied in the passes: every
+  // value of one line carries the same power-of-two factor, so a population with n nonzero
+  // velocity components comes out 2^n times too large, and one multiply by out_scale * 2^-n
+  // at the end gives exactly what halving in every pass would (power-of-two scaling is exact).
+  const C fac[4] = {out_scale, out_scale * C(0.5), out_scale * C(0.25), out_scale * C(0.125)};
 #pragma unroll
   for (int l = 0; l < 9; ++l) {  // z lines: index 3l + c
     const C k0 = K[3 * l], k1 = K[3 * l + 1], k2 = K[3 * l + 2];
     const C m1 = k1 + uz * k0, m2 = k2 + uz * (C(2) * k1 + uz * k0);
     K[3 * l + 1] = k0 - m2;
-    K[3 * l + 2] = half * (m2 + m1);
-    K[3 * l + 0] = half * (m2 - m1);
+    K[3 * l + 2] = m2 + m1;
+    K[3 * l + 0] = m2 - m1;
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -199,16 +205,17 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
       const C k0 = K[o], k1 = K[o + 3], k2 = K[o + 6];
       const C m1 = k1 + uy * k0, m2 = k2 + uy * (C(2) * k1 + uy * k0);
       K[o + 3] = k0 - m2;
-      K[o + 6] = half * (m2 + m1);
-      K[o + 0] = half * (m2 - m1);
+      K[o + 6] = m2 + m1;
+      K[o + 0] = m2 - m1;
     }
 #pragma unroll
   for (int o = 0; o < 9; ++o) {  // x lines: index 9a + o
     const C k0 = K[o], k1 = K[o + 9], k2 = K[o + 18];
     const C m1 = k1 + ux * k0, m2 = k2 + ux * (C(2) * k1 + ux * k0);
-    g[o + 9] = k0 - m2;
-    g[o + 18] = half * (m2 + m1);
-    g[o + 0] = half * (m2 - m1);
+    const int n = (o / 3 != 1) + (o % 3 != 1);  // nonzero y and z components of this line
+    g[o + 9] = (k0 - m2) * fac[n];
+    g[o + 18] = (m2 + m1) * fac[n + 1];
+    g[o + 0] = (m2 - m1) * fac[n + 1];
   }
   drho_out = drho;
   u2_out = ux2 + uy2 + uz2;
@@ -251,15 +258,15 @@ __device__ __forceinline__ void conserve_fp16(typename St<S>::C (&g)[NQ], float 
   }
 }
 
-// The FP16 values conserve_fp16 followed by the storage rounding stores, computed in units of
-// the scaled value s = g * scale (scale is a power of two, so each step gives the same
-// result in either unit and the rounding of each direction is done once).
-__device__ __forceinline__ void fp16_conserving(const float (&g)[NQ], float scale, _Float16 (&h)[NQ]) {
+// The FP16 values conserve_fp16 followed by the storage rounding stores, computed from the
+// scaled values s = g * scale (the collision's output with out_scale = scale; scale is a power
+// of two, so each step gives the same result in either unit and each direction is rounded once).
+__device__ __forceinline__ void fp16_conserving(const float (&s)[NQ], _Float16 (&h)[NQ]) {
   float dm = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
 #pragma unroll
   for (int i = 0; i < NQ; ++i) {
     if (cxq(i) != 0 && cyq(i) != 0 && czq(i) != 0) continue;
-    const float si = g[i] * scale;
+    const float si = s[i];
     h[i] = (_Float16)si;
     const float d = (float)h[i] - si;
     dm += d;
@@ -270,7 +277,7 @@ __device__ __forceinline__ void fp16_conserving(const float (&g)[NQ], float scal
 #pragma unroll
   for (int i = 0; i < NQ; ++i) {
     if (cxq(i) == 0 || cyq(i) == 0 || czq(i) == 0) continue;
-    h[i] = (_Float16)(g[i] * scale - 0.125f * (dm + (float)cxq(i) * dx + (float)cyq(i) * dy + (float)czq(i) * dz));
+    h[i] = (_Float16)(s[i] - 0.125f * (dm + (float)cxq(i) * dx + (float)cyq(i) * dy + (float)czq(i) * dz));
   }
 }
 

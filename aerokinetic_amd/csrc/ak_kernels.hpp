@@ -216,10 +216,11 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         g[q] = ld_g<S>(bq + boff, inv_scale);
       }
       C uc[3];
-      collide<C>(g, ws, wb, drho, u2, uc);
-      // FP16: the mass- and momentum-conserving rounding, each value rounded once
+      // FP16: the collision returns the populations times the storage scale, and the mass- and
+      // momentum-conserving rounding rounds each of them once
+      collide<C>(g, ws, wb, drho, u2, uc, S == 2 ? (C)scale : C(1));
       _Float16 h[S == 2 ? NQ : 1];
-      if constexpr (S == 2) fp16_conserving(g, scale, h);
+      if constexpr (S == 2) fp16_conserving(g, h);
       const long long splane = plane;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
