@@ -538,7 +538,7 @@ int set_exterior(Lattice *s) {
 
 extern "C" {
 
-const char *ak_version(void) { return "aerokinetic-mi355x 0.3 (gfx950 HIP, AA-pattern D3Q27 cumulant)"; }
+const char *ak_version(void) { return "aerokinetic-mi355x 0.4 (gfx950 HIP, AA-pattern D3Q27 cumulant)"; }
 
 int ak_last_error(void *handle, char *buf, size_t len) {
   const std::string &e = handle ? H(handle)->err : g_error;

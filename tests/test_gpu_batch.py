@@ -162,3 +162,18 @@ def test_service_serves_a_flapped_request_of_millions_of_cells():
     by_id = {r.id: r for r in rep.results}
     assert by_id[0].cells >= 2_000_000 and 400 <= by_id[0].steps <= 600
     assert np.isfinite([by_id[0].cl, by_id[0].cd, by_id[0].cm]).all()
+
+
+def test_service_retires_a_converged_case_at_its_window():
+    """Cl-window retirement on the GPU: a symmetric section at zero incidence (Cl stays within
+    1e-5 of zero, below tol x the 0.1 floor) retires as converged at its first full window,
+    while a lifting case at Re 1e4, which sheds vortices, runs to the step cap."""
+    reqs = [S.CaseRequest(0, 0.0, 0.0, 0.4, 0.12, 0.0, 1e4, 16, 8),
+            S.CaseRequest(1, 0.0, 0.0, 0.4, 0.12, 2.0, 1e4, 16, 8)]
+    srv = S.Server(storage="fp32", chunk=100, window=300, tol=0.005, max_steps=1200, builders=2)
+    rep = srv.serve(reqs)
+    by_id = {r.id: r for r in rep.results}
+    print({i: (r.reason, r.steps, r.cl_window_variation) for i, r in by_id.items()}, flush=True)
+    assert by_id[0].reason == "converged" and 300 <= by_id[0].steps <= 400
+    assert by_id[0].cl_window_variation <= 0.005 and abs(by_id[0].cl) < 1e-3
+    assert by_id[1].reason == "max_steps" and by_id[1].steps == 1200
