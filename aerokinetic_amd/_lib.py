@@ -48,6 +48,10 @@ _SIGS = {
     "ak_set_farfield": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "ak_init_seeded": (ctypes.c_int, [_vp, _dp]),
     "ak_set_sweep_tile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "ak_set_sweep_tile_form": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ak_next_sweep_odd": (ctypes.c_int, [_vp]),
+    "ak_populations_written": (ctypes.c_int, [_vp]),
+    "ak_set_sweep_waves": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "ak_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int]),
     "ak_run_profiled": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _dp]),
     "ak_set_graph_mode": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -98,7 +98,14 @@ struct Lattice {
   bool use_graphs = true;
   unsigned long long gen = 1;
   GraphCache graphs;
-  SweepGeom geom{}, geom_small{};
+  // sweep tiles of the even (in-place) and odd (gathering) AA forms; geom_small: the periodic
+  // form's bound of 256-thread blocks
+  SweepGeom geom{}, geom_odd{}, geom_small{};
+  // cap on the sweep's resident waves per SIMD of each AA form (0: none), enforced through the
+  // LDS each sweep block reserves (the kernel uses none): fewer concurrent 27-row streams can
+  // move more bytes per second than a full CU; lds_per_cu: the device's LDS per CU
+  int waves_even = 0, waves_odd = 0;
+  int lds_per_cu = 0;
   std::string err;
   std::vector<void *> owned;
 };
@@ -164,15 +171,29 @@ SweepGeom make_geom(int nx, int ny, int nz, int bx, int by) {
   }
   G.gz = nz;
   G.nblocks = G.gx * G.gy * G.gz;
+  magic_of((unsigned)G.gx, G.m_gx, G.s_gx);
+  magic_of((unsigned)(G.gx * G.gy), G.m_gxy, G.s_gxy);
+  magic_of((unsigned)G.bx, G.m_bx, G.s_bx);
+  magic_of((unsigned)nx, G.m_nx, G.s_nx);
   return G;
 }
 // default: 64 x 4 tiles when rows are whole waves, row-linear blocks of 256 otherwise
 SweepGeom default_geom(int nx, int ny, int nz) {
   return nx % 64 == 0 ? make_geom(nx, ny, nz, 64, 4) : make_geom(nx, ny, nz, 0, 256);
 }
-void set_geom(Lattice *s, int bx, int by) { s->geom = make_geom(s->nx, s->ny, s->nz, bx, by); }
+void set_geom(Lattice *s, int bx, int by) { s->geom = s->geom_odd = make_geom(s->nx, s->ny, s->nz, bx, by); }
 
 // ---------------------------------------------------------------- launches
+
+// Dynamic LDS per sweep block that caps the resident waves per SIMD at `waves` (0: no cap):
+// a CU holds 4 SIMDs, so waves * 4 / (waves per block) blocks share its LDS.
+size_t sweep_lds(const Lattice *s, int waves, int threads) {
+  if (waves <= 0 || s->lds_per_cu <= 0) return 0;
+  const int per_block = (threads + 63) / 64;
+  const int blocks = waves * 4 / per_block;
+  if (blocks < 1) return 0;
+  return (size_t)(s->lds_per_cu / blocks) & ~(size_t)255;
+}
 
 // The perimeter moments the sweep keeps and the farfield reads, or null (no characteristic
 // farfield, or a solid cell on the perimeter: the farfield then reads the lattice).
@@ -205,18 +226,21 @@ void launch_sweep(Lattice *s, bool odd, bool diag, hipEvent_t e0 = nullptr, hipE
   using C = typename St<S>::C;
   T *A = (T *)s->pop;
   const bool pxy = s->ff_kind == AK_FARFIELD_PERIODIC;  // AK_FARFIELD_NONE keeps ghost indexing
-  // the periodic form is bounded to 256-thread blocks; larger tiles fall back to 64x4 there
-  const SweepGeom &G = pxy && s->geom.bx * s->geom.by > 256 ? s->geom_small : s->geom;
+  // each AA form has its own tile; the periodic form is bounded to 256-thread blocks, and
+  // larger tiles fall back to 64x4 there
+  const SweepGeom &GF = odd ? s->geom_odd : s->geom;
+  const SweepGeom &G = pxy && GF.bx * GF.by > 256 ? s->geom_small : GF;
   const dim3 grid(G.nblocks), block(G.bx * G.by);
+  const size_t lds = sweep_lds(s, odd ? s->waves_odd : s->waves_even, G.bx * G.by);
   const C ws = (C)s->ws, wb = (C)s->wb;
   C *pm = perimeter_moments<S>(s);
 #define AK_SWEEP(O, P, D)                                                                                           \
   do {                                                                                                              \
     if (e0)                                                                                                         \
-      hipExtLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, e0, e1, 0, A, s->d_solid, s->L, G, ws, \
+      hipExtLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, lds, s->stream, e0, e1, 0, A, s->d_solid, s->L, G, ws, \
                             wb, s->scale, s->inv_scale, s->d_diag, pm);                                             \
     else                                                                                                            \
-      hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, 0, s->stream, A, s->d_solid, s->L, G, ws, wb, s->scale, \
+      hipLaunchKernelGGL((k_sweep<S, O, P, D>), grid, block, lds, s->stream, A, s->d_solid, s->L, G, ws, wb, s->scale, \
                          s->inv_scale, s->d_diag, pm);                                                              \
   } while (0)
   if (odd) {
@@ -618,8 +642,10 @@ int ak_lattice_create(void **handle, int device, int nx, int ny, int nz, int sto
     return fail(nullptr, "lattice plane of %lld elements too large for 32-bit in-plane offsets", (long long)(P * (ny + 2)));
   }
   s->pop_bytes = s->pop_elems * s->elem_size;
-  s->geom = default_geom(nx, ny, nz);
+  s->geom = s->geom_odd = default_geom(nx, ny, nz);
   s->geom_small = s->geom;
+  if (hipDeviceGetAttribute(&s->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess)
+    s->lds_per_cu = 0;
   s->hist_cap = history_capacity > 0 ? history_capacity : 4096;
   s->hist_width = AK_HIST_WIDTH;
   if (dalloc(s, &s->d_solid, (size_t)(nx + 2) * (ny + 2)) || dalloc(s, &s->d_diag, 1) ||
@@ -664,6 +690,13 @@ int ak_bind_populations(void *handle, void *device_ptr, size_t bytes) {
   // ghosts and padding start at zero
   hipLaunchKernelGGL((k_fill<uint8_t>), dim3(1024), dim3(256), 0, s->stream, (uint8_t *)s->pop, (long long)s->pop_bytes, (uint8_t)0);
   HIPCHK(s, hipGetLastError());
+  return 0;
+}
+
+int ak_populations_written(void *handle) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  s->pm_valid = false;  // the next run call recomputes the perimeter moments from the populations
   return 0;
 }
 
@@ -890,6 +923,41 @@ int ak_set_sweep_tile(void *handle, int bx, int by) {
     return fail(s, "tile %dx%d: bx multiple of 64, bx*by <= 1024", bx, by);
   }
   set_geom(s, bx, by);
+  s->waves_even = s->waves_odd = 0;
+  return 0;
+}
+
+int ak_set_sweep_waves(void *handle, int odd, int waves) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  if (odd != 0 && odd != 1) return fail(s, "AA form %d: 0 (even) or 1 (odd)", odd);
+  if (waves < 0 || waves > 8) return fail(s, "waves per SIMD %d: 0 (no cap) to 8", waves);
+  const SweepGeom &G = odd ? s->geom_odd : s->geom;
+  const int threads = G.bx * G.by;
+  if (waves > 0 && (s->lds_per_cu <= 0 || waves * 4 < (threads + 63) / 64))
+    return fail(s, "a cap of %d waves per SIMD does not fit %d-thread sweep blocks", waves, threads);
+  if (sweep_lds(s, waves, threads) > 65536)
+    return fail(s, "a cap of %d waves per SIMD needs more than 64 KiB of LDS per %d-thread block", waves, threads);
+  s->gen++;  // invalidates captured step graphs
+  (odd ? s->waves_odd : s->waves_even) = waves;
+  return 0;
+}
+
+int ak_next_sweep_odd(void *handle) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  return s->sweeps % 2 == 0 ? 1 : 0;
+}
+
+int ak_set_sweep_tile_form(void *handle, int odd, int bx, int by) {
+  Lattice *s = H(handle);
+  if (!s) return fail(nullptr, "null lattice handle");
+  if (odd != 0 && odd != 1) return fail(s, "AA form %d: 0 (even) or 1 (odd)", odd);
+  const SweepGeom keep = odd ? s->geom : s->geom_odd;
+  const int keep_waves = odd ? s->waves_even : s->waves_odd;
+  if (ak_set_sweep_tile(handle, bx, by)) return -1;
+  (odd ? s->geom : s->geom_odd) = keep;
+  (odd ? s->waves_even : s->waves_odd) = keep_waves;
   return 0;
 }
 

@@ -277,7 +277,12 @@ __device__ __forceinline__ void fp16_conserving(const float (&s)[NQ], _Float16 (
 #pragma unroll
   for (int i = 0; i < NQ; ++i) {
     if (cxq(i) == 0 || cyq(i) == 0 || czq(i) == 0) continue;
-    h[i] = (_Float16)(s[i] - 0.125f * (dm + (float)cxq(i) * dx + (float)cyq(i) * dy + (float)czq(i) * dz));
+    // rounded to FP32 first, then to FP16, in every kernel: the empty asm keeps the compiler from
+    // fusing the subtraction and the conversion into one mixed-precision instruction (a single
+    // rounding), which it does or not depending on the surrounding code
+    float c = s[i] - 0.125f * (dm + (float)cxq(i) * dx + (float)cyq(i) * dy + (float)czq(i) * dz);
+    asm volatile("" : "+v"(c));
+    h[i] = (_Float16)c;
   }
 }
 

@@ -16,7 +16,23 @@ struct SweepGeom {
   int gx, gy, gz;  // tiles per axis; row-linear: gx blocks per plane, gy = 1
   int nblocks;
   int lin;         // 1: row-linear mode
+  // multiply-shift constants of the divisions by gx, gx*gy, bx and nx (magic_of)
+  unsigned m_gx, m_gxy, m_bx, m_nx;
+  int s_gx, s_gxy, s_bx, s_nx;
 };
+
+// Division by an invariant d in [1, 2^31) of any n in [0, 2^31) as one multiply and a shift:
+// with l = ceil(log2 d), m = floor(2^(31+l) / d) + 1 (< 2^32) gives 2^(31+l) < m d <= 2^(31+l) + 2^l,
+// so floor(n m / 2^(31+l)) = floor(n / d) for every n < 2^31 (Granlund and Montgomery, 1994).
+__host__ __device__ inline void magic_of(unsigned d, unsigned &m, int &shift) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  m = (unsigned)((1ull << (31 + l)) / d + 1);
+  shift = 31 + l;
+}
+__host__ __device__ __forceinline__ unsigned magic_div(unsigned n, unsigned m, int shift) {
+  return (unsigned)(((unsigned long long)n * m) >> shift);
+}
 
 // Order-independent maxima/minima as bit patterns of non-negative doubles.  Blocks
 // update one of DIAG_SLOTS slots (block index modulo the slot count) so that atomics
@@ -54,6 +70,93 @@ __device__ __forceinline__ C wave_min(C v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) { C t = __shfl_xor(v, o, 64); v = t < v ? t : v; }
   return v;
+}
+
+// Order-preserving keys of floating-point values: k(a) < k(b) exactly when a < b (finite
+// values and infinities); the complement of a key is the key of the negated value.
+__device__ __forceinline__ unsigned ord_key(float v) {
+  const unsigned b = __float_as_uint(v);
+  return b ^ ((unsigned)((int)b >> 31) | 0x80000000u);
+}
+__device__ __forceinline__ unsigned long long ord_key(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return b ^ ((unsigned long long)((long long)b >> 63) | 0x8000000000000000ull);
+}
+__device__ __forceinline__ float key_value(unsigned k) {
+  return __uint_as_float(k & 0x80000000u ? k ^ 0x80000000u : ~k);
+}
+__device__ __forceinline__ double key_value(unsigned long long k) {
+  return __longlong_as_double((long long)(k & 0x8000000000000000ull ? k ^ 0x8000000000000000ull : ~k));
+}
+__device__ __forceinline__ unsigned key_bits(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ unsigned long long key_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+
+// Wave maximum of unsigned keys through DPP row operations (all 64 lanes must be active):
+// two quad permutations and two row rotations leave every lane of a row of 16 with the row's
+// maximum, and the four row maxima are combined from lanes 0, 16, 32 and 48 in scalar registers.
+template <int CTL>
+__device__ __forceinline__ unsigned dpp_u32(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTL, 0xF, 0xF, false);
+}
+template <int CTL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const unsigned lo = dpp_u32<CTL>((unsigned)v), hi = dpp_u32<CTL>((unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned readlane_key(unsigned v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ unsigned long long readlane_key(unsigned long long v, int lane) {
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, lane), hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), lane);
+  return ((unsigned long long)hi << 32) | lo;
+}
+template <typename K>
+__device__ __forceinline__ K wave_max_key(K v) {
+  auto mx = [](K a, K b) { return a > b ? a : b; };
+  if constexpr (sizeof(K) == 4) {
+    v = mx(v, dpp_u32<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = mx(v, dpp_u32<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = mx(v, dpp_u32<0x124>(v));  // row_ror:4
+    v = mx(v, dpp_u32<0x128>(v));  // row_ror:8
+  } else {
+    v = mx(v, dpp_u64<0xB1>(v));
+    v = mx(v, dpp_u64<0x4E>(v));
+    v = mx(v, dpp_u64<0x124>(v));
+    v = mx(v, dpp_u64<0x128>(v));
+  }
+  return mx(mx(readlane_key(v, 0), readlane_key(v, 16)), mx(readlane_key(v, 32), readlane_key(v, 48)));
+}
+
+// Fused diagnostics of one wave of the sweep (every lane active): largest |u|^2 and |drho|,
+// smallest rho and the non-finite flag over the lanes with `counted`, folded by lane 0 into
+// diagnostics slot `slot` (a slot is only ever touched by blocks of one XCD: slot = block
+// index mod 1024, a multiple of the 8 XCDs).  Each extreme is reduced as an order-preserving
+// integer key; the results equal maxima and minima over the cell values themselves (the
+// smallest rho is 1 + the smallest drho, since rounding of 1 + x is monotone in x).
+template <typename C>
+__device__ __forceinline__ void diag_wave(bool counted, C drho, C u2, DiagAcc *__restrict__ diag, int slot) {
+  using K = decltype(ord_key(C(0)));
+  const bool fu = isfinite(u2), fd = isfinite(drho);
+  const K kd = ord_key(drho);
+  const K ku = wave_max_key<K>(counted && fu ? key_bits(u2) : K(0));  // u2 >= +0: its bits are a key
+  const K khi = wave_max_key<K>(counted && fd ? kd : K(0));           // key of the largest drho
+  const K klo = wave_max_key<K>(counted && fd ? (K)~kd : K(0));       // complement of the smallest drho's key
+  const bool bad = __any(counted && !(fu && fd));
+  if ((threadIdx.x & 63) != 0) return;
+  DiagSlot *sl = &diag->s[slot];
+  if (ku != K(0)) {
+    C u2max;
+    if constexpr (sizeof(K) == 4) u2max = __uint_as_float(ku);
+    else u2max = __longlong_as_double((long long)ku);
+    atomicMax(&sl->u2max, (unsigned long long)__double_as_longlong((double)u2max));
+  }
+  if (khi != K(0)) {  // some lane had a counted finite drho (finite keys are never 0)
+    const C dmax = key_value(khi), dmin = key_value((K)~klo);
+    const C amax = fabs(dmax), amin = fabs(dmin);
+    atomicMax(&sl->drmax, (unsigned long long)__double_as_longlong((double)(amin > amax ? amin : amax)));
+    const C rmin = C(1) + dmin;
+    if (rmin > C(0)) atomicMin(&sl->rhomin, (unsigned long long)__double_as_longlong((double)rmin));
+    else atomicMax(&sl->nonfinite, 2ull);  // non-positive density
+  }
+  if (bad) atomicMax(&sl->nonfinite, 1ull);
 }
 
 // Keep a wave-uniform pointer in scalar registers (the compiler otherwise may carry the
@@ -141,6 +244,15 @@ __device__ __forceinline__ void perimeter_put(C *__restrict__ pm, const Desc &L,
   }
 }
 
+// The 27 loaded populations as register values at this point (an empty asm that "modifies"
+// them): the loads are issued, and waited for, ahead of the mask test that follows, instead
+// of being moved behind it into the branch that uses them.
+template <typename C>
+__device__ __forceinline__ void loads_issued(C (&g)[NQ]) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) asm volatile("" : "+v"(g[q]));
+}
+
 // One tile (block `lb` of the lattice's sweep grid) of the fused AA stream + collide.
 // `slot` picks the diagnostics accumulator slot of this block; with `pm` the cells on the
 // x and y faces also write their perimeter moments.
@@ -153,26 +265,37 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
   using C = typename St<S>::C;
   // the block's tile coordinates are wave-uniform: kept in scalar registers, so that the
   // per-direction plane bases below are scalar arithmetic, not 64-bit vector adds
+  // (divisions by multiply and shift: these index computations precede every load of the block)
   int x, y, z;
+  const unsigned ub = (unsigned)__builtin_amdgcn_readfirstlane(lb);
   if (G.lin) {
-    const int j0 = __builtin_amdgcn_readfirstlane((lb % G.gx) * G.bx);
-    z = __builtin_amdgcn_readfirstlane(lb / G.gx);
+    const unsigned zq = magic_div(ub, G.m_gx, G.s_gx);
+    const int j0 = __builtin_amdgcn_readfirstlane((int)(ub - zq * (unsigned)G.gx) * G.bx);
+    z = __builtin_amdgcn_readfirstlane((int)zq);
     const int j = j0 + (int)threadIdx.x;
-    y = j / L.nx;
+    y = (int)magic_div((unsigned)j, G.m_nx, G.s_nx);
     x = j - y * L.nx;
   } else {
-    const int tx = __builtin_amdgcn_readfirstlane(lb % G.gx);
-    const int ty = __builtin_amdgcn_readfirstlane((lb / G.gx) % G.gy);
-    x = tx * G.bx + (int)(threadIdx.x % G.bx);
-    y = ty * G.by + (int)(threadIdx.x / G.bx);
-    z = __builtin_amdgcn_readfirstlane(lb / (G.gx * G.gy));
+    const unsigned zq = magic_div(ub, G.m_gxy, G.s_gxy), r = ub - zq * (unsigned)(G.gx * G.gy);
+    const unsigned tyq = magic_div(r, G.m_gx, G.s_gx);
+    const int tx = __builtin_amdgcn_readfirstlane((int)(r - tyq * (unsigned)G.gx));
+    const int ty = __builtin_amdgcn_readfirstlane((int)tyq);
+    const unsigned ry = magic_div(threadIdx.x, G.m_bx, G.s_bx);
+    x = tx * G.bx + (int)(threadIdx.x - ry * (unsigned)G.bx);
+    y = ty * G.by + (int)ry;
+    z = __builtin_amdgcn_readfirstlane((int)zq);
   }
-  bool active = x < L.nx && y < L.ny;
+  // mask: 0 fluid, 1 solid, 2 fluid outside the diagnostics, 3 exterior ring (its own kernel).
+  // The mask byte is loaded beside the populations, not before them: every lane inside the
+  // lattice gathers and collides, and only the cells the mask admits store (a solid or ring
+  // cell's slots are read, never written, and no other cell writes them), so no wave waits a
+  // memory round trip before issuing its 27 loads.
+  const bool inside = x < L.nx && y < L.ny;
   uint8_t mk = 1;
-  // mask: 0 fluid, 1 solid, 2 fluid outside the diagnostics, 3 exterior ring (its own kernel)
-  if (active) { mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)]; active = mk != 1 && mk != 3; }
+  if (inside) mk = solid[(long long)(y + 1) * (L.nx + 2) + (x + 1)];
+  bool active = false;
   C drho = 0, u2 = 0;
-  if (active) {
+  if (inside) {
     C g[NQ];
     if constexpr (PXY) {
       const long long cell = L.base + cell_of(L, x, y, z);
@@ -183,14 +306,18 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         else src = cell + (long long)q * L.QS;
         g[q] = ld<S>(A + src, inv_scale);
       }
-      collide<C>(g, ws, wb, drho, u2);
-      conserve_fp16<S>(g, scale, inv_scale);
+      loads_issued(g);
+      active = mk != 1 && mk != 3;
+      if (active) {
+        collide<C>(g, ws, wb, drho, u2);
+        conserve_fp16<S>(g, scale, inv_scale);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        long long dst;
-        if constexpr (ODD) dst = cell + (long long)q * L.QS + nb_off<PXY>(L, x, y, z, cxq(q), cyq(q), czq(q));
-        else dst = cell + (long long)opp(q) * L.QS;
-        stv<S>(A + dst, g[q], scale);
+        for (int q = 0; q < NQ; ++q) {
+          long long dst;
+          if constexpr (ODD) dst = cell + (long long)q * L.QS + nb_off<PXY>(L, x, y, z, cxq(q), cyq(q), czq(q));
+          else dst = cell + (long long)opp(q) * L.QS;
+          stv<S>(A + dst, g[q], scale);
+        }
       }
     } else {
       // A block covers one z plane, so every direction's shift (including the periodic
@@ -215,63 +342,39 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         const gchar *bq = uniform_ptr(A + (plane + d));
         g[q] = ld_g<S>(bq + boff, inv_scale);
       }
-      C uc[3];
-      // FP16: the collision returns the populations times the storage scale, and the mass- and
-      // momentum-conserving rounding rounds each of them once
-      collide<C>(g, ws, wb, drho, u2, uc, S == 2 ? (C)scale : C(1));
-      _Float16 h[S == 2 ? NQ : 1];
-      if constexpr (S == 2) fp16_conserving(g, h);
-      const long long splane = plane;
+      loads_issued(g);
+      active = mk != 1 && mk != 3;
+      if (active) {
+        C uc[3];
+        // FP16: the collision returns the populations times the storage scale, and the mass- and
+        // momentum-conserving rounding rounds each of them once
+        collide<C>(g, ws, wb, drho, u2, uc, S == 2 ? (C)scale : C(1));
+        _Float16 h[S == 2 ? NQ : 1];
+        if constexpr (S == 2) fp16_conserving(g, h);
+        const long long splane = plane;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        long long d;
-        if constexpr (ODD) {
-          const int cz = czq(q);
-          d = (long long)q * L.QS + cxq(q) + (long long)cyq(q) * L.P + (cz == 1 ? zp : cz == -1 ? zm : 0);
-        } else {
-          d = (long long)opp(q) * L.QS;
+        for (int q = 0; q < NQ; ++q) {
+          long long d;
+          if constexpr (ODD) {
+            const int cz = czq(q);
+            d = (long long)q * L.QS + cxq(q) + (long long)cyq(q) * L.P + (cz == 1 ? zp : cz == -1 ? zm : 0);
+          } else {
+            d = (long long)opp(q) * L.QS;
+          }
+          gchar *bq = uniform_ptr(A + (splane + d));
+          if constexpr (S == 2) *(__attribute__((address_space(1))) _Float16 *)(bq + boff) = h[q];
+          else st_g<S>(bq + boff, g[q], scale);
         }
-        gchar *bq = uniform_ptr(A + (splane + d));
-        if constexpr (S == 2) *(__attribute__((address_space(1))) _Float16 *)(bq + boff) = h[q];
-        else st_g<S>(bq + boff, g[q], scale);
+        // cells on the x and y faces keep their density deviation and velocity (conserved by
+        // the collision) for the next farfield pass
+        if (pm != nullptr && (x == 0 || y == 0 || x == L.nx - 1 || y == L.ny - 1))
+          perimeter_put<C>(pm, L, x, y, z, drho, uc);
       }
-      // cells on the x and y faces keep their density deviation and velocity (conserved by
-      // the collision) for the next farfield pass
-      if (pm != nullptr && (x == 0 || y == 0 || x == L.nx - 1 || y == L.ny - 1)) perimeter_put<C>(pm, L, x, y, z, drho, uc);
     }
   }
-  if constexpr (DIAG) {
-    // rho and u are conserved by the collision: the pre-collision moments are the
-    // post-collision state's.  Wave reductions run in the collision's compute type.
-    const bool counted = active && mk == 0;
-    C vu = counted ? u2 : C(0);
-    C vd = counted ? (drho < C(0) ? -drho : drho) : C(0);
-    C vr = counted ? C(1) + drho : C(3.0e38);
-    int bad = counted && !(isfinite(vu) && isfinite(vd)) ? 1 : 0;
-    if (!isfinite(vu)) vu = C(0);
-    if (!isfinite(vd)) vd = C(0);
-    if (!isfinite(vr)) vr = C(3.0e38);
-    vu = wave_max(vu); vd = wave_max(vd); vr = wave_min(vr);
-    bad = __any(bad) ? 1 : 0;
-    __shared__ C su[16], sd[16], sr[16];  // one entry per wave, up to 1024 threads
-    __shared__ int sb[16];
-    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-    if (lane == 0) { su[w] = vu; sd[w] = vd; sr[w] = vr; sb[w] = bad; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int nw = (blockDim.x + 63) / 64;
-      for (int k = 1; k < nw; ++k) {
-        vu = su[k] > vu ? su[k] : vu; vd = sd[k] > vd ? sd[k] : vd;
-        vr = sr[k] < vr ? sr[k] : vr; bad |= sb[k];
-      }
-      DiagSlot *sl = &diag->s[slot];
-      atomicMax(&sl->u2max, (unsigned long long)__double_as_longlong((double)vu));
-      atomicMax(&sl->drmax, (unsigned long long)__double_as_longlong((double)vd));
-      if (vr > C(0)) atomicMin(&sl->rhomin, (unsigned long long)__double_as_longlong((double)vr >= 3.0e38 ? 1e300 : (double)vr));
-      else atomicMax(&sl->nonfinite, 2ull);  // non-positive density
-      if (bad) atomicMax(&sl->nonfinite, 1ull);
-    }
-  }
+  // rho and u are conserved by the collision: the pre-collision moments are the
+  // post-collision state's.  Wave reductions run in the collision's compute type.
+  if constexpr (DIAG) diag_wave<C>(active && mk == 0, drho, u2, diag, slot);
 }
 
 template <int S, bool ODD, bool PXY, bool DIAG>

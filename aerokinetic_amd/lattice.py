@@ -39,11 +39,20 @@ class Lattice:
         self.h = h
         self.nbytes = int(lib.ak_population_bytes(h))
         # caller-owned population buffer; 128-byte aligned view
-        self._buf = torch.empty(self.nbytes + 128, dtype=torch.uint8, device=self.torch_device)
-        base = self._buf.data_ptr()
-        self._offset = (-base) % 128
-        _lib.check(lib.ak_bind_populations(h, _lib._vp(base + self._offset), self.nbytes), h, "ak_bind_populations")
+        self._bind(self.new_buffer())
         self.hist_width = int(lib.ak_history_width(h))
+
+    def new_buffer(self):
+        """A fresh device buffer that can hold this lattice's populations (128-byte aligned view)."""
+        return _torch().empty(self.nbytes + 128, dtype=_torch().uint8, device=self.torch_device)
+
+    def _bind(self, buf):
+        """Make `buf` (from new_buffer) the population buffer; its contents start at zero."""
+        base = buf.data_ptr()
+        offset = (-base) % 128
+        _lib.check(self.lib.ak_bind_populations(self.h, _lib._vp(base + offset), self.nbytes), self.h,
+                   "ak_bind_populations")
+        self._buf, self._offset = buf, offset
 
     def close(self):
         if getattr(self, "h", None):
@@ -87,8 +96,26 @@ class Lattice:
         """Replay each step variant from a captured HIP graph (default) or launch directly."""
         _lib.check(self.lib.ak_set_graph_mode(self.h, 1 if enable else 0), self.h, "ak_set_graph_mode")
 
-    def set_tile(self, bx, by):
-        _lib.check(self.lib.ak_set_sweep_tile(self.h, int(bx), int(by)), self.h, "ak_set_sweep_tile")
+    def set_tile(self, bx, by, odd=None):
+        """Sweep tile of both AA forms, or with odd=True / False of the odd / even form only."""
+        if odd is None:
+            _lib.check(self.lib.ak_set_sweep_tile(self.h, int(bx), int(by)), self.h, "ak_set_sweep_tile")
+        else:
+            _lib.check(self.lib.ak_set_sweep_tile_form(self.h, 1 if odd else 0, int(bx), int(by)), self.h,
+                       "ak_set_sweep_tile_form")
+
+    def set_waves(self, waves, odd):
+        """Cap the resident waves per SIMD of the odd (odd=True) or even sweep (0: no cap)."""
+        _lib.check(self.lib.ak_set_sweep_waves(self.h, 1 if odd else 0, int(waves)), self.h, "ak_set_sweep_waves")
+
+    def populations_written(self):
+        """Declare that the bound population buffer was written directly (not through this
+        object): derived state kept between run calls is rebuilt at the next run call."""
+        _lib.check(self.lib.ak_populations_written(self.h), self.h, "ak_populations_written")
+
+    def next_sweep_odd(self):
+        """True when the next sweep is the odd (gathering) AA form."""
+        return bool(_lib.check(self.lib.ak_next_sweep_odd(self.h), self.h, "ak_next_sweep_odd"))
 
     # ---- stepping
     def run(self, nsteps, first_step, sample_every):

@@ -28,12 +28,16 @@ def _log(msg):
     print(f"[profile_step] {msg}", flush=True)
 
 
-def run_uniform(case, storage, steps, warmup, sample_every, graphs=True):
+def run_uniform(case, storage, steps, warmup, sample_every, graphs=True, launch=None):
     import numpy as np
     import torch
+    from .autotune import apply_launch
     from .solver import Solver
     section, flow, grid, plan = case.build()
     s = Solver(section, flow, grid, storage=storage, plan=plan, graphs=graphs)
+    if launch:
+        apply_launch(s.lattice, launch)
+        _log(f"sweep launch (even, odd) {launch}")
     fluid = int(s._fluid.sum())
     _log(f"{case.name} storage {storage}: {int(np.prod(plan.shape))} cells, {fluid} fluid")
     s.advance(warmup, sample_every)
@@ -99,12 +103,25 @@ def run_copy(gib, steps):
     _log(f"copy {int(gib * 2**30)} bytes: {ms:.4f} ms per copy, {gbs:.1f} GB/s (read + write)")
 
 
-def run_traffic(storages, copy_gib=4.0):
+def _launches(text):
+    """{storage: ((bx, by, waves) even, (bx, by, waves) odd)} from 'fp64:256x1c3/0x256c3,fp16:...'."""
+    from .autotune import parse_launch
+    out = {}
+    for item in (text or "").split(","):
+        if item:
+            st, _, spec = item.partition(":")
+            out[st] = parse_launch(spec)
+    return out
+
+
+def run_traffic(storages, copy_gib=4.0, launches=None):
     """The launches whose memory counters calibrate and measure HBM traffic: the streaming copy
     of `copy_gib` GiB twice, then for each storage one sampled step of each AA form of the
-    1536x1024x64 lattice (direct launches, so every sweep is one dispatch)."""
+    1536x1024x64 lattice (direct launches, so every sweep is one dispatch), with the sweep
+    launch of `launches` {storage: (even, odd)} where given."""
     import torch
     from . import cases
+    from .autotune import apply_launch
     from .solver import Solver
     ms, gbs = copy_bandwidth(copy_gib, 2)
     _log(f"copy {int(copy_gib * 2**30)} bytes x 2: {ms:.4f} ms per copy")
@@ -112,6 +129,8 @@ def run_traffic(storages, copy_gib=4.0):
     section, flow, grid, plan = case.build()
     for storage in storages:
         s = Solver(section, flow, grid, storage=storage, plan=plan, graphs=False)
+        if launches and storage in launches:
+            apply_launch(s.lattice, launches[storage])
         sweep_ms, _ = s.lattice.run_profiled(2, 0, 1)
         s.lattice.read_history()
         _log(f"{case.name} {storage}: 2 sampled steps (one per AA form), sweep {sweep_ms / 2:.4f} ms per launch, "
@@ -201,6 +220,8 @@ def main(argv=None):
     ap.add_argument("--sample-every", type=int, default=1)
     ap.add_argument("--copy-gib", type=float, default=4.0)
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly")
+    ap.add_argument("--launch", default="", help="uniform and traffic workloads: sweep launch per storage, "
+                    "'fp64:BXxBYcW/BXxBYcW,...' (even form / odd form tile and wave cap, as the autotuner picks)")
     ap.add_argument("--cases", type=int, default=24, help="serve / batch: number of requests")
     ap.add_argument("--rate", type=float, default=16.0, help="serve: Poisson arrival rate (1/s)")
     ap.add_argument("--chunk", type=int, default=100, help="serve: coarse steps per batch call")
@@ -209,7 +230,7 @@ def main(argv=None):
     if a.workload == "copy":
         run_copy(a.copy_gib, a.steps)
     elif a.workload == "traffic":
-        run_traffic([x for x in a.storages.split(",") if x], a.copy_gib)
+        run_traffic([x for x in a.storages.split(",") if x], a.copy_gib, _launches(a.launch))
     elif a.workload == "batch":
         run_batch(a.storage, a.cases, a.steps, a.warmup, sweep_grid=a.sweep_grid)
     elif a.workload == "serve":
@@ -221,7 +242,8 @@ def main(argv=None):
     else:
         case = {"uniform-c32": cases.NACA0012_C32_384x256x8, "exterior-c32": cases.NACA0012_C32_384x256x8_EXTERIOR,
                 "uniform-100m": cases.NACA0012_C128_1536x1024x64}[a.workload]
-        run_uniform(case, a.storage, a.steps, a.warmup, a.sample_every, not a.no_graphs)
+        run_uniform(case, a.storage, a.steps, a.warmup, a.sample_every, not a.no_graphs,
+                    _launches(a.launch).get(a.storage))
     return 0
 
 

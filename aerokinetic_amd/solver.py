@@ -20,7 +20,8 @@ boundary kernels before each sweep, loads and low-Mach guards reduced on the
 device into a history buffer read back in batches.  Extra keyword-only
 arguments select the storage precision ("fp64", "fp32", "fp16"), the device, an
 explicit plan (fixed lattice shapes) and the sweep tile (bx, by), or "auto" for the
-tile the autotuner (aerokinetic_amd.autotune) measures fastest on this lattice.
+population buffer (of a few allocations), tile and resident-wave cap of each AA form that the
+autotuner (aerokinetic_amd.autotune) measures fastest on this lattice.
 """
 from __future__ import annotations
 
@@ -112,10 +113,13 @@ class Solver:
                                self.storage_origin[1] / self.plan.dx, grid.cells_per_chord, flow.lattice_speed)
         self._initialize()
         self.tile, self.tile_timings = tile, {}
+        self.placement_ms = []
         if tile == "auto":
-            # time every candidate tile on this lattice (cached per shape and storage), then start
-            # again from the seeded field: the tuning steps advanced the state
-            from .autotune import tune
+            # keep the fastest of a few population buffers, time every candidate tile and wave cap
+            # of each AA form on it (cached per shape and storage), then start again from the
+            # seeded field: the tuning steps advanced the state
+            from .autotune import choose_placement, tune
+            self.placement_ms = choose_placement(self.lattice, self._initialize)
             self.tile, self.tile_timings = tune(self.lattice)
             self._initialize()
         self.last_loads = None

@@ -7,7 +7,10 @@ every step.  One step = boundary phase (Bouzidi walls + loads, farfield) + the f
 stream-collide sweep.  `value` is the FP64-storage MLUPS (cells x steps / s over exactly
 --steps timed steps, populations resident in HBM before timing); the same case in FP32 and
 FP16 shifted storage (g = f - w, FP32 collision arithmetic) is reported beside it
-(`by_storage`), with its loads and rho/u fields compared to FP64.
+(`by_storage`), with its loads and rho/u fields compared to FP64.  With --tile auto (the
+default) each AA form's sweep launch (tile and cap on resident waves per SIMD) is the
+autotuner's pick on the lattice itself (aerokinetic_amd.autotune.tune, reported as
+`sweep_launch`); the solver seeds its initial state again after tuning, untimed.
 
 roofline: the dominant kernel is the sweep (~97 % of a step).  Algorithmic bytes per launch
 = fluid cells x 2 x 27 x bytes per population (one read + one write per population, AA
@@ -90,11 +93,12 @@ def _counter_rows(directory):
     return rows
 
 
-def measure_traffic(storages, timeout=300):
+def measure_traffic(storages, timeout=300, launches=None):
     """HBM bytes per launch of the headline sweep, per storage, from two rocprofv3 counter
     passes (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch) over `profile_step --workload
     traffic` in a child process: the streaming copy of 4 GiB (which reads and writes 4 GiB:
-    the calibration) and one sweep of each AA form of the 1536x1024x64 lattice.  Returns
+    the calibration) and one sweep of each AA form of the 1536x1024x64 lattice, with the
+    sweep launches of the timed run (`launches` {storage: "even/odd"}).  Returns
     {storage: bytes per launch (mean of the two forms), ...} plus the details, or a reason."""
     import shutil
     import subprocess
@@ -109,6 +113,8 @@ def measure_traffic(storages, timeout=300):
         out_dir = tempfile.mkdtemp(prefix=f"ak_pmc_{counter.lower()}_", dir="/tmp")
         cmd = [exe, "--pmc", counter, "-d", out_dir, "-o", "pmc", "--output-format", "csv", "--", sys.executable,
                "-m", "aerokinetic_amd.profile_step", "--workload", "traffic", "--storages", ",".join(storages)]
+        if launches:
+            cmd += ["--launch", ",".join(f"{st}:{spec}" for st, spec in launches.items())]
         env = dict(os.environ, TMPDIR="/tmp", PYTHONPATH=str(ROOT))
         log(f"memory counters: {counter}")
         try:
@@ -197,12 +203,18 @@ def refined_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None, tile
     return out
 
 
-def uniform_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
+def uniform_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None, tile=None, launch=None):
     import numpy as np
     import torch
+    from aerokinetic_amd.autotune import apply_launch
     from aerokinetic_amd.solver import Solver
     section, flow, grid, plan = case.build()
-    s = Solver(section, flow, grid, storage=storage, plan=plan)
+    t_tune = time.perf_counter()
+    s = Solver(section, flow, grid, storage=storage, plan=plan, tile=None if launch else tile)
+    if launch:  # a fixed launch per AA form instead of tuning (profiler runs see only these)
+        apply_launch(s.lattice, launch)
+        s.tile = launch
+    t_tune = time.perf_counter() - t_tune
     cells = int(np.prod(plan.shape))
     fluid = int(s._fluid.sum())
     s.advance(warmup, 1)
@@ -220,12 +232,33 @@ def uniform_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
            "sweep_ms": sweep * 1e3, "boundary_ms": bnd_ms / profile_steps,
            "roofline": roofline(fluid * 2 * 27 * BYTES[storage], sweep, copy_gbs), "fluid_cells": fluid,
            "cells": cells, "steps_compared": warmup + steps,
+           "sweep_launch": _launch_report(s.tile), "setup_and_tuning_s": t_tune,
+           "launch_spec": _launch_spec(s.tile),
+           "placement_sweep_ms": [round(t, 4) for t in s.placement_ms],
            "loads": np.array([[r["cl"], r["cd"], r["cm"]] for r in recs]),
            "fields": (rho, u, s._fluid)}
     s.lattice.close()
     del s
     torch.cuda.empty_cache()
     return out
+
+
+def _launch_spec(tile):
+    """The autotuned launch as profile_step --launch takes it ('even/odd'), or None."""
+    from aerokinetic_amd.autotune import format_launch
+    if tile is not None and len(tile) == 2 and all(isinstance(t, (tuple, list)) for t in tile):
+        return format_launch(tile)
+    return None
+
+
+def _launch_report(tile):
+    """Sweep launch of each AA form: tile (bx x by; bx 0 = row-linear) and resident-wave cap."""
+    if tile is None:
+        return {"tile": "library default (64x4 on whole-wave rows)", "autotuned": False}
+    if len(tile) == 2 and all(isinstance(t, (tuple, list)) for t in tile):
+        return {form: {"tile": f"{t[0]}x{t[1]}", "waves_per_simd_cap": (t[2] if len(t) > 2 else 0) or None}
+                for form, t in zip(("even", "odd"), tile)} | {"autotuned": True}
+    return {"tile": f"{tile[0]}x{tile[1]}", "autotuned": False}
 
 
 def exterior_gpu(case, storage, steps, warmup, profile_steps, copy_gbs=None):
@@ -445,8 +478,10 @@ def main():
     ap.add_argument("--traffic", type=int, default=1, help="0 skips the memory-counter passes")
     ap.add_argument("--two-level-storages", default="fp64,fp32", help="empty to skip the two-level case")
     ap.add_argument("--two-level-steps", type=int, default=2000)
-    ap.add_argument("--tile", default="auto", help="sweep tiles of the two-level case and the serving batch: "
-                    "'auto' (autotuned) or 'default'")
+    ap.add_argument("--tile", default="auto", help="sweep launches of the headline lattice (tile and wave cap per AA "
+                    "form), the two-level case and the serving batch: 'auto' (autotuned) or 'default'")
+    ap.add_argument("--launch", default="", help="headline lattice: a fixed sweep launch per storage instead of "
+                    "tuning, 'fp64:BXxBYcW/BXxBYcW,...' (even / odd form tile and wave cap, as `launch_spec` reports)")
     ap.add_argument("--exterior-storages", default="fp64,fp16", help="empty to skip the exterior-farfield case")
     ap.add_argument("--exterior-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -460,6 +495,8 @@ def main():
     ap.add_argument("--serve-max-steps", type=int, default=3000)
     args = ap.parse_args()
     args.tile = None if args.tile == "default" else args.tile
+    from aerokinetic_amd.profile_step import _launches
+    fixed_launch = _launches(args.launch)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: a case runs on a single MI355X here; "
               "multi-GPU runs are not supported (use --gpus 1)", file=sys.stderr)
@@ -474,7 +511,8 @@ def main():
     uni = {}
     for st in storages:
         log(f"{ucase.name} {st}")
-        uni[st] = uniform_gpu(ucase, st, args.steps, args.warmup, args.profile_steps, copy_gbs)
+        uni[st] = uniform_gpu(ucase, st, args.steps, args.warmup, args.profile_steps, copy_gbs, tile=args.tile,
+                              launch=fixed_launch.get(st))
         r = uni[st]
         log(f"  {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, sweep {r['sweep_ms']:.3f} ms "
             f"({100 * r['roofline']['frac']:.1f}% HBM), boundary {r['boundary_ms']:.3f} ms")
@@ -498,7 +536,8 @@ def main():
     polar = polar_gpu(args) if args.polar_cases > 0 else None
     import torch
     torch.cuda.empty_cache()
-    traffic, why = (measure_traffic(storages) if args.traffic else (None, "skipped (--traffic 0)"))
+    launches = {st: r["launch_spec"] for st, r in uni.items() if r.get("launch_spec")}
+    traffic, why = (measure_traffic(storages, launches=launches) if args.traffic else (None, "skipped (--traffic 0)"))
     if traffic:
         for st in storages:
             uni[st]["roofline"]["traffic"] = traffic[st]["bytes_per_launch"]
@@ -529,7 +568,8 @@ def main():
             "config": {"workload": ucase.name + " (1536x1024x64 = 100,663,296 cells), characteristic farfield, "
                                                "loads every step",
                        "storage": storages[0], "cells": head["cells"], "parallelism": "single GPU",
-                       "step_graphs": True, "sweep_tile": "64x4 (row pitch aligned, XCD-remapped blocks)"},
+                       "step_graphs": True,
+                       "sweep_launch": head.get("sweep_launch")},
             "roofline": head["roofline"], "cpu_baseline": cpu,
             "copy": {"bytes": 4 * 2**30, "ms": copy_ms, "gbs_read_write": copy_gbs},
             "by_storage": {st: strip(r) for st, r in uni.items()},

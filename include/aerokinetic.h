@@ -76,6 +76,12 @@ int ak_lattice_destroy(void *handle);
 size_t ak_population_bytes(void *handle);
 int ak_bind_populations(void *handle, void *device_ptr, size_t bytes);
 
+/* The caller wrote the bound population buffer directly (outside the library): state the
+ * library derives from the populations and keeps between run calls (the characteristic
+ * farfield's perimeter moments) is recomputed at the next run call.  The reference writes
+ * its pdf arrays only through the lbmpy macroscopic setter (solver.py:157-202). */
+int ak_populations_written(void *handle);
+
 /* Cell mask of the interior xy cells (host, nx*ny bytes, x-major: m[x*ny+y]):
  * 0 fluid, 1 solid (not updated), 2 fluid excluded from the diagnostics. */
 int ak_set_solid(void *handle, const uint8_t *mask_xy);
@@ -107,6 +113,24 @@ int ak_init_seeded(void *handle, const double *p);
  * blocks of `by` threads (a multiple of 64) over consecutive cells of a plane; set by the tile
  * autotuner.  Default: 64 x 4 when nx is a multiple of 64, row-linear 256 otherwise. */
 int ak_set_sweep_tile(void *handle, int bx, int by);
+
+/* The same for one AA form only (odd = 1: the gathering sweep, 0: the in-place sweep), so the
+ * two forms can run different tiles (their access patterns differ: the odd form's x-shifted
+ * rows straddle cache lines that neighbouring waves share).  Replaces the sweep launch
+ * configuration that lbmpy's generated kernel fixes at code generation (solver.py:58-91). */
+int ak_set_sweep_tile_form(void *handle, int odd, int bx, int by);
+
+/* Cap on the resident waves per SIMD of one AA form's sweep (0: no cap, else 1..8), enforced
+ * by the LDS each sweep block reserves (the sweep itself uses none).  Fewer concurrent waves,
+ * each streaming 27 population rows in and out, can reach a higher memory rate than a full
+ * CU.  Setting a tile clears the caps.  Like the tile, a launch configuration of the sweep
+ * that replaces the one lbmpy fixes at code generation (solver.py:58-91). */
+int ak_set_sweep_waves(void *handle, int odd, int waves);
+
+/* AA form of the lattice's next sweep: 1 odd (gathering), 0 even (in place), -1 on error.
+ * The reference's two-array step has no such state (solver.py:252-279 swaps src and dst); the
+ * tile autotuner uses it to time each form on its own. */
+int ak_next_sweep_odd(void *handle);
 
 /* Advance `nsteps` reference steps.  Steps with (first_step + k + 1) % sample_every == 0
  * (k = 0..nsteps-1; sample_every <= 0 disables) record a history row. */

@@ -143,3 +143,40 @@ def test_steady_mode_stops_on_residual_targets(tmp_path):
     rows = list(csv.DictReader(open(tmp_path / "r" / "residuals.csv")))
     assert [int(r["step"]) for r in rows] == [20, 40, 60] and rows[-1]["streak"] == "3"
     assert float(rows[0]["l2_jx"]) > 0
+
+
+def test_direct_buffer_write_declared_equals_import():
+    """A caller may write the bound population buffer directly between run calls and declare
+    it (ak_populations_written): the characteristic farfield's perimeter moments, which the
+    sweep keeps between calls, are then rebuilt, and the run equals one whose state came in
+    through the library's import.  Left undeclared, the stale moments give another run."""
+    from aerokinetic_amd.config import Flow, Grid, Plan
+    from aerokinetic_amd.geometry import Section, naca0012
+    from aerokinetic_amd.solver import Solver
+    section = Section((naca0012(41),), 1.0, (0.25, 0.0))
+    flow = Flow(reynolds=10000, alpha_deg=4.0, lattice_speed=0.05, perturbation=1e-3)
+    grid = Grid(cells_per_chord=16, upstream_chords=1.0, downstream_chords=2.0, transverse_chords=1.0,
+                span_chords=0.5, memory_limit_gib=2, farfield="characteristic_transport_v1")
+    plan = Plan.create(section, flow, grid)
+    print("direct write", plan.shape, flush=True)
+    sv = [Solver(section, flow, grid, plan=plan) for _ in range(3)]
+    for s in sv:
+        s.advance(3, 1)
+    written, stale, imported = sv
+    g = imported.populations()
+    g[0, :, :, :] *= 1.001  # the x- face (perimeter) and the interior change
+    g[g.shape[0] // 2, :, :, :] *= 0.999
+    imported.set_populations(g)
+    imported.lattice.synchronize()
+    for s in (written, stale):
+        a, b = s.lattice, imported.lattice
+        a._buf[a._offset:a._offset + a.nbytes].copy_(b._buf[b._offset:b._offset + b.nbytes])
+    written.lattice.populations_written()
+    rw, rs, ri = (s.advance(4, 1) for s in sv)
+    keys = ("step", "cl", "cd", "cm", "local_mach_max", "density_error_max")
+    rows = lambda recs: np.array([[r[k] for k in keys] for r in recs])
+    np.testing.assert_array_equal(rows(rw), rows(ri))
+    np.testing.assert_array_equal(written.populations(), imported.populations())
+    assert not np.array_equal(stale.populations(), imported.populations())
+    for s in sv:
+        s.lattice.close()

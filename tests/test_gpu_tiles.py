@@ -92,3 +92,38 @@ def test_persistent_batched_sweep_matches_one_block_per_tile():
             np.testing.assert_array_equal(p.populations(level), q.populations(level))
     a.close()
     b.close()
+
+
+def test_per_form_tiles_and_wave_caps_match_default_launch():
+    """Each AA form may run its own tile and cap on resident waves (the cap only reserves LDS
+    per block): a run with different launches for the two forms, switched mid-run, equals the
+    library's default launch bit for bit, in every storage and with sampled steps."""
+    from aerokinetic_amd.config import Flow, Grid, Plan
+    from aerokinetic_amd.geometry import Section, naca0012
+    from aerokinetic_amd.solver import Solver
+    section = Section((naca0012(41),), 1.0, (0.25, 0.0))
+    flow = Flow(reynolds=10000, alpha_deg=4.0, lattice_speed=0.05, perturbation=1e-3)
+    grid = Grid(cells_per_chord=32, upstream_chords=1.0, downstream_chords=2.0, transverse_chords=1.0,
+                span_chords=0.25, memory_limit_gib=2, farfield="characteristic_transport_v1")
+    plan = Plan.create(section, flow, grid)
+    for storage in ("fp64", "fp32", "fp16"):
+        print("per-form launches", storage, plan.shape, flush=True)
+        a = Solver(section, flow, grid, plan=plan, storage=storage)
+        b = Solver(section, flow, grid, plan=plan, storage=storage)
+        b.lattice.set_tile(256, 1, odd=True)
+        b.lattice.set_waves(3, odd=True)
+        b.lattice.set_tile(64, 1, odd=False)
+        b.lattice.set_waves(5, odd=False)
+        ra, rb = a.advance(7, 2), b.advance(7, 2)
+        b.lattice.set_tile(0, 256, odd=False)  # a new even tile drops only the even form's cap
+        b.lattice.set_waves(4, odd=False)
+        ra += a.advance(6, 1)
+        rb += b.advance(6, 1)
+        keys = ("step", "cl", "cd", "cm", "local_mach_max", "density_error_max", "density_min", "nonfinite")
+        np.testing.assert_array_equal(np.array([[r[k] for k in keys] for r in ra]),
+                                      np.array([[r[k] for k in keys] for r in rb]))
+        np.testing.assert_array_equal(a.populations(), b.populations())
+        with pytest.raises(Exception, match="waves per SIMD"):
+            b.lattice.set_waves(9, odd=True)
+        a.lattice.close()
+        b.lattice.close()
