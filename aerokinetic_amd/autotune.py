@@ -21,21 +21,26 @@ _cache: dict = {}
 
 
 WAVE_CAPS = (6, 5, 4, 3)
-PLACEMENTS = 4
+PLACEMENTS = 12          # at most this many candidate population buffers
+PLACEMENT_MEMORY = 0.6   # ... holding at most this fraction of the free device memory at once
 
 
-def choose_placement(lattice, initialize, candidates=PLACEMENTS, steps=4):
-    """Keep the fastest of up to `candidates` population buffers of `lattice`.
+def choose_placement(lattice, initialize, candidates=None, steps=4):
+    """Keep the fastest of several population buffers of `lattice`.
 
     The sweep's 27 in-place population rows per cell meet HBM channels through the physical
     pages behind the buffer, and the same launch on a buffer of other pages has measured
     up to 13 % faster or slower; a given buffer keeps its rate.  Each candidate (the
     lattice's own buffer first, then fresh allocations while the earlier ones are still
-    held, as far as memory allows) gets the initial state from `initialize()` and is timed
-    over `steps` single steps; the fastest stays bound, the others are released, and
-    `initialize()` runs again on it.  Returns the mean sweep milliseconds (even and odd form
-    averaged) of every candidate, in order."""
+    held) gets the initial state from `initialize()` and is timed over `steps` single steps;
+    the fastest stays bound, the others are released, and `initialize()` runs again on it.
+    `candidates` defaults to as many buffers as PLACEMENT_MEMORY of the free memory holds, at
+    most PLACEMENTS; allocation also stops at the first failure.  Returns the mean sweep
+    milliseconds (even and odd form averaged) of every candidate, in order."""
     import torch
+    if candidates is None:
+        free, _ = torch.cuda.mem_get_info(lattice.torch_device)
+        candidates = min(PLACEMENTS, 1 + int(free * PLACEMENT_MEMORY // max(lattice.nbytes, 1)))
     held, times = [], []
     for k in range(max(1, candidates)):
         if k:

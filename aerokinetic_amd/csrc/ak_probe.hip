@@ -116,6 +116,44 @@ hipError_t launch_wide(void *A, const Box &B, int layout, hipStream_t st) {
   return hipGetLastError();
 }
 
+// The rows layout with the sweep's own row pitch (nx + 2 x-ghosts, rounded up to a 128-byte
+// line) and any tile of 256 threads (bx along x, 256 / bx rows along y), in the sweep's
+// XCD-contiguous block order.  Same traffic as kp_pattern<T, 0, ODD>.
+struct Box2 {
+  int nx, ny, nz, bx, by, gx, gy, nblocks;
+  long long R;
+};
+template <typename T, bool ODD>
+__global__ __launch_bounds__(256) void kp_rows_tile(T *__restrict__ A, Box2 B) {
+  const int b = xcd_order((int)blockIdx.x, B.nblocks);
+  const int tx = b % B.gx, ty = (b / B.gx) % B.gy, z = b / (B.gx * B.gy);
+  const int x = tx * B.bx + (int)(threadIdx.x % B.bx), y = ty * B.by + (int)(threadIdx.x / B.bx);
+  if (x >= B.nx || y >= B.ny) return;
+  auto at = [&](int xx, int yy, int zz, int q) { return (((long long)zz * B.ny + yy) * 27 + q) * B.R + xx; };
+  T v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if constexpr (ODD)
+      v[q] = A[at(wrapi(x - pcx(q), B.nx), wrapi(y - pcy(q), B.ny), wrapi(z - pcz(q), B.nz), popp(q))];
+    else
+      v[q] = A[at(x, y, z, q)];
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if constexpr (ODD)
+      A[at(wrapi(x + pcx(q), B.nx), wrapi(y + pcy(q), B.ny), wrapi(z + pcz(q), B.nz), q)] = v[q];
+    else
+      A[at(x, y, z, popp(q))] = v[q];
+  }
+}
+
+template <typename T>
+hipError_t launch_rows_tile(void *A, const Box2 &B, bool odd, size_t lds, hipStream_t st) {
+  if (odd) hipLaunchKernelGGL((kp_rows_tile<T, true>), dim3(B.nblocks), dim3(256), lds, st, (T *)A, B);
+  else hipLaunchKernelGGL((kp_rows_tile<T, false>), dim3(B.nblocks), dim3(256), lds, st, (T *)A, B);
+  return hipGetLastError();
+}
+
 char g_err[256] = "";
 
 }  // namespace
@@ -195,6 +233,79 @@ int akp_pattern(int layout, int elem_bytes, int odd, int cells_per_lane, int nx,
   out[0] = ms / reps;
   out[1] = 2.0 * 27.0 * elem_bytes * (double)nx * ny * nz;
   return 0;
+}
+
+// Placement and occupancy: `nbuf` population buffers of an nx x ny x nz box (rows layout, the
+// sweep's row pitch), all allocated and held at once, each timed over `reps` launches of the
+// even and of the odd form with tiles of bx x (256 / bx) threads and at most `waves` resident
+// waves per SIMD (0: no cap; the cap reserves LDS per block, which the kernel does not use).
+// out[2k] / out[2k + 1] = ms per even / odd launch on buffer k; out[2 nbuf] = bytes per launch.
+// Returns the number of buffers timed (allocation stops early when memory runs out), or -1.
+int akp_placements(int elem_bytes, int nx, int ny, int nz, int bx, int waves, int nbuf, int reps, double *out) {
+  if ((elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8) || nx < 2 || ny < 2 || nz < 2 || reps < 1 || !out ||
+      nbuf < 1 || nbuf > 16 || (bx != 64 && bx != 128 && bx != 256) || waves < 0 || waves > 8 ||
+      (waves > 0 && waves < 3)) {
+    snprintf(g_err, sizeof g_err, "bad placement-probe arguments");
+    return -1;
+  }
+  Box2 B{};
+  B.nx = nx; B.ny = ny; B.nz = nz; B.bx = bx; B.by = 256 / bx;
+  const int per_line = 128 / elem_bytes;
+  B.R = ((long long)nx + 2 + per_line - 1) / per_line * per_line;
+  B.gx = (nx + bx - 1) / bx;
+  B.gy = (ny + B.by - 1) / B.by;
+  B.nblocks = B.gx * B.gy * nz;
+  const size_t bytes = (size_t)B.R * ny * nz * 27 * elem_bytes;
+  int lds_per_cu = 0;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+    lds_per_cu = 0;
+  const size_t lds = waves > 0 && lds_per_cu > 0 ? (size_t)(lds_per_cu / waves) & ~(size_t)255 : 0;
+  void *held[16] = {};
+  int got = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreate(&st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  for (int k = 0; k < nbuf && e == hipSuccess; ++k) {
+    if (hipMalloc(&held[k], bytes) != hipSuccess) {
+      (void)hipGetLastError();  // out of memory: time the buffers held so far
+      held[k] = nullptr;
+      break;
+    }
+    e = hipMemset(held[k], 0, bytes);
+    for (int odd = 0; odd < 2 && e == hipSuccess; ++odd) {
+      auto launch = [&]() -> hipError_t {
+        switch (elem_bytes) {
+          case 2: return launch_rows_tile<uint16_t>(held[k], B, odd != 0, lds, st);
+          case 4: return launch_rows_tile<uint32_t>(held[k], B, odd != 0, lds, st);
+          default: return launch_rows_tile<uint64_t>(held[k], B, odd != 0, lds, st);
+        }
+      };
+      for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch();
+      if (e == hipSuccess) e = hipEventRecord(e0, st);
+      for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch();
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      out[2 * k + odd] = ms / reps;
+    }
+    if (e == hipSuccess) got = k + 1;
+  }
+  for (int k = 0; k < 16; ++k)
+    if (held[k]) (void)hipFree(held[k]);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (e != hipSuccess) {
+    snprintf(g_err, sizeof g_err, "%s", hipGetErrorString(e));
+    return -1;
+  }
+  out[2 * nbuf] = 2.0 * 27.0 * elem_bytes * (double)nx * ny * nz;
+  return got;
 }
 
 }  // extern "C"

@@ -2,6 +2,7 @@
 reaches without the collision, per storage precision, layout and AA form.
 
     python -m aerokinetic_amd.pattern_probe --shape 1536,1024,64 --reps 10
+    python -m aerokinetic_amd.pattern_probe --placements 6 --tiles 64,256 --waves 0,4,3
 
 Each line is one launch pattern of `aerokinetic_amd/lib/libaerokinetic_probe.so`
 (csrc/ak_probe.hip): 27 loads and 27 stores per cell, even form (read (x, q), write
@@ -13,6 +14,11 @@ their 27 directions together); `rows x2` / `x4` are the even form with 2 or 4 ce
 (one 4- or 8-byte access per direction).  Rates are algorithmic bytes (2 x 27 x element bytes
 per cell) per second; `mean` is the rate of one even plus one odd launch, which is what a
 run of the sweep alternates.  Prints a markdown table (and one JSON line with --json).
+
+--placements N times the pattern (rows layout with the sweep's own row pitch) on N buffers
+allocated and held at once, for each tile and cap on resident waves per SIMD: the same
+launch on buffers of other physical pages, and under fewer resident waves, moves a
+different number of bytes per second.
 """
 from __future__ import annotations
 
@@ -34,7 +40,39 @@ def _lib():
     L.akp_pattern.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
     L.akp_pattern.restype = ctypes.c_int
     L.akp_last_error.restype = ctypes.c_char_p
+    L.akp_placements.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
+    L.akp_placements.restype = ctypes.c_int
     return L
+
+
+def placements(L, storage, shape, bx, waves, nbuf, reps):
+    """[(even GB/s, odd GB/s)] of `nbuf` population buffers held at once (rows layout with the
+    sweep's row pitch, bx x 256/bx tiles, resident-wave cap `waves`, 0 = none)."""
+    out = (ctypes.c_double * (2 * nbuf + 1))()
+    got = L.akp_placements(BYTES[storage], *shape, bx, waves, nbuf, reps, out)
+    if got < 0:
+        raise RuntimeError(f"placement probe {storage}: {L.akp_last_error().decode()}")
+    nbytes = out[2 * nbuf]
+    return [(nbytes / (out[2 * k] * 1e-3) / 1e9, nbytes / (out[2 * k + 1] * 1e-3) / 1e9) for k in range(got)]
+
+
+def main_placements(a, L, shape):
+    """Table of the sweep pattern's rate on several buffers held at once, per tile and wave cap."""
+    print("| storage | tile | waves/SIMD cap | buffer | even TB/s | odd TB/s | mean TB/s | mean % of 8 TB/s |", flush=True)
+    print("|---|---|---:|---:|---:|---:|---:|---:|", flush=True)
+    rows = []
+    for st in [s for s in a.storages.split(",") if s]:
+        for bx in [int(v) for v in a.tiles.split(",")]:
+            for w in [int(v) for v in a.waves.split(",")]:
+                for k, (ge, go) in enumerate(placements(L, st, shape, bx, w, a.placements, a.reps)):
+                    mean = 2 / (1 / ge + 1 / go)
+                    rows.append({"storage": st, "tile": f"{bx}x{256 // bx}", "waves": w, "buffer": k,
+                                 "even_gbs": ge, "odd_gbs": go, "mean_gbs": mean})
+                    print(f"| {st} | {bx}x{256 // bx} | {w or '-'} | {k} | {ge / 1e3:.2f} | {go / 1e3:.2f} | "
+                          f"{mean / 1e3:.2f} | {mean / PEAK_GBS * 100:.1f} |", flush=True)
+    if a.json:
+        print(json.dumps({"shape": shape, "reps": a.reps, "placement_rows": rows}))
+    return 0
 
 
 def measure(L, layout, storage, odd, shape, reps, cells_per_lane=1):
@@ -52,9 +90,16 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--storages", default="fp64,fp32,fp16")
     ap.add_argument("--json", action="store_true")
+    ap.add_argument("--placements", type=int, default=0,
+                    help="instead of the layout table: this many buffers held at once, each timed (rows layout, "
+                         "the sweep's row pitch) for every tile of --tiles and wave cap of --waves")
+    ap.add_argument("--tiles", default="64,256", help="--placements: tile widths bx (tiles bx x 256/bx)")
+    ap.add_argument("--waves", default="0,4,3", help="--placements: resident-wave caps per SIMD (0: none)")
     a = ap.parse_args(argv)
     shape = tuple(int(v) for v in a.shape.split(","))
     L = _lib()
+    if a.placements:
+        return main_placements(a, L, shape)
     rows = []
     print("| storage | layout | even TB/s | odd TB/s | mean TB/s | mean % of 8 TB/s |", flush=True)
     print("|---|---|---:|---:|---:|---:|", flush=True)
