@@ -21,7 +21,7 @@ _cache: dict = {}
 
 
 WAVE_CAPS = (6, 5, 4, 3)
-PLACEMENTS = 12          # at most this many candidate population buffers
+PLACEMENTS = 32          # at most this many candidate population buffers
 PLACEMENT_MEMORY = 0.6   # ... holding at most this fraction of the free device memory at once
 
 
