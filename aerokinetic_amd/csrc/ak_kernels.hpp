@@ -230,6 +230,19 @@ __device__ __forceinline__ long long pm_y(const Desc &L, int side, int x, int z)
 }
 __host__ __device__ inline size_t pm_count(int nx, int ny, int nz) { return 8 * ((size_t)ny * nz + (size_t)nx * nz); }
 
+// Per-cell copies of nine populations of each x-face cell, at [z][side][y][j] (a cell's nine
+// values together, consecutive cells along y): the outgoing ones (for x = 0 the nine
+// f_out(0, y, z, q) with c_x = -1, q = 0..8, j = q; for x = nx - 1 those with c_x = +1,
+// q = 18..26, j = q - 18), in the storage type and bit for bit as the sweep stored them, and
+// the entering ones (c_x = +1 at x = 0, j = q - 18; c_x = -1 at x = nx - 1, j = q) that the
+// farfield computes for the next sweep.  The farfield's lanes run along y, so both copies
+// replace nine lattice rows per face cell (one 128-byte line per value) by a few lines per
+// wave, and the sweep's one x-face lane per wave reads or writes one or two lines.
+__device__ __forceinline__ long long xo_at(const Desc &L, int side, int j, int y, int z) {
+  return ((long long)(z * 2 + side) * L.ny + y) * 9 + j;
+}
+__host__ __device__ inline size_t xo_count(int ny, int nz) { return (size_t)18 * ny * nz; }
+
 // Writes the perimeter moments of cell (x, y, z).
 template <typename C>
 __device__ __forceinline__ void perimeter_put(C *__restrict__ pm, const Desc &L, int x, int y, int z, C dr,
@@ -261,7 +274,9 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
                                             const Desc &L, const SweepGeom &G, typename St<S>::C ws,
                                             typename St<S>::C wb, float scale, float inv_scale,
                                             DiagAcc *__restrict__ diag, int lb, int slot,
-                                            typename St<S>::C *__restrict__ pm = nullptr) {
+                                            typename St<S>::C *__restrict__ pm = nullptr,
+                                            typename St<S>::T *__restrict__ xo = nullptr,
+                                            const typename St<S>::T *__restrict__ xi = nullptr) {
   using C = typename St<S>::C;
   // the block's tile coordinates are wave-uniform: kept in scalar registers, so that the
   // per-direction plane bases below are scalar arithmetic, not 64-bit vector adds
@@ -342,6 +357,15 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         const gchar *bq = uniform_ptr(A + (plane + d));
         g[q] = ld_g<S>(bq + boff, inv_scale);
       }
+      // x-face cells take their nine populations entering through that face from the farfield's
+      // copy (the farfield no longer writes the lattice slots these lanes gathered above)
+      if (xi != nullptr && __any(x == 0 || x == L.nx - 1)) {
+        const int side = x == 0 ? 0 : (x == L.nx - 1 ? 1 : -1);
+        if (side >= 0) {
+#pragma unroll
+          for (int j = 0; j < 9; ++j) g[side ? j : 18 + j] = ld<S>(xi + xo_at(L, side, j, y, z), inv_scale);
+        }
+      }
       loads_issued(g);
       active = mk != 1 && mk != 3;
       if (active) {
@@ -366,9 +390,19 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
           else st_g<S>(bq + boff, g[q], scale);
         }
         // cells on the x and y faces keep their density deviation and velocity (conserved by
-        // the collision) for the next farfield pass
+        // the collision) for the next farfield pass, and x-face cells their outgoing populations
         if (pm != nullptr && (x == 0 || y == 0 || x == L.nx - 1 || y == L.ny - 1))
           perimeter_put<C>(pm, L, x, y, z, drho, uc);
+        if (xo != nullptr && (x == 0 || x == L.nx - 1)) {
+          const int side = x != 0;
+#pragma unroll
+          for (int j = 0; j < 9; ++j) {
+            typedef __attribute__((address_space(1))) typename St<S>::T gT;
+            const int q = side ? 18 + j : j;
+            if constexpr (S == 2) ((gT *)xo)[xo_at(L, side, j, y, z)] = h[q];
+            else ((gT *)xo)[xo_at(L, side, j, y, z)] = g[q];
+          }
+        }
       }
     }
   }
@@ -382,9 +416,12 @@ __global__ __launch_bounds__(PXY ? 256 : 1024) void k_sweep(typename St<S>::T *_
                                                             const uint8_t *__restrict__ solid, Desc L, SweepGeom G,
                                                             typename St<S>::C ws, typename St<S>::C wb, float scale,
                                                             float inv_scale, DiagAcc *__restrict__ diag,
-                                                            typename St<S>::C *__restrict__ pm) {
+                                                            typename St<S>::C *__restrict__ pm,
+                                                            typename St<S>::T *__restrict__ xo,
+                                                            const typename St<S>::T *__restrict__ xi) {
   sweep_block<S, ODD, PXY, DIAG>(A, solid, L, G, ws, wb, scale, inv_scale, diag, xcd_remap(blockIdx.x, G.nblocks),
-                                 (int)(blockIdx.x % DIAG_SLOTS), PXY ? nullptr : pm);
+                                 (int)(blockIdx.x % DIAG_SLOTS), PXY ? nullptr : pm, PXY ? nullptr : xo,
+                                 PXY ? nullptr : xi);
 }
 
 // One lattice of a batched sweep: its arguments and the first block of its tiles in the
@@ -713,6 +750,8 @@ struct FFItems {
   const double *initial;   // glancing trace rows: [side][z][x], side 0 = y low, 1 = y high
   int n;
   const void *pm;          // perimeter moments of the current state (compute type), or null
+  const void *xo;          // outgoing populations of the x-face cells (storage type, xo_at), or null
+  void *xi;                // their populations entering through the x faces, for the next sweep, or null
 };
 
 struct FFObj {
@@ -838,11 +877,22 @@ __device__ __forceinline__ void farfield_body(typename St<S>::T *__restrict__ A,
   for (int d = 0; d < NQ; ++d) {
     if (!((mask >> d) & 1)) continue;
     const int id = opp(d);
-    const C fo = ld<S>(A + loc_out<PE>(L, foot[0], foot[1], foot[2], id), inv_scale);
+    // an x-face foot's populations leaving through that face come from the sweep's face copy
+    const int xside = foot[0] == 0 && cxq(id) == -1 ? 0 : (foot[0] == L.nx - 1 && cxq(id) == 1 ? 1 : -1);
+    const C fo = I.xo != nullptr && xside >= 0
+                     ? ld<S>((const typename St<S>::T *)I.xo +
+                                 xo_at(L, xside, xside ? id - 18 : id, foot[1], foot[2]), inv_scale)
+                     : ld<S>(A + loc_out<PE>(L, foot[0], foot[1], foot[2], id), inv_scale);
     const C val = fo + (equilibrium<C>(id, rhob, ub[0], ub[1], ub[2]) - equilibrium<C>(id, rho, u[0], u[1], u[2]));
-    // fluid cell x = g - c_d reads direction id
+    // fluid cell x = g - c_d reads direction id; what enters an x-face cell through that face
+    // goes to the per-cell copy (xo_at layout) that the next sweep reads, not to nine lattice rows
     const int xz = ((gz - czq(d)) % L.nz + L.nz) % L.nz;
-    stv<S>(A + loc_in<PE>(L, gx - cxq(d), gy - cyq(d), xz, id), val, scale);
+    const int tx = gx - cxq(d);
+    const int tside = tx == 0 && cxq(id) == 1 ? 0 : (tx == L.nx - 1 && cxq(id) == -1 ? 1 : -1);
+    if (I.xi != nullptr && tside >= 0)
+      stv<S>((typename St<S>::T *)I.xi + xo_at(L, tside, tside ? id : id - 18, gy - cyq(d), xz), val, scale);
+    else
+      stv<S>(A + loc_in<PE>(L, tx, gy - cyq(d), xz, id), val, scale);
   }
 }
 
@@ -881,6 +931,18 @@ __global__ void k_perimeter_fill(const typename St<S>::T *__restrict__ A, Desc L
   C *p = pm + (xface ? pm_x(L, x != 0, y, z) : pm_y(L, y != 0, x, z));
   const int st = xface ? L.ny : L.nx;
   p[0] = dr; p[st] = u[0]; p[2 * st] = u[1]; p[3 * st] = u[2];
+}
+
+// The x-face copy (xo_at) of a state in form PE from the lattice: thread = (y, side, z).  Used
+// when the state was set from outside or loaded (every sweep then keeps the copy).
+template <int S, bool PE>
+__global__ void k_xface_fill(const typename St<S>::T *__restrict__ A, Desc L, typename St<S>::T *__restrict__ xo) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2LL * L.ny * L.nz) return;
+  const int y = (int)(t % L.ny), side = (int)((t / L.ny) % 2), z = (int)(t / (2LL * L.ny));
+  const int x = side ? L.nx - 1 : 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) xo[xo_at(L, side, j, y, z)] = A[loc_out<PE>(L, x, y, z, side ? 18 + j : j)];
 }
 
 // Initial farfield state from the initial populations (post-even form): incoming =
