@@ -359,11 +359,19 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
       }
       // x-face cells take their nine populations entering through that face from the farfield's
       // copy (the farfield no longer writes the lattice slots these lanes gathered above)
+      // (the register index into g stays a compile-time constant, with one select per face:
+      // a lane-dependent index becomes a compare-and-select over all 27 registers)
       if (xi != nullptr && __any(x == 0 || x == L.nx - 1)) {
-        const int side = x == 0 ? 0 : (x == L.nx - 1 ? 1 : -1);
-        if (side >= 0) {
+        const bool lo = x == 0, hi = !lo && x == L.nx - 1;
+        if (lo || hi) {
+          C v[9];
 #pragma unroll
-          for (int j = 0; j < 9; ++j) g[side ? j : 18 + j] = ld<S>(xi + xo_at(L, side, j, y, z), inv_scale);
+          for (int j = 0; j < 9; ++j) v[j] = ld<S>(xi + xo_at(L, hi, j, y, z), inv_scale);
+#pragma unroll
+          for (int j = 0; j < 9; ++j) {
+            g[18 + j] = lo ? v[j] : g[18 + j];
+            g[j] = hi ? v[j] : g[j];
+          }
         }
       }
       loads_issued(g);
@@ -394,13 +402,12 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         if (pm != nullptr && (x == 0 || y == 0 || x == L.nx - 1 || y == L.ny - 1))
           perimeter_put<C>(pm, L, x, y, z, drho, uc);
         if (xo != nullptr && (x == 0 || x == L.nx - 1)) {
-          const int side = x != 0;
+          typedef __attribute__((address_space(1))) typename St<S>::T gT;
+          const bool hi = x != 0;
 #pragma unroll
           for (int j = 0; j < 9; ++j) {
-            typedef __attribute__((address_space(1))) typename St<S>::T gT;
-            const int q = side ? 18 + j : j;
-            if constexpr (S == 2) ((gT *)xo)[xo_at(L, side, j, y, z)] = h[q];
-            else ((gT *)xo)[xo_at(L, side, j, y, z)] = g[q];
+            if constexpr (S == 2) ((gT *)xo)[xo_at(L, hi, j, y, z)] = hi ? h[18 + j] : h[j];
+            else ((gT *)xo)[xo_at(L, hi, j, y, z)] = hi ? g[18 + j] : g[j];
           }
         }
       }

@@ -1,7 +1,7 @@
 """Storage-precision study on the 1536x1024x64 lattice (~100.7M cells).
 
     python -m aerokinetic_amd.precision_study --farfield exterior_regularized_fd_v1 \
-        --steps 20000 --average-last 10000 --out gpurun_out/precision_c3.json
+        --steps 20000 --average-last 10000 --out precision_1536x1024x64.json
 
 NACA0012, chord 128 cells at 8 deg, Re 1e5, Ma 0.1, z periodic, seeded perturbation
 (aerokinetic_amd.cases.NACA0012_C128_1536x1024x64; farfield chosen by --farfield, the

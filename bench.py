@@ -337,13 +337,15 @@ def host_cpu(usable):
     return {"cpu_model": model, "usable_cores": usable, "omp_proc_bind": "close", "omp_places": "cores"}
 
 
-def cpu_uniform_oracle(case_name, seconds, threads, shape=None):
+def cpu_uniform_oracle(case_name, seconds, threads, shape=None, warmup=2, steps=0):
     """The FP64 oracle's uniform step (loads every step) of a named workload on `threads` host
     cores, timed in a child process whose OpenMP threads are pinned (OMP_PROC_BIND=close,
-    OMP_PLACES=cores): 2 untimed steps, then steps until `seconds` have passed (at least 4)."""
+    OMP_PLACES=cores): `warmup` untimed steps, then exactly `steps` timed steps, or (steps=0)
+    steps until `seconds` have passed (at least 4)."""
     import subprocess
     cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--case", case_name, "--seconds", str(seconds),
-           "--threads", str(threads)] + (["--shape", ",".join(map(str, shape))] if shape else [])
+           "--threads", str(threads), "--warmup", str(warmup), "--steps", str(steps)] + \
+          (["--shape", ",".join(map(str, shape))] if shape else [])
     env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads),
                PYTHONPATH=str(ROOT))
     r = subprocess.run(cmd, cwd=str(ROOT), env=env, capture_output=True, text=True, timeout=600)
@@ -555,7 +557,10 @@ def main():
         cpu = cpu_uniform_oracle("NACA0012_C128_1536x1024x64", args.cpu_seconds, cores, shape=(1536, 1024, 8))
         cpu["sample"] += " -- a 1536x1024x8 slice of the headline 1536x1024x64 lattice (per-cell work is the same)"
         cpu["host"] = host_cpu(usable)
-        cpu["uniform_384x256x8"] = {"all_cores": cpu_uniform_oracle("NACA0012_C32_384x256x8", args.cpu_seconds / 3, cores),
+        # the plain case as the reference's own profiling script times it (16 warm-up + 256 timed
+        # steps, loads every step) on all cores; one core is bounded by time instead
+        cpu["uniform_384x256x8"] = {"all_cores": cpu_uniform_oracle("NACA0012_C32_384x256x8", 0, cores,
+                                                                    warmup=16, steps=256),
                                     "one_core": cpu_uniform_oracle("NACA0012_C32_384x256x8", args.cpu_seconds / 3, 1)}
         if two:
             cpu["two_level"] = cpu_baseline_refined(case2, args.cpu_seconds / 2, cores)

@@ -6,7 +6,8 @@
 Runs the oracle's uniform step (the reference's two-array pull data flow, boundary objects
 in installation order, loads every step) on a named workload of aerokinetic_amd.cases,
 optionally on another lattice shape with the same leading-edge cell, for `warmup` untimed
-steps and then until `seconds` have passed, and prints one JSON object:
+steps and then until `seconds` have passed (or for exactly `--steps` timed steps, as the
+reference's own profiling script does: 16 warm-up and 256 timed steps), and prints one JSON object:
 {"value": MLUPS, "cores": threads, "steps": n, "seconds": s, "case": name}.  Started in a
 process of its own by the benchmark, with OMP_PROC_BIND / OMP_PLACES set there, so that its
 OpenMP threads are pinned without touching the benchmark's own process.
@@ -27,6 +28,7 @@ def main(argv=None):
     ap.add_argument("--threads", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--min-steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=0, help="exactly this many timed steps (0: bounded by --seconds)")
     a = ap.parse_args(argv)
     case = getattr(cases, a.case)
     if a.shape:
@@ -38,9 +40,14 @@ def main(argv=None):
     for _ in range(a.warmup):
         o.step(sample=True)
     n, t0 = 0, time.perf_counter()
-    while n < a.min_steps or time.perf_counter() - t0 < a.seconds:
-        o.step(sample=True)
-        n += 1
+    if a.steps > 0:
+        for _ in range(a.steps):
+            o.step(sample=True)
+        n = a.steps
+    else:
+        while n < a.min_steps or time.perf_counter() - t0 < a.seconds:
+            o.step(sample=True)
+            n += 1
     dt = time.perf_counter() - t0
     print(json.dumps({"value": cells * n / dt / 1e6, "cores": a.threads, "steps": n, "warmup": a.warmup,
                       "seconds": dt, "case": case.name, "cells": cells}))

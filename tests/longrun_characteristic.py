@@ -6,9 +6,9 @@ Case: the 1536x1024x64 precision-study case scaled by 1/4 (384x256x16, NACA0012 
 characteristic farfield, seeded perturbation), Cl/Cd/Cm every step.  Not collected by
 pytest (the oracle needs ~20 minutes of host time for it).
 
-    python tests/longrun_characteristic.py gpu --steps 7000 --out gpurun_out/longrun_gpu.npy
+    python tests/longrun_characteristic.py gpu --steps 7000 --out longrun_gpu.npy
     python tests/longrun_characteristic.py oracle --steps 7000 --threads 8 --out longrun_oracle.npy
-    python tests/longrun_characteristic.py compare longrun_oracle.npy gpurun_out/longrun_gpu.npy
+    python tests/longrun_characteristic.py compare longrun_oracle.npy longrun_gpu.npy
 
 Each run stops at the first step whose guards fail (local Mach > 0.3, max |rho - 1| > 0.1 or a
 non-finite state; the oracle checks every 100 steps) and saves the loads of every step.
