@@ -106,11 +106,12 @@ __device__ __forceinline__ long long loc_in(const Desc &L, int x, int y, int z, 
 // subtracted from another: the arithmetic keeps the precision of g itself.
 // Returns drho and |u|^2 for the fused diagnostics, and u (conserved by the collision).  The
 // output populations are multiplied by out_scale (a power of two: the FP16 storage scale, or 1).
+//
+// The forward transform needs the raw moments up to second order only, by three
+// one-dimensional passes.  The first pass (z lines of g[3l..3l+2] = (m, o, p)) gives
+// Z0 = (p + m) + o, Z1 = p - m, Z2 = p + m.
 template <typename C>
-__device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out, C (&u_out)[3],
-                                        C out_scale = C(1)) {
-  // forward: raw moments up to second order by three one-dimensional passes
-  C Z0[9], Z1[9], Z2[9];
+__device__ __forceinline__ void zlines(const C (&g)[NQ], C (&Z0)[9], C (&Z1)[9], C (&Z2)[9]) {
 #pragma unroll
   for (int l = 0; l < 9; ++l) {
     const C m = g[3 * l + 0], o = g[3 * l + 1], p = g[3 * l + 2];
@@ -119,6 +120,52 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
     Z1[l] = p - m;
     Z2[l] = s;
   }
+}
+
+// FP32 sums of FP16 values in one instruction: v_fma_mix_f32 converts its FP16 operands
+// exactly and rounds a * 1.0 + b once, which is bit for bit the FP32 sum of the converted
+// values (a conversion of each operand and an add, three instructions, otherwise).
+__device__ __forceinline__ float add_hh(_Float16 a, _Float16 b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float sub_hh(_Float16 a, _Float16 b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float add_hf(_Float16 a, float b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float sub_hf(_Float16 a, float b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+// The first pass straight from the stored FP16 values (still multiplied by the storage
+// scale): every sum, like every later sum of the forward transform, is then exactly the
+// scale times the sum of the unscaled values (power-of-two factor, far from overflow and
+// underflow), so one multiply of each of the 10 raw moments by the inverse scale gives bit for
+// bit what scaling the 27 populations first would.
+__device__ __forceinline__ void zlines_h(const _Float16 (&h)[NQ], float (&Z0)[9], float (&Z1)[9], float (&Z2)[9]) {
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    const _Float16 m = h[3 * l + 0], o = h[3 * l + 1], p = h[3 * l + 2];
+    const float s = add_hh(p, m);
+    Z0[l] = add_hf(o, s);
+    Z1[l] = sub_hh(p, m);
+    Z2[l] = s;
+  }
+}
+
+// The collision from the first pass.  With SCALED the raw moments are multiplied by in_scale.
+template <typename C, bool SCALED>
+__device__ __forceinline__ void collide_z(const C (&Z0)[9], const C (&Z1)[9], const C (&Z2)[9], C in_scale,
+                                          C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out, C (&u_out)[3],
+                                          C out_scale) {
   C Y00[3], Y10[3], Y20[3], Y01[3], Y11[3], Y02[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -131,13 +178,17 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
     Y11[a] = Z1[b + 2] - Z1[b + 0];
     Y02[a] = Z2[b + 0] + Z2[b + 1] + Z2[b + 2];
   }
-  const C drho = Y00[0] + Y00[1] + Y00[2];
-  const C jx = Y00[2] - Y00[0], sxx = Y00[2] + Y00[0];
-  const C jy = Y10[0] + Y10[1] + Y10[2], sxy = Y10[2] - Y10[0];
-  const C syy = Y20[0] + Y20[1] + Y20[2];
-  const C jz = Y01[0] + Y01[1] + Y01[2], sxz = Y01[2] - Y01[0];
-  const C syz = Y11[0] + Y11[1] + Y11[2];
-  const C szz = Y02[0] + Y02[1] + Y02[2];
+  C drho = Y00[0] + Y00[1] + Y00[2];
+  C jx = Y00[2] - Y00[0], sxx = Y00[2] + Y00[0];
+  C jy = Y10[0] + Y10[1] + Y10[2], sxy = Y10[2] - Y10[0];
+  C syy = Y20[0] + Y20[1] + Y20[2];
+  C jz = Y01[0] + Y01[1] + Y01[2], sxz = Y01[2] - Y01[0];
+  C syz = Y11[0] + Y11[1] + Y11[2];
+  C szz = Y02[0] + Y02[1] + Y02[2];
+  if constexpr (SCALED) {
+    drho *= in_scale; jx *= in_scale; jy *= in_scale; jz *= in_scale;
+    sxx *= in_scale; sxy *= in_scale; syy *= in_scale; sxz *= in_scale; syz *= in_scale; szz *= in_scale;
+  }
 
   const C third = C(1) / C(3);
   const C ir = C(1) / (C(1) + drho);
@@ -222,6 +273,23 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
   u_out[0] = ux; u_out[1] = uy; u_out[2] = uz;
 }
 template <typename C>
+__device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out, C (&u_out)[3],
+                                        C out_scale = C(1)) {
+  C Z0[9], Z1[9], Z2[9];
+  zlines<C>(g, Z0, Z1, Z2);
+  collide_z<C, false>(Z0, Z1, Z2, C(1), g, ws, wb, drho_out, u2_out, u_out, out_scale);
+}
+// The collision of FP16-stored populations h (the shifted populations times the storage
+// scale, 1 / inv_scale), bit for bit collide() of the populations loaded as FP32 and
+// multiplied by inv_scale; the result goes to g.
+__device__ __forceinline__ void collide_h(const _Float16 (&h)[NQ], float inv_scale, float (&g)[NQ], float ws,
+                                          float wb, float &drho_out, float &u2_out, float (&u_out)[3],
+                                          float out_scale) {
+  float Z0[9], Z1[9], Z2[9];
+  zlines_h(h, Z0, Z1, Z2);
+  collide_z<float, true>(Z0, Z1, Z2, inv_scale, g, ws, wb, drho_out, u2_out, u_out, out_scale);
+}
+template <typename C>
 __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &u2_out) {
   C u[3];
   collide<C>(g, ws, wb, drho_out, u2_out, u);
@@ -268,7 +336,7 @@ __device__ __forceinline__ void fp16_conserving(const float (&s)[NQ], _Float16 (
     if (cxq(i) != 0 && cyq(i) != 0 && czq(i) != 0) continue;
     const float si = s[i];
     h[i] = (_Float16)si;
-    const float d = (float)h[i] - si;
+    const float d = sub_hf(h[i], si);  // (float)h[i] - si
     dm += d;
     if (cxq(i)) dx += (float)cxq(i) * d;
     if (cyq(i)) dy += (float)cyq(i) * d;

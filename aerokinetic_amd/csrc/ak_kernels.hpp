@@ -345,6 +345,11 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
       const long long plane = ODD ? L.base + (long long)z * L.PS : fresh_sgpr64(L.base + (long long)z * L.PS);
       const long long zp = (z == L.nz - 1) ? -(long long)(L.nz - 1) * L.PS : L.PS;
       const long long zm = (z == 0) ? (long long)(L.nz - 1) * L.PS : -L.PS;
+      // FP16 storage: the loads keep the stored halves, which the collision's first pass adds in
+      // mixed precision (collide_h); FP64 / FP32: the values themselves
+      using RT = std::conditional_t<S == 2, _Float16, C>;
+      typedef __attribute__((address_space(1))) const typename St<S>::T gT;
+      RT r[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         long long d;
@@ -355,32 +360,38 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
           d = (long long)q * L.QS;
         }
         const gchar *bq = uniform_ptr(A + (plane + d));
-        g[q] = ld_g<S>(bq + boff, inv_scale);
+        r[q] = *(gT *)(bq + boff);
       }
       // x-face cells take their nine populations entering through that face from the farfield's
       // copy (the farfield no longer writes the lattice slots these lanes gathered above)
-      // (the register index into g stays a compile-time constant, with one select per face:
+      // (the register index into r stays a compile-time constant, with one select per face:
       // a lane-dependent index becomes a compare-and-select over all 27 registers)
       if (xi != nullptr && __any(x == 0 || x == L.nx - 1)) {
         const bool lo = x == 0, hi = !lo && x == L.nx - 1;
         if (lo || hi) {
-          C v[9];
+          RT v[9];
 #pragma unroll
-          for (int j = 0; j < 9; ++j) v[j] = ld<S>(xi + xo_at(L, hi, j, y, z), inv_scale);
+          for (int j = 0; j < 9; ++j) v[j] = ((gT *)xi)[xo_at(L, hi, j, y, z)];
 #pragma unroll
           for (int j = 0; j < 9; ++j) {
-            g[18 + j] = lo ? v[j] : g[18 + j];
-            g[j] = hi ? v[j] : g[j];
+            r[18 + j] = lo ? v[j] : r[18 + j];
+            r[j] = hi ? v[j] : r[j];
           }
         }
       }
-      loads_issued(g);
+      loads_issued(r);
       active = mk != 1 && mk != 3;
       if (active) {
         C uc[3];
         // FP16: the collision returns the populations times the storage scale, and the mass- and
         // momentum-conserving rounding rounds each of them once
-        collide<C>(g, ws, wb, drho, u2, uc, S == 2 ? (C)scale : C(1));
+        if constexpr (S == 2) {
+          collide_h(r, inv_scale, g, ws, wb, drho, u2, uc, scale);
+        } else {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) g[q] = r[q];
+          collide<C>(g, ws, wb, drho, u2, uc, C(1));
+        }
         _Float16 h[S == 2 ? NQ : 1];
         if constexpr (S == 2) fp16_conserving(g, h);
         const long long splane = plane;
