@@ -395,6 +395,11 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
         _Float16 h[S == 2 ? NQ : 1];
         if constexpr (S == 2) fp16_conserving(g, h);
         const long long splane = plane;
+        // the 32-bit offset taken afresh in this block, so that each store, like each load, is a
+        // scalar base plus this offset (otherwise the stores get 64-bit vector addresses, one
+        // 64-bit add each, from the offset widened in the loads' block)
+        unsigned soff = boff;
+        asm volatile("" : "+v"(soff));
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           long long d;
@@ -405,8 +410,8 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
             d = (long long)opp(q) * L.QS;
           }
           gchar *bq = uniform_ptr(A + (splane + d));
-          if constexpr (S == 2) *(__attribute__((address_space(1))) _Float16 *)(bq + boff) = h[q];
-          else st_g<S>(bq + boff, g[q], scale);
+          if constexpr (S == 2) *(__attribute__((address_space(1))) _Float16 *)(bq + soff) = h[q];
+          else st_g<S>(bq + soff, g[q], scale);
         }
         // cells on the x and y faces keep their density deviation and velocity (conserved by
         // the collision) for the next farfield pass, and x-face cells their outgoing populations
