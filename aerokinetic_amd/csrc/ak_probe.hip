@@ -308,4 +308,81 @@ int akp_placements(int elem_bytes, int nx, int ny, int nz, int bx, int waves, in
   return got;
 }
 
+
+// The same pattern on `nbuf` buffers (held at once, each allocated `max offset` bytes larger),
+// each timed with the box starting at every one of `noff` byte offsets into the buffer
+// (multiples of 256): does a buffer's rate depend on where in its pages the box starts?
+// out[(k * noff + j) * 2 + odd] = ms per launch (buffer k, offset j); returns the number of
+// buffers timed, or -1 with akp_last_error() set.
+int akp_offsets(int elem_bytes, int nx, int ny, int nz, int bx, int nbuf, int noff, const long long *offsets,
+                int reps, double *out) {
+  if ((elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8) || nx < 2 || ny < 2 || nz < 2 || reps < 1 || !out ||
+      !offsets || nbuf < 1 || nbuf > 16 || noff < 1 || noff > 64 || (bx != 64 && bx != 128 && bx != 256)) {
+    snprintf(g_err, sizeof g_err, "bad offset-probe arguments");
+    return -1;
+  }
+  long long max_off = 0;
+  for (int j = 0; j < noff; ++j) {
+    if (offsets[j] < 0 || offsets[j] % 256) {
+      snprintf(g_err, sizeof g_err, "offsets must be non-negative multiples of 256");
+      return -1;
+    }
+    max_off = offsets[j] > max_off ? offsets[j] : max_off;
+  }
+  Box2 B{};
+  B.nx = nx; B.ny = ny; B.nz = nz; B.bx = bx; B.by = 256 / bx;
+  const int per_line = 128 / elem_bytes;
+  B.R = ((long long)nx + 2 + per_line - 1) / per_line * per_line;
+  B.gx = (nx + bx - 1) / bx;
+  B.gy = (ny + B.by - 1) / B.by;
+  B.nblocks = B.gx * B.gy * nz;
+  const size_t bytes = (size_t)B.R * ny * nz * 27 * elem_bytes + (size_t)max_off;
+  void *held[16] = {};
+  int got = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreate(&st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  for (int k = 0; k < nbuf && e == hipSuccess; ++k) {
+    if (hipMalloc(&held[k], bytes) != hipSuccess) {
+      (void)hipGetLastError();  // out of memory: time the buffers held so far
+      held[k] = nullptr;
+      break;
+    }
+    e = hipMemset(held[k], 0, bytes);
+    for (int j = 0; j < noff && e == hipSuccess; ++j) {
+      void *A = (char *)held[k] + offsets[j];
+      for (int odd = 0; odd < 2 && e == hipSuccess; ++odd) {
+        auto launch = [&]() -> hipError_t {
+          switch (elem_bytes) {
+            case 2: return launch_rows_tile<uint16_t>(A, B, odd != 0, 0, st);
+            case 4: return launch_rows_tile<uint32_t>(A, B, odd != 0, 0, st);
+            default: return launch_rows_tile<uint64_t>(A, B, odd != 0, 0, st);
+          }
+        };
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        out[((size_t)k * noff + j) * 2 + odd] = ms / reps;
+      }
+    }
+    if (e == hipSuccess) got = k + 1;
+  }
+  for (int k = 0; k < 16; ++k)
+    if (held[k]) (void)hipFree(held[k]);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (e != hipSuccess) {
+    snprintf(g_err, sizeof g_err, "%s", hipGetErrorString(e));
+    return -1;
+  }
+  return got;
+}
+
 }  // extern "C"

@@ -19,6 +19,9 @@ run of the sweep alternates.  Prints a markdown table (and one JSON line with --
 allocated and held at once, for each tile and cap on resident waves per SIMD: the same
 launch on buffers of other physical pages, and under fewer resident waves, moves a
 different number of bytes per second.
+
+--offsets O1,O2,... times it on --buffers buffers held at once with the box starting at each
+byte offset into the buffer: whether a buffer's rate follows where in its pages the data starts.
 """
 from __future__ import annotations
 
@@ -42,7 +45,42 @@ def _lib():
     L.akp_last_error.restype = ctypes.c_char_p
     L.akp_placements.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(ctypes.c_double)]
     L.akp_placements.restype = ctypes.c_int
+    L.akp_offsets.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_longlong), ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_double)]
+    L.akp_offsets.restype = ctypes.c_int
     return L
+
+
+def offsets(L, storage, shape, bx, nbuf, offs, reps):
+    """[[(even GB/s, odd GB/s) per offset] per buffer]: the pattern on `nbuf` buffers held at
+    once, the box starting `offs[j]` bytes into each."""
+    n = len(offs)
+    arr = (ctypes.c_longlong * n)(*offs)
+    out = (ctypes.c_double * (2 * n * nbuf))()
+    got = L.akp_offsets(BYTES[storage], *shape, bx, nbuf, n, arr, reps, out)
+    if got < 0:
+        raise RuntimeError(f"offset probe {storage}: {L.akp_last_error().decode()}")
+    nbytes = 2.0 * 27 * BYTES[storage] * shape[0] * shape[1] * shape[2]
+    return [[(nbytes / (out[(k * n + j) * 2] * 1e-3) / 1e9, nbytes / (out[(k * n + j) * 2 + 1] * 1e-3) / 1e9)
+             for j in range(n)] for k in range(got)]
+
+
+def main_offsets(a, L, shape):
+    """Table of the sweep pattern's rate on several buffers, each with the box at several offsets."""
+    offs = [int(v) for v in a.offsets.split(",")]
+    print("| storage | buffer | offset | even TB/s | odd TB/s | mean % of 8 TB/s |", flush=True)
+    print("|---|---:|---:|---:|---:|---:|", flush=True)
+    rows = []
+    for st in [s for s in a.storages.split(",") if s]:
+        for k, per in enumerate(offsets(L, st, shape, a.tiles_bx, a.buffers, offs, a.reps)):
+            for off, (ge, go) in zip(offs, per):
+                mean = 2 * ge * go / (ge + go)
+                rows.append({"storage": st, "buffer": k, "offset": off, "even_gbs": ge, "odd_gbs": go})
+                print(f"| {st} | {k} | {off} | {ge / 1e3:.2f} | {go / 1e3:.2f} | {mean / PEAK_GBS * 100:.1f} |",
+                      flush=True)
+    if a.json:
+        print(json.dumps({"shape": shape, "reps": a.reps, "offsets": rows}))
+    return 0
 
 
 def placements(L, storage, shape, bx, waves, nbuf, reps):
@@ -95,9 +133,15 @@ def main(argv=None):
                          "the sweep's row pitch) for every tile of --tiles and wave cap of --waves")
     ap.add_argument("--tiles", default="64,256", help="--placements: tile widths bx (tiles bx x 256/bx)")
     ap.add_argument("--waves", default="0,4,3", help="--placements: resident-wave caps per SIMD (0: none)")
+    ap.add_argument("--offsets", default="", help="instead of the layout table: byte offsets (multiples of 256) at "
+                    "which the box starts in each of --buffers buffers held at once (tiles --tile-bx x 256/bx)")
+    ap.add_argument("--buffers", type=int, default=4)
+    ap.add_argument("--tile-bx", dest="tiles_bx", type=int, default=256)
     a = ap.parse_args(argv)
     shape = tuple(int(v) for v in a.shape.split(","))
     L = _lib()
+    if a.offsets:
+        return main_offsets(a, L, shape)
     if a.placements:
         return main_placements(a, L, shape)
     rows = []
