@@ -48,7 +48,122 @@ def _lib():
     L.akp_offsets.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_longlong), ctypes.c_int,
                                                    ctypes.POINTER(ctypes.c_double)]
     L.akp_offsets.restype = ctypes.c_int
+    L.akp_vmm.argtypes = [ctypes.c_int] * 6 + [ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_double)]
+    L.akp_vmm.restype = ctypes.c_int
+    L.akp_screen.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.akp_screen.restype = ctypes.c_int
+    L.akp_orders.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_double)]
+    L.akp_orders.restype = ctypes.c_int
+    L.akp_affinity.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_double)]
+    L.akp_affinity.restype = ctypes.c_int
     return L
+
+
+def main_affinity(a, L):
+    """Read GB/s of each XCD alone from each region of one buffer."""
+    n = a.affinity
+    out = (ctypes.c_double * (8 * n))()
+    got = L.akp_affinity(a.piece_bytes, n, a.xcd_blocks, a.reps, out)
+    if got < 0:
+        raise RuntimeError(f"affinity probe: {L.akp_last_error().decode()}")
+    print("| region | " + " | ".join(f"XCD {k}" for k in range(8)) + " |", flush=True)
+    print("|---:|" + "---:|" * 8, flush=True)
+    rows = []
+    for r in range(got):
+        v = [out[r * 8 + k] for k in range(8)]
+        rows.append(v)
+        print(f"| {r} | " + " | ".join(f"{x:.0f}" for x in v) + " |", flush=True)
+    if a.json:
+        print(json.dumps({"region_bytes": a.piece_bytes, "affinity_gbs": rows}))
+    return 0
+
+
+def main_orders(a, L, shape):
+    """Table of the pattern on several buffers under several block orders (0: one contiguous slab per
+    XCD, -1: hardware order, C: chunks of C tiles dealt to the XCDs in turn)."""
+    orders = [int(v) for v in a.orders.split(",")]
+    print("| storage | buffer | order | even TB/s | odd TB/s | mean % of 8 TB/s |", flush=True)
+    print("|---|---:|---:|---:|---:|---:|", flush=True)
+    rows = []
+    for st in [s for s in a.storages.split(",") if s]:
+        n = len(orders)
+        arr = (ctypes.c_int * n)(*orders)
+        out = (ctypes.c_double * (2 * n * a.buffers))()
+        got = L.akp_orders(BYTES[st], *shape, a.tiles_bx, a.buffers, n, arr, a.reps, out)
+        if got < 0:
+            raise RuntimeError(f"order probe {st}: {L.akp_last_error().decode()}")
+        nbytes = 2.0 * 27 * BYTES[st] * shape[0] * shape[1] * shape[2]
+        for k in range(got):
+            for j, o in enumerate(orders):
+                ge = nbytes / (out[(k * n + j) * 2] * 1e-3) / 1e9
+                go = nbytes / (out[(k * n + j) * 2 + 1] * 1e-3) / 1e9
+                mean = 2 * ge * go / (ge + go)
+                rows.append({"storage": st, "buffer": k, "order": o, "even_gbs": ge, "odd_gbs": go})
+                print(f"| {st} | {k} | {o} | {ge / 1e3:.2f} | {go / 1e3:.2f} | {mean / PEAK_GBS * 100:.1f} |", flush=True)
+    if a.json:
+        print(json.dumps({"shape": shape, "reps": a.reps, "orders": rows}))
+    return 0
+
+
+def main_screen(a, L, shape):
+    """Per-piece rates of --screen physical pieces, and buffers made of the fastest / slowest pieces."""
+    rows = {}
+    for st in [s for s in a.storages.split(",") if s]:
+        n = a.screen
+        out = (ctypes.c_double * (2 * n + 4))()
+        got = L.akp_screen(BYTES[st], shape[0], shape[1], a.nz_piece, shape[2], a.tiles_bx, n, a.piece_bytes, a.reps,
+                           out)
+        if got < 0:
+            raise RuntimeError(f"screen probe {st}: {L.akp_last_error().decode()}")
+        pb = 2.0 * 27 * BYTES[st] * shape[0] * shape[1] * a.nz_piece
+        fb = 2.0 * 27 * BYTES[st] * shape[0] * shape[1] * shape[2]
+        mean = lambda nbytes, te, to: 2 * nbytes / ((te + to) * 1e-3) / 1e9 / PEAK_GBS * 100
+        pieces = [mean(pb, out[2 * i], out[2 * i + 1]) for i in range(got)]
+        fast = mean(fb, out[2 * n], out[2 * n + 1]) if out[2 * n] > 0 else None
+        slow = mean(fb, out[2 * n + 2], out[2 * n + 3]) if out[2 * n + 2] > 0 else None
+        srt = sorted(pieces)
+        print(f"{st}: {got} pieces of {a.piece_bytes} bytes; piece rates % of 8 TB/s: min {srt[0]:.1f} "
+              f"p25 {srt[len(srt) // 4]:.1f} median {srt[len(srt) // 2]:.1f} p75 {srt[3 * len(srt) // 4]:.1f} "
+              f"max {srt[-1]:.1f}", flush=True)
+        print(f"{st}: in allocation order: " + " ".join(f"{v:.0f}" for v in pieces), flush=True)
+        print(f"{st}: buffer of the fastest pieces {fast if fast is None else round(fast, 1)} %, of the slowest "
+              f"{slow if slow is None else round(slow, 1)} %", flush=True)
+        rows[st] = {"pieces": pieces, "fastest_buffer_pct": fast, "slowest_buffer_pct": slow}
+    if a.json:
+        print(json.dumps({"shape": shape, "piece_bytes": a.piece_bytes, "nz_piece": a.nz_piece, "screen": rows}))
+    return 0
+
+
+def main_vmm(a, L, shape):
+    """Table of the pattern on buffers built from physical pieces of --vmm-chunk bytes, each reached
+    through two address ranges mapped onto the same physical memory."""
+    print("| storage | piece | buffer | range | even TB/s | odd TB/s | mean % of 8 TB/s |", flush=True)
+    print("|---|---:|---:|---:|---:|---:|---:|", flush=True)
+    rows = []
+    for st in [s for s in a.storages.split(",") if s]:
+        for chunk in [int(v) for v in a.vmm.split(",")]:
+            out = (ctypes.c_double * (4 * a.buffers + 1))()
+            got = L.akp_vmm(BYTES[st], *shape, a.tiles_bx, a.buffers, chunk, a.ranges, a.reps, out)
+            if got < 0:
+                raise RuntimeError(f"vmm probe {st}: {L.akp_last_error().decode()}")
+            if got < a.buffers:
+                print(f"(piece {chunk}: {got} of {a.buffers} buffers made: {L.akp_last_error().decode()})", flush=True)
+            nbytes = 2.0 * 27 * BYTES[st] * shape[0] * shape[1] * shape[2]
+            for k in range(got):
+                for r in range(a.ranges):
+                    ge = nbytes / (out[4 * k + 2 * r] * 1e-3) / 1e9
+                    go = nbytes / (out[4 * k + 2 * r + 1] * 1e-3) / 1e9
+                    mean = 2 * ge * go / (ge + go)
+                    rows.append({"storage": st, "chunk": chunk, "buffer": k, "range": r, "even_gbs": ge, "odd_gbs": go,
+                                 "granularity": out[4 * a.buffers]})
+                    print(f"| {st} | {chunk or 'whole'} | {k} | {r} | {ge / 1e3:.2f} | {go / 1e3:.2f} | "
+                          f"{mean / PEAK_GBS * 100:.1f} |", flush=True)
+    if a.json:
+        print(json.dumps({"shape": shape, "reps": a.reps, "vmm": rows}))
+    return 0
 
 
 def offsets(L, storage, shape, bx, nbuf, offs, reps):
@@ -136,12 +251,32 @@ def main(argv=None):
     ap.add_argument("--offsets", default="", help="instead of the layout table: byte offsets (multiples of 256) at "
                     "which the box starts in each of --buffers buffers held at once (tiles --tile-bx x 256/bx)")
     ap.add_argument("--buffers", type=int, default=4)
+    ap.add_argument("--affinity", type=int, default=0, help="instead of the layout table: read GB/s of each XCD "
+                    "alone from each of this many --piece-bytes regions of one buffer")
+    ap.add_argument("--xcd-blocks", type=int, default=1024, help="--affinity: blocks per XCD")
+    ap.add_argument("--orders", default="", help="instead of the layout table: block orders (0 one slab per XCD, -1 "
+                    "hardware order, C chunks of C tiles) timed on --buffers buffers held at once")
+    ap.add_argument("--screen", type=int, default=0, help="instead of the layout table: this many physical pieces "
+                    "timed one by one (--nz-piece planes each), then buffers of the fastest and the slowest pieces")
+    ap.add_argument("--piece-bytes", type=int, default=1 << 30)
+    ap.add_argument("--nz-piece", type=int, default=3)
+    ap.add_argument("--ranges", type=int, default=2, help="--vmm: address ranges mapped onto each buffer (1 or 2)")
+    ap.add_argument("--vmm", default="", help="instead of the layout table: piece sizes in bytes (0: one piece) of "
+                    "--buffers buffers built with the virtual-memory interface, each mapped twice")
     ap.add_argument("--tile-bx", dest="tiles_bx", type=int, default=256)
     a = ap.parse_args(argv)
     shape = tuple(int(v) for v in a.shape.split(","))
     L = _lib()
     if a.offsets:
         return main_offsets(a, L, shape)
+    if a.vmm:
+        return main_vmm(a, L, shape)
+    if a.screen:
+        return main_screen(a, L, shape)
+    if a.orders:
+        return main_orders(a, L, shape)
+    if a.affinity:
+        return main_affinity(a, L)
     if a.placements:
         return main_placements(a, L, shape)
     rows = []
