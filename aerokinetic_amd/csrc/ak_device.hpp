@@ -295,6 +295,7 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
   collide<C>(g, ws, wb, drho_out, u2_out, u);
 }
 
+
 // FP16 storage of a cell's 27 post-collision populations that keeps its mass and momentum.
 // Rounding each shifted population to FP16 on its own changes the cell's sum and first
 // moments by the sum of 27 rounding errors, a few units in the last place of the largest
@@ -329,14 +330,18 @@ __device__ __forceinline__ void conserve_fp16(typename St<S>::C (&g)[NQ], float 
 // The FP16 values conserve_fp16 followed by the storage rounding stores, computed from the
 // scaled values s = g * scale (the collision's output with out_scale = scale; scale is a power
 // of two, so each step gives the same result in either unit and each direction is rounded once).
-__device__ __forceinline__ void fp16_conserving(const float (&s)[NQ], _Float16 (&h)[NQ]) {
+// store(i, h) receives direction i's value as soon as it is final: the non-corner directions
+// in index order (their deficits summed in that order), then the corners.
+template <class Store>
+__device__ __forceinline__ void fp16_conserving_each(const float (&s)[NQ], Store &&store) {
   float dm = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
 #pragma unroll
   for (int i = 0; i < NQ; ++i) {
     if (cxq(i) != 0 && cyq(i) != 0 && czq(i) != 0) continue;
     const float si = s[i];
-    h[i] = (_Float16)si;
-    const float d = sub_hf(h[i], si);  // (float)h[i] - si
+    const _Float16 hi = (_Float16)si;
+    store(i, hi);
+    const float d = sub_hf(hi, si);  // (float)hi - si
     dm += d;
     if (cxq(i)) dx += (float)cxq(i) * d;
     if (cyq(i)) dy += (float)cyq(i) * d;
@@ -350,7 +355,7 @@ __device__ __forceinline__ void fp16_conserving(const float (&s)[NQ], _Float16 (
     // rounding), which it does or not depending on the surrounding code
     float c = s[i] - 0.125f * (dm + (float)cxq(i) * dx + (float)cyq(i) * dy + (float)czq(i) * dz);
     asm volatile("" : "+v"(c));
-    h[i] = (_Float16)c;
+    store(i, (_Float16)c);
   }
 }
 

@@ -366,7 +366,7 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
       // copy (the farfield no longer writes the lattice slots these lanes gathered above)
       // (the register index into r stays a compile-time constant, with one select per face:
       // a lane-dependent index becomes a compare-and-select over all 27 registers)
-      if (xi != nullptr && __any(x == 0 || x == L.nx - 1)) {
+      if (S != 2 && xi != nullptr && __any(x == 0 || x == L.nx - 1)) {
         const bool lo = x == 0, hi = !lo && x == L.nx - 1;
         if (lo || hi) {
           RT v[9];
@@ -392,16 +392,13 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
           for (int q = 0; q < NQ; ++q) g[q] = r[q];
           collide<C>(g, ws, wb, drho, u2, uc, C(1));
         }
-        _Float16 h[S == 2 ? NQ : 1];
-        if constexpr (S == 2) fp16_conserving(g, h);
         const long long splane = plane;
         // the 32-bit offset taken afresh in this block, so that each store, like each load, is a
         // scalar base plus this offset (otherwise the stores get 64-bit vector addresses, one
         // 64-bit add each, from the offset widened in the loads' block)
         unsigned soff = boff;
         asm volatile("" : "+v"(soff));
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
+        auto dst = [&](int q) -> gchar * {
           long long d;
           if constexpr (ODD) {
             const int cz = czq(q);
@@ -409,21 +406,26 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
           } else {
             d = (long long)opp(q) * L.QS;
           }
-          gchar *bq = uniform_ptr(A + (splane + d));
-          if constexpr (S == 2) *(__attribute__((address_space(1))) _Float16 *)(bq + soff) = h[q];
-          else st_g<S>(bq + soff, g[q], scale);
+          return uniform_ptr(A + (splane + d)) + soff;
+        };
+        if constexpr (S == 2) {
+          // each value stored as soon as the conserving rounding has made it final
+          fp16_conserving_each(g, [&](int q, _Float16 v) { *(__attribute__((address_space(1))) _Float16 *)dst(q) = v; });
+        } else {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) st_g<S>(dst(q), g[q], scale);
         }
         // cells on the x and y faces keep their density deviation and velocity (conserved by
         // the collision) for the next farfield pass, and x-face cells their outgoing populations
+        // (FP64 / FP32 storage only)
         if (pm != nullptr && (x == 0 || y == 0 || x == L.nx - 1 || y == L.ny - 1))
           perimeter_put<C>(pm, L, x, y, z, drho, uc);
-        if (xo != nullptr && (x == 0 || x == L.nx - 1)) {
-          typedef __attribute__((address_space(1))) typename St<S>::T gT;
-          const bool hi = x != 0;
+        if constexpr (S != 2) {
+          if (xo != nullptr && (x == 0 || x == L.nx - 1)) {
+            typedef __attribute__((address_space(1))) typename St<S>::T gT;
+            const bool hi = x != 0;
 #pragma unroll
-          for (int j = 0; j < 9; ++j) {
-            if constexpr (S == 2) ((gT *)xo)[xo_at(L, hi, j, y, z)] = hi ? h[18 + j] : h[j];
-            else ((gT *)xo)[xo_at(L, hi, j, y, z)] = hi ? g[18 + j] : g[j];
+            for (int j = 0; j < 9; ++j) ((gT *)xo)[xo_at(L, hi, j, y, z)] = hi ? g[18 + j] : g[j];
           }
         }
       }
