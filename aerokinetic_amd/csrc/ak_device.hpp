@@ -191,7 +191,11 @@ __device__ __forceinline__ void collide_z(const C (&Z0)[9], const C (&Z1)[9], co
   }
 
   const C third = C(1) / C(3);
-  const C ir = C(1) / (C(1) + drho);
+  // FP32: the hardware reciprocal (within 1 ulp) instead of the correctly rounded division
+  // sequence (about ten instructions); FP64 keeps the division
+  C ir;
+  if constexpr (sizeof(C) == 4) ir = __builtin_amdgcn_rcpf(C(1) + drho);
+  else ir = C(1) / (C(1) + drho);
   const C ux = jx * ir, uy = jy * ir, uz = jz * ir;
   // second central moments minus rho/3 (A) and off-diagonals (B)
   const C d3 = drho * third;
@@ -240,10 +244,23 @@ __device__ __forceinline__ void collide_z(const C (&Z0)[9], const C (&Z1)[9], co
   // velocity components comes out 2^n times too large, and one multiply by out_scale * 2^-n
   // at the end gives exactly what halving in every pass would (power-of-two scaling is exact).
   const C fac[4] = {out_scale, out_scale * C(0.5), out_scale * C(0.25), out_scale * C(0.125)};
+  // one line's two raw moments from its central ones, m1 = k1 + u k0 and m2 = k2 + 2 u k1 + u^2 k0:
+  // FP64 keeps the nested form; FP32 takes three fused multiply-adds with u^2 and 2u formed once
+  // per axis (one instruction fewer per line, 27 per cell)
+  auto raw = [](C u, C uu, C k0, C k1, C k2, C &m1, C &m2) {
+    if constexpr (sizeof(C) == 4) {
+      m1 = __builtin_fmaf(u, k0, k1);
+      m2 = __builtin_fmaf(uu, k0, __builtin_fmaf(C(2) * u, k1, k2));
+    } else {
+      m1 = k1 + u * k0;
+      m2 = k2 + u * (C(2) * k1 + u * k0);
+    }
+  };
 #pragma unroll
   for (int l = 0; l < 9; ++l) {  // z lines: index 3l + c
     const C k0 = K[3 * l], k1 = K[3 * l + 1], k2 = K[3 * l + 2];
-    const C m1 = k1 + uz * k0, m2 = k2 + uz * (C(2) * k1 + uz * k0);
+    C m1, m2;
+    raw(uz, uz2, k0, k1, k2, m1, m2);
     K[3 * l + 1] = k0 - m2;
     K[3 * l + 2] = m2 + m1;
     K[3 * l + 0] = m2 - m1;
@@ -254,7 +271,8 @@ __device__ __forceinline__ void collide_z(const C (&Z0)[9], const C (&Z1)[9], co
     for (int c = 0; c < 3; ++c) {  // y lines: index 9a + 3b + c
       const int o = 9 * a + c;
       const C k0 = K[o], k1 = K[o + 3], k2 = K[o + 6];
-      const C m1 = k1 + uy * k0, m2 = k2 + uy * (C(2) * k1 + uy * k0);
+      C m1, m2;
+      raw(uy, uy2, k0, k1, k2, m1, m2);
       K[o + 3] = k0 - m2;
       K[o + 6] = m2 + m1;
       K[o + 0] = m2 - m1;
@@ -262,7 +280,8 @@ __device__ __forceinline__ void collide_z(const C (&Z0)[9], const C (&Z1)[9], co
 #pragma unroll
   for (int o = 0; o < 9; ++o) {  // x lines: index 9a + o
     const C k0 = K[o], k1 = K[o + 9], k2 = K[o + 18];
-    const C m1 = k1 + ux * k0, m2 = k2 + ux * (C(2) * k1 + ux * k0);
+    C m1, m2;
+    raw(ux, ux2, k0, k1, k2, m1, m2);
     const int n = (o / 3 != 1) + (o % 3 != 1);  // nonzero y and z components of this line
     g[o + 9] = (k0 - m2) * fac[n];
     g[o + 18] = (m2 + m1) * fac[n + 1];
@@ -309,15 +328,27 @@ __device__ __forceinline__ void collide(C (&g)[NQ], C ws, C wb, C &drho_out, C &
 template <int S>
 __device__ __forceinline__ void conserve_fp16(typename St<S>::C (&g)[NQ], float scale, float inv_scale) {
   if constexpr (S == 2) {
+    // the deficits summed as fp16_conserving_each sums them
+    auto deficit = [&](int i) { return (float)(_Float16)(g[i] * scale) * inv_scale - g[i]; };
     float dm = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+    constexpr int full[5] = {1, 3, 4, 5, 7}, ends[4] = {0, 2, 6, 8};
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-      if (cxq(i) != 0 && cyq(i) != 0 && czq(i) != 0) continue;
-      const float d = (float)(_Float16)(g[i] * scale) * inv_scale - g[i];
-      dm += d;
-      if (cxq(i)) dx += (float)cxq(i) * d;
-      if (cyq(i)) dy += (float)cyq(i) * d;
-      if (czq(i)) dz += (float)czq(i) * d;
+    for (int k = 0; k < 5; ++k) {
+      const int o = full[k];
+      const float a = deficit(o), b = deficit(o + 9), c = deficit(o + 18);
+      const float line = a + b + c;
+      dm += line;
+      dx += c - a;
+      if (cyq(o)) dy += (float)cyq(o) * line;
+      if (czq(o)) dz += (float)czq(o) * line;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int o = ends[k];
+      const float b = deficit(o + 9);
+      dm += b;
+      dy += (float)cyq(o) * b;
+      dz += (float)czq(o) * b;
     }
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
@@ -330,22 +361,37 @@ __device__ __forceinline__ void conserve_fp16(typename St<S>::C (&g)[NQ], float 
 // The FP16 values conserve_fp16 followed by the storage rounding stores, computed from the
 // scaled values s = g * scale (the collision's output with out_scale = scale; scale is a power
 // of two, so each step gives the same result in either unit and each direction is rounded once).
-// store(i, h) receives direction i's value as soon as it is final: the non-corner directions
-// in index order (their deficits summed in that order), then the corners.
+// store(i, h) receives direction i's value as soon as it is final, the 19 non-corner
+// directions first, then the corners.
 template <class Store>
 __device__ __forceinline__ void fp16_conserving_each(const float (&s)[NQ], Store &&store) {
-  float dm = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
-#pragma unroll
-  for (int i = 0; i < NQ; ++i) {
-    if (cxq(i) != 0 && cyq(i) != 0 && czq(i) != 0) continue;
-    const float si = s[i];
-    const _Float16 hi = (_Float16)si;
+  // rounding deficit (float)h - s of a non-corner direction, its value stored
+  auto deficit = [&](int i) {
+    const _Float16 hi = (_Float16)s[i];
     store(i, hi);
-    const float d = sub_hf(hi, si);  // (float)hi - si
-    dm += d;
-    if (cxq(i)) dx += (float)cxq(i) * d;
-    if (cyq(i)) dy += (float)cyq(i) * d;
-    if (czq(i)) dz += (float)czq(i) * d;
+    return sub_hf(hi, s[i]);
+  };
+  // the deficits summed per x line (directions o, o + 9, o + 18 at c_x = -1, 0, +1): the five
+  // lines without a corner first, then the middle values of the four lines that end in corners
+  float dm = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+  constexpr int full[5] = {1, 3, 4, 5, 7}, ends[4] = {0, 2, 6, 8};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const int o = full[k];
+    const float a = deficit(o), b = deficit(o + 9), c = deficit(o + 18);
+    const float line = a + b + c;
+    dm += line;
+    dx += c - a;
+    if (cyq(o)) dy += (float)cyq(o) * line;
+    if (czq(o)) dz += (float)czq(o) * line;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int o = ends[k];
+    const float b = deficit(o + 9);
+    dm += b;
+    dy += (float)cyq(o) * b;
+    dz += (float)czq(o) * b;
   }
 #pragma unroll
   for (int i = 0; i < NQ; ++i) {
