@@ -134,3 +134,23 @@ def test_exterior_plan_keeps_the_physical_domain_and_rejects_unsupported_models(
         Plan.create(p.section, replace(p.flow, sgs="smagorinsky"), explicit.grid)
     with pytest.raises(ValueError, match="uniform"):
         RefinedPlan.create(explicit, RefinementSettings())
+
+
+def test_profile_summary_takes_the_last_dispatches_in_start_order(tmp_path, capsys):
+    """The kernel-trace summary used for the benchmark profiles: per kernel name, the mean of all
+    dispatches and of the last N, in start order whatever the file order."""
+    from aerokinetic_amd import profile_summary as P
+    trace = tmp_path / "t_kernel_trace.csv"
+    trace.write_text(
+        "Kind,Agent_Id,Kernel_Name,Start_Timestamp,End_Timestamp\n"
+        'K,1,"void ak::k_sweep<0, true, false, true>(double*, int)",9000,11000\n'
+        'K,1,"void ak::k_sweep<0, true, false, true>(double*, int)",100,8100\n'
+        'K,1,"void ak::k_sweep<0, true, false, true>(double*, int)",20000,23000\n'
+        'K,1,"ak::k_fill<unsigned char>(unsigned char*, long long)",1,2001\n', encoding="utf-8")
+    d = P.durations(str(trace))
+    assert d['void ak::k_sweep<0, true, false, true>(double*, int)'] == [8.0, 2.0, 3.0]
+    assert P.short('void ak::k_sweep<0, true, false, true>(double*, int)') == "ak::k_sweep<0, true, false, true>"
+    P.main([str(trace), "--last", "2", "--match", "k_sweep"])
+    out = capsys.readouterr().out
+    assert "| `ak::k_sweep<0, true, false, true>` | 3 | 4.3 | 2.5 | 2.0–3.0 |" in out
+    assert "k_fill" not in out
