@@ -209,17 +209,16 @@ typename St<S>::C *perimeter_moments(Lattice *s) {
   return on ? (typename St<S>::C *)s->d_pm : nullptr;
 }
 // The x-face copies of the outgoing and of the entering populations, used where the perimeter
-// moments are, for FP64 and FP32 storage.  Measured on the 1536x1024x64 lattice they cut the
-// boundary phase from 0.100 / 0.115 to 0.082 / 0.087 ms and the sweep by 2.1 / 2.8 % (the farfield
-// no longer leaves partly written lines in the rows the sweep streams).  FP16 keeps the lattice
-// slots: its step time came out the same, with the sweep 4 % slower in direct launches.
+// moments are.  Measured on the 1536x1024x64 lattice they cut the boundary phase from 0.100 /
+// 0.115 / 0.118 to 0.082 / 0.087 / 0.085 ms (FP64 / FP32 / FP16) and the sweep by 2.1 / 2.8 / 2.4 %
+// (the farfield no longer leaves partly written lines in the rows the sweep streams).
 template <int S>
 typename St<S>::T *xface_copy(Lattice *s) {
-  return S != 2 && perimeter_moments<S>(s) && s->d_xo ? (typename St<S>::T *)s->d_xo : nullptr;
+  return perimeter_moments<S>(s) && s->d_xo ? (typename St<S>::T *)s->d_xo : nullptr;
 }
 template <int S>
 typename St<S>::T *xface_in(Lattice *s) {
-  return S != 2 && perimeter_moments<S>(s) && s->d_xi ? (typename St<S>::T *)s->d_xi : nullptr;
+  return perimeter_moments<S>(s) && s->d_xi ? (typename St<S>::T *)s->d_xi : nullptr;
 }
 
 // Perimeter moments of the current populations, when the state was set from outside since
@@ -894,8 +893,7 @@ int ak_set_farfield(void *handle, int kind, double U) {
     std::vector<double> rows((size_t)2 * nz * nx, 0.0);
     if (upload(s, &s->d_rows, rows)) return -1;
     if (dalloc(s, &s->d_pm, pm_count(nx, ny, nz) * (s->storage == 0 ? sizeof(double) : sizeof(float))) ||
-        (s->storage != 2 && (dalloc(s, &s->d_xo, xo_count(ny, nz) * s->elem_size) ||
-                             dalloc(s, &s->d_xi, xo_count(ny, nz) * s->elem_size))))
+        dalloc(s, &s->d_xo, xo_count(ny, nz) * s->elem_size) || dalloc(s, &s->d_xi, xo_count(ny, nz) * s->elem_size))
       return -1;
     FFItems I{s->d_ffg, s->d_ffz, s->d_ffobj, s->d_ffmask, s->d_inc, s->d_der, s->d_clock, s->d_rows, s->nff, nullptr,
               nullptr, nullptr};

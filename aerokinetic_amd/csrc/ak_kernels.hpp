@@ -366,7 +366,7 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
       // copy (the farfield no longer writes the lattice slots these lanes gathered above)
       // (the register index into r stays a compile-time constant, with one select per face:
       // a lane-dependent index becomes a compare-and-select over all 27 registers)
-      if (S != 2 && xi != nullptr && __any(x == 0 || x == L.nx - 1)) {
+      if (xi != nullptr && __any(x == 0 || x == L.nx - 1)) {
         const bool lo = x == 0, hi = !lo && x == L.nx - 1;
         if (lo || hi) {
           RT v[9];
@@ -409,15 +409,30 @@ __device__ __forceinline__ void sweep_block(typename St<S>::T *__restrict__ A, c
           return uniform_ptr(A + (splane + d)) + soff;
         };
         if constexpr (S == 2) {
-          // each value stored as soon as the conserving rounding has made it final
-          fp16_conserving_each(g, [&](int q, _Float16 v) { *(__attribute__((address_space(1))) _Float16 *)dst(q) = v; });
+          // each value stored as soon as the conserving rounding has made it final; waves with an
+          // x-face cell also put its nine outgoing values into the face copy
+          typedef __attribute__((address_space(1))) _Float16 hT;
+          auto put = [&](int q, _Float16 v) { *(hT *)dst(q) = v; };
+          if (xo != nullptr && __any(x == 0 || x == L.nx - 1)) {
+            const bool lo = x == 0, hi = !lo && x == L.nx - 1;
+            fp16_conserving_each(g, [&](int q, _Float16 v) {
+              put(q, v);
+              if (q < 9) {
+                if (lo) ((hT *)xo)[xo_at(L, 0, q, y, z)] = v;
+              } else if (q >= 18) {
+                if (hi) ((hT *)xo)[xo_at(L, 1, q - 18, y, z)] = v;
+              }
+            });
+          } else {
+            fp16_conserving_each(g, put);
+          }
         } else {
 #pragma unroll
           for (int q = 0; q < NQ; ++q) st_g<S>(dst(q), g[q], scale);
         }
         // cells on the x and y faces keep their density deviation and velocity (conserved by
         // the collision) for the next farfield pass, and x-face cells their outgoing populations
-        // (FP64 / FP32 storage only)
+        // (FP16 storage: above, as they are rounded)
         if (pm != nullptr && (x == 0 || y == 0 || x == L.nx - 1 || y == L.ny - 1))
           perimeter_put<C>(pm, L, x, y, z, drho, uc);
         if constexpr (S != 2) {
