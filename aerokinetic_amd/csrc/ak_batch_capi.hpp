@@ -26,6 +26,8 @@ struct Batch {
   BatchMember *d_desc = nullptr;    // [4][cap]: coarse sweep (lead parity 0, 1), fine sweep (substep 0, 1)
   CaseDev *d_cases = nullptr;       // [cap]
   int *d_first = nullptr;           // [BP_COUNT][cap + 1] first block of each member per kernel
+  uint16_t *d_bmem = nullptr;       // member of every sweep block: [nb_coarse] coarse, then [nb_fine] fine
+  size_t bmem_cap = 0;
   int cap = 0, nb_coarse = 0, nb_fine = 0;
   int nblk[BP_COUNT] = {};
   size_t yz_lds = 0;                // dynamic LDS of the y/z prolongation (largest member)
@@ -126,6 +128,25 @@ int build_descriptors(Batch *b) {
     b->nb_fine = first;
   }
   BCHK(hipMemcpy(b->d_desc, h.data(), sizeof(BatchMember) * h.size(), hipMemcpyHostToDevice));
+  // the member of every sweep block (one load per block instead of a bisection over the
+  // descriptors): the coarse sweeps' blocks, then the fine sweeps'
+  if (n > 65535) return bfail("%d members: at most 65535 in a batch", n);
+  std::vector<uint16_t> bm((size_t)b->nb_coarse + b->nb_fine);
+  for (int lev = 0; lev < 2; ++lev) {
+    size_t k = lev ? (size_t)b->nb_coarse : 0;
+    for (int i = 0; i < n; ++i) {
+      const Lattice *l = lev ? b->members[i]->f : b->members[i]->c;
+      for (int j = 0; j < tile_geom(l).nblocks; ++j) bm[k++] = (uint16_t)i;
+    }
+  }
+  if (bm.size() > b->bmem_cap) {
+    if (b->d_bmem) (void)hipFree(b->d_bmem);
+    b->d_bmem = nullptr;
+    b->bmem_cap = 0;
+    BCHK(hipMalloc((void **)&b->d_bmem, sizeof(uint16_t) * bm.size()));
+    b->bmem_cap = bm.size();
+  }
+  BCHK(hipMemcpy(b->d_bmem, bm.data(), sizeof(uint16_t) * bm.size(), hipMemcpyHostToDevice));
   // per-member arguments of the step kernels and their work-list tables
   std::vector<CaseDev> cd((size_t)b->cap);
   std::vector<int> first((size_t)BP_COUNT * (b->cap + 1), 0);
@@ -177,10 +198,9 @@ int build_descriptors(Batch *b) {
 }
 
 template <int S>
-void batched_sweep(Batch *b, const BatchMember *d, int nblocks) {
+void batched_sweep(Batch *b, const BatchMember *d, const uint16_t *bmem, int nblocks) {
   const int grid = b->sweep_grid > 0 ? std::min(nblocks, b->sweep_grid) : nblocks;
-  hipLaunchKernelGGL((k_sweep_batch<S, false>), dim3(grid), dim3(256), 0, b->stream, d, (int)b->members.size(),
-                     nblocks);
+  hipLaunchKernelGGL((k_sweep_batch<S, false>), dim3(grid), dim3(256), 0, b->stream, d, bmem, nblocks);
 }
 
 // Donor moments of every member's current state at the start of a batch run (each step's
@@ -202,7 +222,7 @@ void batch_step(Batch *b, bool sample) {
   const CaseDev *C = b->d_cases;
   auto F = [b](int k) { return b->d_first + (size_t)k * (b->cap + 1); };
   auto go = [&](int k) { return b->nblk[k] > 0; };
-  batched_sweep<S>(b, b->d_desc + (size_t)lead * nc, b->nb_coarse);
+  batched_sweep<S>(b, b->d_desc + (size_t)lead * nc, b->d_bmem, b->nb_coarse);
   for (Refined *r : b->members) r->c->sweeps++;
   // face flux, farfield and donor moments of the new coarse state: one launch (disjoint slots)
   {
@@ -215,7 +235,7 @@ void batch_step(Batch *b, bool sample) {
   if (go(BP_PX)) hipLaunchKernelGGL((kb_prolong_x<S>), dim3(b->nblk[BP_PX]), dim3(64), 0, st, C, F(BP_PX), nm);
   for (int sub = 0; sub < 2; ++sub) {
     if (go(BP_PYZ)) hipLaunchKernelGGL((kb_prolong_yz<S>), dim3(b->nblk[BP_PYZ]), dim3(64), b->yz_lds, st, C, F(BP_PYZ), nm, sub);
-    batched_sweep<S>(b, b->d_desc + (size_t)(2 + sub) * nc, b->nb_fine);
+    batched_sweep<S>(b, b->d_desc + (size_t)(2 + sub) * nc, b->d_bmem + b->nb_coarse, b->nb_fine);
     for (Refined *r : b->members) r->f->sweeps++;
     // fine face flux and walls + loads: one launch
     if (b->nblk[BP_FLUXF] + b->nblk[BP_WALLS] > 0)
@@ -282,7 +302,7 @@ int ak_batch_destroy(void *handle) {
   b->graphs.clear();
   for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
   (void)hipEventDestroy(b->fork);
-  for (void *p : {(void *)b->d_desc, (void *)b->d_cases, (void *)b->d_first})
+  for (void *p : {(void *)b->d_desc, (void *)b->d_cases, (void *)b->d_first, (void *)b->d_bmem})
     if (p) (void)hipFree(p);
   if (b->own_stream) (void)hipStreamDestroy(b->stream);
   delete b;

@@ -481,26 +481,21 @@ struct BatchMember {
 
 // The sweeps of several lattices of one storage type (uneven sizes) in one launch over
 // their concatenated tile lists (256-thread tiles; x/y ghosts kept, z periodic).  The
-// member of a tile is found by bisection over the members' first blocks; each member
+// member of a tile comes from a per-block table (bmem[b] = member index); each member
 // keeps its own parity, relaxation rates, scale and diagnostics.  The grid is either one
 // block per tile or a persistent grid (a multiple of 8 blocks, at most the tile count) whose
 // block i takes tiles i, i + grid, ...: every such index has the same residue mod 8, so the
 // XCD-contiguous remap still keeps each XCD on its own slab of tiles.  Without diagnostics
 // the tile body has no barrier, so blocks may loop over different numbers of tiles.
 template <int S, bool DIAG>
-__global__ __launch_bounds__(256) void k_sweep_batch(const BatchMember *__restrict__ M, int nm, int nblocks) {
+__global__ __launch_bounds__(256) void k_sweep_batch(const BatchMember *__restrict__ M,
+                                                     const uint16_t *__restrict__ bmem, int nblocks) {
   static_assert(!DIAG, "the persistent loop assumes a tile body without block barriers");
   using T = typename St<S>::T;
   using C = typename St<S>::C;
   for (int vb = (int)blockIdx.x; vb < nblocks; vb += (int)gridDim.x) {
     const int b = xcd_remap(vb, nblocks);
-    int lo = 0, hi = nm - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (M[mid].first_block <= b) lo = mid;
-      else hi = mid - 1;
-    }
-    const BatchMember &m = M[lo];
+    const BatchMember &m = M[bmem[b]];
     const int lb = b - m.first_block;
     if (m.odd)
       sweep_block<S, true, false, DIAG>((T *)m.A, m.solid, m.L, m.G, (C)m.ws, (C)m.wb, m.scale, m.inv_scale, m.diag,

@@ -151,14 +151,16 @@ def run_serve(storage, cases, rate, chunk, window, max_steps):
          f"batch steps {rep.gpu_steps}, serving-thread seconds {s['host_s']}")
 
 
-def run_batch(storage, ncases, steps, warmup, chunk=50, sweep_grid=0):
-    """A fixed batch of `ncases` two-level requests (coarse chord 16-24, span 8, Re 1e4-3e4;
-    seed 2026) stepped together in chunks of `chunk` coarse steps; wall time per batch step."""
+def run_batch(storage, ncases, steps, warmup, chunk=50, sweep_grid=0, chord=(16, 24), span=(8, 8),
+              reynolds=(1e4, 3e4)):
+    """A fixed batch of `ncases` two-level requests (coarse chord, span and Re ranges as given,
+    by default 16-24, 8, 1e4-3e4; seed 2026) stepped together in chunks of `chunk` coarse steps;
+    wall time per batch step."""
     import torch
     from . import serving as S
     from .batch import RefinedBatch
     from .refined import RefinedSolver
-    reqs = S.polar_requests(ncases, seed=2026, chord=(16, 24), span=(8, 8), reynolds=(1e4, 3e4))
+    reqs = S.polar_requests(ncases, seed=2026, chord=chord, span=span, reynolds=reynolds)
     members = []
     for r in reqs:
         project, settings, plan = S.project_of(r)
@@ -226,13 +228,18 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, default=16.0, help="serve: Poisson arrival rate (1/s)")
     ap.add_argument("--chunk", type=int, default=100, help="serve: coarse steps per batch call")
     ap.add_argument("--sweep-grid", type=int, default=0, help="batch: persistent sweep grid (0: one block per tile)")
+    ap.add_argument("--chord", default="16,24", help="batch: coarse chord range in cells (the polar sweep: 32,96)")
+    ap.add_argument("--span", default="8,8", help="batch: span range in cells (the polar sweep: 8,32)")
+    ap.add_argument("--reynolds", default="1e4,3e4", help="batch: Reynolds number range (the polar sweep: 1e4,1e5)")
     a = ap.parse_args(argv)
     if a.workload == "copy":
         run_copy(a.copy_gib, a.steps)
     elif a.workload == "traffic":
         run_traffic([x for x in a.storages.split(",") if x], a.copy_gib, _launches(a.launch))
     elif a.workload == "batch":
-        run_batch(a.storage, a.cases, a.steps, a.warmup, sweep_grid=a.sweep_grid)
+        pair = lambda v, f: tuple(f(x) for x in v.split(","))
+        run_batch(a.storage, a.cases, a.steps, a.warmup, sweep_grid=a.sweep_grid, chord=pair(a.chord, int),
+                  span=pair(a.span, int), reynolds=pair(a.reynolds, float))
     elif a.workload == "serve":
         run_serve(a.storage, a.cases, a.rate, a.chunk, 400, a.steps)
     elif a.workload in ("slab-y", "slab-x"):
