@@ -154,3 +154,24 @@ def test_profile_summary_takes_the_last_dispatches_in_start_order(tmp_path, caps
     out = capsys.readouterr().out
     assert "| `ak::k_sweep<0, true, false, true>` | 3 | 4.3 | 2.5 | 2.0–3.0 |" in out
     assert "k_fill" not in out
+
+
+def test_profile_summary_counter_mode_per_wave_and_issue_shares(tmp_path, capsys):
+    """The --pmc summary: counters per wave, the clock implied by SQ_BUSY_CYCLES over the shader
+    engines, and the VALU issue share = VALU instructions per CU / cycles of the dispatch."""
+    from aerokinetic_amd import profile_summary as P
+    f = tmp_path / "c_counter_collection.csv"
+    head = "Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value,Start_Timestamp,End_Timestamp\n"
+    k = '"void ak::k_sweep<2, false, false, true>(ak::St)"'
+    # 2 CUs, 1 engine, 1000 ns at 2 GHz = 2000 cycles; 4 waves x 500 VALU = 2000 -> 1000 per CU = 50 %
+    rows = [f"7,{k},{c},{v},1000,2000" for c, v in
+            (("SQ_WAVES", 4), ("SQ_INSTS_VALU", 2000), ("SQ_INSTS_SALU", 400), ("SQ_BUSY_CYCLES", 2000))]
+    f.write_text(head + "\n".join(rows) + "\n", encoding="utf-8")
+    r, = P.issue_rows(str(f), "k_sweep", cus=2, engines=1)
+    assert r["dispatches"] == 1 and r["us"] == 1.0
+    assert r["SQ_INSTS_VALU"] == 500.0 and r["SQ_INSTS_SALU"] == 100.0
+    assert abs(r["ghz"] - 2.0) < 1e-12
+    assert abs(r["valu_issue"] - 0.5) < 1e-12 and abs(r["salu_issue"] - 0.1) < 1e-12
+    P.main([str(f), "--counters", "--cus", "2", "--engines", "1"])
+    assert "| `ak::k_sweep<2, false, false, true>` | 1 | 1.0 | 100.0 | 500.0 | 2.00 | 50.0% | 10.0% |" \
+        in capsys.readouterr().out
